@@ -1,0 +1,880 @@
+// FP64 CPU oracle for the wholeCellFE per-step mechanics hot path.
+//
+// TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py as the checker.  The product never links it.
+//
+// Part 1 restates the reference functions (file:line cited per function).
+// Part 2 is the planar Newton model the reference lacks; its tangent is the
+// forward-mode derivative (dual numbers) of the Part 1 residual, so it is an
+// independent derivation from the GPU kernels' hand-derived tangents.
+//
+// Compiled with -ffp-contract=off: the reference build had no FMA.
+#include "wcfe_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+// ------------------------------------------------------------------------
+// Scalars: double, or a forward-mode dual number carrying ND derivatives.
+// ------------------------------------------------------------------------
+template <int ND> struct Dual {
+  double v;
+  double d[ND];
+  Dual() : v(0) { for (int i = 0; i < ND; ++i) d[i] = 0; }
+  Dual(double a) : v(a) { for (int i = 0; i < ND; ++i) d[i] = 0; }
+};
+template <int ND> Dual<ND> operator+(const Dual<ND> &a, const Dual<ND> &b) {
+  Dual<ND> r(a.v + b.v);
+  for (int i = 0; i < ND; ++i) r.d[i] = a.d[i] + b.d[i];
+  return r;
+}
+template <int ND> Dual<ND> operator-(const Dual<ND> &a, const Dual<ND> &b) {
+  Dual<ND> r(a.v - b.v);
+  for (int i = 0; i < ND; ++i) r.d[i] = a.d[i] - b.d[i];
+  return r;
+}
+template <int ND> Dual<ND> operator-(const Dual<ND> &a) {
+  Dual<ND> r(-a.v);
+  for (int i = 0; i < ND; ++i) r.d[i] = -a.d[i];
+  return r;
+}
+template <int ND> Dual<ND> operator*(const Dual<ND> &a, const Dual<ND> &b) {
+  Dual<ND> r(a.v * b.v);
+  for (int i = 0; i < ND; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+  return r;
+}
+template <int ND> Dual<ND> operator/(const Dual<ND> &a, const Dual<ND> &b) {
+  Dual<ND> r(a.v / b.v);
+  for (int i = 0; i < ND; ++i) r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v);
+  return r;
+}
+template <int ND> Dual<ND> operator+(const Dual<ND> &a, double b) { return a + Dual<ND>(b); }
+template <int ND> Dual<ND> operator+(double a, const Dual<ND> &b) { return Dual<ND>(a) + b; }
+template <int ND> Dual<ND> operator-(const Dual<ND> &a, double b) { return a - Dual<ND>(b); }
+template <int ND> Dual<ND> operator-(double a, const Dual<ND> &b) { return Dual<ND>(a) - b; }
+template <int ND> Dual<ND> operator*(const Dual<ND> &a, double b) { return a * Dual<ND>(b); }
+template <int ND> Dual<ND> operator*(double a, const Dual<ND> &b) { return Dual<ND>(a) * b; }
+template <int ND> Dual<ND> operator/(const Dual<ND> &a, double b) { return a / Dual<ND>(b); }
+template <int ND> Dual<ND> operator/(double a, const Dual<ND> &b) { return Dual<ND>(a) / b; }
+template <int ND> Dual<ND> sqrt(const Dual<ND> &a) {
+  double s = ::sqrt(a.v);
+  Dual<ND> r(s);
+  for (int i = 0; i < ND; ++i) r.d[i] = a.d[i] / (2.0 * s);
+  return r;
+}
+inline double sqrt(double a) { return ::sqrt(a); }
+inline double val(double a) { return a; }
+template <int ND> double val(const Dual<ND> &a) { return a.v; }
+
+typedef Dual<6> D6;
+
+// 3-vectors and 3x3 matrices (row-major m[3*i+j]) over a scalar S.
+template <class S> struct V3 { S a[3]; };
+template <class S> struct M3 { S m[9]; };
+
+template <class S> V3<S> cross(const V3<S> &u, const V3<S> &w) {
+  V3<S> r;
+  r.a[0] = u.a[1] * w.a[2] - u.a[2] * w.a[1];
+  r.a[1] = u.a[2] * w.a[0] - u.a[0] * w.a[2];
+  r.a[2] = u.a[0] * w.a[1] - u.a[1] * w.a[0];
+  return r;
+}
+template <class S> S norm(const V3<S> &u) { return sqrt(u.a[0] * u.a[0] + (u.a[1] * u.a[1] + u.a[2] * u.a[2])); }
+template <class S> S dot3(const V3<S> &u, const V3<S> &w) { return u.a[0] * w.a[0] + (u.a[1] * w.a[1] + u.a[2] * w.a[2]); }
+
+// Eigen 3.3.4 LU/Determinant.h:18-23,61-69 (row-0 expansion).
+template <class S> S det3(const M3<S> &M) {
+  const S *m = M.m;
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+// Eigen 3.3.4 LU/InverseImpl.h:126-170: cofactors of column 0, det from
+// column 0, invdet, adjugate * invdet.
+template <class S> S cof(const M3<S> &M, int i, int j) {
+  int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+  return M.m[3 * i1 + j1] * M.m[3 * i2 + j2] - M.m[3 * i1 + j2] * M.m[3 * i2 + j1];
+}
+template <class S> M3<S> inv3(const M3<S> &M) {
+  S c0 = cof(M, 0, 0), c1 = cof(M, 1, 0), c2 = cof(M, 2, 0);
+  S det = c0 * M.m[0] + (c1 * M.m[3] + c2 * M.m[6]);
+  S invdet = S(1.0) / det;
+  M3<S> R;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R.m[3 * i + j] = cof(M, j, i) * invdet;
+  return R;
+}
+template <class S> M3<S> matmul(const M3<S> &A, const M3<S> &B) {
+  M3<S> R;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      R.m[3 * i + j] = A.m[3 * i] * B.m[j] + (A.m[3 * i + 1] * B.m[3 + j] + A.m[3 * i + 2] * B.m[6 + j]);
+  return R;
+}
+template <class S> M3<S> transpose(const M3<S> &A) {
+  M3<S> R;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R.m[3 * i + j] = A.m[3 * j + i];
+  return R;
+}
+
+// ------------------------------------------------------------------------
+// UL:710-928 evalElementRe (cell overload), generic in the scalar of the
+// current positions x.
+// ------------------------------------------------------------------------
+// Material constants come from ul_constants() for the reference restatement
+// (hard-coded there) and from the cell parameters for the planar model.
+void ul_constants(int mat, double time, double *mu, double *t_act, double *thick) {
+  *mu = 0;  // UL:721-740
+  if (mat == 1) *mu = 1000;
+  else if (mat == 2) *mu = 1000000;
+  *t_act = 0;  // UL:743-763
+  if (mat == 1) *t_act = (time < 2) ? time * 200.0 : 400;
+  *thick = 1;  // UL:768
+}
+
+template <class S>
+void ul_element(const double *Xd, const S *x, const double *ea, const double *cA, const double *cC, double dt_real,
+                double mu, double t_act, double thick, const double *centroid, S *Re, S *Re_ea, S *MMe, S *ReA,
+                S *ReC) {
+  const double thetag_dot = 0.0;  // UL:714
+
+  V3<S> X0, X1, X2, x0, x1, x2;
+  for (int c = 0; c < 3; ++c) {
+    X0.a[c] = S(Xd[c]); X1.a[c] = S(Xd[3 + c]); X2.a[c] = S(Xd[6 + c]);
+    x0.a[c] = x[c]; x1.a[c] = x[3 + c]; x2.a[c] = x[6 + c];
+  }
+  // reference metric and normal UL:771-774
+  V3<S> G1, G2;
+  for (int c = 0; c < 3; ++c) { G1.a[c] = X1.a[c] - X0.a[c]; G2.a[c] = X2.a[c] - X0.a[c]; }
+  V3<S> G1xG2 = cross(G1, G2);
+  S nG = norm(G1xG2);
+  V3<S> N;
+  for (int c = 0; c < 3; ++c) N.a[c] = G1xG2.a[c] / nG;
+  // centroid strain and be_t UL:778-792
+  double third = 1.0 / 3.0;
+  double eav[6];
+  for (int q = 0; q < 6; ++q) eav[q] = (third * ea[q] + third * ea[6 + q]) + third * ea[12 + q];
+  M3<S> aux;
+  const int vo[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+  for (int k = 0; k < 9; ++k) aux.m[k] = S(((k % 4) == 0 ? 1.0 : 0.0) - 2.0 * eav[vo[k]]);
+  M3<S> be_t = inv3(aux);
+  M3<S> NN, be_t_s;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      NN.m[3 * i + j] = N.a[i] * N.a[j];
+      be_t_s.m[3 * i + j] = be_t.m[3 * i + j] - NN.m[3 * i + j];
+    }
+  // deformed metric UL:802-806
+  V3<S> g1, g2;
+  for (int c = 0; c < 3; ++c) { g1.a[c] = x1.a[c] - x0.a[c]; g2.a[c] = x2.a[c] - x0.a[c]; }
+  V3<S> g12 = cross(g1, g2);
+  S ng = norm(g12);
+  V3<S> n;
+  for (int c = 0; c < 3; ++c) n.a[c] = g12.a[c] / ng;
+  // G, g, inverses, F, C UL:810-832
+  M3<S> G, g;
+  for (int r = 0; r < 3; ++r) {
+    G.m[3 * r + 0] = G1.a[r]; G.m[3 * r + 1] = G2.a[r]; G.m[3 * r + 2] = N.a[r];
+    g.m[3 * r + 0] = g1.a[r]; g.m[3 * r + 1] = g2.a[r]; g.m[3 * r + 2] = n.a[r];
+  }
+  M3<S> Ginv = inv3(G);
+  V3<S> G1i, G2i;
+  for (int c = 0; c < 3; ++c) { G1i.a[c] = Ginv.m[c]; G2i.a[c] = Ginv.m[3 + c]; }
+  M3<S> F;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      F.m[3 * i + j] = (g1.a[i] * G1i.a[j] + g2.a[i] * G2i.a[j]) + n.a[i] * N.a[j];
+  M3<S> Ft = transpose(F);
+  M3<S> C = matmul(Ft, F);
+  M3<S> Cinv = inv3(C);
+  M3<S> Cinv_s;
+  for (int k = 0; k < 9; ++k) Cinv_s.m[k] = Cinv.m[k] - NN.m[k];
+  // concentration gradients UL:835-845
+  M3<S> ginv = inv3(g);
+  V3<S> g1i, g2i;
+  for (int c = 0; c < 3; ++c) { g1i.a[c] = ginv.m[c]; g2i.a[c] = ginv.m[3 + c]; }
+  V3<S> gradA, gradC;
+  for (int c = 0; c < 3; ++c) {
+    S m12 = -g1i.a[c] - g2i.a[c];
+    gradA.a[c] = (cA[0] * g1i.a[c] + cA[1] * g2i.a[c]) + cA[2] * m12;
+    gradC.a[c] = (cC[0] * g1i.a[c] + cC[1] * g2i.a[c]) + cC[2] * m12;
+  }
+  double c_A = (1. / 3.) * (cA[0] + cA[1] + cA[2]);
+  double c_C = (1. / 3.) * (cC[0] + cC[1] + cC[2]);
+  // area changes UL:848-862
+  S theta_e_t = sqrt(det3(be_t));
+  double theta_g = 1.0 + thetag_dot * dt_real;
+  S theta = det3(F);
+  S theta_e = theta * theta_e_t / theta_g;
+  S lamda_N = S(1.0) / theta_e;
+  S p = -mu * (lamda_N * lamda_N);
+  // stresses UL:865-880
+  M3<S> Spas;
+  for (int k = 0; k < 9; ++k) Spas.m[k] = (mu / theta_g) * be_t_s.m[k] + theta_g * p * Cinv_s.m[k];
+  double kappa = 0.5;
+  V3<S> a0;
+  for (int c = 0; c < 3; ++c) a0.a[c] = (1. / 3.) * ((x0.a[c] + x1.a[c]) + x2.a[c]) - centroid[c];
+  S na0 = norm(a0);
+  for (int c = 0; c < 3; ++c) a0.a[c] = a0.a[c] / na0;
+  M3<S> Ssum;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      S sact = t_act * ((i == j ? kappa : 0.0) + (1 - 2 * kappa) * (a0.a[i] * a0.a[j]));
+      Ssum.m[3 * i + j] = sact + Spas.m[3 * i + j];
+    }
+  // updated strain UL:884-887
+  M3<S> be = matmul(matmul(F, be_t), Ft);
+  for (int k = 0; k < 9; ++k) be.m[k] = (1.0 / (theta_g * theta_g)) * be.m[k];
+  M3<S> bei = inv3(be);
+  S eas[6];
+  {
+    M3<S> es;
+    for (int k = 0; k < 9; ++k) es.m[k] = 0.5 * (((k % 4) == 0 ? 1.0 : 0.0) - bei.m[k]);
+    eas[0] = es.m[0]; eas[1] = es.m[4]; eas[2] = es.m[8];
+    eas[3] = es.m[1]; eas[4] = es.m[2]; eas[5] = es.m[5];
+  }
+  // residual UL:893-927
+  S Sv[6] = {Ssum.m[0], Ssum.m[4], Ssum.m[8], Ssum.m[1], Ssum.m[2], Ssum.m[5]};
+  for (int i = 0; i < 3; ++i) {
+    for (int pp = 0; pp < 3; ++pp) {
+      double vg1[3] = {0, 0, 0}, vg2[3] = {0, 0, 0};
+      if (i == 0) { vg1[pp] = -1.0; vg2[pp] = -1.0; }
+      if (i == 1) vg1[pp] = 1.0;
+      if (i == 2) vg2[pp] = 1.0;
+      M3<S> varF;
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) varF.m[3 * r + c] = vg1[r] * G1i.a[c] + vg2[r] * G2i.a[c];
+      M3<S> vC1 = matmul(transpose(varF), F), vC2 = matmul(Ft, varF);
+      M3<S> varE;
+      for (int k = 0; k < 9; ++k) varE.m[k] = 0.5 * (vC1.m[k] + vC2.m[k]);
+      S vEv[6] = {varE.m[0], varE.m[4], varE.m[8], 2 * varE.m[1], 2 * varE.m[2], 2 * varE.m[5]};
+      S dotv = Sv[0] * vEv[0];
+      for (int q = 1; q < 6; ++q) dotv = dotv + Sv[q] * vEv[q];
+      Re[i * 3 + pp] = dotv * nG / 2.0 * thick;
+    }
+    for (int q = 0; q < 6; ++q) Re_ea[i * 6 + q] = (1. / 3.) * (eas[q] - ea[6 * i + q]) * nG / 2.0 * thick;
+    MMe[i] = (1. / 3.) * nG / 2.0 * thick;
+    V3<S> gradNi;
+    for (int c = 0; c < 3; ++c) {
+      S m12 = -g1i.a[c] - g2i.a[c];
+      gradNi.a[c] = (i == 0 ? g1i.a[c] : (i == 1 ? g2i.a[c] : m12));
+    }
+    const double D_A = 1., D_C = 1e-10;
+    ReA[i] = (c_A * (1. / 3.) - dt_real * D_A * dot3(gradA, gradNi)) * nG / 2.0 * thick;
+    ReC[i] = (c_C * (1. / 3.) - dt_real * D_C * dot3(gradC, gradNi)) * nG / 2.0 * thick;
+  }
+}
+
+// MD table UL:1207-1214 (kMD variant, 6 knots).
+const int MD_N = 6;
+const double MD_PX[MD_N] = {0., 2.51250583e-03, 4.40265201e-03, 5.91408228e-03, 11.76817302e-03, 17.001e-03};
+const double MD_PY[MD_N] = {-7.15514497, 71.80501308, 356.62420075, 356.96656556, 126.21811585, 528.40481124};
+const double MD_K[MD_N] = {31426.8556, 99485.86235, 67080.67086, -7908.39043, 21978.66923, 76858.39733};
+
+// Table lookup exactly as UL:1258-1264 (later matches overwrite earlier ones).
+template <class S>
+S table_force(int npts, const double *px, const double *py, const double *k, const S &D, bool *found) {
+  S f(0.0);
+  *found = false;
+  for (int j = 0; j < npts - 1; j++) {
+    if (val(D) < px[j + 1] && val(D) >= px[j]) {
+      f = py[j] + k[j] * (D - px[j]);
+      *found = true;
+    } else if (val(D) >= px[npts - 1]) {
+      f = py[npts - 1] + k[npts - 1] * (D - px[npts - 1]);
+      *found = true;
+    }
+  }
+  return f;
+}
+
+}  // namespace
+
+// ==========================================================================
+// Part 1: reference restatement
+// ==========================================================================
+extern "C" void orc_eval_element_re(const double *X, const double *x, const double *ea, const double *cA,
+                                    const double *cC, double dt, double time, const double *centroid, int mat,
+                                    double *Re, double *Re_ea, double *MMe, double *ReA, double *ReC) {
+  double mu, t_act, thick;
+  ul_constants(mat, time, &mu, &t_act, &thick);
+  ul_element<double>(X, x, ea, cA, cC, dt, mu, t_act, thick, centroid, Re, Re_ea, MMe, ReA, ReC);
+}
+
+extern "C" void orc_area_tot(int n_tris, const int32_t *tris, const double *x, double *out) {
+  // UL:1662-1688
+  double area_tot = 0, cen[3] = {0, 0, 0};
+  for (int e = 0; e < n_tris; ++e) {
+    const double *p0 = x + 3 * tris[3 * e], *p1 = x + 3 * tris[3 * e + 1], *p2 = x + 3 * tris[3 * e + 2];
+    V3<double> a, b;
+    for (int c = 0; c < 3; ++c) { a.a[c] = p1[c] - p0[c]; b.a[c] = p2[c] - p0[c]; }
+    V3<double> cr = cross(a, b);
+    for (int c = 0; c < 3; ++c) cr.a[c] *= 0.5;
+    double ar = norm(cr);
+    for (int c = 0; c < 3; ++c) cen[c] += (1. / 3. * (p0[c] + p1[c] + p2[c])) * ar;
+    area_tot += ar;
+  }
+  out[0] = area_tot;
+  for (int c = 0; c < 3; ++c) out[1 + c] = cen[c] / area_tot;
+}
+
+extern "C" void orc_calculate_res(int n_nodes, int n_tris, const int32_t *tris, const double *X, const double *x,
+                                  const double *ea, const double *cA, const double *cC, double dt, double time,
+                                  const double *centroid, int mat, double *Res, double *Res_ea, double *MM,
+                                  double *ResA, double *ResC) {
+  // UL:632-707: zero, element loop, element-order += scatter.
+  memset(Res, 0, sizeof(double) * 3 * n_nodes);
+  memset(Res_ea, 0, sizeof(double) * 6 * n_nodes);
+  memset(MM, 0, sizeof(double) * n_nodes);
+  memset(ResA, 0, sizeof(double) * n_nodes);
+  memset(ResC, 0, sizeof(double) * n_nodes);
+  for (int e = 0; e < n_tris; ++e) {
+    int v[3] = {tris[3 * e], tris[3 * e + 1], tris[3 * e + 2]};
+    double Xe[9], xe[9], eae[18], ca[3], cc[3];
+    for (int a = 0; a < 3; ++a) {
+      for (int c = 0; c < 3; ++c) { Xe[3 * a + c] = X[3 * v[a] + c]; xe[3 * a + c] = x[3 * v[a] + c]; }
+      for (int q = 0; q < 6; ++q) eae[6 * a + q] = ea[6 * v[a] + q];
+      ca[a] = cA[v[a]];
+      cc[a] = cC[v[a]];
+    }
+    double Re[9], Rea[18], MMe[3], RA[3], RC[3], mu, t_act, thick;
+    ul_constants(mat, time, &mu, &t_act, &thick);
+    ul_element<double>(Xe, xe, eae, ca, cc, dt, mu, t_act, thick, centroid, Re, Rea, MMe, RA, RC);
+    for (int a = 0; a < 3; ++a) {
+      for (int c = 0; c < 3; ++c) Res[3 * v[a] + c] += Re[3 * a + c];
+      for (int q = 0; q < 6; ++q) Res_ea[6 * v[a] + q] += Rea[6 * a + q];
+      MM[v[a]] += MMe[a];
+      ResA[v[a]] += RA[a];
+      ResC[v[a]] += RC[a];
+    }
+  }
+}
+
+namespace {
+// Node one-ring lists in increasing triangle order (nondestructivetrimesh.cpp:489-506).
+void one_rings(int n_nodes, int n_tris, const int32_t *tris, std::vector<int> &start, std::vector<int> &list) {
+  start.assign(n_nodes + 1, 0);
+  for (int e = 0; e < n_tris; ++e)
+    for (int a = 0; a < 3; ++a) start[tris[3 * e + a] + 1]++;
+  for (int i = 0; i < n_nodes; ++i) start[i + 1] += start[i];
+  list.assign(start[n_nodes], 0);
+  std::vector<int> fill(start.begin(), start.end() - 1);
+  for (int e = 0; e < n_tris; ++e)
+    for (int a = 0; a < 3; ++a) list[fill[tris[3 * e + a]]++] = e;
+}
+}  // namespace
+
+extern "C" void orc_force_fa(int n_nodes, int n_tris, const int32_t *tris, const double *x, const double *cC,
+                             const double *fau, int law, double kint, double *F) {
+  // UL:1198-1303 (law 1) and UL1:1196-1295 (law 0)
+  const double rho_i_max = 100;
+  std::vector<int> st, li;
+  one_rings(n_nodes, n_tris, tris, st, li);
+  for (int ni = 0; ni < n_nodes; ++ni) {
+    double Asum = 0.0;
+    for (int k = st[ni]; k < st[ni + 1]; ++k) {
+      int e = li[k];
+      const double *p0 = x + 3 * tris[3 * e], *p1 = x + 3 * tris[3 * e + 1], *p2 = x + 3 * tris[3 * e + 2];
+      V3<double> E1, E2;
+      for (int c = 0; c < 3; ++c) { E1.a[c] = p1[c] - p0[c]; E2.a[c] = p2[c] - p0[c]; }
+      double area = 0.5 * norm(cross(E1, E2));
+      Asum += area / 3;
+    }
+    const double *u = fau + 3 * ni;
+    double fs[3] = {0, 0, 0};
+    if (law == 0) {
+      for (int c = 0; c < 3; ++c) fs[c] = kint * u[c];
+    } else {
+      double D = ::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      double uD[3] = {0, 0, 0};
+      if (D != 0)
+        for (int c = 0; c < 3; ++c) uD[c] = u[c] / D;
+      bool found;
+      double f = table_force<double>(MD_N, MD_PX, MD_PY, MD_K, D, &found);
+      for (int c = 0; c < 3; ++c) fs[c] = f * uD[c];
+    }
+    for (int c = 0; c < 3; ++c) F[3 * ni + c] = -cC[ni] * fs[c] * Asum * rho_i_max;
+  }
+}
+
+extern "C" void orc_update_fa(int n_nodes, int n_tris, const int32_t *tris, const double *X, const double *x,
+                              double *cA, double *cC, double *fau, double dt, int n_snodes, int n_stris,
+                              const int32_t *stris, const double *SX, const double *Sx, int law, double kint,
+                              int *n_unlocated) {
+  // UL:2026-2174 (law 1) and UL1:2005-2115 (law 0)
+  (void)n_tris; (void)tris; (void)n_snodes;
+  int unloc = 0;
+  for (int ni = 0; ni < n_nodes; ++ni) {
+    double cell_u[3], sX[3], subs_u[3] = {0, 0, 0};
+    bool located = false;
+    for (int c = 0; c < 3; ++c) {
+      cell_u[c] = x[3 * ni + c] - X[3 * ni + c];
+      sX[c] = X[3 * ni + c] - fau[3 * ni + c];
+    }
+    // brute-force point location, last containing triangle wins (UL:2061-2100)
+    for (int e = 0; e < n_stris; ++e) {
+      const int *t = stris + 3 * e;
+      V3<double> e1, e2, ep;
+      for (int c = 0; c < 3; ++c) {
+        e1.a[c] = SX[3 * t[1] + c] - SX[3 * t[0] + c];
+        e2.a[c] = SX[3 * t[2] + c] - SX[3 * t[0] + c];
+        ep.a[c] = sX[c] - SX[3 * t[0] + c];
+      }
+      double A_tri = 0.5 * norm(cross(e1, e2));
+      V3<double> A3n = cross(e1, ep);
+      double A3 = -1;
+      if (A3n.a[2] >= 0) A3 = 0.5 * norm(A3n);
+      double A2 = -1;
+      V3<double> A2n = cross(ep, e2);
+      if (A2n.a[2] >= 0) A2 = 0.5 * norm(A2n);
+      double A1 = A_tri - A2 - A3;
+      if (A1 >= 0 && A2 >= 0 && A3 >= 0) {
+        for (int c = 0; c < 3; ++c)
+          subs_u[c] = ((A1 / A_tri) * (Sx[3 * t[0] + c] - SX[3 * t[0] + c]) +
+                       (A2 / A_tri) * (Sx[3 * t[1] + c] - SX[3 * t[1] + c])) +
+                      (A3 / A_tri) * (Sx[3 * t[2] + c] - SX[3 * t[2] + c]);
+        located = true;
+      }
+    }
+    if (!located) ++unloc;  // the reference leaves subs_u uninitialised here; defined as 0
+    for (int c = 0; c < 3; ++c) fau[3 * ni + c] = (fau[3 * ni + c] + cell_u[c]) - subs_u[c];
+    double force_FA;
+    const double *u = fau + 3 * ni;
+    if (law == 0) {
+      force_FA = kint * ::sqrt(u[0] * u[0] + u[1] * u[1]);  // 2D norm, UL1:2077
+    } else {
+      double D = ::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      double uD[3] = {0, 0, 0};
+      if (D != 0)
+        for (int c = 0; c < 3; ++c) uD[c] = u[c] / D;
+      bool found;
+      double f = table_force<double>(MD_N, MD_PX, MD_PY, MD_K, D, &found);
+      double fs[3] = {f * uD[0], f * uD[1], f * uD[2]};
+      force_FA = ::sqrt(fs[0] * fs[0] + fs[1] * fs[1] + fs[2] * fs[2]);
+    }
+    const double k_on = 1, ka = 0.004, kb = 10, Fa = 15, Fb = 15;  // UL:2139-2143
+    double k_off = ka * ::exp(force_FA / Fa) + kb * ::exp(-force_FA / Fb);
+    double c_C_old = cC[ni];
+    cC[ni] = cC[ni] + dt * k_on * (1 - cC[ni]) - dt * k_off * cC[ni];
+    if (cC[ni] < 0) cC[ni] = 0;
+    if (cA[ni] < 0) cA[ni] = 0;
+    if (c_C_old - dt * k_off * c_C_old > 0 && cC[ni] > 0) {
+      for (int c = 0; c < 3; ++c) fau[3 * ni + c] = fau[3 * ni + c] * (c_C_old - dt * k_off * c_C_old) / cC[ni];
+    } else {
+      for (int c = 0; c < 3; ++c) fau[3 * ni + c] = 0.0 * fau[3 * ni + c];
+    }
+  }
+  if (n_unlocated) *n_unlocated = unloc;
+}
+
+// ==========================================================================
+// Part 2: planar Newton model
+// ==========================================================================
+namespace {
+
+double t_act_of(double time) { return (time < 2) ? time * 200.0 : 400.0; }  // UL:743-757
+
+// Spring force vector for extension u (2D): (D == 0) ? k0 * u : f(D) * u / D.
+template <class S> void spring_vec(const orc_cell *c, const S *u, S *out) {
+  if (c->spring_law == ORC_SPRING_LINEAR) {
+    out[0] = c->k_lin * u[0];
+    out[1] = c->k_lin * u[1];
+    return;
+  }
+  S D2 = u[0] * u[0] + u[1] * u[1];
+  if (val(D2) == 0.0) {
+    out[0] = c->tab_k[0] * u[0];
+    out[1] = c->tab_k[0] * u[1];
+    return;
+  }
+  S D = sqrt(D2);
+  bool found;
+  S f = table_force<S>(c->tab_n, c->tab_px, c->tab_py, c->tab_k, D, &found);
+  out[0] = f * u[0] / D;
+  out[1] = f * u[1] / D;
+}
+
+// Element equilibrium contribution G_e(x_e) (6 entries), generic in the scalar.
+template <class S>
+void element_G(const orc_cell *c, int e, const S *xe /*6*/, const std::vector<int> &sst, const std::vector<int> &sli,
+               S *Ge) {
+  const int32_t *t = c->tris + 3 * e;
+  for (int k = 0; k < 6; ++k) Ge[k] = S(0.0);
+  if (c->law == ORC_LAW_UL) {
+    double Xe[9], ea6[18], zero3[3] = {0, 0, 0};
+    S x9[9];
+    for (int a = 0; a < 3; ++a) {
+      Xe[3 * a] = c->X[2 * t[a]];
+      Xe[3 * a + 1] = c->X[2 * t[a] + 1];
+      Xe[3 * a + 2] = 0;
+      x9[3 * a] = xe[2 * a];
+      x9[3 * a + 1] = xe[2 * a + 1];
+      x9[3 * a + 2] = S(0.0);
+      const double *E3 = c->ea + 3 * t[a];
+      double *E6 = ea6 + 6 * a;
+      E6[0] = E3[0]; E6[1] = E3[1]; E6[2] = 0; E6[3] = E3[2]; E6[4] = 0; E6[5] = 0;
+    }
+    // far-away centroid: (1 - 2 kappa) = 0 removes a0 (UL:872-878); avoids the
+    // 0/0 of an element centroid coinciding with the cell centroid.
+    const double far[3] = {1e30, 1e30, 0};
+    S Re[9], Rea[18], MMe[3], RA[3], RC[3];
+    ul_element<S>(Xe, x9, ea6, zero3, zero3, 0.0, c->mu, t_act_of(c->time), c->thick, far, Re, Rea, MMe, RA, RC);
+    for (int a = 0; a < 3; ++a) {
+      Ge[2 * a] = Re[3 * a];
+      Ge[2 * a + 1] = Re[3 * a + 1];
+    }
+  } else {
+    // Linear plane-stress CST (small strain), active isotropic stress 0.5 t_act.
+    double X0 = c->X[2 * t[0]], Y0 = c->X[2 * t[0] + 1];
+    double X1 = c->X[2 * t[1]], Y1 = c->X[2 * t[1] + 1];
+    double X2 = c->X[2 * t[2]], Y2 = c->X[2 * t[2] + 1];
+    double det = (X1 - X0) * (Y2 - Y0) - (X2 - X0) * (Y1 - Y0);
+    double A = 0.5 * fabs(det);
+    // shape-function gradients
+    double b[3] = {(Y1 - Y2) / det, (Y2 - Y0) / det, (Y0 - Y1) / det};
+    double d[3] = {(X2 - X1) / det, (X0 - X2) / det, (X1 - X0) / det};
+    S ux[3], uy[3];
+    for (int a = 0; a < 3; ++a) {
+      ux[a] = xe[2 * a] - c->X[2 * t[a]];
+      uy[a] = xe[2 * a + 1] - c->X[2 * t[a] + 1];
+    }
+    S exx = b[0] * ux[0] + b[1] * ux[1] + b[2] * ux[2];
+    S eyy = d[0] * uy[0] + d[1] * uy[1] + d[2] * uy[2];
+    S gxy = (d[0] * ux[0] + d[1] * ux[1] + d[2] * ux[2]) + (b[0] * uy[0] + b[1] * uy[1] + b[2] * uy[2]);
+    double Dc = c->E / (1.0 - c->nu * c->nu);
+    double sa = 0.5 * t_act_of(c->time);
+    S sxx = Dc * (exx + c->nu * eyy) + sa;
+    S syy = Dc * (c->nu * exx + eyy) + sa;
+    S sxy = Dc * 0.5 * (1.0 - c->nu) * gxy;
+    double At = A * c->thick;
+    for (int a = 0; a < 3; ++a) {
+      Ge[2 * a] = At * (b[a] * sxx + d[a] * sxy);
+      Ge[2 * a + 1] = At * (d[a] * syy + b[a] * sxy);
+    }
+  }
+  // barycentric springs hosted by this element: G += b_a * spring_vec(p(x) - anchor) * state
+  for (int k = sst[e]; k < sst[e + 1]; ++k) {
+    int s = sli[k];
+    double st = c->spr_state[s];
+    if (st == 0.0) continue;
+    const double *bb = c->spr_bary + 3 * s;
+    S u[2];
+    for (int q = 0; q < 2; ++q)
+      u[q] = (bb[0] * xe[q] + bb[1] * xe[2 + q] + bb[2] * xe[4 + q]) - c->spr_anchor[2 * s + q];
+    S f[2];
+    spring_vec<S>(c, u, f);
+    for (int a = 0; a < 3; ++a)
+      for (int q = 0; q < 2; ++q) Ge[2 * a + q] = Ge[2 * a + q] + (bb[a] * st) * f[q];
+  }
+  // node-attached springs (UL:1226-1269): G_a += (area_e / 3) * cC rho f(u_a)
+  if (c->node_springs) {
+    S cz = (xe[2] - xe[0]) * (xe[5] - xe[1]) - (xe[3] - xe[1]) * (xe[4] - xe[0]);
+    S area = 0.5 * sqrt(cz * cz);
+    for (int a = 0; a < 3; ++a) {
+      int v = t[a];
+      S u[2];
+      for (int q = 0; q < 2; ++q) u[q] = c->fau0[2 * v + q] + (xe[2 * a + q] - c->X[2 * v + q]);
+      S f[2];
+      spring_vec<S>(c, u, f);
+      for (int q = 0; q < 2; ++q) Ge[2 * a + q] = Ge[2 * a + q] + (area / 3.0) * (c->cC[v] * c->rho_max) * f[q];
+    }
+  }
+}
+
+void spring_lists(const orc_cell *c, std::vector<int> &st, std::vector<int> &li) {
+  st.assign(c->n_tris + 1, 0);
+  for (int s = 0; s < c->n_spr; ++s) st[c->spr_tri[s] + 1]++;
+  for (int e = 0; e < c->n_tris; ++e) st[e + 1] += st[e];
+  li.assign(c->n_spr, 0);
+  std::vector<int> fill(st.begin(), st.end() - 1);
+  for (int s = 0; s < c->n_spr; ++s) li[fill[c->spr_tri[s]]++] = s;
+}
+
+void element_system_impl(const orc_cell *c, int e, const std::vector<int> &sst, const std::vector<int> &sli,
+                         double *Ge, double *Ke) {
+  const int32_t *t = c->tris + 3 * e;
+  D6 xe[6];
+  for (int a = 0; a < 3; ++a)
+    for (int q = 0; q < 2; ++q) {
+      xe[2 * a + q] = D6(c->x[2 * t[a] + q]);
+      xe[2 * a + q].d[2 * a + q] = 1.0;
+    }
+  D6 G[6];
+  element_G<D6>(c, e, xe, sst, sli, G);
+  for (int r = 0; r < 6; ++r) {
+    Ge[r] = G[r].v;
+    if (Ke)
+      for (int k = 0; k < 6; ++k) Ke[6 * r + k] = G[r].d[k];
+  }
+}
+
+}  // namespace
+
+extern "C" void orc_spring_law(const orc_cell *c, double D, double *f, double *df) {
+  if (c->spring_law == ORC_SPRING_LINEAR) {
+    *f = c->k_lin * D;
+    *df = c->k_lin;
+    return;
+  }
+  Dual<1> d(D);
+  d.d[0] = 1.0;
+  bool found;
+  Dual<1> r = table_force<Dual<1>>(c->tab_n, c->tab_px, c->tab_py, c->tab_k, d, &found);
+  *f = r.v;
+  *df = r.d[0];
+}
+
+extern "C" int orc_color_elements(int n_nodes, int n_tris, const int32_t *tris, int32_t *color) {
+  std::vector<uint64_t> used(n_nodes, 0);
+  int nc = 0;
+  for (int e = 0; e < n_tris; ++e) {
+    uint64_t forb = used[tris[3 * e]] | used[tris[3 * e + 1]] | used[tris[3 * e + 2]];
+    int col = 0;
+    while (col < 64 && ((forb >> col) & 1ull)) ++col;
+    if (col == 64) return -1;
+    color[e] = col;
+    for (int a = 0; a < 3; ++a) used[tris[3 * e + a]] |= (1ull << col);
+    nc = std::max(nc, col + 1);
+  }
+  return nc;
+}
+
+extern "C" void orc_element_system(const orc_cell *c, int e, const int32_t *spr_start, double *Ge, double *Ke) {
+  (void)spr_start;
+  std::vector<int> sst, sli;
+  spring_lists(c, sst, sli);
+  element_system_impl(c, e, sst, sli, Ge, Ke);
+}
+
+extern "C" void orc_assemble(const orc_cell *c, const int32_t *color, int n_colors, double *G, double *K) {
+  const int n = 2 * c->n_nodes;
+  memset(G, 0, sizeof(double) * n);
+  if (K) memset(K, 0, sizeof(double) * (size_t)n * n);
+  std::vector<int> sst, sli;
+  spring_lists(c, sst, sli);
+  int passes = color ? n_colors : 1;
+  for (int pc = 0; pc < passes; ++pc) {
+    for (int e = 0; e < c->n_tris; ++e) {
+      if (color && color[e] != pc) continue;
+      double Ge[6], Ke[36];
+      element_system_impl(c, e, sst, sli, Ge, K ? Ke : nullptr);
+      const int32_t *t = c->tris + 3 * e;
+      for (int r = 0; r < 6; ++r) {
+        int gr = 2 * t[r / 2] + (r % 2);
+        G[gr] += Ge[r];
+        if (K)
+          for (int k = 0; k < 6; ++k) K[(size_t)gr * n + 2 * t[k / 2] + (k % 2)] += Ke[6 * r + k];
+      }
+    }
+  }
+}
+
+template <class T> static int lu_factor_t(int n, T *A, int32_t *ipiv, int pivoting) {
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    if (pivoting) {
+      T best = fabs(A[(size_t)k * n + k]);
+      for (int i = k + 1; i < n; ++i) {
+        T v = fabs(A[(size_t)i * n + k]);
+        if (v > best) { best = v; p = i; }
+      }
+      ipiv[k] = p;
+      if (p != k)
+        for (int j = 0; j < n; ++j) std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]);
+    } else if (ipiv) {
+      ipiv[k] = k;
+    }
+    T piv = A[(size_t)k * n + k];
+    if (piv == 0) return k + 1;
+    for (int i = k + 1; i < n; ++i) {
+      T l = A[(size_t)i * n + k] / piv;
+      A[(size_t)i * n + k] = l;
+      if (l != 0)
+        for (int j = k + 1; j < n; ++j) A[(size_t)i * n + j] -= l * A[(size_t)k * n + j];
+    }
+  }
+  return 0;
+}
+
+extern "C" int orc_lu_factor(int n, double *A, int32_t *ipiv, int pivoting) { return lu_factor_t<double>(n, A, ipiv, pivoting); }
+extern "C" int orc_lu_factor_f32(int n, float *A, int32_t *ipiv, int pivoting) { return lu_factor_t<float>(n, A, ipiv, pivoting); }
+
+extern "C" void orc_lu_solve(int n, const double *LU, const int32_t *ipiv, int pivoting, double *b) {
+  if (pivoting)
+    for (int k = 0; k < n; ++k)
+      if (ipiv[k] != k) std::swap(b[k], b[ipiv[k]]);
+  for (int i = 0; i < n; ++i) {
+    double s = b[i];
+    for (int j = 0; j < i; ++j) s -= LU[(size_t)i * n + j] * b[j];
+    b[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= LU[(size_t)i * n + j] * b[j];
+    b[i] = s / LU[(size_t)i * n + i];
+  }
+}
+
+extern "C" int orc_pcg_csr(int n, const int32_t *rowptr, const int32_t *col, const double *val_, const double *b,
+                           double *x, double rtol, int maxit) {
+  std::vector<double> r(n), z(n), p(n), q(n), dinv(n);
+  for (int i = 0; i < n; ++i) {
+    double d = 0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if (col[k] == i) d = val_[k];
+    dinv[i] = d != 0 ? 1.0 / d : 1.0;
+  }
+  auto spmv = [&](const double *v, double *out) {
+    for (int i = 0; i < n; ++i) {
+      double s = 0;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) s += val_[k] * v[col[k]];
+      out[i] = s;
+    }
+  };
+  spmv(x, q.data());
+  double bn = 0;
+  for (int i = 0; i < n; ++i) { r[i] = b[i] - q[i]; bn += b[i] * b[i]; }
+  bn = ::sqrt(bn);
+  if (bn == 0) bn = 1;
+  double rz = 0;
+  for (int i = 0; i < n; ++i) { z[i] = dinv[i] * r[i]; p[i] = z[i]; rz += r[i] * z[i]; }
+  for (int it = 0; it < maxit; ++it) {
+    double rn = 0;
+    for (int i = 0; i < n; ++i) rn += r[i] * r[i];
+    if (::sqrt(rn) <= rtol * bn) return it;
+    spmv(p.data(), q.data());
+    double pq = 0;
+    for (int i = 0; i < n; ++i) pq += p[i] * q[i];
+    double alpha = rz / pq;
+    for (int i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
+    double rz2 = 0;
+    for (int i = 0; i < n; ++i) { z[i] = dinv[i] * r[i]; rz2 += r[i] * z[i]; }
+    double beta = rz2 / rz;
+    rz = rz2;
+    for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+  }
+  return -1;
+}
+
+extern "C" int orc_newton_solve(orc_cell *c, double *x, double rtol, double atol, int maxit, double *res_hist) {
+  const int n = 2 * c->n_nodes;
+  std::vector<double> G(n), K((size_t)n * n);
+  std::vector<int32_t> ipiv(n);
+  std::vector<int32_t> color(c->n_tris);
+  int nc = orc_color_elements(c->n_nodes, c->n_tris, c->tris, color.data());
+  const double *xsave = c->x;
+  c->x = x;
+  double r0 = -1;
+  int ret = -1;
+  for (int it = 0; it <= maxit; ++it) {
+    orc_assemble(c, color.data(), nc, G.data(), K.data());
+    double rn = 0;
+    for (int i = 0; i < n; ++i) rn += G[i] * G[i];
+    rn = ::sqrt(rn);
+    if (res_hist) res_hist[it] = rn;
+    if (r0 < 0) r0 = rn;
+    if (rn <= rtol * r0 || rn <= atol) { ret = it; break; }
+    if (it == maxit) break;
+    if (orc_lu_factor(n, K.data(), ipiv.data(), 1)) break;
+    for (int i = 0; i < n; ++i) G[i] = -G[i];
+    orc_lu_solve(n, K.data(), ipiv.data(), 1, G.data());
+    for (int i = 0; i < n; ++i) x[i] += G[i];
+  }
+  c->x = xsave;
+  return ret;
+}
+
+// Philox4x32-10, Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11: multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key increments
+// 0x9E3779B9 / 0xBB67AE85, 10 rounds.
+extern "C" void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr_in[0], ctr_in[1], ctr_in[2], ctr_in[3]};
+  uint32_t k[2] = {key_in[0], key_in[1]};
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k[0] += 0x9E3779B9u; k[1] += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k[0];
+    uint32_t n2 = hi0 ^ c[3] ^ k[1];
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  }
+  for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
+extern "C" double orc_uniform(uint32_t key_lo, uint32_t key_hi, uint32_t a, uint32_t b, uint32_t cc) {
+  uint32_t ctr[4] = {a, b, cc, 0}, key[2] = {key_lo, key_hi}, o[4];
+  orc_philox4x32_10(ctr, key, o);
+  return ((double)o[0] + 0.5) * (1.0 / 4294967296.0);
+}
+
+extern "C" int orc_bell_update(const orc_cell *c, double *state, double *anchor, double dt, double k_on, double k_off0,
+                               double f_b, uint32_t key_lo, uint32_t key_hi, uint32_t replica, uint32_t step,
+                               double *min_margin) {
+  int transitions = 0;
+  double mm = 1.0;
+  for (int s = 0; s < c->n_spr; ++s) {
+    const int32_t *t = c->tris + 3 * c->spr_tri[s];
+    const double *bb = c->spr_bary + 3 * s;
+    double p[2];
+    for (int q = 0; q < 2; ++q)
+      p[q] = bb[0] * c->x[2 * t[0] + q] + bb[1] * c->x[2 * t[1] + q] + bb[2] * c->x[2 * t[2] + q];
+    double u = orc_uniform(key_lo, key_hi, replica, (uint32_t)s, step);
+    double prob;
+    if (state[s] != 0.0) {
+      double d0 = p[0] - anchor[2 * s], d1 = p[1] - anchor[2 * s + 1];
+      double D = ::sqrt(d0 * d0 + d1 * d1), f, df;
+      orc_spring_law(c, D, &f, &df);
+      double koff = k_off0 * ::exp(fabs(f) / f_b);
+      prob = 1.0 - ::exp(-koff * dt);
+      if (u < prob) { state[s] = 0.0; ++transitions; }
+    } else {
+      prob = 1.0 - ::exp(-k_on * dt);
+      if (u < prob) {
+        state[s] = 1.0;
+        anchor[2 * s] = p[0];
+        anchor[2 * s + 1] = p[1];
+        ++transitions;
+      }
+    }
+    mm = std::min(mm, fabs(u - prob));
+  }
+  if (min_margin) *min_margin = mm;
+  return transitions;
+}
+
+extern "C" void orc_commit_ul_strain(const orc_cell *c, double *ea_out) {
+  // UL:607-612: ea += Res_ea / MM, Res_ea/MM assembled in element order (UL:692-705).
+  const int N = c->n_nodes;
+  std::vector<double> Rea(6 * N, 0.0), MM(N, 0.0);
+  const double far[3] = {1e30, 1e30, 0}, zero3[3] = {0, 0, 0};
+  for (int e = 0; e < c->n_tris; ++e) {
+    const int32_t *t = c->tris + 3 * e;
+    double Xe[9], xe[9], ea6[18];
+    for (int a = 0; a < 3; ++a) {
+      Xe[3 * a] = c->X[2 * t[a]]; Xe[3 * a + 1] = c->X[2 * t[a] + 1]; Xe[3 * a + 2] = 0;
+      xe[3 * a] = c->x[2 * t[a]]; xe[3 * a + 1] = c->x[2 * t[a] + 1]; xe[3 * a + 2] = 0;
+      const double *E3 = c->ea + 3 * t[a];
+      double *E6 = ea6 + 6 * a;
+      E6[0] = E3[0]; E6[1] = E3[1]; E6[2] = 0; E6[3] = E3[2]; E6[4] = 0; E6[5] = 0;
+    }
+    double Re[9], Re_ea[18], MMe[3], RA[3], RC[3];
+    ul_element<double>(Xe, xe, ea6, zero3, zero3, 0.0, c->mu, t_act_of(c->time), c->thick, far, Re, Re_ea, MMe, RA,
+                       RC);
+    for (int a = 0; a < 3; ++a) {
+      for (int q = 0; q < 6; ++q) Rea[6 * t[a] + q] += Re_ea[6 * a + q];
+      MM[t[a]] += MMe[a];
+    }
+  }
+  for (int i = 0; i < N; ++i) {
+    ea_out[3 * i + 0] = c->ea[3 * i + 0] + Rea[6 * i + 0] / MM[i];
+    ea_out[3 * i + 1] = c->ea[3 * i + 1] + Rea[6 * i + 1] / MM[i];
+    ea_out[3 * i + 2] = c->ea[3 * i + 2] + Rea[6 * i + 3] / MM[i];
+  }
+}

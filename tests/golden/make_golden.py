@@ -1,0 +1,152 @@
+"""Generate golden vectors from the compiled reference (this container only).
+
+Runs oracle/_ref/refharness_{k1,k31,kMD} (built by `make -C oracle ref` from the
+reference's own UL_growth_explicit.cpp + nondestructivetrimesh.cpp) on seeded
+inputs over the reference's shipped meshes, and stores inputs and outputs as
+small compressed .npz fixtures next to this script.  The fixtures are data
+(mesh coordinates/connectivity read from the reference's scripts/*.txt input
+files, seeded inputs, and the reference's outputs); nothing of the reference's
+source travels.
+
+Usage:  python tests/golden/make_golden.py
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+SCRIPTS = os.path.join(REF, "wholeCellFE_kMD", "cellBP", "scripts")
+HARNESS = os.path.join(REPO, "oracle", "_ref", "refharness_{}")
+
+
+def read_ecomsol(path):
+    """Parse the reference 'ecomsol' mesh format (iomesh.cpp:491-565); z := 0."""
+    lines = open(path).read().split("\n")
+    i = next(k for k, l in enumerate(lines) if "Nodes" in l)
+    n = int(lines[i + 1].split()[0])
+    pts = np.zeros((n, 3))
+    for k in range(n):
+        v = lines[i + 2 + k].split()
+        pts[k, 0], pts[k, 1] = float(v[0]), float(v[1])
+    j = next(k for k, l in enumerate(lines) if "Elements" in l)
+    m = int(lines[j + 1].split()[0])
+    tris = np.array([[int(v) for v in lines[j + 2 + k].split()[:3]] for k in range(m)], dtype=np.int64)
+    return pts, tris
+
+
+def run(variant, mode, inputs, meta):
+    d = tempfile.mkdtemp(prefix="wcfe_golden_")
+    try:
+        for k, v in inputs.items():
+            np.ascontiguousarray(v).tofile(os.path.join(d, k + ".bin"))
+        with open(os.path.join(d, "meta.txt"), "w") as f:
+            for k, v in meta.items():
+                f.write("%s %.17g\n" % (k, v))
+        subprocess.run([HARNESS.format(variant), mode, d], check=True)
+        out = {}
+        for fn in os.listdir(d):
+            if fn.startswith("out_"):
+                out[fn[4:-4]] = np.fromfile(os.path.join(d, fn), dtype=np.float64)
+        return out
+    finally:
+        shutil.rmtree(d)
+
+
+def deform(X, rng, amp):
+    A = np.array([[1.04, 0.03], [-0.02, 0.96]])
+    x = X.copy()
+    x[:, :2] = X[:, :2] @ A.T + amp * rng.standard_normal((len(X), 2))
+    return x
+
+
+def main():
+    if not os.path.exists(HARNESS.format("kMD")):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    rng = np.random.default_rng(20240601)
+    disk296, t296 = read_ecomsol(os.path.join(SCRIPTS, "disk_20um_diam_edited.txt"))
+    disk483, t483 = read_ecomsol(os.path.join(SCRIPTS, "disk_20um_diam_N1025.txt"))
+    subs, tsubs = read_ecomsol(os.path.join(SCRIPTS, "sq_subs_100um.txt"))
+
+    # --- element residual / calculateRes (law identical in all variants) ---
+    for name, X, T, time, ea3d in [("ul_disk296_t1p3", disk296, t296, 1.3, False),
+                                   ("ul_disk483_t3", disk483, t483, 3.0, False),
+                                   ("ul_disk296_ea3d", disk296, t296, 0.7, True)]:
+        N = len(X)
+        x = deform(X, rng, 0.02)
+        ea = np.zeros((N, 6))
+        ea[:, [0, 1, 3]] = 0.05 * rng.standard_normal((N, 3))
+        if ea3d:
+            ea[:, [2, 4, 5]] = 0.02 * rng.standard_normal((N, 3))
+        cA = rng.uniform(0, 100, N)
+        cC = rng.uniform(0, 1, N)
+        inp = dict(X=X, x=x, ea=ea, cA=cA, cC=cC, tris=T)
+        meta = dict(dt=0.005, time=time)
+        o1 = run("kMD", "elem", inp, meta)
+        o2 = run("kMD", "calcres", inp, meta)
+        o3 = run("kMD", "areatot", inp, meta)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), dt=0.005, time=time, **inp,
+                            **{"elem_" + k: v for k, v in o1.items()},
+                            **{"res_" + k: v for k, v in o2.items()},
+                            areatot=o3["areatot"])
+        print("wrote", name)
+
+    # --- calculateForce_FA (linear k1/k31, MD table kMD) ---
+    N = len(disk296)
+    for name, variant, kint, planar in [("forcefa_k1", "k1", 1000.0, True), ("forcefa_k31", "k31", 31000.0, True),
+                                        ("forcefa_kMD", "kMD", -1.0, True), ("forcefa_kMD_3d", "kMD", -1.0, False)]:
+        x = deform(disk296, rng, 0.02)
+        D = rng.uniform(0.0, 0.022, N)
+        D[:4] = [0.0, 2.51250583e-03, 17.001e-03, 4.40265201e-03]  # exact knots and D == 0
+        th = rng.uniform(0, 2 * np.pi, N)
+        fau = np.zeros((N, 3))
+        if planar:
+            fau[:, 0], fau[:, 1] = D * np.cos(th), D * np.sin(th)
+        else:
+            ph = rng.uniform(0.2, np.pi - 0.2, N)
+            fau[:, 0], fau[:, 1], fau[:, 2] = D * np.cos(th) * np.sin(ph), D * np.sin(th) * np.sin(ph), D * np.cos(ph)
+        cC = rng.uniform(0, 1, N)
+        inp = dict(X=disk296, x=x, cC=cC, fau=fau, tris=t296)
+        o = run(variant, "forcefa", inp, dict(kint=kint))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), kint=kint, **inp, F_FA=o["F_FA"])
+        print("wrote", name)
+
+    # --- updateFA (11-argument form) with the 100 um substrate ---
+    for name, variant, kint, moving in [("updatefa_k1_rigid", "k1", 1000.0, False),
+                                        ("updatefa_k31_moving", "k31", 31000.0, True),
+                                        ("updatefa_kMD_rigid", "kMD", -1.0, False),
+                                        ("updatefa_kMD_moving", "kMD", -1.0, True)]:
+        N = len(disk296)
+        X = disk296.copy()
+        x = X.copy()
+        x[:, :2] += 0.004 * rng.standard_normal((N, 2))
+        D = rng.uniform(0.0, 0.02, N)
+        th = rng.uniform(0, 2 * np.pi, N)
+        fau = np.zeros((N, 3))
+        fau[:, 0], fau[:, 1] = D * np.cos(th), D * np.sin(th)
+        cA = np.full(N, 100.0)
+        cC = rng.uniform(0, 1, N)
+        cC[:3] = [0.0, 1.0, 1e-9]
+        SX = subs.copy()
+        Sx = SX.copy()
+        if moving:
+            Sx[:, 0] += 0.01 * np.sin(SX[:, 1] / 7.0)
+            Sx[:, 1] += 0.01 * np.cos(SX[:, 0] / 5.0)
+        inp = dict(X=X, x=x, cA=cA, cC=cC, fau=fau, SX=SX, Sx=Sx, tris=t296, Stris=tsubs)
+        o = run(variant, "updatefa", inp, dict(dt=0.005, kint=kint))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), dt=0.005, kint=kint, **inp,
+                            out_cA=o["cA"], out_cC=o["cC"], out_fau=o["fau"])
+        print("wrote", name)
+
+    # config-1 template mesh (483 nodes / 905 triangles) for the bench and tests
+    np.savez_compressed(os.path.join(HERE, "mesh_disk483.npz"), X=disk483[:, :2], tris=t483.astype(np.int32))
+    print("wrote mesh_disk483")
+
+
+if __name__ == "__main__":
+    main()
