@@ -1,0 +1,191 @@
+/* wcfe.h -- C ABI of the MI355X-native per-step cell mechanics hot path.
+ *
+ * The reference (dredremontes/wholeCellFE) advances each cell with explicit
+ * central differences; its hot path is a set of free C++ functions called from
+ * advance_sim (wholeCellFE_kMD/cellBP/main.cpp:356-363):
+ *   updateFA               UL_growth_explicit.cpp:2026 (kMD) / :2005 (k1, k31)
+ *   updateForces_Explicit  UL_growth_explicit.cpp:508  (cell overload)
+ *     -> calculateRes      UL_growth_explicit.cpp:632
+ *     -> evalElementRe     UL_growth_explicit.cpp:710
+ *     -> calculateForce_FA UL_growth_explicit.cpp:1198 (kMD) / :1196 (k1, k31)
+ * This library replaces them with HIP kernels for gfx950 behind plain C entry
+ * points (pointers and sizes only).  Two families:
+ *
+ *  (1) Reference-shaped, stateless calls with the reference's argument meaning
+ *      (3 components per node, host arrays in and out): wcfe_calculate_res,
+ *      wcfe_force_fa, wcfe_update_fa.  They are what include/wcfe_compat.hpp
+ *      forwards the reference call sites to.
+ *  (2) The implicit (Newton) step the reference lacks: a context holding a
+ *      batch of cells resident in HBM, assembled with an analytic tangent,
+ *      colour-ordered scatter, dense blocked LU (FP32 MFMA, FP64 iterative
+ *      refinement) and an FP64 residual/convergence test.
+ *
+ * Units follow the reference: um, pN, s, Pa = pN/um^2.
+ * Every function returns WCFE_OK (0) or a WCFE_ERR_* code; the message of the
+ * last error of a context is available from wcfe_last_error().
+ * A context is not thread safe: one host thread per context.
+ */
+#ifndef WCFE_H
+#define WCFE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define WCFE_OK 0
+#define WCFE_ERR_ARG 1            /* invalid argument or size */
+#define WCFE_ERR_HIP 2            /* HIP runtime error (message has details) */
+#define WCFE_ERR_OOM 3            /* device allocation failed */
+#define WCFE_ERR_DEGENERATE 4     /* zero-area or inverted element */
+#define WCFE_ERR_ZERO_PIVOT 5     /* exact zero pivot in partial-pivot LU */
+#define WCFE_ERR_NOT_CONVERGED 6  /* Newton or refinement did not converge */
+#define WCFE_ERR_STATE 7          /* call out of order (e.g. before finalize) */
+#define WCFE_ERR_NO_DEVICE 8      /* no usable gfx950 device */
+
+/* ---- enumerations ---- */
+enum wcfe_law { WCFE_LAW_UL = 0, WCFE_LAW_CST = 1 };
+enum wcfe_spring_law { WCFE_SPRING_LINEAR = 0, WCFE_SPRING_TABLE = 1 };
+enum wcfe_kinetics { WCFE_KIN_NONE = 0, WCFE_KIN_MEANFIELD = 1, WCFE_KIN_BELL = 2 };
+enum wcfe_precision { WCFE_PREC_MIXED = 0, WCFE_PREC_FP64 = 1 };
+enum wcfe_pivoting { WCFE_PIVOT_STATIC = 0, WCFE_PIVOT_PARTIAL = 1 };
+
+typedef struct wcfe_ctx wcfe_ctx;
+
+typedef struct wcfe_options {
+  int32_t device;        /* HIP device ordinal */
+  int32_t precision;     /* wcfe_precision: MIXED = FP32 assembly + LU on MFMA, FP64 residual/IR */
+  int32_t pivoting;      /* wcfe_pivoting for the dense LU */
+  int32_t batched;       /* 1: one launch sequence over all cells; 0: one cell at a time */
+  int32_t use_graph;     /* 1: capture a Newton step in a hipGraph and replay it */
+  int32_t max_refine;    /* max iterative-refinement sweeps per linear solve */
+  double lin_rtol;       /* ||b - K d|| <= lin_rtol ||b|| (FP64) ends refinement */
+  double newton_rtol;    /* ||G(x)|| <= newton_rtol ||G(x0)|| ends a load step */
+  double newton_atol;    /* ... or ||G(x)|| <= newton_atol */
+  int32_t max_newton;    /* a load step that needs more iterations is reported */
+  double dt;             /* load/time step [s] */
+  double t0;             /* initial time [s] (active stress schedule UL:743-757) */
+  uint32_t seed_lo, seed_hi; /* Philox key for Bell kinetics */
+  double k_on, k_off0, f_bell; /* Bell kinetics: binding rate, unloaded off-rate, force scale */
+  double kappa;          /* active-stress isotropy (reference: 0.5, UL:872) */
+} wcfe_options;
+
+typedef struct wcfe_cell_desc {
+  int32_t n_nodes, n_tris;
+  const double *X;        /* 2N reference positions: start of step (UL law) / rest (CST law) */
+  const double *x;        /* 2N current positions, NULL = X */
+  const int32_t *tris;    /* 3T node indices, 0-based */
+  int32_t law;            /* wcfe_law */
+  double mu;              /* UL law shear modulus [Pa] (reference 1000, UL:724) */
+  double E, nu;           /* CST law */
+  double thick;           /* [um] (reference 1, UL:768) */
+  const double *ea;       /* 3N planar Euler-Almansi strain [xx,yy,xy] (UL law), NULL = 0 */
+  /* reference node-attached integrin springs (UL:1198-1303) */
+  int32_t node_springs;   /* 0/1 */
+  const double *cC;       /* N bound fraction */
+  const double *fau;      /* 2N spring extension fa_u */
+  double rho_max;         /* [um^-2] (reference 100, UL:1218) */
+  /* discrete springs at barycentric points of cell triangles */
+  int32_t n_spr;
+  const int32_t *spr_tri;   /* host triangle of each spring */
+  const double *spr_bary;   /* 3 barycentric coordinates per spring */
+  const double *spr_anchor; /* 2 per spring, substrate point */
+  const double *spr_state;  /* 1 per spring: 1 bound / 0 unbound, NULL = all bound */
+  /* force law shared by both spring kinds */
+  int32_t spring_law;     /* wcfe_spring_law */
+  double k_lin;           /* [pN/um] linear law */
+  int32_t table_id;       /* table law: id given to wcfe_set_table */
+  int32_t kinetics;       /* wcfe_kinetics */
+} wcfe_cell_desc;
+
+typedef struct wcfe_step_stats {
+  int32_t n_cells;
+  int32_t n_converged;    /* cells whose load step converged at this Newton step */
+  int32_t max_refine_used;
+  int32_t n_static_pivots;/* tiny pivots replaced (static pivoting) */
+  int32_t n_failed;       /* cells whose refinement missed lin_rtol */
+  double max_lin_relres;  /* max over cells of the final ||b - K d|| / ||b|| */
+  double max_newton_rel;  /* max over cells of ||G(x_new)|| / ||G(x0)|| */
+} wcfe_step_stats;
+
+/* ---- lifecycle ---- */
+void wcfe_default_options(wcfe_options *opt);
+int wcfe_create(const wcfe_options *opt, wcfe_ctx **out);
+int wcfe_destroy(wcfe_ctx *ctx);
+const char *wcfe_last_error(const wcfe_ctx *ctx);
+/* gfx arch name of the context's device, e.g. "gfx950" */
+int wcfe_device_arch(const wcfe_ctx *ctx, char *buf, int len);
+
+/* ---- batch definition (before wcfe_finalize) ---- */
+/* Force table in the reference's (px, py, k) form (UL:1204-1214): for
+ * D in [px_j, px_{j+1}) the force is py_j + k_j (D - px_j); beyond the last
+ * knot the last segment extrapolates.  n <= 64. */
+int wcfe_set_table(wcfe_ctx *ctx, int32_t table_id, int32_t n, const double *px, const double *py,
+                   const double *k);
+int wcfe_add_cell(wcfe_ctx *ctx, const wcfe_cell_desc *desc, int32_t *cell_id);
+/* Colour elements, lay the batch out in HBM, allocate the dense systems. */
+int wcfe_finalize(wcfe_ctx *ctx);
+
+/* ---- Newton path (after wcfe_finalize) ---- */
+/* One batched Newton iteration on every cell: assemble K, G at x; factor;
+ * solve K d = -G with refinement to lin_rtol; x += d; FP64 residual at the new
+ * x; cells meeting the convergence test commit their load step (kinetics,
+ * strain update, time += dt).  stats may be NULL (then nothing is copied back
+ * and the call does not synchronise). */
+int wcfe_newton_step(wcfe_ctx *ctx, wcfe_step_stats *stats);
+/* Newton iterations until every cell has committed one load step. */
+int wcfe_load_step(wcfe_ctx *ctx, int32_t *iterations, wcfe_step_stats *stats);
+int wcfe_synchronize(wcfe_ctx *ctx);
+
+/* ---- state access ---- */
+int wcfe_get_positions(wcfe_ctx *ctx, int32_t cell, double *x /* 2N */);
+int wcfe_set_positions(wcfe_ctx *ctx, int32_t cell, const double *x /* 2N */);
+int wcfe_get_ul_state(wcfe_ctx *ctx, int32_t cell, double *X /* 2N */, double *ea /* 3N */, double *cC /* N */,
+                      double *fau /* 2N */);
+int wcfe_get_springs(wcfe_ctx *ctx, int32_t cell, double *state /* n_spr */, double *anchor /* 2 n_spr */);
+int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step, int32_t *newton_iter,
+                       int32_t *n_colors, int32_t *ndof);
+/* FP64 residual G(x) = R_int(x) - F_spring(x) at the current positions (2N). */
+int wcfe_residual_fp64(wcfe_ctx *ctx, int32_t cell, double *G);
+/* Element systems in the cell's element order: Ge (6 per element) and Ke
+ * (36 per element, row-major) with springs folded in, FP64. */
+int wcfe_element_systems(wcfe_ctx *ctx, int32_t cell, double *Ge, double *Ke);
+/* Assembled dense tangent (FP64 copy of the working-precision matrix, ndof x
+ * ndof, row-major) and residual, as the solver sees them. */
+int wcfe_assembled_system(wcfe_ctx *ctx, int32_t cell, double *K, double *G);
+/* Element colours (int32 per element) in the cell's element order. */
+int wcfe_get_colors(wcfe_ctx *ctx, int32_t cell, int32_t *color);
+
+/* ---- dense LU on its own (parity tests, autotuning) ----
+ * Factor and solve a batch of row-major n_i x n_i systems in the context's
+ * working precision (FP32 for MIXED, FP64 otherwise).  A holds the matrices
+ * back to back; b the right-hand sides; x receives the solutions after
+ * refinement in FP64 against the FP64 A.  relres (per system) receives
+ * ||b - A x|| / ||b||, iters the refinement sweeps used. */
+int wcfe_dense_solve(wcfe_ctx *ctx, int32_t n_sys, const int32_t *n, const double *A, const double *b, double *x,
+                     double *relres, int32_t *iters, int32_t *ipiv_out /* optional, sum n */);
+
+/* ---- reference-shaped stateless calls (3 components per node) ----
+ * wcfe_calculate_res: calculateRes (UL:632-707) for the cell law (mat = 1,
+ * UL:721-768 constants) on planar meshes (z = 0, planar ea).  X, x: 3N; ea:
+ * 6N Voigt [xx,yy,zz,xy,xz,yz]; outputs Res 3N, Res_ea 6N, MM N. */
+int wcfe_calculate_res(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *X,
+                       const double *x, const double *ea, double time, double *Res, double *Res_ea, double *MM);
+/* calculateForce_FA (UL:1198 kMD table, law 1; UL1:1196 linear kint, law 0).
+ * x: 3N, cC: N, fau: 3N, F: 3N. */
+int wcfe_force_fa(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
+                  const double *cC, const double *fau, int32_t law, double kint, double *F);
+/* updateFA 11-argument form (UL:2026 kMD, law 1; UL1:2005 k1/k31, law 0).
+ * cA, cC, fau are updated in place.  n_unlocated (optional) receives the
+ * number of anchors outside every substrate triangle (subs_u := 0 there; the
+ * reference leaves it uninitialised). */
+int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double *x, double *cA, double *cC,
+                   double *fau, double dt, int32_t n_snodes, int32_t n_stris, const int32_t *stris, const double *SX,
+                   const double *Sx, int32_t law, double kint, int32_t *n_unlocated);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,316 @@
+"""wholecellfe_amd -- MI355X (gfx950) per-step cell mechanics hot path.
+
+Thin ctypes binding of libwcfe.so (the C ABI declared in include/wcfe.h).
+The compute lives in the HIP library; this module only marshals numpy arrays.
+It raises if the library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwcfe.so")
+
+OK = 0
+ERR_NAMES = {1: "ARG", 2: "HIP", 3: "OOM", 4: "DEGENERATE", 5: "ZERO_PIVOT", 6: "NOT_CONVERGED", 7: "STATE",
+             8: "NO_DEVICE"}
+LAW_UL, LAW_CST = 0, 1
+SPRING_LINEAR, SPRING_TABLE = 0, 1
+KIN_NONE, KIN_MEANFIELD, KIN_BELL = 0, 1, 2
+PREC_MIXED, PREC_FP64 = 0, 1
+PIVOT_STATIC, PIVOT_PARTIAL = 0, 1
+
+D = C.POINTER(C.c_double)
+I32 = C.POINTER(C.c_int32)
+
+# every symbol include/wcfe.h declares
+EXPORTS = [
+    "wcfe_default_options", "wcfe_create", "wcfe_destroy", "wcfe_last_error", "wcfe_device_arch", "wcfe_set_table",
+    "wcfe_add_cell", "wcfe_finalize", "wcfe_newton_step", "wcfe_load_step", "wcfe_synchronize",
+    "wcfe_get_positions", "wcfe_set_positions", "wcfe_get_ul_state", "wcfe_get_springs", "wcfe_get_cell_info",
+    "wcfe_residual_fp64", "wcfe_element_systems", "wcfe_assembled_system", "wcfe_get_colors", "wcfe_dense_solve",
+    "wcfe_calculate_res", "wcfe_force_fa", "wcfe_update_fa",
+]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32), ("precision", C.c_int32), ("pivoting", C.c_int32), ("batched", C.c_int32),
+        ("use_graph", C.c_int32), ("max_refine", C.c_int32), ("lin_rtol", C.c_double),
+        ("newton_rtol", C.c_double), ("newton_atol", C.c_double), ("max_newton", C.c_int32), ("dt", C.c_double),
+        ("t0", C.c_double), ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32), ("k_on", C.c_double),
+        ("k_off0", C.c_double), ("f_bell", C.c_double), ("kappa", C.c_double),
+    ]
+
+
+class CellDesc(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int32), ("n_tris", C.c_int32), ("X", D), ("x", D), ("tris", I32), ("law", C.c_int32),
+        ("mu", C.c_double), ("E", C.c_double), ("nu", C.c_double), ("thick", C.c_double), ("ea", D),
+        ("node_springs", C.c_int32), ("cC", D), ("fau", D), ("rho_max", C.c_double), ("n_spr", C.c_int32),
+        ("spr_tri", I32), ("spr_bary", D), ("spr_anchor", D), ("spr_state", D), ("spring_law", C.c_int32),
+        ("k_lin", C.c_double), ("table_id", C.c_int32), ("kinetics", C.c_int32),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int32), ("n_converged", C.c_int32), ("max_refine_used", C.c_int32),
+        ("n_static_pivots", C.c_int32), ("n_failed", C.c_int32), ("max_lin_relres", C.c_double),
+        ("max_newton_rel", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+class WcfeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("wcfe error %d (%s): %s" % (code, ERR_NAMES.get(code, "?"), msg))
+        self.code = code
+
+
+def lib():
+    """Load libwcfe.so; raise if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libwcfe.so is not built: run `python -m wholecellfe_amd.build` "
+                              "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.wcfe_last_error.restype = C.c_char_p
+        L.wcfe_create.argtypes = [C.POINTER(Options), C.POINTER(C.c_void_p)]
+        for name in EXPORTS:
+            getattr(L, name)  # raises AttributeError if missing
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(D)
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(I32)
+
+
+def default_options(**kw):
+    o = Options()
+    lib().wcfe_default_options(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """A batch of cells resident on one GPU (wcfe_ctx)."""
+
+    def __init__(self, **options):
+        self._L = lib()
+        self.opt = default_options(**options)
+        h = C.c_void_p()
+        rc = self._L.wcfe_create(C.byref(self.opt), C.byref(h))
+        self._h = h
+        if rc != OK:
+            msg = self._L.wcfe_last_error(h).decode() if h.value else "no device"
+            if h.value:
+                self._L.wcfe_destroy(h)
+            self._h = C.c_void_p()
+            raise WcfeError(rc, msg)
+        self._keep = []
+        self.cells = []
+
+    def _check(self, rc):
+        if rc != OK:
+            raise WcfeError(rc, self._L.wcfe_last_error(self._h).decode())
+
+    def close(self):
+        if self._h and self._h.value:
+            self._L.wcfe_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def arch(self):
+        buf = C.create_string_buffer(64)
+        self._check(self._L.wcfe_device_arch(self._h, buf, 64))
+        return buf.value.decode()
+
+    def set_table(self, table_id, px, py, k):
+        px, py, k = _f64(px), _f64(py), _f64(k)
+        self._check(self._L.wcfe_set_table(self._h, table_id, len(px), _dp(px), _dp(py), _dp(k)))
+
+    def add_cell(self, X, tris, law=LAW_CST, x=None, mu=1000.0, E=1000.0, nu=0.3, thick=0.2, ea=None,
+                 node_springs=0, cC=None, fau=None, rho_max=100.0, spr_tri=None, spr_bary=None, spr_anchor=None,
+                 spr_state=None, spring_law=SPRING_LINEAR, k_lin=1000.0, table_id=0, kinetics=KIN_NONE):
+        X = _f64(X).reshape(-1)
+        tris = _i32(tris).reshape(-1)
+        arrs = dict(X=X, x=_f64(x), tris=tris, ea=_f64(ea), cC=_f64(cC), fau=_f64(fau), spr_tri=_i32(spr_tri),
+                    spr_bary=_f64(spr_bary), spr_anchor=_f64(spr_anchor), spr_state=_f64(spr_state))
+        for k in ("x", "ea", "cC", "fau", "spr_bary", "spr_anchor", "spr_state"):
+            if arrs[k] is not None:
+                arrs[k] = arrs[k].reshape(-1)
+        if arrs["spr_tri"] is not None:
+            arrs["spr_tri"] = arrs["spr_tri"].reshape(-1)
+        self._keep.append(arrs)
+        d = CellDesc()
+        d.n_nodes = len(X) // 2
+        d.n_tris = len(tris) // 3
+        d.X, d.x, d.tris = _dp(X), _dp(arrs["x"]), _ip(tris)
+        d.law, d.mu, d.E, d.nu, d.thick = law, mu, E, nu, thick
+        d.ea = _dp(arrs["ea"])
+        d.node_springs, d.cC, d.fau, d.rho_max = node_springs, _dp(arrs["cC"]), _dp(arrs["fau"]), rho_max
+        d.n_spr = 0 if arrs["spr_tri"] is None else len(arrs["spr_tri"])
+        d.spr_tri, d.spr_bary, d.spr_anchor, d.spr_state = (_ip(arrs["spr_tri"]), _dp(arrs["spr_bary"]),
+                                                            _dp(arrs["spr_anchor"]), _dp(arrs["spr_state"]))
+        d.spring_law, d.k_lin, d.table_id, d.kinetics = spring_law, k_lin, table_id, kinetics
+        cid = C.c_int32(-1)
+        self._check(self._L.wcfe_add_cell(self._h, C.byref(d), C.byref(cid)))
+        self.cells.append(dict(n_nodes=d.n_nodes, n_tris=d.n_tris, n_spr=d.n_spr))
+        return cid.value
+
+    def finalize(self):
+        self._check(self._L.wcfe_finalize(self._h))
+        self._keep = []
+
+    def newton_step(self, stats=True):
+        if not stats:
+            self._check(self._L.wcfe_newton_step(self._h, None))
+            return None
+        st = StepStats()
+        self._check(self._L.wcfe_newton_step(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def load_step(self):
+        st = StepStats()
+        it = C.c_int32(0)
+        self._check(self._L.wcfe_load_step(self._h, C.byref(it), C.byref(st)))
+        d = st.as_dict()
+        d["iterations"] = it.value
+        return d
+
+    def synchronize(self):
+        self._check(self._L.wcfe_synchronize(self._h))
+
+    def positions(self, cell):
+        n = self.cells[cell]["n_nodes"]
+        x = np.zeros(2 * n)
+        self._check(self._L.wcfe_get_positions(self._h, cell, _dp(x)))
+        return x
+
+    def set_positions(self, cell, x):
+        x = _f64(x).reshape(-1)
+        self._check(self._L.wcfe_set_positions(self._h, cell, _dp(x)))
+
+    def ul_state(self, cell):
+        n = self.cells[cell]["n_nodes"]
+        X, ea, cC, fau = np.zeros(2 * n), np.zeros(3 * n), np.zeros(n), np.zeros(2 * n)
+        self._check(self._L.wcfe_get_ul_state(self._h, cell, _dp(X), _dp(ea), _dp(cC), _dp(fau)))
+        return X, ea, cC, fau
+
+    def springs(self, cell):
+        n = self.cells[cell]["n_spr"]
+        st, an = np.zeros(n), np.zeros(2 * n)
+        if n:
+            self._check(self._L.wcfe_get_springs(self._h, cell, _dp(st), _dp(an)))
+        return st, an
+
+    def cell_info(self, cell):
+        t = C.c_double(0)
+        step, nit, nc, ndof = C.c_int32(0), C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        self._check(self._L.wcfe_get_cell_info(self._h, cell, C.byref(t), C.byref(step), C.byref(nit), C.byref(nc),
+                                               C.byref(ndof)))
+        return dict(time=t.value, step=step.value, newton_iter=nit.value, n_colors=nc.value, ndof=ndof.value)
+
+    def residual(self, cell):
+        G = np.zeros(2 * self.cells[cell]["n_nodes"])
+        self._check(self._L.wcfe_residual_fp64(self._h, cell, _dp(G)))
+        return G
+
+    def element_systems(self, cell):
+        T = self.cells[cell]["n_tris"]
+        Ge, Ke = np.zeros(6 * T), np.zeros(36 * T)
+        self._check(self._L.wcfe_element_systems(self._h, cell, _dp(Ge), _dp(Ke)))
+        return Ge.reshape(T, 6), Ke.reshape(T, 6, 6)
+
+    def assembled_system(self, cell):
+        n = 2 * self.cells[cell]["n_nodes"]
+        K, G = np.zeros((n, n)), np.zeros(n)
+        self._check(self._L.wcfe_assembled_system(self._h, cell, _dp(K), _dp(G)))
+        return K, G
+
+    def colors(self, cell):
+        c = np.zeros(self.cells[cell]["n_tris"], dtype=np.int32)
+        self._check(self._L.wcfe_get_colors(self._h, cell, _ip(c)))
+        return c
+
+    def dense_solve(self, mats, rhss):
+        n = np.array([m.shape[0] for m in mats], dtype=np.int32)
+        A = np.concatenate([np.ascontiguousarray(m, dtype=np.float64).reshape(-1) for m in mats])
+        b = np.concatenate([np.ascontiguousarray(r, dtype=np.float64).reshape(-1) for r in rhss])
+        x = np.zeros_like(b)
+        rel = np.zeros(len(mats))
+        it = np.zeros(len(mats), dtype=np.int32)
+        piv = np.zeros(int(n.sum()), dtype=np.int32)
+        self._check(self._L.wcfe_dense_solve(self._h, len(mats), _ip(n), _dp(A), _dp(b), _dp(x), _dp(rel), _ip(it),
+                                             _ip(piv)))
+        out, o = [], 0
+        for k in n:
+            out.append(x[o:o + k])
+            o += k
+        return out, rel, it, piv
+
+    # ---- reference-shaped stateless calls (3 components per node) ----
+    def calculate_res(self, tris, X, x, ea, time):
+        """GPU counterpart of calculateRes (UL_growth_explicit.cpp:632) for the
+        cell law; returns Res (3N), Res_ea (6N), MM (N)."""
+        tris, X, x, ea = _i32(tris).reshape(-1), _f64(X).reshape(-1), _f64(x).reshape(-1), _f64(ea).reshape(-1)
+        n = len(X) // 3
+        Res, Rea, MM = np.zeros(3 * n), np.zeros(6 * n), np.zeros(n)
+        self._check(self._L.wcfe_calculate_res(self._h, n, len(tris) // 3, _ip(tris), _dp(X), _dp(x), _dp(ea),
+                                               C.c_double(time), _dp(Res), _dp(Rea), _dp(MM)))
+        return Res, Rea, MM
+
+    def force_fa(self, tris, x, cC, fau, law, kint):
+        """GPU counterpart of calculateForce_FA (UL:1198 / UL1:1196)."""
+        tris, x, cC, fau = _i32(tris).reshape(-1), _f64(x).reshape(-1), _f64(cC), _f64(fau).reshape(-1)
+        n = len(x) // 3
+        F = np.zeros(3 * n)
+        self._check(self._L.wcfe_force_fa(self._h, n, len(tris) // 3, _ip(tris), _dp(x), _dp(cC), _dp(fau), law,
+                                          C.c_double(kint), _dp(F)))
+        return F
+
+    def update_fa(self, X, x, cA, cC, fau, dt, stris, SX, Sx, law, kint):
+        """GPU counterpart of updateFA, 11-argument form (UL:2026 / UL1:2005)."""
+        X, x = _f64(X).reshape(-1), _f64(x).reshape(-1)
+        cA, cC, fau = _f64(cA).copy(), _f64(cC).copy(), _f64(fau).reshape(-1).copy()
+        stris, SX, Sx = _i32(stris).reshape(-1), _f64(SX).reshape(-1), _f64(Sx).reshape(-1)
+        un = C.c_int32(0)
+        self._check(self._L.wcfe_update_fa(self._h, len(X) // 3, _dp(X), _dp(x), _dp(cA), _dp(cC), _dp(fau),
+                                           C.c_double(dt), len(SX) // 3, len(stris) // 3, _ip(stris), _dp(SX),
+                                           _dp(Sx), law, C.c_double(kint), C.byref(un)))
+        return cA, cC, fau, un.value

@@ -1,0 +1,461 @@
+// Assembly, FP64 residual/refinement vectors, Newton control and commit
+// kernels for gfx950.
+//
+// Assembly is colour ordered: one launch per colour; within a colour no two
+// elements share a node, so every += into the dense system and the residual
+// has a single writer and the summation order per entry is fixed by the
+// colour sequence (run-to-run bitwise deterministic, no atomics).
+#include "wcfe_internal.hpp"
+
+namespace wcfe {
+
+// ---------------------------------------------------------------------------
+// dense system fill: zero, identity on the padded diagonal (rows >= ndof)
+// ---------------------------------------------------------------------------
+template <class T> struct Vec16;
+template <> struct Vec16<float> { typedef float4 type; static constexpr int n = 4; };
+template <> struct Vec16<double> { typedef double2 type; static constexpr int n = 2; };
+
+template <class T>
+__global__ __launch_bounds__(256) void k_fill_systems(const CellParams *cp, const int *cells, int ncell, T *K) {
+  const int ci = blockIdx.y;
+  if (ci >= ncell) return;
+  const CellParams p = cp[cells[ci]];
+  typedef typename Vec16<T>::type V;
+  constexpr int VN = Vec16<T>::n;
+  const int nv = p.npad / VN;
+  for (int r = blockIdx.x; r < p.npad; r += gridDim.x) {
+    V *row = reinterpret_cast<V *>(K + p.k_off + (long long)r * p.npad);
+    for (int c = threadIdx.x; c < nv; c += blockDim.x) {
+      T vals[VN];
+#pragma unroll
+      for (int k = 0; k < VN; ++k) vals[k] = (r >= p.ndof && r == c * VN + k) ? T(1) : T(0);
+      V w;
+      memcpy(&w, vals, sizeof(V));
+      row[c] = w;
+    }
+  }
+}
+
+template <class T>
+void launch_fill_systems(hipStream_t s, const CellParams *cp, const int *cells, int ncell, int max_npad, T *K) {
+  if (ncell <= 0) return;
+  int gx = max_npad < 1024 ? max_npad : 1024;
+  if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(k_fill_systems<T>, dim3(gx, ncell), dim3(256), 0, s, cp, cells, ncell, K);
+}
+template void launch_fill_systems<float>(hipStream_t, const CellParams *, const int *, int, int, float *);
+template void launch_fill_systems<double>(hipStream_t, const CellParams *, const int *, int, int, double *);
+
+__global__ __launch_bounds__(256) void k_fill(double *p, long long n, double value) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+void launch_fill(hipStream_t s, double *p, long long n, double value) {
+  if (n <= 0) return;
+  long long b = (n + 255) / 256;
+  int g = (int)(b < 4096 ? b : 4096);
+  hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, p, n, value);
+}
+
+__global__ __launch_bounds__(256) void k_neg_copy(const double *src, double *dst, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = -src[i];
+}
+void launch_neg_copy(hipStream_t s, const double *src, double *dst, long long n) {
+  if (n <= 0) return;
+  long long b = (n + 255) / 256;
+  int g = (int)(b < 4096 ? b : 4096);
+  hipLaunchKernelGGL(k_neg_copy, dim3(g), dim3(256), 0, s, src, dst, n);
+}
+
+// ---------------------------------------------------------------------------
+// element assembly (one colour per launch)
+// ---------------------------------------------------------------------------
+template <class T, int MODE>
+__global__ __launch_bounds__(128) void k_assemble(View v, int e_begin, int e_end, T *K, double *G, double *Ke64,
+                                                  double *Rea, double *MM, CellState *cs) {
+  const int e = e_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e_end) return;
+  const int c = v.el_cell[e];
+  const int n[3] = {v.el_tri[3 * e], v.el_tri[3 * e + 1], v.el_tri[3 * e + 2]};
+  if (MODE == ASM_STRAIN) {
+    if (!v.cs[c].converged || v.cp[c].law != LAW_UL) return;
+    double re[9], mm[3];
+    element_strain(v, e, re, mm);
+    for (int a = 0; a < 3; ++a) {
+      for (int q = 0; q < 3; ++q) Rea[3 * n[a] + q] += re[3 * a + q];
+      MM[n[a]] += mm[a];
+    }
+    return;
+  }
+  double Ge[6], Ke[36];
+  bool ok = element_system<MODE == ASM_FULL>(v, e, Ge, Ke);
+  if (!ok) cs[c].degenerate = 1;
+  for (int a = 0; a < 3; ++a) {
+    G[2 * n[a]] += Ge[2 * a];
+    G[2 * n[a] + 1] += Ge[2 * a + 1];
+  }
+  if (MODE == ASM_FULL) {
+    const CellParams &p = v.cp[c];
+    T *A = K + p.k_off;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int pp = 0; pp < 2; ++pp) {
+        T *Ar = A + (long long)(2 * (n[a] - p.node_off) + pp) * p.npad;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const int col = 2 * (n[b] - p.node_off);
+          Ar[col] += T(Ke[(2 * a + pp) * 6 + 2 * b]);
+          Ar[col + 1] += T(Ke[(2 * a + pp) * 6 + 2 * b + 1]);
+        }
+      }
+    if (Ke64) {
+      double2 *dst = reinterpret_cast<double2 *>(Ke64 + 36 * (long long)e);
+#pragma unroll
+      for (int k = 0; k < 18; ++k) dst[k] = make_double2(Ke[2 * k], Ke[2 * k + 1]);
+    }
+  }
+}
+
+template <class T>
+void launch_assemble(hipStream_t s, const View &v, int e_begin, int e_end, int mode, T *K, double *G, double *Ke64,
+                     double *Rea, double *MM, int *) {
+  const int n = e_end - e_begin;
+  if (n <= 0) return;
+  dim3 grid((n + 127) / 128), block(128);
+  CellState *cs = const_cast<CellState *>(v.cs);
+  if (mode == ASM_FULL)
+    hipLaunchKernelGGL((k_assemble<T, ASM_FULL>), grid, block, 0, s, v, e_begin, e_end, K, G, Ke64, Rea, MM, cs);
+  else if (mode == ASM_RESIDUAL)
+    hipLaunchKernelGGL((k_assemble<T, ASM_RESIDUAL>), grid, block, 0, s, v, e_begin, e_end, K, G, Ke64, Rea, MM, cs);
+  else
+    hipLaunchKernelGGL((k_assemble<T, ASM_STRAIN>), grid, block, 0, s, v, e_begin, e_end, K, G, Ke64, Rea, MM, cs);
+}
+template void launch_assemble<float>(hipStream_t, const View &, int, int, int, float *, double *, double *, double *,
+                                     double *, int *);
+template void launch_assemble<double>(hipStream_t, const View &, int, int, int, double *, double *, double *,
+                                      double *, double *, int *);
+
+__global__ __launch_bounds__(128) void k_element_dump(View v, int e_begin, int e_end, double *Ge, double *Ke) {
+  const int e = e_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e_end) return;
+  double g[6], k[36];
+  element_system<true>(v, e, g, k);
+  const long long o = e - e_begin;
+  for (int i = 0; i < 6; ++i) Ge[6 * o + i] = g[i];
+  for (int i = 0; i < 36; ++i) Ke[36 * o + i] = k[i];
+}
+void launch_element_dump(hipStream_t s, const View &v, int e_begin, int e_end, double *Ge, double *Ke) {
+  const int n = e_end - e_begin;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_element_dump, dim3((n + 127) / 128), dim3(128), 0, s, v, e_begin, e_end, Ge, Ke);
+}
+
+// ---------------------------------------------------------------------------
+// per-cell FP64 norms (fixed-order tree reduction, deterministic)
+// which: 0 = residual after assembly (sets gnorm, bnorm and g0 at the first
+//        iterate), 1 = refinement residual (rnorm), 2 = residual after the
+//        update (gnorm)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cell_norm(const CellParams *cp, CellState *cs, const int *cells,
+                                                   const double *vec, int which, int only_active) {
+  const int c = cells[blockIdx.x];
+  if (only_active && cs[c].ir_done) return;
+  const CellParams &p = cp[c];
+  const double *v = vec + (long long)p.vec_off;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < p.ndof; i += 256) acc += v[i] * v[i];
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(red[0]);
+    CellState &s = cs[c];
+    if (which == 0) {
+      s.gnorm = nrm;
+      s.bnorm = nrm;
+      if (s.newton_it == 0) s.g0 = nrm;
+    } else if (which == 1) {
+      s.rnorm = nrm;
+    } else {
+      s.gnorm = nrm;
+    }
+  }
+}
+void launch_cell_norm(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell,
+                      const double *vec, int which, int only_active) {
+  if (ncell <= 0) return;
+  hipLaunchKernelGGL(k_cell_norm, dim3(ncell), dim3(256), 0, s, cp, cs, cells, vec, which, only_active);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 refinement residual r = -G - K d, element by element from the stored
+// FP64 element matrices, gathered per node over its incident elements in
+// element order (deterministic, no atomics)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_matvec_ir(const int *el_tri, const int *node_cell, const int *adj_start,
+                                                   const int *adj, const double *Ke64, const double *G,
+                                                   const double *delta, double *r, int node_begin, int node_end,
+                                                   const CellState *cs) {
+  const int i = node_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= node_end) return;
+  if (cs[node_cell[i]].ir_done) return;
+  const double2 *d2 = reinterpret_cast<const double2 *>(delta);
+  double y0 = 0.0, y1 = 0.0;
+  for (int k = adj_start[i]; k < adj_start[i + 1]; ++k) {
+    const int pk = adj[k];
+    const int e = pk >> 2, a = pk & 3;
+    const double *Ke = Ke64 + 36 * (long long)e + 12 * a;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const double2 d = d2[el_tri[3 * e + b]];
+      y0 += Ke[2 * b] * d.x + Ke[2 * b + 1] * d.y;
+      y1 += Ke[6 + 2 * b] * d.x + Ke[6 + 2 * b + 1] * d.y;
+    }
+  }
+  r[2 * i] = -G[2 * i] - y0;
+  r[2 * i + 1] = -G[2 * i + 1] - y1;
+}
+void launch_matvec_ir(hipStream_t s, const View &v, const int *node_cell, const int *adj_start, const int *adj,
+                      const double *Ke64, const double *G, const double *delta, double *r, int node_begin,
+                      int node_end, const CellState *cs) {
+  const int n = node_end - node_begin;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_matvec_ir, dim3((n + 255) / 256), dim3(256), 0, s, v.el_tri, node_cell, adj_start, adj, Ke64,
+                     G, delta, r, node_begin, node_end, cs);
+}
+
+__global__ void k_ir_control(CellState *cs, const int *cells, int ncell, double lin_rtol, int reset) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncell) return;
+  CellState &s = cs[cells[i]];
+  if (reset) {
+    s.ir_done = 0;
+    s.ir_iters = 0;
+    s.lin_relres = 0.0;
+    return;
+  }
+  if (s.ir_done) return;
+  s.ir_iters += 1;
+  s.lin_relres = s.bnorm > 0.0 ? s.rnorm / s.bnorm : 0.0;
+  if (s.rnorm <= lin_rtol * s.bnorm) s.ir_done = 1;
+}
+void launch_ir_control(hipStream_t s, CellState *cs, const int *cells, int ncell, double lin_rtol, int reset) {
+  if (ncell <= 0) return;
+  hipLaunchKernelGGL(k_ir_control, dim3((ncell + 63) / 64), dim3(64), 0, s, cs, cells, ncell, lin_rtol, reset);
+}
+
+__global__ __launch_bounds__(256) void k_update_x(double2 *x, const double *delta, int node_begin, int node_end) {
+  const int i = node_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= node_end) return;
+  double2 p = x[i];
+  p.x += delta[2 * i];
+  p.y += delta[2 * i + 1];
+  x[i] = p;
+}
+void launch_update_x(hipStream_t s, double2 *x, const double *delta, int node_begin, int node_end) {
+  const int n = node_end - node_begin;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_update_x, dim3((n + 255) / 256), dim3(256), 0, s, x, delta, node_begin, node_end);
+}
+
+__global__ void k_newton_control(CellState *cs, const int *cells, int ncell, double rtol, double atol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncell) return;
+  CellState &s = cs[cells[i]];
+  s.newton_it += 1;
+  s.converged = (s.gnorm <= rtol * s.g0 || s.gnorm <= atol) ? 1 : 0;
+  if (!s.ir_done) s.lin_failed = 1;
+}
+void launch_newton_control(hipStream_t s, CellState *cs, const int *cells, int ncell, double rtol, double atol) {
+  if (ncell <= 0) return;
+  hipLaunchKernelGGL(k_newton_control, dim3((ncell + 63) / 64), dim3(64), 0, s, cs, cells, ncell, rtol, atol);
+}
+
+// ---------------------------------------------------------------------------
+// commit of a converged load step
+// ---------------------------------------------------------------------------
+// UL:607-612 strain update; UL:2046-2171 mean-field bond kinetics with a rigid
+// substrate (subs_u = 0): fa_u += x - X, catch-slip k_off, forward Euler c_C,
+// dissipative rescale; the new reference configuration is the converged x.
+__global__ __launch_bounds__(256) void k_commit_nodes(View v, const int *node_cell, double2 *X, double2 *x, double *ea,
+                                                      double *cC, double2 *fau, const double *Rea, const double *MM,
+                                                      int node_begin, int node_end, double dt) {
+  const int i = node_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= node_end) return;
+  const int c = node_cell[i];
+  if (!v.cs[c].converged) return;
+  const CellParams &p = v.cp[c];
+  if (p.law == LAW_UL) {
+    const double m = MM[i];
+    for (int q = 0; q < 3; ++q) ea[3 * i + q] += Rea[3 * i + q] / m;
+  }
+  const double2 xi = x[i], Xi = X[i];
+  if (p.node_springs) {
+    double2 u = fau[i];
+    u.x = u.x + (xi.x - Xi.x);
+    u.y = u.y + (xi.y - Xi.y);
+    if (p.kinetics == KIN_MEANFIELD) {
+      double force;
+      if (p.spring_law == SPRING_LINEAR) {
+        force = p.k_lin * sqrt(u.x * u.x + u.y * u.y);  // 2D norm, UL1:2077
+      } else {
+        double f[2], J[4];
+        spring_force(v, p, u.x, u.y, f, J);
+        force = sqrt(f[0] * f[0] + f[1] * f[1]);  // UL:2137
+      }
+      const double k_on = 1.0, ka = 0.004, kb = 10.0, Fa = 15.0, Fb = 15.0;  // UL:2139-2143
+      const double k_off = ka * exp(force / Fa) + kb * exp(-force / Fb);
+      const double cold = cC[i];
+      double cnew = cold + dt * k_on * (1.0 - cold) - dt * k_off * cold;
+      if (cnew < 0.0) cnew = 0.0;
+      cC[i] = cnew;
+      const double keep = cold - dt * k_off * cold;
+      if (keep > 0.0 && cnew > 0.0) {
+        u.x = u.x * keep / cnew;
+        u.y = u.y * keep / cnew;
+      } else {
+        u.x = 0.0 * u.x;
+        u.y = 0.0 * u.y;
+      }
+    }
+    fau[i] = u;
+  }
+  if (p.law == LAW_UL) X[i] = xi;
+}
+void launch_commit_nodes(hipStream_t s, const View &v, const int *node_cell, double2 *X, double2 *x, double *ea,
+                         double *cC, double2 *fau, const double *Rea, const double *MM, int node_begin, int node_end,
+                         double dt) {
+  const int n = node_end - node_begin;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_commit_nodes, dim3((n + 255) / 256), dim3(256), 0, s, v, node_cell, X, x, ea, cC, fau, Rea, MM,
+                     node_begin, node_end, dt);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox4x32_10(unsigned c[4], unsigned k0, unsigned k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c[0];
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c[2];
+    const unsigned n0 = (unsigned)(p1 >> 32) ^ c[1] ^ k0;
+    const unsigned n2 = (unsigned)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (unsigned)p1;
+    c[3] = (unsigned)p0;
+    c[0] = n0;
+    c[2] = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Bell kinetics per barycentric spring after a converged load step:
+// bound -> unbound with p = 1 - exp(-k_off0 e^{|F|/f_b} dt), unbound -> bound
+// with p = 1 - exp(-k_on dt), re-anchored at the current attachment point.
+// Uniform u = (w + 0.5) 2^-32, w = Philox(ctr = (replica, spring, step, 0)).
+__global__ __launch_bounds__(256) void k_commit_springs(View v, const int *sp_cell, const int *sp_elem,
+                                                        const int *sp_local, double *state, double *anchor,
+                                                        int s_begin, int s_end, double dt, double k_on,
+                                                        double k_off0, double f_b, unsigned key_lo, unsigned key_hi,
+                                                        const int *replica_of_cell, int *transitions) {
+  const int s = s_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= s_end) return;
+  const int c = sp_cell[s];
+  const CellState &cst = v.cs[c];
+  if (!cst.converged) return;
+  const CellParams &p = v.cp[c];
+  if (p.kinetics != KIN_BELL) return;
+  const int e = sp_elem[s];
+  const double *bb = v.sp_bary + 3 * s;
+  double px = 0.0, py = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double2 xa = v.x[v.el_tri[3 * e + a]];
+    px += bb[a] * xa.x;
+    py += bb[a] * xa.y;
+  }
+  unsigned ctr[4] = {(unsigned)replica_of_cell[c], (unsigned)sp_local[s], (unsigned)cst.step, 0u};
+  philox4x32_10(ctr, key_lo, key_hi);
+  const double u = ((double)ctr[0] + 0.5) * (1.0 / 4294967296.0);
+  if (state[s] != 0.0) {
+    const double ux = px - anchor[2 * s], uy = py - anchor[2 * s + 1];
+    double f[2], J[4];
+    spring_force(v, p, ux, uy, f, J);
+    const double F = sqrt(f[0] * f[0] + f[1] * f[1]);
+    const double prob = 1.0 - exp(-k_off0 * exp(F / f_b) * dt);
+    if (u < prob) {
+      state[s] = 0.0;
+      atomicAdd(transitions, 1);
+    }
+  } else {
+    const double prob = 1.0 - exp(-k_on * dt);
+    if (u < prob) {
+      state[s] = 1.0;
+      anchor[2 * s] = px;
+      anchor[2 * s + 1] = py;
+      atomicAdd(transitions, 1);
+    }
+  }
+}
+void launch_commit_springs(hipStream_t s, const View &v, const int *sp_cell, const int *sp_elem, const int *sp_local,
+                           double *state, double *anchor, int s_begin, int s_end, double dt, double k_on,
+                           double k_off0, double f_b, unsigned key_lo, unsigned key_hi, const int *replica_of_cell,
+                           int *transitions) {
+  const int n = s_end - s_begin;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_commit_springs, dim3((n + 255) / 256), dim3(256), 0, s, v, sp_cell, sp_elem, sp_local, state,
+                     anchor, s_begin, s_end, dt, k_on, k_off0, f_b, key_lo, key_hi, replica_of_cell, transitions);
+}
+
+__global__ void k_commit_cells(CellState *cs, const int *cells, int ncell, double dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncell) return;
+  CellState &s = cs[cells[i]];
+  if (!s.converged) return;
+  s.time += dt;
+  s.step += 1;
+  s.newton_it = 0;
+}
+void launch_commit_cells(hipStream_t s, CellState *cs, const int *cells, int ncell, double dt) {
+  if (ncell <= 0) return;
+  hipLaunchKernelGGL(k_commit_cells, dim3((ncell + 63) / 64), dim3(64), 0, s, cs, cells, ncell, dt);
+}
+
+// ---------------------------------------------------------------------------
+// static-pivot threshold: tau = sqrt(eps) * max_i |K_ii| per cell
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_diag_tau(const CellParams *cp, CellState *cs, const int *cells, const T *K) {
+  const int c = cells[blockIdx.x];
+  const CellParams &p = cp[c];
+  const T *A = K + p.k_off;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < p.ndof; i += 256) m = fmax(m, fabs((double)A[(long long)i * p.npad + i]));
+  __shared__ double red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double eps = sizeof(T) == 4 ? 5.9604644775390625e-08 : 1.1102230246251565e-16;
+    cs[c].tau = sqrt(eps) * red[0];
+    cs[c].n_static = 0;
+    cs[c].zero_pivot = 0;
+  }
+}
+template <class T>
+void launch_diag_tau(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell, const T *K) {
+  if (ncell <= 0) return;
+  hipLaunchKernelGGL(k_diag_tau<T>, dim3(ncell), dim3(256), 0, s, cp, cs, cells, K);
+}
+template void launch_diag_tau<float>(hipStream_t, const CellParams *, CellState *, const int *, int, const float *);
+template void launch_diag_tau<double>(hipStream_t, const CellParams *, CellState *, const int *, int, const double *);
+
+}  // namespace wcfe

@@ -1,0 +1,1272 @@
+// C ABI of the MI355X per-step cell mechanics hot path (include/wcfe.h):
+// batch layout in HBM, colouring, LU work lists, the Newton step sequence and
+// its hipGraph capture.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/wcfe.h"
+#include "wcfe_internal.hpp"
+
+using namespace wcfe;
+
+namespace {
+
+struct HipError {
+  hipError_t e;
+  const char *what;
+};
+#define HIPCHECK(x)                             \
+  do {                                          \
+    hipError_t err_ = (x);                      \
+    if (err_ != hipSuccess) throw HipError{err_, #x}; \
+  } while (0)
+
+struct ApiError {
+  int code;
+  std::string msg;
+};
+[[noreturn]] void fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw ApiError{code, buf};
+}
+
+template <class T> T *dalloc(size_t n) {
+  if (n == 0) n = 1;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) fail(WCFE_ERR_OOM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+  return static_cast<T *>(p);
+}
+template <class T> T *dupload(const std::vector<T> &h) {
+  T *d = dalloc<T>(h.size());
+  if (!h.empty()) HIPCHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+struct HostCell {
+  int n_nodes = 0, n_tris = 0, law = 0, node_springs = 0, spring_law = 0, table_id = 0, kinetics = 0, n_spr = 0;
+  double mu = 1000, E = 1000, nu = 0.3, thick = 1, rho_max = 100, k_lin = 1000;
+  std::vector<double> X, x, ea, cC, fau, spr_bary, spr_anchor, spr_state;
+  std::vector<int> tris, spr_tri;
+};
+
+struct Table {
+  std::vector<double> px, py, k;
+};
+
+// Greedy first-fit element colouring in element-index order (64 colours max).
+int color_elements(int n_nodes, int n_tris, const int *tris, std::vector<int> &color) {
+  std::vector<unsigned long long> used(n_nodes, 0ull);
+  color.assign(n_tris, 0);
+  int nc = 0;
+  for (int e = 0; e < n_tris; ++e) {
+    const unsigned long long forb = used[tris[3 * e]] | used[tris[3 * e + 1]] | used[tris[3 * e + 2]];
+    if (forb == ~0ull) return -1;
+    const int col = __builtin_ctzll(~forb);
+    color[e] = col;
+    for (int a = 0; a < 3; ++a) used[tris[3 * e + a]] |= (1ull << col);
+    nc = std::max(nc, col + 1);
+  }
+  return nc;
+}
+
+// LU work lists: for every block step, the exclusive prefix sums of work items
+// over the cells of a list (all cells batched, or one cell at a time).
+struct StepWL {
+  int off[5];
+  int tot[5];
+};
+enum { WL_DIAG = 0, WL_TRSM = 1, WL_GEMM = 2, WL_PANEL = 3, WL_SWP = 4 };
+
+struct LuPlan {
+  int max_nblk = 0;
+  int *d_prefix = nullptr;
+  std::vector<StepWL> batched;
+  std::vector<std::vector<StepWL>> single;
+
+  void build(const std::vector<CellParams> &cp, int pivoting) {
+    const int n = (int)cp.size();
+    std::vector<int> buf;
+    auto items = [&](const CellParams &p, int k, int kind) -> int {
+      if (k >= p.nblk) return 0;
+      const int m = p.nblk - k - 1;
+      switch (kind) {
+        case WL_DIAG: return pivoting ? 0 : 1;
+        case WL_TRSM: return pivoting ? m : 2 * m;
+        case WL_GEMM: return m * m;
+        case WL_PANEL: return pivoting ? 1 : 0;
+        default: return (pivoting && m > 0) ? (m * NB + 255) / 256 : 0;
+      }
+    };
+    max_nblk = 0;
+    for (auto &p : cp) max_nblk = std::max(max_nblk, p.nblk);
+    batched.assign(max_nblk, StepWL{});
+    for (int k = 0; k < max_nblk; ++k)
+      for (int kind = 0; kind < 5; ++kind) {
+        batched[k].off[kind] = (int)buf.size();
+        int acc = 0;
+        for (int c = 0; c < n; ++c) {
+          buf.push_back(acc);
+          acc += items(cp[c], k, kind);
+        }
+        buf.push_back(acc);
+        batched[k].tot[kind] = acc;
+      }
+    single.assign(n, {});
+    for (int c = 0; c < n; ++c) {
+      single[c].assign(cp[c].nblk, StepWL{});
+      for (int k = 0; k < cp[c].nblk; ++k)
+        for (int kind = 0; kind < 5; ++kind) {
+          single[c][k].off[kind] = (int)buf.size();
+          const int t = items(cp[c], k, kind);
+          buf.push_back(0);
+          buf.push_back(t);
+          single[c][k].tot[kind] = t;
+        }
+    }
+    d_prefix = dupload(buf);
+  }
+};
+
+// A batch of cells laid out in HBM.
+struct Batch {
+  int ncell = 0, N = 0, T = 0, S = 0, ncolor = 0, max_npad = 0, any_ul = 0, any_bell = 0;
+  long long kpool = 0;
+  std::vector<CellParams> cp_h;
+  std::vector<int> color_start;                 // global colour-major ranges (ncolor + 1)
+  std::vector<std::vector<int>> cell_color;     // per cell: ncolor + 1 global offsets
+  std::vector<std::vector<int>> elem_pos;       // per cell: original element -> global id
+  std::vector<std::vector<int>> elem_color;     // per cell: colour of each original element
+  std::vector<std::vector<int>> spr_pos;        // per cell: original spring -> global id
+  std::vector<int> ncolors_cell;
+  // device
+  double2 *X = nullptr, *x = nullptr, *fau = nullptr;
+  double *ea = nullptr, *cC = nullptr;
+  int *el_tri = nullptr, *el_cell = nullptr, *el_spr = nullptr;
+  double *sp_bary = nullptr, *sp_anchor = nullptr, *sp_state = nullptr;
+  int *sp_cell = nullptr, *sp_elem = nullptr, *sp_local = nullptr;
+  CellParams *cp = nullptr;
+  CellState *cs = nullptr;
+  int *node_cell = nullptr, *adj_start = nullptr, *adj = nullptr;
+  double *Ke64 = nullptr, *G = nullptr, *r = nullptr, *delta = nullptr, *Rea = nullptr, *MM = nullptr;
+  void *K = nullptr;
+  int *ipiv = nullptr;
+  int *cells = nullptr;
+  int *replica = nullptr;
+  int *transitions = nullptr;
+  LuPlan plan;
+
+  void release() {
+    void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
+                    sp_local, cp, cs, node_cell, adj_start, adj, Ke64, G, r, delta, Rea, MM, K, ipiv, cells,
+                    replica, transitions, plan.d_prefix};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+    *this = Batch();
+  }
+};
+
+}  // namespace
+
+struct wcfe_ctx {
+  wcfe_options opt;
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string arch;
+  size_t max_lds = 65536;
+  std::map<int, Table> tables;
+  std::vector<HostCell> cells;
+  bool finalized = false;
+  Batch b;
+  double *tab_px = nullptr, *tab_py = nullptr, *tab_k = nullptr;
+  std::map<int, std::pair<int, int>> table_off;  // id -> (offset, n)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+int set_err(wcfe_ctx *ctx, int code, const std::string &m) {
+  if (ctx) ctx->err = m;
+  return code;
+}
+
+template <class F> int guarded(wcfe_ctx *ctx, F &&f) {
+  try {
+    f();
+    return WCFE_OK;
+  } catch (const ApiError &e) {
+    return set_err(ctx, e.code, e.msg);
+  } catch (const HipError &e) {
+    return set_err(ctx, WCFE_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
+  } catch (const std::bad_alloc &) {
+    return set_err(ctx, WCFE_ERR_OOM, "host allocation failed");
+  }
+}
+
+bool mixed(const wcfe_ctx *ctx) { return ctx->opt.precision == WCFE_PREC_MIXED; }
+
+View make_view(const wcfe_ctx *ctx, const Batch &b) {
+  View v;
+  v.X = b.X; v.x = b.x; v.ea = b.ea; v.cC = b.cC; v.fau = b.fau;
+  v.el_tri = b.el_tri; v.el_cell = b.el_cell; v.el_spr = b.el_spr;
+  v.sp_bary = b.sp_bary; v.sp_anchor = b.sp_anchor; v.sp_state = b.sp_state;
+  v.cp = b.cp; v.cs = b.cs;
+  v.tab_px = ctx->tab_px; v.tab_py = ctx->tab_py; v.tab_k = ctx->tab_k;
+  v.kappa = ctx->opt.kappa;
+  return v;
+}
+
+// Lay out cells in HBM.  Elements are stored colour-major (colour k of every
+// cell, then colour k+1, ...), so one colour of the whole batch, or of one
+// cell, is a contiguous range.
+void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int elem_size) {
+  b.ncell = (int)hc.size();
+  std::vector<std::vector<int>> colors(b.ncell);
+  b.ncolors_cell.assign(b.ncell, 0);
+  b.ncolor = 0;
+  for (int c = 0; c < b.ncell; ++c) {
+    const int nc = color_elements(hc[c].n_nodes, hc[c].n_tris, hc[c].tris.data(), colors[c]);
+    if (nc < 0) fail(WCFE_ERR_ARG, "cell %d needs more than 64 element colours", c);
+    b.ncolors_cell[c] = nc;
+    b.ncolor = std::max(b.ncolor, nc);
+  }
+  b.cp_h.assign(b.ncell, CellParams{});
+  int N = 0, S = 0, sys = 0;
+  long long kp = 0;
+  b.max_npad = 0;
+  b.any_ul = b.any_bell = 0;
+  for (int c = 0; c < b.ncell; ++c) {
+    const HostCell &h = hc[c];
+    CellParams &p = b.cp_h[c];
+    p.law = h.law;
+    p.node_springs = h.node_springs;
+    p.spring_law = h.spring_law;
+    p.kinetics = h.kinetics;
+    if (h.spring_law == WCFE_SPRING_TABLE) {
+      auto it = ctx->table_off.find(h.table_id);
+      if (it == ctx->table_off.end()) fail(WCFE_ERR_ARG, "cell %d uses undefined table %d", c, h.table_id);
+      p.table_off = it->second.first;
+      p.table_n = it->second.second;
+    }
+    p.node_off = N;
+    p.n_nodes = h.n_nodes;
+    p.n_elems = h.n_tris;
+    p.spr_off = S;
+    p.n_spr = h.n_spr;
+    p.ndof = 2 * h.n_nodes;
+    p.npad = ((p.ndof + NB - 1) / NB) * NB;
+    p.nblk = p.npad / NB;
+    p.sys_off = sys;
+    p.vec_off = 2 * N;
+    p.k_off = kp;
+    p.mu = h.mu; p.E = h.E; p.nu = h.nu; p.thick = h.thick; p.k_lin = h.k_lin; p.rho_max = h.rho_max;
+    N += h.n_nodes;
+    S += h.n_spr;
+    sys += p.npad;
+    kp += (long long)p.npad * p.npad;
+    b.max_npad = std::max(b.max_npad, p.npad);
+    b.any_ul |= (h.law == WCFE_LAW_UL);
+    b.any_bell |= (h.kinetics == WCFE_KIN_BELL);
+  }
+  b.N = N;
+  b.S = S;
+  b.kpool = kp;
+  // colour-major element order
+  b.color_start.assign(b.ncolor + 1, 0);
+  b.cell_color.assign(b.ncell, std::vector<int>(b.ncolor + 1, 0));
+  b.elem_pos.assign(b.ncell, {});
+  b.elem_color = colors;
+  int T = 0;
+  for (int c = 0; c < b.ncell; ++c) {
+    T += hc[c].n_tris;
+    b.elem_pos[c].assign(hc[c].n_tris, -1);
+  }
+  b.T = T;
+  std::vector<int> el_tri(3 * (size_t)T), el_cell(T);
+  int g = 0;
+  for (int k = 0; k < b.ncolor; ++k) {
+    b.color_start[k] = g;
+    for (int c = 0; c < b.ncell; ++c) {
+      b.cell_color[c][k] = g;
+      const HostCell &h = hc[c];
+      for (int e = 0; e < h.n_tris; ++e) {
+        if (colors[c][e] != k) continue;
+        b.elem_pos[c][e] = g;
+        for (int a = 0; a < 3; ++a) el_tri[3 * (size_t)g + a] = b.cp_h[c].node_off + h.tris[3 * e + a];
+        el_cell[g] = c;
+        ++g;
+      }
+    }
+  }
+  b.color_start[b.ncolor] = g;
+  for (int c = 0; c < b.ncell; ++c) b.cell_color[c][b.ncolor] = g;
+  // fix per-cell colour ranges: cell_color[c][k] .. next colour start of the same cell
+  std::vector<std::vector<int>> ranges(b.ncell, std::vector<int>(2 * b.ncolor, 0));
+  for (int c = 0; c < b.ncell; ++c)
+    for (int k = 0; k < b.ncolor; ++k) {
+      int cnt = 0;
+      for (int e = 0; e < hc[c].n_tris; ++e) cnt += colors[c][e] == k;
+      ranges[c][2 * k] = b.cell_color[c][k];
+      ranges[c][2 * k + 1] = b.cell_color[c][k] + cnt;
+    }
+  b.cell_color = ranges;  // now 2 entries (begin, end) per colour
+  // springs: per cell, ordered by host element global id, then original index
+  std::vector<int> el_spr(2 * (size_t)T, 0), sp_cell(S), sp_elem(S), sp_local(S);
+  std::vector<double> sp_bary(3 * (size_t)S), sp_anchor(2 * (size_t)S), sp_state(S);
+  b.spr_pos.assign(b.ncell, {});
+  for (int c = 0; c < b.ncell; ++c) {
+    const HostCell &h = hc[c];
+    std::vector<int> order(h.n_spr);
+    for (int s = 0; s < h.n_spr; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int bb) {
+      return b.elem_pos[c][h.spr_tri[a]] < b.elem_pos[c][h.spr_tri[bb]];
+    });
+    b.spr_pos[c].assign(h.n_spr, 0);
+    int base = b.cp_h[c].spr_off;
+    for (int i = 0; i < h.n_spr; ++i) {
+      const int s = order[i], gs = base + i;
+      const int ge = b.elem_pos[c][h.spr_tri[s]];
+      b.spr_pos[c][s] = gs;
+      sp_cell[gs] = c;
+      sp_elem[gs] = ge;
+      sp_local[gs] = s;
+      for (int q = 0; q < 3; ++q) sp_bary[3 * (size_t)gs + q] = h.spr_bary[3 * s + q];
+      for (int q = 0; q < 2; ++q) sp_anchor[2 * (size_t)gs + q] = h.spr_anchor[2 * s + q];
+      sp_state[gs] = h.spr_state.empty() ? 1.0 : h.spr_state[s];
+      if (el_spr[2 * (size_t)ge + 1] == 0 && el_spr[2 * (size_t)ge] == 0) el_spr[2 * (size_t)ge] = gs;
+      el_spr[2 * (size_t)ge + 1] = gs + 1;
+    }
+  }
+  // node -> (element, corner) adjacency in increasing element order
+  std::vector<int> adj_start(N + 1, 0), adj(3 * (size_t)T), node_cell(N);
+  for (int e = 0; e < T; ++e)
+    for (int a = 0; a < 3; ++a) adj_start[el_tri[3 * (size_t)e + a] + 1]++;
+  for (int i = 0; i < N; ++i) adj_start[i + 1] += adj_start[i];
+  {
+    std::vector<int> fill(adj_start.begin(), adj_start.end() - 1);
+    for (int e = 0; e < T; ++e)
+      for (int a = 0; a < 3; ++a) adj[fill[el_tri[3 * (size_t)e + a]]++] = 4 * e + a;
+  }
+  for (int c = 0; c < b.ncell; ++c)
+    for (int i = 0; i < hc[c].n_nodes; ++i) node_cell[b.cp_h[c].node_off + i] = c;
+  // node state
+  std::vector<double2> X(N), x(N), fau(N);
+  std::vector<double> ea(3 * (size_t)N, 0.0), cC(N, 0.0);
+  for (int c = 0; c < b.ncell; ++c) {
+    const HostCell &h = hc[c];
+    const int o = b.cp_h[c].node_off;
+    for (int i = 0; i < h.n_nodes; ++i) {
+      X[o + i] = make_double2(h.X[2 * i], h.X[2 * i + 1]);
+      x[o + i] = h.x.empty() ? X[o + i] : make_double2(h.x[2 * i], h.x[2 * i + 1]);
+      fau[o + i] = h.fau.empty() ? make_double2(0, 0) : make_double2(h.fau[2 * i], h.fau[2 * i + 1]);
+      if (!h.ea.empty())
+        for (int q = 0; q < 3; ++q) ea[3 * (size_t)(o + i) + q] = h.ea[3 * i + q];
+      if (!h.cC.empty()) cC[o + i] = h.cC[i];
+    }
+  }
+  std::vector<CellState> cs(b.ncell);
+  for (auto &s : cs) {
+    memset(&s, 0, sizeof s);
+    s.time = ctx->opt.t0;
+  }
+  std::vector<int> ident(b.ncell), replica(b.ncell);
+  for (int c = 0; c < b.ncell; ++c) ident[c] = replica[c] = c;
+  // upload
+  b.X = dupload(X);
+  b.x = dupload(x);
+  b.fau = dupload(fau);
+  b.ea = dupload(ea);
+  b.cC = dupload(cC);
+  b.el_tri = dupload(el_tri);
+  b.el_cell = dupload(el_cell);
+  b.el_spr = dupload(el_spr);
+  b.sp_bary = dupload(sp_bary);
+  b.sp_anchor = dupload(sp_anchor);
+  b.sp_state = dupload(sp_state);
+  b.sp_cell = dupload(sp_cell);
+  b.sp_elem = dupload(sp_elem);
+  b.sp_local = dupload(sp_local);
+  b.cp = dupload(b.cp_h);
+  b.cs = dupload(cs);
+  b.node_cell = dupload(node_cell);
+  b.adj_start = dupload(adj_start);
+  b.adj = dupload(adj);
+  b.cells = dupload(ident);
+  b.replica = dupload(replica);
+  b.transitions = dalloc<int>(1);
+  HIPCHECK(hipMemset(b.transitions, 0, sizeof(int)));
+  b.Ke64 = dalloc<double>(36 * (size_t)T);
+  b.G = dalloc<double>(2 * (size_t)N);
+  b.r = dalloc<double>(2 * (size_t)N);
+  b.delta = dalloc<double>(2 * (size_t)N);
+  b.Rea = dalloc<double>(3 * (size_t)N);
+  b.MM = dalloc<double>(N);
+  b.ipiv = dalloc<int>(std::max(sys, 1));
+  HIPCHECK(hipMemset(b.ipiv, 0, sizeof(int) * std::max(sys, 1)));
+  b.K = (void *)dalloc<unsigned char>((size_t)kp * elem_size);
+  b.plan.build(b.cp_h, ctx->opt.pivoting);
+  if (lu_solve_lds_bytes(b.max_npad, elem_size) > ctx->max_lds)
+    fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
+         ctx->max_lds);
+}
+
+struct Scope {
+  const int *cells;
+  int ncell;
+  int node_begin, node_end, spr_begin, spr_end;
+  int cell;  // -1 = whole batch
+};
+
+Scope scope_all(const Batch &b) { return Scope{b.cells, b.ncell, 0, b.N, 0, b.S, -1}; }
+Scope scope_cell(const Batch &b, int c) {
+  const CellParams &p = b.cp_h[c];
+  return Scope{b.cells + c, 1, p.node_off, p.node_off + p.n_nodes, p.spr_off, p.spr_off + p.n_spr, c};
+}
+
+void color_range(const Batch &b, const Scope &sc, int k, int &e0, int &e1) {
+  if (sc.cell < 0) {
+    e0 = b.color_start[k];
+    e1 = b.color_start[k + 1];
+  } else {
+    e0 = b.cell_color[sc.cell][2 * k];
+    e1 = b.cell_color[sc.cell][2 * k + 1];
+  }
+}
+
+WorkList worklist(const Batch &b, const Scope &sc, int k, int kind, bool *any) {
+  const StepWL *w;
+  if (sc.cell < 0) {
+    if (k >= (int)b.plan.batched.size()) { *any = false; return WorkList{}; }
+    w = &b.plan.batched[k];
+  } else {
+    if (k >= (int)b.plan.single[sc.cell].size()) { *any = false; return WorkList{}; }
+    w = &b.plan.single[sc.cell][k];
+  }
+  *any = w->tot[kind] > 0;
+  return WorkList{sc.cells, b.plan.d_prefix + w->off[kind], sc.ncell, w->tot[kind]};
+}
+
+template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &sc, int mode) {
+  hipStream_t s = ctx->stream;
+  View v = make_view(ctx, b);
+  T *K = static_cast<T *>(b.K);
+  for (int k = 0; k < b.ncolor; ++k) {
+    int e0, e1;
+    color_range(b, sc, k, e0, e1);
+    launch_assemble<T>(s, v, e0, e1, mode, K, b.G, b.Ke64, b.Rea, b.MM, nullptr);
+  }
+}
+
+template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc) {
+  hipStream_t s = ctx->stream;
+  T *K = static_cast<T *>(b.K);
+  const int nsteps = sc.cell < 0 ? b.plan.max_nblk : b.cp_h[sc.cell].nblk;
+  const bool piv = ctx->opt.pivoting == WCFE_PIVOT_PARTIAL;
+  if (!piv) launch_diag_tau<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K);
+  for (int k = 0; k < nsteps; ++k) {
+    bool any;
+    if (piv) {
+      WorkList wp = worklist(b, sc, k, WL_PANEL, &any);
+      if (any) launch_lu_panel_pp<T>(s, b.cp, b.cs, wp, k, K, b.ipiv);
+      WorkList ws = worklist(b, sc, k, WL_SWP, &any);
+      if (any) launch_lu_laswp<T>(s, b.cp, ws, k, K, b.ipiv);
+    } else {
+      WorkList wd = worklist(b, sc, k, WL_DIAG, &any);
+      if (any) launch_lu_diag<T>(s, b.cp, b.cs, wd, k, K);
+    }
+    WorkList wt = worklist(b, sc, k, WL_TRSM, &any);
+    if (any) launch_lu_trsm<T>(s, b.cp, wt, k, K, piv ? 0 : 1);
+    WorkList wg = worklist(b, sc, k, WL_GEMM, &any);
+    if (any) launch_lu_gemm<T>(s, b.cp, wg, k, K);
+  }
+}
+
+// The refinement residual for a mesh batch: r = -G - K d from the FP64
+// element matrices.
+template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scope &sc) {
+  hipStream_t s = ctx->stream;
+  T *K = static_cast<T *>(b.K);
+  View v = make_view(ctx, b);
+  const int piv = ctx->opt.pivoting == WCFE_PIVOT_PARTIAL;
+  const long long v0 = 2LL * sc.node_begin, vn = 2LL * (sc.node_end - sc.node_begin);
+  launch_ir_control(s, b.cs, sc.cells, sc.ncell, ctx->opt.lin_rtol, 1);
+  launch_neg_copy(s, b.G + v0, b.r + v0, vn);
+  const int npad = sc.cell < 0 ? b.max_npad : b.cp_h[sc.cell].npad;
+  for (int it = 0; it < std::max(1, ctx->opt.max_refine); ++it) {
+    launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, b.ipiv, piv, b.r, b.delta, npad, it > 0);
+    launch_matvec_ir(s, v, b.node_cell, b.adj_start, b.adj, b.Ke64, b.G, b.delta, b.r, sc.node_begin, sc.node_end,
+                     b.cs);
+    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.r, 1, 1);
+    launch_ir_control(s, b.cs, sc.cells, sc.ncell, ctx->opt.lin_rtol, 0);
+  }
+}
+
+template <class T> void enqueue_newton(wcfe_ctx *ctx, Batch &b, const Scope &sc) {
+  hipStream_t s = ctx->stream;
+  T *K = static_cast<T *>(b.K);
+  const long long v0 = 2LL * sc.node_begin, vn = 2LL * (sc.node_end - sc.node_begin);
+  const int npad = sc.cell < 0 ? b.max_npad : b.cp_h[sc.cell].npad;
+  // 1. assemble K(x), G(x)
+  launch_fill_systems<T>(s, b.cp, sc.cells, sc.ncell, npad, K);
+  launch_fill(s, b.G + v0, vn, 0.0);
+  enqueue_assemble<T>(ctx, b, sc, ASM_FULL);
+  launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.G, 0, 0);
+  // 2. factor
+  enqueue_factor<T>(ctx, b, sc);
+  // 3. solve K d = -G with FP64 refinement
+  enqueue_solve_refine<T>(ctx, b, sc);
+  // 4. x += d, FP64 residual at the new x, convergence test
+  launch_update_x(s, b.x, b.delta, sc.node_begin, sc.node_end);
+  launch_fill(s, b.G + v0, vn, 0.0);
+  enqueue_assemble<T>(ctx, b, sc, ASM_RESIDUAL);
+  launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.G, 2, 0);
+  launch_newton_control(s, b.cs, sc.cells, sc.ncell, ctx->opt.newton_rtol, ctx->opt.newton_atol);
+  // 5. commit converged cells
+  const bool any_ul = sc.cell < 0 ? b.any_ul : (b.cp_h[sc.cell].law == LAW_UL);
+  if (any_ul) {
+    launch_fill(s, b.Rea + 3LL * sc.node_begin, 3LL * (sc.node_end - sc.node_begin), 0.0);
+    launch_fill(s, b.MM + sc.node_begin, sc.node_end - sc.node_begin, 0.0);
+    enqueue_assemble<T>(ctx, b, sc, ASM_STRAIN);
+  }
+  View v = make_view(ctx, b);
+  launch_commit_nodes(s, v, b.node_cell, b.X, b.x, b.ea, b.cC, b.fau, b.Rea, b.MM, sc.node_begin, sc.node_end,
+                      ctx->opt.dt);
+  const bool any_bell = sc.cell < 0 ? b.any_bell : (b.cp_h[sc.cell].kinetics == KIN_BELL);
+  if (any_bell)
+    launch_commit_springs(s, v, b.sp_cell, b.sp_elem, b.sp_local, b.sp_state, b.sp_anchor, sc.spr_begin, sc.spr_end,
+                          ctx->opt.dt, ctx->opt.k_on, ctx->opt.k_off0, ctx->opt.f_bell, ctx->opt.seed_lo,
+                          ctx->opt.seed_hi, b.replica, b.transitions);
+  launch_commit_cells(s, b.cs, sc.cells, sc.ncell, ctx->opt.dt);
+}
+
+void enqueue_step(wcfe_ctx *ctx) {
+  Batch &b = ctx->b;
+  if (ctx->opt.batched) {
+    if (mixed(ctx)) enqueue_newton<float>(ctx, b, scope_all(b));
+    else enqueue_newton<double>(ctx, b, scope_all(b));
+  } else {
+    for (int c = 0; c < b.ncell; ++c) {
+      if (mixed(ctx)) enqueue_newton<float>(ctx, b, scope_cell(b, c));
+      else enqueue_newton<double>(ctx, b, scope_cell(b, c));
+    }
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+void check_cell(const wcfe_ctx *ctx, int cell) {
+  if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
+  if (cell < 0 || cell >= ctx->b.ncell) fail(WCFE_ERR_ARG, "cell %d out of range [0, %d)", cell, ctx->b.ncell);
+}
+
+void collect_stats(wcfe_ctx *ctx, wcfe_step_stats *st, int *err_code) {
+  Batch &b = ctx->b;
+  std::vector<CellState> cs(b.ncell);
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  HIPCHECK(hipMemcpy(cs.data(), b.cs, sizeof(CellState) * b.ncell, hipMemcpyDeviceToHost));
+  wcfe_step_stats s;
+  memset(&s, 0, sizeof s);
+  s.n_cells = b.ncell;
+  int degenerate = -1, zero = -1;
+  for (int c = 0; c < b.ncell; ++c) {
+    const CellState &q = cs[c];
+    s.n_converged += q.converged;
+    s.max_refine_used = std::max(s.max_refine_used, q.ir_iters);
+    s.n_static_pivots += q.n_static;
+    s.n_failed += q.ir_done ? 0 : 1;
+    s.max_lin_relres = std::max(s.max_lin_relres, q.lin_relres);
+    const double rel = q.g0 > 0 ? q.gnorm / q.g0 : 0.0;
+    s.max_newton_rel = std::max(s.max_newton_rel, rel);
+    if (q.degenerate && degenerate < 0) degenerate = c;
+    if (q.zero_pivot && zero < 0) zero = c;
+  }
+  if (st) *st = s;
+  *err_code = WCFE_OK;
+  if (degenerate >= 0) {
+    *err_code = WCFE_ERR_DEGENERATE;
+    ctx->err = "cell " + std::to_string(degenerate) + " has a zero-area or inverted element";
+  } else if (zero >= 0) {
+    *err_code = WCFE_ERR_ZERO_PIVOT;
+    ctx->err = "cell " + std::to_string(zero) + ": exactly singular column in partial-pivot LU";
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void wcfe_default_options(wcfe_options *o) {
+  memset(o, 0, sizeof *o);
+  o->device = 0;
+  o->precision = WCFE_PREC_MIXED;
+  o->pivoting = WCFE_PIVOT_STATIC;
+  o->batched = 1;
+  o->use_graph = 1;
+  o->max_refine = 8;
+  o->lin_rtol = 1e-8;
+  o->newton_rtol = 1e-8;
+  o->newton_atol = 1e-12;
+  o->max_newton = 20;
+  o->dt = 0.005;
+  o->t0 = 0.5;
+  o->seed_lo = 5;
+  o->seed_hi = 0;
+  o->k_on = 1.0;
+  o->k_off0 = 0.1;
+  o->f_bell = 4.0;
+  o->kappa = 0.5;
+}
+
+int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
+  if (!out) return WCFE_ERR_ARG;
+  *out = nullptr;
+  wcfe_ctx *ctx = new (std::nothrow) wcfe_ctx();
+  if (!ctx) return WCFE_ERR_OOM;
+  if (opt) ctx->opt = *opt;
+  else wcfe_default_options(&ctx->opt);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= ctx->opt.device) {
+    delete ctx;
+    return WCFE_ERR_NO_DEVICE;
+  }
+  int rc = guarded(ctx, [&] {
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, ctx->opt.device));
+    ctx->arch = prop.gcnArchName;
+    if (ctx->arch.find("gfx950") == std::string::npos)
+      fail(WCFE_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", ctx->opt.device,
+           prop.gcnArchName);
+    int lds = 0;
+    HIPCHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->opt.device));
+    ctx->max_lds = (size_t)lds;
+    lu_prepare_attributes(ctx->max_lds);
+    HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  });
+  if (rc != WCFE_OK) {
+    // keep the context so the caller can read the message, but report the code
+    *out = ctx;
+    return rc;
+  }
+  *out = ctx;
+  return WCFE_OK;
+}
+
+int wcfe_destroy(wcfe_ctx *ctx) {
+  if (!ctx) return WCFE_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->b.release();
+  if (ctx->tab_px) (void)hipFree(ctx->tab_px);
+  if (ctx->tab_py) (void)hipFree(ctx->tab_py);
+  if (ctx->tab_k) (void)hipFree(ctx->tab_k);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return WCFE_OK;
+}
+
+const char *wcfe_last_error(const wcfe_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int wcfe_device_arch(const wcfe_ctx *ctx, char *buf, int len) {
+  if (!ctx || !buf || len <= 0) return WCFE_ERR_ARG;
+  snprintf(buf, len, "%s", ctx->arch.c_str());
+  return WCFE_OK;
+}
+
+int wcfe_set_table(wcfe_ctx *ctx, int32_t id, int32_t n, const double *px, const double *py, const double *k) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (ctx->finalized) fail(WCFE_ERR_STATE, "tables must be set before wcfe_finalize");
+    if (n < 1 || n > MAX_TABLE || !px || !py || !k) fail(WCFE_ERR_ARG, "table %d: need 1..%d knots", id, MAX_TABLE);
+    for (int j = 1; j < n; ++j)
+      if (!(px[j] > px[j - 1])) fail(WCFE_ERR_ARG, "table %d: knots must increase", id);
+    Table t;
+    t.px.assign(px, px + n);
+    t.py.assign(py, py + n);
+    t.k.assign(k, k + n);
+    ctx->tables[id] = t;
+  });
+}
+
+int wcfe_add_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d, int32_t *cell_id) {
+  if (!ctx || !d) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (ctx->finalized) fail(WCFE_ERR_STATE, "cells must be added before wcfe_finalize");
+    if (d->n_nodes <= 0 || d->n_tris <= 0 || !d->X || !d->tris) fail(WCFE_ERR_ARG, "empty mesh");
+    if (d->law != WCFE_LAW_UL && d->law != WCFE_LAW_CST) fail(WCFE_ERR_ARG, "unknown law %d", d->law);
+    if (d->node_springs && d->law != WCFE_LAW_UL)
+      fail(WCFE_ERR_ARG, "node-attached springs follow the reference model and need the UL law");
+    if (d->node_springs && (!d->cC || !d->fau)) fail(WCFE_ERR_ARG, "node springs need cC and fau");
+    if (d->n_spr < 0 || (d->n_spr > 0 && (!d->spr_tri || !d->spr_bary || !d->spr_anchor)))
+      fail(WCFE_ERR_ARG, "barycentric springs need spr_tri, spr_bary and spr_anchor");
+    if (d->kinetics == WCFE_KIN_MEANFIELD && !d->node_springs)
+      fail(WCFE_ERR_ARG, "mean-field kinetics act on node-attached springs");
+    if (d->kinetics == WCFE_KIN_BELL && d->n_spr == 0) fail(WCFE_ERR_ARG, "Bell kinetics act on barycentric springs");
+    if (d->spring_law == WCFE_SPRING_TABLE && !ctx->tables.count(d->table_id))
+      fail(WCFE_ERR_ARG, "table %d has not been set", d->table_id);
+    HostCell h;
+    h.n_nodes = d->n_nodes;
+    h.n_tris = d->n_tris;
+    h.law = d->law;
+    h.mu = d->mu; h.E = d->E; h.nu = d->nu; h.thick = d->thick;
+    h.node_springs = d->node_springs;
+    h.rho_max = d->rho_max;
+    h.spring_law = d->spring_law;
+    h.k_lin = d->k_lin;
+    h.table_id = d->table_id;
+    h.kinetics = d->kinetics;
+    h.n_spr = d->n_spr;
+    h.X.assign(d->X, d->X + 2 * d->n_nodes);
+    if (d->x) h.x.assign(d->x, d->x + 2 * d->n_nodes);
+    h.tris.assign(d->tris, d->tris + 3 * d->n_tris);
+    for (int v : h.tris)
+      if (v < 0 || v >= d->n_nodes) fail(WCFE_ERR_ARG, "triangle node index %d out of range", v);
+    if (d->ea) h.ea.assign(d->ea, d->ea + 3 * d->n_nodes);
+    if (d->cC) h.cC.assign(d->cC, d->cC + d->n_nodes);
+    if (d->fau) h.fau.assign(d->fau, d->fau + 2 * d->n_nodes);
+    if (d->n_spr > 0) {
+      h.spr_tri.assign(d->spr_tri, d->spr_tri + d->n_spr);
+      for (int t : h.spr_tri)
+        if (t < 0 || t >= d->n_tris) fail(WCFE_ERR_ARG, "spring host triangle %d out of range", t);
+      h.spr_bary.assign(d->spr_bary, d->spr_bary + 3 * d->n_spr);
+      h.spr_anchor.assign(d->spr_anchor, d->spr_anchor + 2 * d->n_spr);
+      if (d->spr_state) h.spr_state.assign(d->spr_state, d->spr_state + d->n_spr);
+    }
+    ctx->cells.push_back(std::move(h));
+    if (cell_id) *cell_id = (int32_t)ctx->cells.size() - 1;
+  });
+}
+
+int wcfe_finalize(wcfe_ctx *ctx) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (ctx->finalized) fail(WCFE_ERR_STATE, "already finalized");
+    if (ctx->cells.empty()) fail(WCFE_ERR_ARG, "no cells");
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    // tables
+    std::vector<double> px, py, kk;
+    for (auto &kv : ctx->tables) {
+      ctx->table_off[kv.first] = {(int)px.size(), (int)kv.second.px.size()};
+      px.insert(px.end(), kv.second.px.begin(), kv.second.px.end());
+      py.insert(py.end(), kv.second.py.begin(), kv.second.py.end());
+      kk.insert(kk.end(), kv.second.k.begin(), kv.second.k.end());
+    }
+    ctx->tab_px = dupload(px);
+    ctx->tab_py = dupload(py);
+    ctx->tab_k = dupload(kk);
+    build_batch(ctx, ctx->b, ctx->cells, mixed(ctx) ? 4 : 8);
+    ctx->finalized = true;
+  });
+}
+
+int wcfe_newton_step(wcfe_ctx *ctx, wcfe_step_stats *stats) {
+  if (!ctx) return WCFE_ERR_ARG;
+  int code = WCFE_OK;
+  int rc = guarded(ctx, [&] {
+    if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    if (ctx->opt.use_graph && ctx->opt.batched) {
+      if (!ctx->gexec) {
+        HIPCHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue_step(ctx);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(ctx->stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHECK(hipStreamEndCapture(ctx->stream, &ctx->graph));
+        HIPCHECK(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
+      }
+      HIPCHECK(hipGraphLaunch(ctx->gexec, ctx->stream));
+    } else {
+      enqueue_step(ctx);
+    }
+    if (stats) collect_stats(ctx, stats, &code);
+  });
+  return rc != WCFE_OK ? rc : code;
+}
+
+int wcfe_load_step(wcfe_ctx *ctx, int32_t *iterations, wcfe_step_stats *stats) {
+  if (!ctx) return WCFE_ERR_ARG;
+  int code = WCFE_OK;
+  int rc = guarded(ctx, [&] {
+    if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
+    Batch &b = ctx->b;
+    std::vector<CellState> cs(b.ncell);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(cs.data(), b.cs, sizeof(CellState) * b.ncell, hipMemcpyDeviceToHost));
+    std::vector<int> step0(b.ncell);
+    for (int c = 0; c < b.ncell; ++c) step0[c] = cs[c].step;
+    int it = 0;
+    wcfe_step_stats st;
+    memset(&st, 0, sizeof st);
+    for (;;) {
+      int r = wcfe_newton_step(ctx, &st);
+      if (r != WCFE_OK) fail(r, "%s", ctx->err.c_str());
+      ++it;
+      HIPCHECK(hipMemcpy(cs.data(), b.cs, sizeof(CellState) * b.ncell, hipMemcpyDeviceToHost));
+      bool all = true;
+      for (int c = 0; c < b.ncell; ++c) all &= cs[c].step > step0[c];
+      if (all) break;
+      if (it >= ctx->opt.max_newton) {
+        code = WCFE_ERR_NOT_CONVERGED;
+        ctx->err = "load step did not converge in " + std::to_string(it) + " Newton iterations";
+        break;
+      }
+    }
+    if (iterations) *iterations = it;
+    if (stats) *stats = st;
+  });
+  return rc != WCFE_OK ? rc : code;
+}
+
+int wcfe_synchronize(wcfe_ctx *ctx) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] { HIPCHECK(hipStreamSynchronize(ctx->stream)); });
+}
+
+int wcfe_get_positions(wcfe_ctx *ctx, int32_t cell, double *x) {
+  if (!ctx || !x) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    const CellParams &p = ctx->b.cp_h[cell];
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(x, ctx->b.x + p.node_off, sizeof(double2) * p.n_nodes, hipMemcpyDeviceToHost));
+  });
+}
+
+int wcfe_set_positions(wcfe_ctx *ctx, int32_t cell, const double *x) {
+  if (!ctx || !x) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    const CellParams &p = ctx->b.cp_h[cell];
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(ctx->b.x + p.node_off, x, sizeof(double2) * p.n_nodes, hipMemcpyHostToDevice));
+  });
+}
+
+int wcfe_get_ul_state(wcfe_ctx *ctx, int32_t cell, double *X, double *ea, double *cC, double *fau) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    const CellParams &p = ctx->b.cp_h[cell];
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if (X) HIPCHECK(hipMemcpy(X, ctx->b.X + p.node_off, sizeof(double2) * p.n_nodes, hipMemcpyDeviceToHost));
+    if (ea) HIPCHECK(hipMemcpy(ea, ctx->b.ea + 3LL * p.node_off, sizeof(double) * 3 * p.n_nodes, hipMemcpyDeviceToHost));
+    if (cC) HIPCHECK(hipMemcpy(cC, ctx->b.cC + p.node_off, sizeof(double) * p.n_nodes, hipMemcpyDeviceToHost));
+    if (fau) HIPCHECK(hipMemcpy(fau, ctx->b.fau + p.node_off, sizeof(double2) * p.n_nodes, hipMemcpyDeviceToHost));
+  });
+}
+
+int wcfe_get_springs(wcfe_ctx *ctx, int32_t cell, double *state, double *anchor) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    const CellParams &p = ctx->b.cp_h[cell];
+    if (p.n_spr == 0) return;
+    std::vector<double> st(p.n_spr), an(2 * (size_t)p.n_spr);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(st.data(), ctx->b.sp_state + p.spr_off, sizeof(double) * p.n_spr, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(an.data(), ctx->b.sp_anchor + 2LL * p.spr_off, sizeof(double) * 2 * p.n_spr,
+                       hipMemcpyDeviceToHost));
+    for (int s = 0; s < p.n_spr; ++s) {
+      const int g = ctx->b.spr_pos[cell][s] - p.spr_off;
+      if (state) state[s] = st[g];
+      if (anchor) { anchor[2 * s] = an[2 * g]; anchor[2 * s + 1] = an[2 * g + 1]; }
+    }
+  });
+}
+
+int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step, int32_t *newton_iter,
+                       int32_t *n_colors, int32_t *ndof) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    CellState s;
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(&s, ctx->b.cs + cell, sizeof s, hipMemcpyDeviceToHost));
+    if (time) *time = s.time;
+    if (step) *step = s.step;
+    if (newton_iter) *newton_iter = s.newton_it;
+    if (n_colors) *n_colors = ctx->b.ncolors_cell[cell];
+    if (ndof) *ndof = ctx->b.cp_h[cell].ndof;
+  });
+}
+
+int wcfe_residual_fp64(wcfe_ctx *ctx, int32_t cell, double *G) {
+  if (!ctx || !G) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    Batch &b = ctx->b;
+    Scope sc = scope_cell(b, cell);
+    const CellParams &p = b.cp_h[cell];
+    launch_fill(ctx->stream, b.G + 2LL * p.node_off, 2LL * p.n_nodes, 0.0);
+    if (mixed(ctx)) enqueue_assemble<float>(ctx, b, sc, ASM_RESIDUAL);
+    else enqueue_assemble<double>(ctx, b, sc, ASM_RESIDUAL);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(G, b.G + 2LL * p.node_off, sizeof(double) * 2 * p.n_nodes, hipMemcpyDeviceToHost));
+  });
+}
+
+int wcfe_element_systems(wcfe_ctx *ctx, int32_t cell, double *Ge, double *Ke) {
+  if (!ctx || !Ge || !Ke) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    Batch &b = ctx->b;
+    const CellParams &p = b.cp_h[cell];
+    const int T = p.n_elems;
+    double *dG = dalloc<double>(6 * (size_t)T), *dK = dalloc<double>(36 * (size_t)T);
+    View v = make_view(ctx, b);
+    std::vector<double> hG(6 * (size_t)T), hK(36 * (size_t)T);
+    int done = 0;
+    std::vector<int> order;  // global ids in dump order
+    for (int k = 0; k < b.ncolor; ++k) {
+      const int e0 = b.cell_color[cell][2 * k], e1 = b.cell_color[cell][2 * k + 1];
+      if (e1 <= e0) continue;
+      launch_element_dump(ctx->stream, v, e0, e1, dG + 6LL * done, dK + 36LL * done);
+      for (int e = e0; e < e1; ++e) order.push_back(e);
+      done += e1 - e0;
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(hG.data(), dG, sizeof(double) * hG.size(), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(hK.data(), dK, sizeof(double) * hK.size(), hipMemcpyDeviceToHost));
+    (void)hipFree(dG);
+    (void)hipFree(dK);
+    std::map<int, int> slot;
+    for (int i = 0; i < (int)order.size(); ++i) slot[order[i]] = i;
+    for (int e = 0; e < T; ++e) {
+      const int i = slot[b.elem_pos[cell][e]];
+      memcpy(Ge + 6LL * e, hG.data() + 6LL * i, sizeof(double) * 6);
+      memcpy(Ke + 36LL * e, hK.data() + 36LL * i, sizeof(double) * 36);
+    }
+  });
+}
+
+int wcfe_assembled_system(wcfe_ctx *ctx, int32_t cell, double *K, double *G) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    Batch &b = ctx->b;
+    Scope sc = scope_cell(b, cell);
+    const CellParams &p = b.cp_h[cell];
+    launch_fill(ctx->stream, b.G + 2LL * p.node_off, 2LL * p.n_nodes, 0.0);
+    const size_t es = mixed(ctx) ? 4 : 8;
+    if (mixed(ctx)) {
+      launch_fill_systems<float>(ctx->stream, b.cp, sc.cells, 1, p.npad, static_cast<float *>(b.K));
+      enqueue_assemble<float>(ctx, b, sc, ASM_FULL);
+    } else {
+      launch_fill_systems<double>(ctx->stream, b.cp, sc.cells, 1, p.npad, static_cast<double *>(b.K));
+      enqueue_assemble<double>(ctx, b, sc, ASM_FULL);
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if (G) HIPCHECK(hipMemcpy(G, b.G + 2LL * p.node_off, sizeof(double) * p.ndof, hipMemcpyDeviceToHost));
+    if (K) {
+      std::vector<unsigned char> raw((size_t)p.npad * p.npad * es);
+      HIPCHECK(hipMemcpy(raw.data(), static_cast<unsigned char *>(b.K) + (size_t)p.k_off * es, raw.size(),
+                         hipMemcpyDeviceToHost));
+      for (int i = 0; i < p.ndof; ++i)
+        for (int j = 0; j < p.ndof; ++j) {
+          const size_t o = (size_t)i * p.npad + j;
+          K[(size_t)i * p.ndof + j] =
+              es == 4 ? (double)reinterpret_cast<float *>(raw.data())[o] : reinterpret_cast<double *>(raw.data())[o];
+        }
+    }
+  });
+}
+
+int wcfe_get_colors(wcfe_ctx *ctx, int32_t cell, int32_t *color) {
+  if (!ctx || !color) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    const auto &c = ctx->b.elem_color[cell];
+    for (size_t e = 0; e < c.size(); ++e) color[e] = c[e];
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device buffers owned for the duration of one stateless call.
+struct Scratch {
+  std::vector<void *> ptrs;
+  template <class T> T *up(const T *h, size_t n) {
+    T *d = dalloc<T>(n);
+    ptrs.push_back(d);
+    if (n && h) HIPCHECK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+  }
+  template <class T> T *zeros(size_t n) {
+    T *d = dalloc<T>(n);
+    ptrs.push_back(d);
+    HIPCHECK(hipMemset(d, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    return d;
+  }
+  ~Scratch() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+};
+
+template <class T>
+void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *A, const double *b, double *x,
+                      double *relres, int32_t *iters, int32_t *ipiv_out) {
+  hipStream_t s = ctx->stream;
+  std::vector<CellParams> cp(n_sys);
+  std::vector<long long> a_off(n_sys);
+  long long kp = 0, ap = 0;
+  int vo = 0, sys = 0, max_npad = 0;
+  for (int i = 0; i < n_sys; ++i) {
+    if (n[i] <= 0) fail(WCFE_ERR_ARG, "system %d has size %d", i, n[i]);
+    CellParams &p = cp[i];
+    memset(&p, 0, sizeof p);
+    p.ndof = n[i];
+    p.npad = ((n[i] + NB - 1) / NB) * NB;
+    p.nblk = p.npad / NB;
+    p.k_off = kp;
+    p.sys_off = sys;
+    p.vec_off = vo;
+    a_off[i] = ap;
+    kp += (long long)p.npad * p.npad;
+    ap += (long long)n[i] * n[i];
+    vo += n[i];
+    sys += p.npad;
+    max_npad = std::max(max_npad, p.npad);
+  }
+  if (lu_solve_lds_bytes(max_npad, sizeof(T)) > ctx->max_lds)
+    fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve", max_npad);
+  std::vector<T> Kh((size_t)kp, T(0));
+  for (int i = 0; i < n_sys; ++i) {
+    const CellParams &p = cp[i];
+    for (int r = 0; r < p.npad; ++r)
+      for (int c = 0; c < p.npad; ++c)
+        Kh[(size_t)p.k_off + (size_t)r * p.npad + c] =
+            (r < p.ndof && c < p.ndof) ? (T)A[a_off[i] + (long long)r * p.ndof + c] : (r == c ? T(1) : T(0));
+  }
+  std::vector<CellState> cs(n_sys);
+  for (auto &q : cs) memset(&q, 0, sizeof q);
+  std::vector<int> ident(n_sys);
+  for (int i = 0; i < n_sys; ++i) ident[i] = i;
+  Scratch sc;
+  T *dK = sc.up(Kh.data(), Kh.size());
+  double *dA = sc.up(A, (size_t)ap);
+  long long *dAoff = sc.up(a_off.data(), a_off.size());
+  double *db = sc.up(b, (size_t)vo);
+  double *dr = sc.zeros<double>((size_t)vo);
+  double *dx = sc.zeros<double>((size_t)vo);
+  CellParams *dcp = sc.up(cp.data(), cp.size());
+  CellState *dcs = sc.up(cs.data(), cs.size());
+  int *dcells = sc.up(ident.data(), ident.size());
+  int *dipiv = sc.zeros<int>((size_t)std::max(sys, 1));
+  LuPlan plan;
+  plan.build(cp, ctx->opt.pivoting);
+  sc.ptrs.push_back(plan.d_prefix);
+  const bool piv = ctx->opt.pivoting == WCFE_PIVOT_PARTIAL;
+  if (!piv) launch_diag_tau<T>(s, dcp, dcs, dcells, n_sys, dK);
+  for (int k = 0; k < plan.max_nblk; ++k) {
+    const StepWL &w = plan.batched[k];
+    auto wl = [&](int kind) { return WorkList{dcells, plan.d_prefix + w.off[kind], n_sys, w.tot[kind]}; };
+    if (piv) {
+      launch_lu_panel_pp<T>(s, dcp, dcs, wl(WL_PANEL), k, dK, dipiv);
+      launch_lu_laswp<T>(s, dcp, wl(WL_SWP), k, dK, dipiv);
+    } else {
+      launch_lu_diag<T>(s, dcp, dcs, wl(WL_DIAG), k, dK);
+    }
+    launch_lu_trsm<T>(s, dcp, wl(WL_TRSM), k, dK, piv ? 0 : 1);
+    launch_lu_gemm<T>(s, dcp, wl(WL_GEMM), k, dK);
+  }
+  // refinement: r = b; repeat { d = LU^-1 r; x += d; r = b - A x (FP64) }
+  launch_cell_norm(s, dcp, dcs, dcells, n_sys, db, 0, 0);
+  launch_ir_control(s, dcs, dcells, n_sys, ctx->opt.lin_rtol, 1);
+  HIPCHECK(hipMemcpyAsync(dr, db, sizeof(double) * vo, hipMemcpyDeviceToDevice, s));
+  for (int it = 0; it < std::max(1, ctx->opt.max_refine); ++it) {
+    launch_lu_solve<T>(s, dcp, dcs, dcells, n_sys, dK, dipiv, piv, dr, dx, max_npad, it > 0);
+    launch_dense_residual(s, dcp, dcs, dcells, n_sys, dA, dAoff, db, dx, dr);
+    launch_cell_norm(s, dcp, dcs, dcells, n_sys, dr, 1, 1);
+    launch_ir_control(s, dcs, dcells, n_sys, ctx->opt.lin_rtol, 0);
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  HIPCHECK(hipMemcpy(x, dx, sizeof(double) * vo, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(cs.data(), dcs, sizeof(CellState) * n_sys, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n_sys; ++i) {
+    if (relres) relres[i] = cs[i].lin_relres;
+    if (iters) iters[i] = cs[i].ir_iters;
+    if (cs[i].zero_pivot) fail(WCFE_ERR_ZERO_PIVOT, "system %d is exactly singular", i);
+  }
+  if (ipiv_out) {
+    std::vector<int> pv(std::max(sys, 1));
+    HIPCHECK(hipMemcpy(pv.data(), dipiv, sizeof(int) * pv.size(), hipMemcpyDeviceToHost));
+    int o = 0;
+    for (int i = 0; i < n_sys; ++i)
+      for (int j = 0; j < n[i]; ++j) ipiv_out[o++] = pv[cp[i].sys_off + j];
+  }
+}
+
+// one-ring lists (node -> triangles) in increasing triangle order, as
+// NonDestructiveTriMesh builds m_vertex_to_triangle_map
+// (nondestructivetrimesh.cpp:489-506)
+void one_rings(int n_nodes, int n_tris, const int32_t *tris, std::vector<int> &start, std::vector<int> &list) {
+  start.assign(n_nodes + 1, 0);
+  for (int e = 0; e < n_tris; ++e)
+    for (int a = 0; a < 3; ++a) start[tris[3 * e + a] + 1]++;
+  for (int i = 0; i < n_nodes; ++i) start[i + 1] += start[i];
+  list.assign(start[n_nodes], 0);
+  std::vector<int> fill(start.begin(), start.end() - 1);
+  for (int e = 0; e < n_tris; ++e)
+    for (int a = 0; a < 3; ++a) list[fill[tris[3 * e + a]]++] = e;
+}
+
+void check_mesh(int n_nodes, int n_tris, const int32_t *tris) {
+  if (n_nodes <= 0 || n_tris < 0 || (n_tris > 0 && !tris)) fail(WCFE_ERR_ARG, "invalid mesh sizes");
+  for (long long i = 0; i < 3LL * n_tris; ++i)
+    if (tris[i] < 0 || tris[i] >= n_nodes) fail(WCFE_ERR_ARG, "triangle node index %d out of range", tris[i]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int wcfe_dense_solve(wcfe_ctx *ctx, int32_t n_sys, const int32_t *n, const double *A, const double *b, double *x,
+                     double *relres, int32_t *iters, int32_t *ipiv_out) {
+  if (!ctx || n_sys <= 0 || !n || !A || !b || !x) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    if (mixed(ctx)) dense_solve_impl<float>(ctx, n_sys, n, A, b, x, relres, iters, ipiv_out);
+    else dense_solve_impl<double>(ctx, n_sys, n, A, b, x, relres, iters, ipiv_out);
+  });
+}
+
+int wcfe_calculate_res(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *X,
+                       const double *x, const double *ea, double time, double *Res, double *Res_ea, double *MM) {
+  if (!ctx || !X || !x || !ea || !Res || !Res_ea || !MM) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_mesh(n_nodes, n_tris, tris);
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    HostCell h;
+    h.n_nodes = n_nodes;
+    h.n_tris = n_tris;
+    h.law = WCFE_LAW_UL;
+    h.mu = 1000.0;  // UL:724
+    h.thick = 1.0;  // UL:768
+    h.X.resize(2 * (size_t)n_nodes);
+    h.x.resize(2 * (size_t)n_nodes);
+    h.ea.resize(3 * (size_t)n_nodes);
+    for (int i = 0; i < n_nodes; ++i) {
+      if (X[3 * i + 2] != 0.0 || x[3 * i + 2] != 0.0)
+        fail(WCFE_ERR_ARG, "node %d is off the plane z = 0 (the reference's planar cell, iomesh.cpp:519)", i);
+      const double *e6 = ea + 6 * (size_t)i;
+      if (fabs(e6[2]) > 1e-10 || fabs(e6[4]) > 1e-10 || fabs(e6[5]) > 1e-10)
+        fail(WCFE_ERR_ARG, "node %d carries out-of-plane strain", i);
+      h.X[2 * i] = X[3 * i]; h.X[2 * i + 1] = X[3 * i + 1];
+      h.x[2 * i] = x[3 * i]; h.x[2 * i + 1] = x[3 * i + 1];
+      h.ea[3 * i] = e6[0]; h.ea[3 * i + 1] = e6[1]; h.ea[3 * i + 2] = e6[3];
+    }
+    h.tris.assign(tris, tris + 3 * (size_t)n_tris);
+    Batch tb;
+    try {
+      build_batch(ctx, tb, std::vector<HostCell>{h}, 8);
+      CellState cs;
+      memset(&cs, 0, sizeof cs);
+      cs.time = time;
+      cs.converged = 1;
+      HIPCHECK(hipMemcpy(tb.cs, &cs, sizeof cs, hipMemcpyHostToDevice));
+      Scope sc = scope_all(tb);
+      launch_fill(ctx->stream, tb.G, 2LL * n_nodes, 0.0);
+      enqueue_assemble<double>(ctx, tb, sc, ASM_RESIDUAL);
+      launch_fill(ctx->stream, tb.Rea, 3LL * n_nodes, 0.0);
+      launch_fill(ctx->stream, tb.MM, n_nodes, 0.0);
+      enqueue_assemble<double>(ctx, tb, sc, ASM_STRAIN);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+      std::vector<double> G(2 * (size_t)n_nodes), R(3 * (size_t)n_nodes);
+      HIPCHECK(hipMemcpy(G.data(), tb.G, sizeof(double) * G.size(), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(R.data(), tb.Rea, sizeof(double) * R.size(), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(MM, tb.MM, sizeof(double) * n_nodes, hipMemcpyDeviceToHost));
+      int degenerate = 0;
+      HIPCHECK(hipMemcpy(&cs, tb.cs, sizeof cs, hipMemcpyDeviceToHost));
+      degenerate = cs.degenerate;
+      for (int i = 0; i < n_nodes; ++i) {
+        Res[3 * i] = G[2 * i]; Res[3 * i + 1] = G[2 * i + 1]; Res[3 * i + 2] = 0.0;
+        double *o = Res_ea + 6 * (size_t)i;
+        o[0] = R[3 * i]; o[1] = R[3 * i + 1]; o[2] = 0.0; o[3] = R[3 * i + 2]; o[4] = 0.0; o[5] = 0.0;
+      }
+      tb.release();
+      if (degenerate) fail(WCFE_ERR_DEGENERATE, "zero-area or inverted element");
+    } catch (...) {
+      tb.release();
+      throw;
+    }
+  });
+}
+
+int wcfe_force_fa(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
+                  const double *cC, const double *fau, int32_t law, double kint, double *F) {
+  if (!ctx || !x || !cC || !fau || !F) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_mesh(n_nodes, n_tris, tris);
+    if (law != 0 && law != 1) fail(WCFE_ERR_ARG, "law must be 0 (linear kint) or 1 (MD table)");
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    std::vector<int> st, li;
+    one_rings(n_nodes, n_tris, tris, st, li);
+    Scratch sc;
+    int *dst = sc.up(st.data(), st.size()), *dli = sc.up(li.data(), li.size());
+    int *dtr = sc.up(tris, 3 * (size_t)n_tris);
+    double *dx = sc.up(x, 3 * (size_t)n_nodes), *dc = sc.up(cC, (size_t)n_nodes);
+    double *df = sc.up(fau, 3 * (size_t)n_nodes), *dF = sc.zeros<double>(3 * (size_t)n_nodes);
+    launch_ref_force_fa(ctx->stream, n_nodes, dst, dli, dtr, dx, dc, df, law, kint, dF);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(F, dF, sizeof(double) * 3 * n_nodes, hipMemcpyDeviceToHost));
+  });
+}
+
+int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double *x, double *cA, double *cC,
+                   double *fau, double dt, int32_t n_snodes, int32_t n_stris, const int32_t *stris, const double *SX,
+                   const double *Sx, int32_t law, double kint, int32_t *n_unlocated) {
+  if (!ctx || !X || !x || !cA || !cC || !fau || !SX || !Sx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (n_nodes <= 0) fail(WCFE_ERR_ARG, "no cell nodes");
+    check_mesh(n_snodes, n_stris, stris);
+    if (law != 0 && law != 1) fail(WCFE_ERR_ARG, "law must be 0 (linear kint) or 1 (MD table)");
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    Scratch sc;
+    double *dX = sc.up(X, 3 * (size_t)n_nodes), *dx = sc.up(x, 3 * (size_t)n_nodes);
+    double *dA = sc.up(cA, (size_t)n_nodes), *dC = sc.up(cC, (size_t)n_nodes), *dF = sc.up(fau, 3 * (size_t)n_nodes);
+    int *dst = sc.up(stris, 3 * (size_t)n_stris);
+    double *dSX = sc.up(SX, 3 * (size_t)n_snodes), *dSx = sc.up(Sx, 3 * (size_t)n_snodes);
+    int *dun = sc.zeros<int>(1);
+    launch_ref_update_fa(ctx->stream, n_nodes, dX, dx, dA, dC, dF, dt, n_stris, dst, dSX, dSx, law, kint, dun);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(cA, dA, sizeof(double) * n_nodes, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(cC, dC, sizeof(double) * n_nodes, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(fau, dF, sizeof(double) * 3 * n_nodes, hipMemcpyDeviceToHost));
+    int un = 0;
+    HIPCHECK(hipMemcpy(&un, dun, sizeof(int), hipMemcpyDeviceToHost));
+    if (n_unlocated) *n_unlocated = un;
+  });
+}
+
+}  // extern "C"

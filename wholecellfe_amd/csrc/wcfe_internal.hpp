@@ -1,0 +1,76 @@
+// Host-side launcher declarations shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "wcfe_device.hpp"
+
+namespace wcfe {
+
+// Work list of one launch over a set of cells: `cells` holds cell ids and
+// `prefix` the exclusive prefix sum (length ncell + 1) of work items per cell.
+struct WorkList {
+  const int *cells;
+  const int *prefix;
+  int ncell;
+  int total;
+};
+
+// ---- assembly and vector kernels (k_assemble.hip) ----
+enum AsmMode : int { ASM_FULL = 0, ASM_RESIDUAL = 1, ASM_STRAIN = 2 };
+
+template <class T>
+void launch_fill_systems(hipStream_t s, const CellParams *cp, const int *cells, int ncell, int max_npad, T *K);
+void launch_fill(hipStream_t s, double *p, long long n, double value);
+template <class T>
+void launch_assemble(hipStream_t s, const View &v, int e_begin, int e_end, int mode, T *K, double *G, double *Ke64,
+                     double *Rea, double *MM, int *cs_degenerate_base);
+void launch_element_dump(hipStream_t s, const View &v, int e_begin, int e_end, double *Ge, double *Ke);
+void launch_cell_norm(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell,
+                      const double *vec, int which, int only_active);
+void launch_neg_copy(hipStream_t s, const double *src, double *dst, long long n);
+void launch_matvec_ir(hipStream_t s, const View &v, const int *node_cell, const int *adj_start, const int *adj,
+                      const double *Ke64, const double *G, const double *delta, double *r, int node_begin,
+                      int node_end, const CellState *cs);
+void launch_ir_control(hipStream_t s, CellState *cs, const int *cells, int ncell, double lin_rtol, int reset);
+void launch_update_x(hipStream_t s, double2 *x, const double *delta, int node_begin, int node_end);
+void launch_newton_control(hipStream_t s, CellState *cs, const int *cells, int ncell, double rtol, double atol);
+void launch_commit_nodes(hipStream_t s, const View &v, const int *node_cell, double2 *X, double2 *x, double *ea,
+                         double *cC, double2 *fau, const double *Rea, const double *MM, int node_begin, int node_end,
+                         double dt);
+void launch_commit_springs(hipStream_t s, const View &v, const int *sp_cell, const int *sp_elem, const int *sp_local,
+                           double *state, double *anchor, int s_begin, int s_end, double dt, double k_on,
+                           double k_off0, double f_b, unsigned key_lo, unsigned key_hi, const int *replica_of_cell,
+                           int *transitions);
+void launch_commit_cells(hipStream_t s, CellState *cs, const int *cells, int ncell, double dt);
+template <class T>
+void launch_diag_tau(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell, const T *K);
+
+// ---- dense LU and triangular solves (k_lu.hip) ----
+template <class T>
+void launch_lu_diag(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K);
+template <class T>
+void launch_lu_panel_pp(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K,
+                        int *ipiv);
+template <class T>
+void launch_lu_laswp(hipStream_t s, const CellParams *cp, const WorkList &wl, int k, T *K, const int *ipiv);
+template <class T>
+void launch_lu_trsm(hipStream_t s, const CellParams *cp, const WorkList &wl, int k, T *K, int with_lower);
+template <class T>
+void launch_lu_gemm(hipStream_t s, const CellParams *cp, const WorkList &wl, int k, T *K);
+template <class T>
+void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
+                     const T *K, const int *ipiv, int pivoting, const double *r, double *delta, int max_npad,
+                     int accumulate);
+size_t lu_solve_lds_bytes(int npad, int elem_size);
+void lu_prepare_attributes(size_t max_lds);
+
+// ---- reference-shaped stateless kernels (k_reference.hip) ----
+void launch_ref_force_fa(hipStream_t s, int n_nodes, const int *ring_start, const int *ring, const int *tris,
+                         const double *x, const double *cC, const double *fau, int law, double kint, double *F);
+void launch_ref_update_fa(hipStream_t s, int n_nodes, const double *X, const double *x, double *cA, double *cC,
+                          double *fau, double dt, int n_stris, const int *stris, const double *SX, const double *Sx,
+                          int law, double kint, int *n_unlocated);
+void launch_dense_residual(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
+                           const double *A64, const long long *a_off, const double *b, const double *x, double *r);
+
+}  // namespace wcfe
