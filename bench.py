@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark: Newton steps/s of the batched mixed-precision GPU form vs the
+plain FP64 GPU form, at <= 1e-8 relative linear residual (FP64 refinement).
+
+Workload (default): 256 independent disk cells in one batch, triangle counts
+log-uniform in [200, 5000], E uniform in [0.5, 5] kPa, nu = 0.3, thickness
+0.2 um, spring law per cell drawn from {1 pN/nm, 31 pN/nm, 64-knot stiffening
+table}, T/3 springs per cell; dense FP32 blocked LU (MFMA) + FP64 iterative
+refinement; one hipGraph replay per batched Newton step.
+
+A step is one Newton iteration on every cell of the batch (assemble, factor,
+solve with refinement to 1e-8, update, FP64 residual and convergence test,
+commit of converged load steps).  value = Newton steps/s summed over cells.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "Newton steps/s (batched mixed-prec vs plain FP64 GPU) at 1e-8 rel residual"
+MFMA_FP32_PEAK = 157.3e12
+HBM_PEAK = 8.0e12
+
+
+def cpu_baseline(cells, budget_s=20.0):
+    """The FP64 oracle (oracle/liboracle.so, one thread) timed on a sample of
+    the batch's cells: one Newton step each (assemble K and G, partial-pivot
+    LU, solve, update).  The batch rate is extrapolated with the measured
+    per-cell cost scaled by (n / n_sample)^3 (dense LU dominates)."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_py as O
+    from wholecellfe_amd import mesh
+
+    order = sorted(range(len(cells)), key=lambda i: len(cells[i]["X"]))
+    sample, spent, rec = [], 0.0, []
+    px, py, kk = mesh.stiffening_table()
+    for idx in order[:: max(1, len(order) // 24)]:
+        c = cells[idx]
+        n = 2 * len(c["X"])
+        if n > 3200 or spent > budget_s:
+            continue
+        d = dict(X=c["X"].reshape(-1), x=c["X"].reshape(-1).copy(), tris=c["tris"].reshape(-1), law=1, E=c["E"],
+                 nu=c["nu"], thick=c["thick"], time=0.5, spr_tri=c["spr_tri"], spr_bary=c["spr_bary"].reshape(-1),
+                 spr_anchor=c["spr_anchor"].reshape(-1), spr_state=np.ones(len(c["spr_tri"])),
+                 spring_law=c["spring_law"], k_lin=c["k_lin"], tab_px=px, tab_py=py, tab_k=kk)
+        oc = O.make_cell(d)
+        t0 = time.perf_counter()
+        G, K = O.assemble(oc)
+        LU, piv, _ = O.lu_factor(K, pivoting=1)
+        dx = O.lu_solve(LU, piv, -G, pivoting=1)
+        _ = d["x"] + dx
+        dt = time.perf_counter() - t0
+        spent += dt
+        rec.append((n, dt))
+        sample.append(idx)
+    rec.sort()
+    big = rec[len(rec) // 2:]
+    coef = float(np.median([t / n ** 3 for n, t in big]))
+    total = sum(coef * (2 * len(c["X"])) ** 3 for c in cells)
+    return {"value": len(cells) / total, "unit": "Newton steps/s", "cores": 1, "kind": "port",
+            "sample": "FP64 oracle Newton step (assemble, partial-pivot LU, solve) on %d of %d cells with n <= %d "
+                      "(%.1f s measured); batch time extrapolated as sum of c n^3, c = %.3e s" %
+                      (len(rec), len(cells), max(n for n, _ in rec), spent, coef)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=256)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pivoting", choices=["static", "partial"], default="static")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (independent cells would be "
+              "partitioned across GPUs with no exchange, see DESIGN.md)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import wholecellfe_amd as W
+    from wholecellfe_amd import configs
+
+    log = lambda *m: print("[bench]", *m, file=sys.stderr, flush=True)
+    t_setup = time.perf_counter()
+    cells, tables = configs.batched_cells(n_cells=a.cells)
+    n_tri = sum(len(c["tris"]) for c in cells)
+    ndofs = [2 * len(c["X"]) for c in cells]
+    log("built %d cells: %d triangles, dof %d..%d in %.1fs" % (len(cells), n_tri, min(ndofs), max(ndofs),
+                                                               time.perf_counter() - t_setup))
+    piv = W.PIVOT_STATIC if a.pivoting == "static" else W.PIVOT_PARTIAL
+    common = dict(lin_rtol=1e-8, newton_rtol=1e-7, newton_atol=1e-12, t0=0.5, dt=0.005, pivoting=piv)
+
+    # ---- batched mixed precision, one hipGraph replay per Newton step ----
+    ctx = W.Context(precision=W.PREC_MIXED, batched=1, use_graph=1, max_refine=8, **common)
+    configs.load(ctx, cells, tables)
+    log("mixed context ready")
+    for _ in range(a.warmup):
+        ctx.newton_step(stats=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ctx.newton_step(stats=False)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / a.steps
+    value = len(cells) * a.steps / (t1 - t0)
+    log("mixed: %.2f ms/step, %.0f Newton steps/s" % (ms_step, value))
+    st = ctx.newton_step(stats=True)
+    prof = ctx.profile_step()
+    ctx.close()
+    log("check step:", st)
+    log("phases:", {k: round(v["ms"], 3) for k, v in prof.items()})
+    g = prof["lu_gemm"]
+    tile_flops = 2.0 * 64 ** 3
+    achieved = g["work"] * tile_flops / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
+    roofline = {"kernel": "k_lu_gemm<float> (LU trailing update, v_mfma_f32_16x16x4f32)", "bound": "mfma",
+                "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / MFMA_FP32_PEAK, "traffic": None,
+                "launches_per_step": g["launches"], "tiles_per_step": g["work"],
+                "avg_launch_us": g["ms"] * 1e3 / max(g["launches"], 1),
+                "flops_per_tile": tile_flops}
+    lu_alg = configs.lu_flops(cells)
+    lu_ms = prof["lu_gemm"]["ms"] + prof["lu_panel"]["ms"]
+
+    # ---- plain FP64 GPU form: same kernels in FP64, one cell at a time, no graph ----
+    fp64 = None
+    if not a.no_fp64:
+        c64 = W.Context(precision=W.PREC_FP64, batched=0, use_graph=0, max_refine=3, **common)
+        configs.load(c64, cells, tables)
+        c64.newton_step(stats=False)
+        c64.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.fp64_steps):
+            c64.newton_step(stats=False)
+        c64.synchronize()
+        t1 = time.perf_counter()
+        st64 = c64.newton_step(stats=True)
+        c64.close()
+        v64 = len(cells) * a.fp64_steps / (t1 - t0)
+        fp64 = {"value": v64, "unit": "Newton steps/s", "ms_per_step": (t1 - t0) * 1e3 / a.fp64_steps,
+                "steps": a.fp64_steps, "lin_relres_max": st64["max_lin_relres"]}
+        log("fp64 plain: %.1f ms/step, %.0f Newton steps/s" % (fp64["ms_per_step"], v64))
+
+    cpu = None if a.no_cpu else cpu_baseline(cells)
+    if cpu:
+        log("cpu baseline:", cpu)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "Newton steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": "batched %d disk cells, T log-uniform 200-5000 (total %d triangles), E 0.5-5 kPa, "
+                               "k in {1, 31 pN/nm, 64-knot table}, T/3 springs/cell, dense FP32 LU (MFMA, %s "
+                               "pivoting) + FP64 refinement to 1e-8, hipGraph" % (len(cells), n_tri, a.pivoting),
+                   "cells": len(cells), "triangles": n_tri, "dof_min": min(ndofs), "dof_max": max(ndofs),
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fp64_plain": fp64,
+        "speedup_vs_fp64_plain": (value / fp64["value"]) if fp64 else None,
+        "element_assemblies_per_s": n_tri * a.steps / ((ms_step * a.steps) * 1e-3),
+        "lin_relres_max": st["max_lin_relres"],
+        "refine_sweeps_max": st["max_refine_used"],
+        "static_pivots": st["n_static_pivots"],
+        "lu_fraction_of_mfma_peak": (lu_alg / (lu_ms * 1e-3)) / MFMA_FP32_PEAK if lu_ms > 0 else None,
+        "phases_ms": {k: round(v["ms"], 4) for k, v in prof.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -139,6 +139,24 @@ int wcfe_newton_step(wcfe_ctx *ctx, wcfe_step_stats *stats);
 int wcfe_load_step(wcfe_ctx *ctx, int32_t *iterations, wcfe_step_stats *stats);
 int wcfe_synchronize(wcfe_ctx *ctx);
 
+/* Phases of a Newton step, for wcfe_profile_step. */
+#define WCFE_PH_FILL 0     /* zero the dense systems and residuals */
+#define WCFE_PH_ASSEMBLE 1 /* colour-ordered element assembly of K and G (work = elements) */
+#define WCFE_PH_LU_GEMM 2  /* LU trailing updates on MFMA (work = 64x64 tiles) */
+#define WCFE_PH_LU_PANEL 3 /* LU diagonal/panel factorisation, interchanges, block solves */
+#define WCFE_PH_SOLVE 4    /* triangular solves */
+#define WCFE_PH_REFINE 5   /* FP64 refinement residuals, norms, control */
+#define WCFE_PH_RESIDUAL 6 /* update x, FP64 residual at the new x, convergence test */
+#define WCFE_PH_COMMIT 7   /* strain update, bond kinetics, time advance */
+#define WCFE_NPHASES 8
+/* Run one Newton step eagerly with HIP events around every phase; per phase:
+ * summed kernel milliseconds, launches, and work units (see above).  The
+ * state advances exactly as in wcfe_newton_step.  Diagnostics: the extra
+ * events make this slower than a plain step. */
+int wcfe_profile_step(wcfe_ctx *ctx, double *phase_ms /* [WCFE_NPHASES] */,
+                      int32_t *phase_launches /* [WCFE_NPHASES], optional */,
+                      double *phase_work /* [WCFE_NPHASES], optional */);
+
 /* ---- state access ---- */
 int wcfe_get_positions(wcfe_ctx *ctx, int32_t cell, double *x /* 2N */);
 int wcfe_set_positions(wcfe_ctx *ctx, int32_t cell, const double *x /* 2N */);

@@ -1,0 +1,149 @@
+// wcfe_compat.hpp -- adapters with the reference's call signatures.
+//
+// Drop-in for the hot-path call sites of wholeCellFE's advance_sim
+// (wholeCellFE_kMD/cellBP/main.cpp:356-363) and for updateForces_Explicit's
+// inner calls (UL_growth_explicit.cpp:543, :562).  Each adapter has the
+// signature and argument meaning of the reference function it replaces and
+// forwards to the C ABI in wcfe.h (GPU kernels for gfx950); El Topo meshes,
+// VTK output and everything else stay in the host application.
+//
+// The adapters are templates over the reference's types so this header needs
+// none of the reference's headers: Vec3 is anything with operator[] (Vec3d,
+// vec.h:23), VecX anything with operator()(i) and resize(n) (Eigen::VectorXd),
+// Mesh anything with num_triangles() and get_triangle(i)[k]
+// (NonDestructiveTriMesh, nondestructivetrimesh.h:49-50).
+//
+// Error behaviour: the reference functions return void and never fail; the
+// adapters throw wcfe_compat::error (a std::runtime_error carrying the wcfe
+// error code) when the library reports one, e.g. an out-of-plane node.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "wcfe.h"
+
+namespace wcfe_compat {
+
+struct error : std::runtime_error {
+  int code;
+  error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+// One library context per host thread (the reference is single threaded).
+inline wcfe_ctx *context() {
+  thread_local struct Holder {
+    wcfe_ctx *ctx = nullptr;
+    ~Holder() {
+      if (ctx) wcfe_destroy(ctx);
+    }
+  } h;
+  if (!h.ctx) {
+    int rc = wcfe_create(nullptr, &h.ctx);
+    if (rc != WCFE_OK) {
+      std::string m = h.ctx ? wcfe_last_error(h.ctx) : "no gfx950 device";
+      if (h.ctx) wcfe_destroy(h.ctx);
+      h.ctx = nullptr;
+      throw error(rc, m);
+    }
+  }
+  return h.ctx;
+}
+
+inline void check(int rc) {
+  if (rc != WCFE_OK) throw error(rc, wcfe_last_error(context()));
+}
+
+// Force law of calculateForce_FA / updateFA: the reference selects it by
+// compiling a different folder (k1, k31: linear kint, UL1:1259-1261;
+// kMD: 6-knot MD table, UL:1204-1264).  0 = linear, 1 = MD table.
+inline int &force_law() {
+  thread_local int law = 1;
+  return law;
+}
+
+template <class Vec3> std::vector<double> flat3(const std::vector<Vec3> &v) {
+  std::vector<double> o(3 * v.size());
+  for (size_t i = 0; i < v.size(); ++i)
+    for (int c = 0; c < 3; ++c) o[3 * i + c] = v[i][c];
+  return o;
+}
+
+template <class Mesh> std::vector<int32_t> flat_tris(const Mesh &mesh) {
+  const size_t T = mesh.num_triangles();
+  std::vector<int32_t> t(3 * T);
+  for (size_t e = 0; e < T; ++e)
+    for (int a = 0; a < 3; ++a) t[3 * e + a] = (int32_t)mesh.get_triangle(e)[a];
+  return t;
+}
+
+// calculateRes, cell overload (UL_growth_explicit.cpp:632): FP64 residual,
+// strain increment and lumped mass of the cell law (mat = 1) on the GPU.
+// Res_Adif / Res_Cdif (diffusion terms the reference computes but never uses,
+// UL:617-618) are returned as zero vectors.
+template <class Vec3, class VecX, class Mesh>
+void calculateRes(const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t, std::vector<VecX> &ea_t,
+                  const std::vector<double> &c_A, const std::vector<double> &c_C, const Mesh &mesh, double dt_real,
+                  VecX &Res, VecX &Res_ea, VecX &MM, VecX &Res_Adif, VecX &Res_Cdif, int mat, const void *centroid,
+                  double time) {
+  (void)c_A; (void)c_C; (void)dt_real; (void)centroid;
+  if (mat != 1) throw error(WCFE_ERR_ARG, "calculateRes adapter: only the cell law (mat = 1) runs on the GPU");
+  const int n = (int)X_t.size();
+  std::vector<double> X = flat3(X_t), x = flat3(x_t), ea(6 * (size_t)n);
+  for (int i = 0; i < n; ++i)
+    for (int q = 0; q < 6; ++q) ea[6 * i + q] = ea_t[i](q);
+  std::vector<int32_t> tris = flat_tris(mesh);
+  std::vector<double> R(3 * (size_t)n), Rea(6 * (size_t)n), M(n);
+  check(wcfe_calculate_res(context(), n, (int32_t)(tris.size() / 3), tris.data(), X.data(), x.data(), ea.data(),
+                           time, R.data(), Rea.data(), M.data()));
+  Res.resize(3 * n);
+  Res_ea.resize(6 * n);
+  MM.resize(n);
+  Res_Adif.resize(n);
+  Res_Cdif.resize(n);
+  for (int i = 0; i < 3 * n; ++i) Res(i) = R[i];
+  for (int i = 0; i < 6 * n; ++i) Res_ea(i) = Rea[i];
+  for (int i = 0; i < n; ++i) {
+    MM(i) = M[i];
+    Res_Adif(i) = 0.0;
+    Res_Cdif(i) = 0.0;
+  }
+}
+
+// calculateForce_FA (UL_growth_explicit.cpp:1198 / UL1:1196).
+template <class VecX, class Vec3, class Mesh>
+void calculateForce_FA(VecX &F_constraint, const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t,
+                       const std::vector<double> &c_C, std::vector<Vec3> &fa_u, const Mesh &mesh, double kint) {
+  (void)X_t;
+  const int n = (int)x_t.size();
+  std::vector<double> x = flat3(x_t), fau = flat3(fa_u), F(3 * (size_t)n);
+  std::vector<int32_t> tris = flat_tris(mesh);
+  check(wcfe_force_fa(context(), n, (int32_t)(tris.size() / 3), tris.data(), x.data(), c_C.data(), fau.data(),
+                      force_law(), kint, F.data()));
+  F_constraint.resize(3 * n);
+  for (int i = 0; i < 3 * n; ++i) F_constraint(i) = F[i];
+}
+
+// updateFA, 11-argument form (UL_growth_explicit.cpp:2026 / UL1:2005), as
+// called at main.cpp:356.
+template <class Vec3, class Mesh>
+void updateFA(std::vector<double> &c_A, std::vector<double> &c_C, std::vector<Vec3> &fa_u, double dt_real,
+              const std::vector<Vec3> &X_t_cell, const std::vector<Vec3> &x_t_cell, const Mesh &mesh_cell,
+              const std::vector<Vec3> &X_t_subs, const std::vector<Vec3> &x_t_subs, const Mesh &mesh_subs,
+              double kint) {
+  (void)mesh_cell;
+  const int n = (int)X_t_cell.size();
+  std::vector<double> X = flat3(X_t_cell), x = flat3(x_t_cell), fau = flat3(fa_u);
+  std::vector<double> SX = flat3(X_t_subs), Sx = flat3(x_t_subs);
+  std::vector<int32_t> st = flat_tris(mesh_subs);
+  int32_t unlocated = 0;
+  check(wcfe_update_fa(context(), n, X.data(), x.data(), c_A.data(), c_C.data(), fau.data(), dt_real,
+                       (int32_t)X_t_subs.size(), (int32_t)(st.size() / 3), st.data(), SX.data(), Sx.data(),
+                       force_law(), kint, &unlocated));
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) fa_u[i][c] = fau[3 * i + c];
+}
+
+}  // namespace wcfe_compat

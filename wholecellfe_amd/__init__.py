@@ -30,8 +30,9 @@ EXPORTS = [
     "wcfe_add_cell", "wcfe_finalize", "wcfe_newton_step", "wcfe_load_step", "wcfe_synchronize",
     "wcfe_get_positions", "wcfe_set_positions", "wcfe_get_ul_state", "wcfe_get_springs", "wcfe_get_cell_info",
     "wcfe_residual_fp64", "wcfe_element_systems", "wcfe_assembled_system", "wcfe_get_colors", "wcfe_dense_solve",
-    "wcfe_calculate_res", "wcfe_force_fa", "wcfe_update_fa",
+    "wcfe_calculate_res", "wcfe_force_fa", "wcfe_update_fa", "wcfe_profile_step",
 ]
+PHASES = ["fill", "assemble", "lu_gemm", "lu_panel", "solve", "refine", "residual", "commit"]
 
 
 class Options(C.Structure):
@@ -212,6 +213,15 @@ class Context:
         d = st.as_dict()
         d["iterations"] = it.value
         return d
+
+    def profile_step(self):
+        """One eager Newton step with per-phase HIP-event timing."""
+        n = len(PHASES)
+        ms = np.zeros(n)
+        launches = np.zeros(n, dtype=np.int32)
+        work = np.zeros(n)
+        self._check(self._L.wcfe_profile_step(self._h, _dp(ms), _ip(launches), _dp(work)))
+        return {p: dict(ms=float(ms[i]), launches=int(launches[i]), work=float(work[i])) for i, p in enumerate(PHASES)}
 
     def synchronize(self):
         self._check(self._L.wcfe_synchronize(self._h))

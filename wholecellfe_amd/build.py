@@ -15,6 +15,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwcfe.so")
+REPO = os.path.dirname(HERE)
+COMPAT_SRC = os.path.join(REPO, "tests", "compat_check.cpp")
+COMPAT_BIN = os.path.join(BUILD, "compat_check")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -63,6 +66,13 @@ def build(force=True, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    # host-only check program for include/wcfe_compat.hpp (links libwcfe.so)
+    if os.path.exists(COMPAT_SRC):
+        cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(REPO, "include"), COMPAT_SRC, "-o", COMPAT_BIN,
+               "-L" + HERE, "-lwcfe", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("compat_check build failed:\n%s\n%s" % (r.stdout, r.stderr))
     if verbose:
         print("built", LIB)
     return LIB

@@ -373,11 +373,15 @@ __global__ __launch_bounds__(256) void k_commit_springs(View v, const int *sp_ce
   if (p.kinetics != KIN_BELL) return;
   const int e = sp_elem[s];
   const double *bb = v.sp_bary + 3 * s;
-  double px = 0.0, py = 0.0;
-  for (int a = 0; a < 3; ++a) {
-    const double2 xa = v.x[v.el_tri[3 * e + a]];
-    px += bb[a] * xa.x;
-    py += bb[a] * xa.y;
+  // attachment point (b0 x0 + b1 x1) + b2 x2, rounded term by term without
+  // contraction so a re-anchored bond lands on the same bits as a plain FP64
+  // evaluation of the same expression
+  const double2 x0 = v.x[v.el_tri[3 * e]], x1 = v.x[v.el_tri[3 * e + 1]], x2 = v.x[v.el_tri[3 * e + 2]];
+  double px, py;
+  {
+#pragma clang fp contract(off)
+    px = bb[0] * x0.x + bb[1] * x1.x + bb[2] * x2.x;
+    py = bb[0] * x0.y + bb[1] * x1.y + bb[2] * x2.y;
   }
   unsigned ctr[4] = {(unsigned)replica_of_cell[c], (unsigned)sp_local[s], (unsigned)cst.step, 0u};
   philox4x32_10(ctr, key_lo, key_hi);

@@ -151,6 +151,8 @@ __global__ __launch_bounds__(512) void k_lu_panel_pp(const CellParams *cp, CellS
       int i = ri[0];
       for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
         if (rv[w] > b || (rv[w] == b && ri[w] < i)) { b = rv[w]; i = ri[w]; }
+      // a column of NaNs yields no candidate: keep the diagonal row
+      if (i < col || i >= p.npad) i = col;
       s_piv = i;
       piv_out[col] = i;
       if (b == 0.0) cs[c].zero_pivot = 1;
@@ -274,8 +276,12 @@ void launch_lu_trsm(hipStream_t s, const CellParams *cp, const WorkList &wl, int
 // trailing update on MFMA: C(64x64) -= L(64x64) U(64x64)
 // 4 waves, each a 32 x 32 quadrant = 2 x 2 MFMA 16x16 tiles.
 // Operand layout of v_mfma_*_16x16x4: lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; accumulator register r of lane l is D[4(l/16)+r][l%16].
+// B[l/16][l%16].  Accumulator register r of lane l holds
+//   f32: D[4(l/16) + r][l%16]      f64: D[l/16 + 4r][l%16]
+// (both measured on gfx950 with row/column-tagged operands).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int acc_row(float, int lk, int r) { return 4 * lk + r; }
+__device__ __forceinline__ int acc_row(double, int lk, int r) { return lk + 4 * r; }
 __device__ __forceinline__ floatx4 mfma16(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
@@ -313,7 +319,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, const int
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[ti][tj][r] = C[(long long)(wr + ti * 16 + 4 * lk + r) * ld + wc + tj * 16 + lr];
+      for (int r = 0; r < 4; ++r) acc[ti][tj][r] = C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr];
   __syncthreads();
 #pragma unroll 4
   for (int kk = 0; kk < 64; kk += 4) {
@@ -332,7 +338,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, const int
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) C[(long long)(wr + ti * 16 + 4 * lk + r) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
+      for (int r = 0; r < 4; ++r) C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
 }
 template <class T>
 void launch_lu_gemm(hipStream_t s, const CellParams *cp, const WorkList &wl, int k, T *K) {

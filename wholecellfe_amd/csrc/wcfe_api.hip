@@ -178,6 +178,30 @@ struct Batch {
   }
 };
 
+// Per-phase kernel timing with HIP events on the library's stream (only
+// while wcfe_profile_step runs; never inside a captured graph).
+struct Profiler {
+  struct Rec {
+    int phase;
+    hipEvent_t a, b;
+    long long work;
+  };
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<Rec> recs;
+  hipEvent_t get() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      HIPCHECK(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  ~Profiler() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
 }  // namespace
 
 struct wcfe_ctx {
@@ -195,6 +219,7 @@ struct wcfe_ctx {
   std::map<int, std::pair<int, int>> table_off;  // id -> (offset, n)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  Profiler *prof = nullptr;
 };
 
 namespace {
@@ -218,6 +243,25 @@ template <class F> int guarded(wcfe_ctx *ctx, F &&f) {
 }
 
 bool mixed(const wcfe_ctx *ctx) { return ctx->opt.precision == WCFE_PREC_MIXED; }
+
+struct PhaseScope {
+  wcfe_ctx *ctx;
+  int phase;
+  long long work;
+  hipEvent_t a = nullptr;
+  PhaseScope(wcfe_ctx *c, int ph, long long w = 0) : ctx(c), phase(ph), work(w) {
+    if (ctx->prof) {
+      a = ctx->prof->get();
+      HIPCHECK(hipEventRecord(a, ctx->stream));
+    }
+  }
+  ~PhaseScope() {
+    if (ctx->prof) {
+      hipEvent_t b = ctx->prof->get();
+      if (hipEventRecord(b, ctx->stream) == hipSuccess) ctx->prof->recs.push_back({phase, a, b, work});
+    }
+  }
+};
 
 View make_view(const wcfe_ctx *ctx, const Batch &b) {
   View v;
@@ -476,22 +520,31 @@ template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc)
   T *K = static_cast<T *>(b.K);
   const int nsteps = sc.cell < 0 ? b.plan.max_nblk : b.cp_h[sc.cell].nblk;
   const bool piv = ctx->opt.pivoting == WCFE_PIVOT_PARTIAL;
-  if (!piv) launch_diag_tau<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K);
+  if (!piv) {
+    PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
+    launch_diag_tau<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K);
+  }
   for (int k = 0; k < nsteps; ++k) {
     bool any;
-    if (piv) {
-      WorkList wp = worklist(b, sc, k, WL_PANEL, &any);
-      if (any) launch_lu_panel_pp<T>(s, b.cp, b.cs, wp, k, K, b.ipiv);
-      WorkList ws = worklist(b, sc, k, WL_SWP, &any);
-      if (any) launch_lu_laswp<T>(s, b.cp, ws, k, K, b.ipiv);
-    } else {
-      WorkList wd = worklist(b, sc, k, WL_DIAG, &any);
-      if (any) launch_lu_diag<T>(s, b.cp, b.cs, wd, k, K);
+    {
+      PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
+      if (piv) {
+        WorkList wp = worklist(b, sc, k, WL_PANEL, &any);
+        if (any) launch_lu_panel_pp<T>(s, b.cp, b.cs, wp, k, K, b.ipiv);
+        WorkList ws = worklist(b, sc, k, WL_SWP, &any);
+        if (any) launch_lu_laswp<T>(s, b.cp, ws, k, K, b.ipiv);
+      } else {
+        WorkList wd = worklist(b, sc, k, WL_DIAG, &any);
+        if (any) launch_lu_diag<T>(s, b.cp, b.cs, wd, k, K);
+      }
+      WorkList wt = worklist(b, sc, k, WL_TRSM, &any);
+      if (any) launch_lu_trsm<T>(s, b.cp, wt, k, K, piv ? 0 : 1);
     }
-    WorkList wt = worklist(b, sc, k, WL_TRSM, &any);
-    if (any) launch_lu_trsm<T>(s, b.cp, wt, k, K, piv ? 0 : 1);
     WorkList wg = worklist(b, sc, k, WL_GEMM, &any);
-    if (any) launch_lu_gemm<T>(s, b.cp, wg, k, K);
+    if (any) {
+      PhaseScope ph(ctx, WCFE_PH_LU_GEMM, wg.total);
+      launch_lu_gemm<T>(s, b.cp, wg, k, K);
+    }
   }
 }
 
@@ -503,11 +556,18 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
   View v = make_view(ctx, b);
   const int piv = ctx->opt.pivoting == WCFE_PIVOT_PARTIAL;
   const long long v0 = 2LL * sc.node_begin, vn = 2LL * (sc.node_end - sc.node_begin);
-  launch_ir_control(s, b.cs, sc.cells, sc.ncell, ctx->opt.lin_rtol, 1);
-  launch_neg_copy(s, b.G + v0, b.r + v0, vn);
+  {
+    PhaseScope ph(ctx, WCFE_PH_REFINE);
+    launch_ir_control(s, b.cs, sc.cells, sc.ncell, ctx->opt.lin_rtol, 1);
+    launch_neg_copy(s, b.G + v0, b.r + v0, vn);
+  }
   const int npad = sc.cell < 0 ? b.max_npad : b.cp_h[sc.cell].npad;
   for (int it = 0; it < std::max(1, ctx->opt.max_refine); ++it) {
-    launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, b.ipiv, piv, b.r, b.delta, npad, it > 0);
+    {
+      PhaseScope ph(ctx, WCFE_PH_SOLVE);
+      launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, b.ipiv, piv, b.r, b.delta, npad, it > 0);
+    }
+    PhaseScope ph(ctx, WCFE_PH_REFINE);
     launch_matvec_ir(s, v, b.node_cell, b.adj_start, b.adj, b.Ke64, b.G, b.delta, b.r, sc.node_begin, sc.node_end,
                      b.cs);
     launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.r, 1, 1);
@@ -521,21 +581,31 @@ template <class T> void enqueue_newton(wcfe_ctx *ctx, Batch &b, const Scope &sc)
   const long long v0 = 2LL * sc.node_begin, vn = 2LL * (sc.node_end - sc.node_begin);
   const int npad = sc.cell < 0 ? b.max_npad : b.cp_h[sc.cell].npad;
   // 1. assemble K(x), G(x)
-  launch_fill_systems<T>(s, b.cp, sc.cells, sc.ncell, npad, K);
-  launch_fill(s, b.G + v0, vn, 0.0);
-  enqueue_assemble<T>(ctx, b, sc, ASM_FULL);
-  launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.G, 0, 0);
+  {
+    PhaseScope ph(ctx, WCFE_PH_FILL);
+    launch_fill_systems<T>(s, b.cp, sc.cells, sc.ncell, npad, K);
+    launch_fill(s, b.G + v0, vn, 0.0);
+  }
+  {
+    PhaseScope ph(ctx, WCFE_PH_ASSEMBLE, sc.cell < 0 ? b.T : b.cp_h[sc.cell].n_elems);
+    enqueue_assemble<T>(ctx, b, sc, ASM_FULL);
+    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.G, 0, 0);
+  }
   // 2. factor
   enqueue_factor<T>(ctx, b, sc);
   // 3. solve K d = -G with FP64 refinement
   enqueue_solve_refine<T>(ctx, b, sc);
   // 4. x += d, FP64 residual at the new x, convergence test
-  launch_update_x(s, b.x, b.delta, sc.node_begin, sc.node_end);
-  launch_fill(s, b.G + v0, vn, 0.0);
-  enqueue_assemble<T>(ctx, b, sc, ASM_RESIDUAL);
-  launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.G, 2, 0);
-  launch_newton_control(s, b.cs, sc.cells, sc.ncell, ctx->opt.newton_rtol, ctx->opt.newton_atol);
+  {
+    PhaseScope ph(ctx, WCFE_PH_RESIDUAL);
+    launch_update_x(s, b.x, b.delta, sc.node_begin, sc.node_end);
+    launch_fill(s, b.G + v0, vn, 0.0);
+    enqueue_assemble<T>(ctx, b, sc, ASM_RESIDUAL);
+    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.G, 2, 0);
+    launch_newton_control(s, b.cs, sc.cells, sc.ncell, ctx->opt.newton_rtol, ctx->opt.newton_atol);
+  }
   // 5. commit converged cells
+  PhaseScope ph(ctx, WCFE_PH_COMMIT);
   const bool any_ul = sc.cell < 0 ? b.any_ul : (b.cp_h[sc.cell].law == LAW_UL);
   if (any_ul) {
     launch_fill(s, b.Rea + 3LL * sc.node_begin, 3LL * (sc.node_end - sc.node_begin), 0.0);
@@ -837,6 +907,39 @@ int wcfe_load_step(wcfe_ctx *ctx, int32_t *iterations, wcfe_step_stats *stats) {
     if (stats) *stats = st;
   });
   return rc != WCFE_OK ? rc : code;
+}
+
+int wcfe_profile_step(wcfe_ctx *ctx, double *phase_ms, int32_t *phase_launches, double *phase_work) {
+  if (!ctx || !phase_ms) return WCFE_ERR_ARG;
+  Profiler prof;
+  int rc = guarded(ctx, [&] {
+    if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->prof = &prof;
+    try {
+      enqueue_step(ctx);
+    } catch (...) {
+      ctx->prof = nullptr;
+      throw;
+    }
+    ctx->prof = nullptr;
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < WCFE_NPHASES; ++i) {
+      phase_ms[i] = 0.0;
+      if (phase_launches) phase_launches[i] = 0;
+      if (phase_work) phase_work[i] = 0.0;
+    }
+    for (const Profiler::Rec &r : prof.recs) {
+      float ms = 0.0f;
+      HIPCHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      phase_ms[r.phase] += ms;
+      if (phase_launches) phase_launches[r.phase] += 1;
+      if (phase_work) phase_work[r.phase] += (double)r.work;
+    }
+  });
+  ctx->prof = nullptr;
+  return rc;
 }
 
 int wcfe_synchronize(wcfe_ctx *ctx) {
