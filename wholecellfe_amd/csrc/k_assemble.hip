@@ -431,7 +431,9 @@ void launch_commit_cells(hipStream_t s, CellState *cs, const int *cells, int nce
 }
 
 // ---------------------------------------------------------------------------
-// static-pivot threshold: tau = sqrt(eps) * max_i |K_ii| per cell
+// static-pivot threshold: tau = 64 eps max_i |K_ii| per cell (the stiffness
+// matrices are SPD in the configurations, so only a nearly singular system
+// has pivots that small)
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void k_diag_tau(const CellParams *cp, CellState *cs, const int *cells, const T *K) {
@@ -449,7 +451,7 @@ __global__ __launch_bounds__(256) void k_diag_tau(const CellParams *cp, CellStat
   }
   if (threadIdx.x == 0) {
     const double eps = sizeof(T) == 4 ? 5.9604644775390625e-08 : 1.1102230246251565e-16;
-    cs[c].tau = sqrt(eps) * red[0];
+    cs[c].tau = 64.0 * eps * (red[0] > 0.0 ? red[0] : 1.0);
     cs[c].n_static = 0;
     cs[c].zero_pivot = 0;
   }

@@ -1,19 +1,31 @@
 // Batched dense blocked LU for gfx950: right-looking, 64 x 64 blocks,
 // row-major matrices padded to a multiple of 64 (identity on the padding).
 //
+// Every diagonal block k is factored once (L_kk \ U_kk stored in place) and
+// the explicit inverses L_kk^-1, U_kk^-1 are kept in a side buffer (Dinv).
+// That turns every block triangular solve of the factorisation into a
+// 64 x 64 x 64 product on MFMA, and the diagonal solves of the triangular
+// sweeps into small matrix-vector products.
+//
 // Per block step k, over every cell of the work list at once:
-//   static pivoting : k_lu_diag (one workgroup factors the diagonal block in
-//                     LDS; pivots below tau = sqrt(eps) max|K_ii| are
-//                     replaced by +-tau, Li & Demmel's static pivoting, and
-//                     the error is removed by FP64 iterative refinement)
-//                     -> k_lu_trsm (U row panel and L column panel)
-//   partial pivoting: k_lu_panel_pp (one workgroup factors the 64-wide
-//                     column panel with row interchanges, LAPACK getf2
-//                     semantics inside the panel) -> k_lu_laswp (interchanges
-//                     applied to the trailing columns only, LINPACK style)
-//                     -> k_lu_trsm (U row panel)
-//   then            : k_lu_gemm (trailing update A_ij -= L_ik U_kj on MFMA:
-//                     v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64)
+//   static pivoting : [k = 0: k_lu_diag_inv]
+//                     k_lu_trsm_mm   U_kj = L_kk^-1 A_kj, L_ik = A_ik U_kk^-1
+//                     k_lu_gemm      A_ij -= L_ik U_kj (i, j > k); the
+//                                    workgroup that finishes block (k+1, k+1)
+//                                    goes on to factor it and invert its
+//                                    triangles for step k+1.
+//                     Pivots below tau (64 eps max|K_ii|) are replaced by
+//                     +-tau (static pivoting, Li & Demmel); FP64 iterative
+//                     refinement removes the perturbation.
+//   partial pivoting: k_lu_panel_pp  one workgroup factors the 64-wide
+//                                    column panel with row interchanges
+//                                    (LAPACK getf2 order inside the panel)
+//                                    and inverts its diagonal triangles
+//                     k_lu_laswp     interchanges on the trailing columns
+//                                    only (LINPACK order)
+//                     k_lu_trsm_mm   U_kj = L_kk^-1 A_kj
+//                     k_lu_gemm
+// MFMA: v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64.
 // k_lu_solve applies the factors to one right-hand side per cell with the
 // whole vector resident in LDS.
 #include "wcfe_internal.hpp"
@@ -32,10 +44,16 @@ __device__ __forceinline__ int find_item_cell(const int *prefix, int ncell, int 
   return lo;
 }
 
+// Dinv layout: per cell, per diagonal block k: L_kk^-1 (64 x 64 row-major,
+// unit diagonal included) then U_kk^-1.
+__device__ __forceinline__ long long dinv_off(const CellParams &p, int k) {
+  return (long long)p.sys_off * 128 + (long long)k * 8192;
+}
+
 template <class T> struct Pitch { static constexpr int v = 68; };
 template <> struct Pitch<double> { static constexpr int v = 66; };
 
-// 64 x 64 block (row-major, leading dimension ld) -> LDS tile with pitch P.
+// 64 x 64 block (row-major, leading dimension ld) <-> LDS tile with pitch P.
 template <class T, int P>
 __device__ __forceinline__ void load_block(T *s, const T *g, long long ld) {
   if (sizeof(T) == 4) {
@@ -68,60 +86,128 @@ __device__ __forceinline__ void store_block(T *g, const T *s, long long ld) {
 }
 
 // ---------------------------------------------------------------------------
-// static pivoting: factor the diagonal block in LDS
+// Diagonal block: factorisation without interchanges (static pivoting) and
+// explicit triangular inverses.  `a` is a 64 x 64 LDS tile (pitch P) holding
+// the block; `prow` a 2 x 64 LDS scratch.  Wave 0 owns the rows (lane r ->
+// row r in registers, loops fully unrolled); all threads of the workgroup
+// call these (the barrier count is uniform).  Returns the number of pivots
+// lane `threadIdx.x` replaced (0 or 1).
 // ---------------------------------------------------------------------------
+template <class T, int P>
+__device__ __forceinline__ int factor_diag(T *a, T *prow, T tau) {
+  const int lane = threadIdx.x;
+  const bool w0 = lane < 64;
+  T row[64];
+  int fixed = 0;
+  if (w0) {
+#pragma unroll
+    for (int c = 0; c < 64; ++c) row[c] = a[lane * P + c];
+  }
+#pragma unroll
+  for (int j = 0; j < 64; ++j) {
+    T *pr = prow + (j & 1) * 64;
+    if (lane == j) {
+      if (!(fabs((double)row[j]) >= (double)tau)) {
+        row[j] = row[j] >= T(0) ? tau : -tau;
+        fixed = 1;
+      }
+#pragma unroll
+      for (int c = j; c < 64; ++c) pr[c] = row[c];
+    }
+    __syncthreads();
+    if (w0 && lane > j) {
+      const T l = row[j] / pr[j];
+      row[j] = l;
+#pragma unroll
+      for (int c = j + 1; c < 64; ++c) row[c] -= l * pr[c];
+    }
+  }
+  if (w0) {
+#pragma unroll
+    for (int c = 0; c < 64; ++c) a[lane * P + c] = row[c];
+  }
+  __syncthreads();
+  return fixed;
+}
+
+// Inverses of the unit lower and the upper triangle of the factored block in
+// `a` (LDS), written row-major to Li and Ui (global).  Wave 0: column c of
+// L^-1 by forward substitution; wave 1: column c of U^-1 by back substitution.
+template <class T, int P>
+__device__ __forceinline__ void invert_triangles(const T *a, T *Li, T *Ui) {
+  const int w = threadIdx.x >> 6, c = threadIdx.x & 63;
+  T x[64];
+  if (w == 0) {
+#pragma unroll
+    for (int i = 0; i < 64; ++i) x[i] = (i == c) ? T(1) : T(0);
+#pragma unroll
+    for (int i = 1; i < 64; ++i) {
+      T s = x[i];
+#pragma unroll
+      for (int m = 0; m < i; ++m) s -= a[i * P + m] * x[m];
+      x[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 64; ++i) Li[i * 64 + c] = x[i];
+  } else if (w == 1) {
+#pragma unroll
+    for (int i = 63; i >= 0; --i) {
+      T s = (i == c) ? T(1) : T(0);
+#pragma unroll
+      for (int m = i + 1; m < 64; ++m) s -= a[i * P + m] * x[m];
+      x[i] = s / a[i * P + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 64; ++i) Ui[i * 64 + c] = x[i];
+  }
+}
+
+// k = 0 of static pivoting: factor block (0, 0) and invert its triangles.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_diag(const CellParams *cp, CellState *cs, const int *cells,
-                                                 const int *prefix, int ncell, int k, T *K) {
-  constexpr int P = Pitch<T>::v;
-  __shared__ __attribute__((aligned(16))) T a[64 * P];
+__global__ __launch_bounds__(256) void k_lu_diag_inv(const CellParams *cp, CellState *cs, const int *cells,
+                                                     const int *prefix, int ncell, int k, T *K, T *Dinv) {
+  constexpr int P = 65;
+  __shared__ T a[64 * P];
+  __shared__ T prow[128];
   const int ci = find_item_cell(prefix, ncell, blockIdx.x);
   const int c = cells[ci];
   const CellParams p = cp[c];
-  T *blk = K + p.k_off + (long long)(k * NB) * p.npad + k * NB;
-  load_block<T, P>(a, blk, p.npad);
-  const T tau = (T)cs[c].tau;
-  int nstatic = 0;
+  const long long ld = p.npad;
+  T *blk = K + p.k_off + (long long)(k * NB) * ld + k * NB;
+  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
+    a[(idx >> 6) * P + (idx & 63)] = blk[(idx >> 6) * ld + (idx & 63)];
   __syncthreads();
-  for (int j = 0; j < 64; ++j) {
-    if (threadIdx.x == 0) {
-      T piv = a[j * P + j];
-      if (!(fabs((double)piv) >= (double)tau)) {
-        a[j * P + j] = piv >= T(0) ? tau : -tau;
-        ++nstatic;
-      }
-    }
-    __syncthreads();
-    const T piv = a[j * P + j];
-    if (threadIdx.x > j && threadIdx.x < 64) a[threadIdx.x * P + j] /= piv;
-    __syncthreads();
-    const int m = 63 - j;
-    for (int idx = threadIdx.x; idx < m * m; idx += 256) {
-      const int r = j + 1 + idx / m, cc = j + 1 + idx % m;
-      a[r * P + cc] -= a[r * P + j] * a[j * P + cc];
-    }
-    __syncthreads();
+  const int fixed = factor_diag<T, P>(a, prow, (T)cs[c].tau);
+  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
+    blk[(idx >> 6) * ld + (idx & 63)] = a[(idx >> 6) * P + (idx & 63)];
+  T *D = Dinv + dinv_off(p, k);
+  invert_triangles<T, P>(a, D, D + 4096);
+  if (threadIdx.x < 64) {
+    const int cnt = __popcll(__ballot(fixed != 0));
+    if (threadIdx.x == 0 && cnt) atomicAdd(&cs[c].n_static, cnt);
   }
-  store_block<T, P>(blk, a, p.npad);
-  if (threadIdx.x == 0 && nstatic) atomicAdd(&cs[c].n_static, nstatic);
 }
-
 template <class T>
-void launch_lu_diag(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K) {
+void launch_lu_diag_inv(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K,
+                        T *Dinv) {
   if (wl.total <= 0) return;
-  hipLaunchKernelGGL(k_lu_diag<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.cells, wl.prefix, wl.ncell, k, K);
+  hipLaunchKernelGGL(k_lu_diag_inv<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.cells, wl.prefix, wl.ncell, k, K,
+                     Dinv);
 }
 
 // ---------------------------------------------------------------------------
-// partial pivoting: panel factorisation (one workgroup per cell)
+// partial pivoting: panel factorisation (one workgroup per cell), then the
+// inverses of the diagonal triangles
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(512) void k_lu_panel_pp(const CellParams *cp, CellState *cs, const int *cells,
-                                                     const int *prefix, int ncell, int k, T *K, int *ipiv) {
+                                                     const int *prefix, int ncell, int k, T *K, int *ipiv, T *Dinv) {
+  constexpr int P = 65;
   __shared__ T prow[64];
   __shared__ double rv[16];
   __shared__ int ri[16];
   __shared__ int s_piv;
+  __shared__ T a[64 * P];
   const int ci = find_item_cell(prefix, ncell, blockIdx.x);
   const int c = cells[ci];
   const CellParams p = cp[c];
@@ -179,13 +265,19 @@ __global__ __launch_bounds__(512) void k_lu_panel_pp(const CellParams *cp, CellS
     }
     __syncthreads();
   }
+  T *blk = A + (long long)k0 * ld + k0;
+  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
+    a[(idx >> 6) * P + (idx & 63)] = blk[(idx >> 6) * ld + (idx & 63)];
+  __syncthreads();
+  T *D = Dinv + dinv_off(p, k);
+  invert_triangles<T, P>(a, D, D + 4096);
 }
 template <class T>
 void launch_lu_panel_pp(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K,
-                        int *ipiv) {
+                        int *ipiv, T *Dinv) {
   if (wl.total <= 0) return;
   hipLaunchKernelGGL(k_lu_panel_pp<T>, dim3(wl.total), dim3(512), 0, s, cp, cs, wl.cells, wl.prefix, wl.ncell, k, K,
-                     ipiv);
+                     ipiv, Dinv);
 }
 
 // row interchanges of block k applied to the trailing columns [k0+64, npad)
@@ -218,15 +310,54 @@ void launch_lu_laswp(hipStream_t s, const CellParams *cp, const WorkList &wl, in
 }
 
 // ---------------------------------------------------------------------------
-// triangular block solves: U_kj = L_kk^-1 A_kj (items [0, m)) and, for static
-// pivoting, L_ik = A_ik U_kk^-1 (items [m, 2m)), m = nblk - k - 1
+// 64 x 64 x 64 products on MFMA.  4 waves, each a 32 x 32 quadrant = 2 x 2
+// MFMA 16x16 tiles.  Operand layout of v_mfma_*_16x16x4: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16].  Accumulator register r of lane l holds
+//   f32: D[4(l/16) + r][l%16]      f64: D[l/16 + 4r][l%16]
+// (both measured on gfx950 with row/column-tagged operands).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ floatx4 mfma16(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ int acc_row(float, int lk, int r) { return 4 * lk + r; }
+__device__ __forceinline__ int acc_row(double, int lk, int r) { return lk + 4 * r; }
+template <class T> struct Acc4;
+template <> struct Acc4<float> { typedef floatx4 type; };
+template <> struct Acc4<double> { typedef doublex4 type; };
+
+// acc += sign * X(64x64, LDS pitch PX) Y(64x64, LDS pitch PY) for this wave's quadrant
+template <class T, int PX, int PY>
+__device__ __forceinline__ void mma_64(const T *xs, const T *ys, typename Acc4<T>::type acc[2][2], T sign) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int kk = 0; kk < 64; kk += 4) {
+    T a[2], b[2];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti) a[ti] = sign * xs[(wr + ti * 16 + lr) * PX + kk + lk];
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) b[tj] = ys[(kk + lk) * PY + wc + tj * 16 + lr];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = mfma16(a[ti], b[tj], acc[ti][tj]);
+  }
+}
+
+// block triangular solves as products with the stored inverses:
+//   items [0, m):  U_kj = L_kk^-1 A_kj
+//   items [m, 2m): L_ik = A_ik U_kk^-1   (static pivoting only)
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_trsm(const CellParams *cp, const int *cells, const int *prefix, int ncell,
-                                                 int k, T *K) {
+__global__ __launch_bounds__(256) void k_lu_trsm_mm(const CellParams *cp, const int *cells, const int *prefix,
+                                                    int ncell, int k, T *K, const T *Dinv) {
   constexpr int P = Pitch<T>::v;
-  __shared__ __attribute__((aligned(16))) T d[64 * P];
-  __shared__ __attribute__((aligned(16))) T b[64 * P];
+  typedef typename Acc4<T>::type acc_t;
+  __shared__ __attribute__((aligned(16))) T xs[64 * P];
+  __shared__ __attribute__((aligned(16))) T ys[64 * P];
   const int ci = find_item_cell(prefix, ncell, blockIdx.x);
   const int c = cells[ci];
   const CellParams p = cp[c];
@@ -234,71 +365,54 @@ __global__ __launch_bounds__(256) void k_lu_trsm(const CellParams *cp, const int
   const int m = p.nblk - k - 1;
   const bool upper = t < m;
   const int other = k + 1 + (upper ? t : t - m);
-  T *A = K + p.k_off;
   const long long ld = p.npad;
-  T *diag = A + (long long)(k * NB) * ld + k * NB;
+  T *A = K + p.k_off;
+  const T *D = Dinv + dinv_off(p, k);
   T *blk = upper ? A + (long long)(k * NB) * ld + other * NB : A + (long long)(other * NB) * ld + k * NB;
-  load_block<T, P>(d, diag, ld);
-  load_block<T, P>(b, blk, ld);
-  __syncthreads();
   if (upper) {
-    // forward substitution with the unit lower triangle, all 64 columns at once
-    for (int i = 0; i < 63; ++i) {
-      const int nr = 63 - i;
-      for (int idx = threadIdx.x; idx < nr * 64; idx += 256) {
-        const int r = i + 1 + (idx >> 6), cc = idx & 63;
-        b[r * P + cc] -= d[r * P + i] * b[i * P + cc];
-      }
-      __syncthreads();
-    }
+    load_block<T, P>(xs, D, 64);
+    load_block<T, P>(ys, blk, ld);
   } else {
-    // X U = B, column by column
-    for (int j = 0; j < 64; ++j) {
-      if (threadIdx.x < 64) b[threadIdx.x * P + j] /= d[j * P + j];
-      __syncthreads();
-      const int nc = 63 - j;
-      for (int idx = threadIdx.x; idx < nc * 64; idx += 256) {
-        const int r = idx / nc, cc = j + 1 + idx % nc;
-        b[r * P + cc] -= b[r * P + j] * d[j * P + cc];
-      }
-      __syncthreads();
-    }
+    load_block<T, P>(xs, blk, ld);
+    load_block<T, P>(ys, D + 4096, 64);
   }
-  store_block<T, P>(blk, b, ld);
+  __syncthreads();
+  acc_t acc[2][2];
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
+  mma_64<T, P, P>(xs, ys, acc, T(1));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        blk[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
 }
 template <class T>
-void launch_lu_trsm(hipStream_t s, const CellParams *cp, const WorkList &wl, int k, T *K, int) {
+void launch_lu_trsm(hipStream_t s, const CellParams *cp, CellState *, const WorkList &wl, int k, T *K,
+                    const T *Dinv) {
   if (wl.total <= 0) return;
-  hipLaunchKernelGGL(k_lu_trsm<T>, dim3(wl.total), dim3(256), 0, s, cp, wl.cells, wl.prefix, wl.ncell, k, K);
+  hipLaunchKernelGGL(k_lu_trsm_mm<T>, dim3(wl.total), dim3(256), 0, s, cp, wl.cells, wl.prefix, wl.ncell, k, K, Dinv);
 }
 
-// ---------------------------------------------------------------------------
-// trailing update on MFMA: C(64x64) -= L(64x64) U(64x64)
-// 4 waves, each a 32 x 32 quadrant = 2 x 2 MFMA 16x16 tiles.
-// Operand layout of v_mfma_*_16x16x4: lane l supplies A[l%16][l/16] and
-// B[l/16][l%16].  Accumulator register r of lane l holds
-//   f32: D[4(l/16) + r][l%16]      f64: D[l/16 + 4r][l%16]
-// (both measured on gfx950 with row/column-tagged operands).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int acc_row(float, int lk, int r) { return 4 * lk + r; }
-__device__ __forceinline__ int acc_row(double, int lk, int r) { return lk + 4 * r; }
-__device__ __forceinline__ floatx4 mfma16(float a, float b, floatx4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-template <class T> struct Acc4;
-template <> struct Acc4<float> { typedef floatx4 type; };
-template <> struct Acc4<double> { typedef doublex4 type; };
-
+// trailing update C = A_ij - L_ik U_kj; with `next_diag` (static pivoting)
+// the workgroup of block (k+1, k+1) then factors that block and stores the
+// inverses of its triangles for step k+1
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, const int *cells, const int *prefix, int ncell,
-                                                 int k, T *K) {
+__global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState *cs, const int *cells,
+                                                 const int *prefix, int ncell, int k, T *K, T *Dinv, int next_diag) {
   constexpr int P = Pitch<T>::v;
   typedef typename Acc4<T>::type acc_t;
   __shared__ __attribute__((aligned(16))) T ls[64 * P];
   __shared__ __attribute__((aligned(16))) T us[64 * P];
+  __shared__ T prow[128];
   const int ci = find_item_cell(prefix, ncell, blockIdx.x);
   const int c = cells[ci];
   const CellParams p = cp[c];
@@ -319,42 +433,102 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, const int
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[ti][tj][r] = C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr];
+      for (int r = 0; r < 4; ++r)
+        acc[ti][tj][r] = C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr];
   __syncthreads();
-#pragma unroll 4
-  for (int kk = 0; kk < 64; kk += 4) {
-    T a[2], b[2];
-#pragma unroll
-    for (int ti = 0; ti < 2; ++ti) a[ti] = -ls[(wr + ti * 16 + lr) * P + kk + lk];
-#pragma unroll
-    for (int tj = 0; tj < 2; ++tj) b[tj] = us[(kk + lk) * P + wc + tj * 16 + lr];
+  mma_64<T, P, P>(ls, us, acc, T(-1));
+  const bool diag_next = next_diag && bi == k + 1 && bj == k + 1;
+  if (!diag_next) {
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-      for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = mfma16(a[ti], b[tj], acc[ti][tj]);
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
+    return;
   }
+  // block (k+1, k+1) is final for step k+1: factor it here
+  __syncthreads();
+  constexpr int PA = 65;
+  T *a = ls;  // reuse: 64 x 65 fits in 64 x P
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
+      for (int r = 0; r < 4; ++r) a[(wr + ti * 16 + acc_row(T(0), lk, r)) * PA + wc + tj * 16 + lr] = acc[ti][tj][r];
+  __syncthreads();
+  const int fixed = factor_diag<T, PA>(a, prow, (T)cs[c].tau);
+  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
+    C[(idx >> 6) * ld + (idx & 63)] = a[(idx >> 6) * PA + (idx & 63)];
+  T *D = Dinv + dinv_off(p, k + 1);
+  invert_triangles<T, PA>(a, D, D + 4096);
+  if (threadIdx.x < 64) {
+    const int cnt = __popcll(__ballot(fixed != 0));
+    if (threadIdx.x == 0 && cnt) atomicAdd(&cs[c].n_static, cnt);
+  }
 }
 template <class T>
-void launch_lu_gemm(hipStream_t s, const CellParams *cp, const WorkList &wl, int k, T *K) {
+void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
+                    int next_diag) {
   if (wl.total <= 0) return;
-  hipLaunchKernelGGL(k_lu_gemm<T>, dim3(wl.total), dim3(256), 0, s, cp, wl.cells, wl.prefix, wl.ncell, k, K);
+  hipLaunchKernelGGL(k_lu_gemm<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.cells, wl.prefix, wl.ncell, k, K, Dinv,
+                     next_diag);
 }
 
 // ---------------------------------------------------------------------------
 // triangular solves with the factors, one right-hand side per cell, the
-// whole vector in LDS; one workgroup (4 waves) per cell
+// whole vector in LDS; diagonal blocks through the stored inverses
 // ---------------------------------------------------------------------------
+constexpr int SOLVE_THREADS = 512;
+
+// y[k0 .. k0+63] <- Minv (64 x 64, row-major) * y[k0 .. k0+63]; 8 threads per row
+template <class T>
+__device__ __forceinline__ void diag_matvec(T *y, int k0, const T *Minv, T *tmp) {
+  const int r = threadIdx.x >> 3, q = threadIdx.x & 7;
+  T s = T(0);
+  if (r < 64) {
+    const T *row = Minv + r * 64 + q * 8;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) s += row[c] * y[k0 + q * 8 + c];
+  }
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  s += __shfl_xor(s, 4, 64);
+  if (r < 64 && q == 0) tmp[r] = s;
+  __syncthreads();
+  if (threadIdx.x < 64) y[k0 + threadIdx.x] = tmp[threadIdx.x];
+  __syncthreads();
+}
+
+// A[rr, k0 .. k0+63] . y[k0 .. k0+63] for one row
+template <class T>
+__device__ __forceinline__ T row_dot64(const T *row, const T *yk) {
+  T sacc = T(0);
+  if (sizeof(T) == 4) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const float4 l4 = reinterpret_cast<const float4 *>(row)[q];
+      sacc += (T)l4.x * yk[4 * q] + (T)l4.y * yk[4 * q + 1] + (T)l4.z * yk[4 * q + 2] + (T)l4.w * yk[4 * q + 3];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+      const double2 l2 = reinterpret_cast<const double2 *>(row)[q];
+      sacc += (T)l2.x * yk[2 * q] + (T)l2.y * yk[2 * q + 1];
+    }
+  }
+  return sacc;
+}
+
 template <class T, bool PIVOT>
-__global__ __launch_bounds__(256) void k_lu_solve(const CellParams *cp, const CellState *cs, const int *cells,
-                                                  const T *K, const int *ipiv, const double *rhs, double *delta,
-                                                  int accumulate) {
+__global__ __launch_bounds__(SOLVE_THREADS) void k_lu_solve(const CellParams *cp, const CellState *cs,
+                                                           const int *cells, const T *K, const T *Dinv,
+                                                           const int *ipiv, const double *rhs, double *delta,
+                                                           int accumulate) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ T tmp[64];
   T *y = reinterpret_cast<T *>(smem);
   const int c = cells[blockIdx.x];
   if (cs[c].ir_done) return;
@@ -362,10 +536,8 @@ __global__ __launch_bounds__(256) void k_lu_solve(const CellParams *cp, const Ce
   const T *A = K + p.k_off;
   const long long ld = p.npad;
   const double *r = rhs + (long long)p.vec_off;
-  for (int i = threadIdx.x; i < p.npad; i += 256) y[i] = i < p.ndof ? (T)r[i] : T(0);
+  for (int i = threadIdx.x; i < p.npad; i += SOLVE_THREADS) y[i] = i < p.ndof ? (T)r[i] : T(0);
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const bool w0 = threadIdx.x < 64;
   // forward: P, unit lower L (LINPACK order: block interchanges, then block)
   for (int kb = 0; kb < p.nblk; ++kb) {
     const int k0 = kb * NB;
@@ -375,123 +547,74 @@ __global__ __launch_bounds__(256) void k_lu_solve(const CellParams *cp, const Ce
         for (int j = 0; j < 64; ++j) {
           const int q = pv[k0 + j];
           if (q != k0 + j) {
-            const T tmp = y[k0 + j];
+            const T tt = y[k0 + j];
             y[k0 + j] = y[q];
-            y[q] = tmp;
+            y[q] = tt;
           }
         }
       }
       __syncthreads();
     }
-    if (w0) {
-      T bv = y[k0 + lane];
-      const T *row = A + (long long)(k0 + lane) * ld + k0;
-#pragma unroll 8
-      for (int cc = 0; cc < 63; ++cc) {
-        const T yc = __shfl(bv, cc, 64);
-        if (lane > cc) bv -= row[cc] * yc;
-      }
-      y[k0 + lane] = bv;
-    }
-    __syncthreads();
-    for (int rr = k0 + NB + threadIdx.x; rr < p.npad; rr += 256) {
-      const T *row = A + (long long)rr * ld + k0;
-      T sacc = T(0);
-      if (sizeof(T) == 4) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const float4 l4 = reinterpret_cast<const float4 *>(row)[q];
-          sacc += (T)l4.x * y[k0 + 4 * q] + (T)l4.y * y[k0 + 4 * q + 1] + (T)l4.z * y[k0 + 4 * q + 2] +
-                  (T)l4.w * y[k0 + 4 * q + 3];
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < 32; ++q) {
-          const double2 l2 = reinterpret_cast<const double2 *>(row)[q];
-          sacc += (T)l2.x * y[k0 + 2 * q] + (T)l2.y * y[k0 + 2 * q + 1];
-        }
-      }
-      y[rr] -= sacc;
-    }
+    diag_matvec<T>(y, k0, Dinv + dinv_off(p, kb), tmp);
+    for (int rr = k0 + NB + threadIdx.x; rr < p.npad; rr += SOLVE_THREADS)
+      y[rr] -= row_dot64<T>(A + (long long)rr * ld + k0, y + k0);
     __syncthreads();
   }
   // backward: U with its diagonal
   for (int kb = p.nblk - 1; kb >= 0; --kb) {
     const int k0 = kb * NB;
-    if (w0) {
-      T bv = y[k0 + lane];
-      const T *row = A + (long long)(k0 + lane) * ld + k0;
-      const T dg = row[lane];
-#pragma unroll 8
-      for (int cc = 63; cc >= 0; --cc) {
-        if (lane == cc) bv = bv / dg;
-        const T yc = __shfl(bv, cc, 64);
-        if (lane < cc) bv -= row[cc] * yc;
-      }
-      y[k0 + lane] = bv;
-    }
-    __syncthreads();
-    for (int rr = threadIdx.x; rr < k0; rr += 256) {
-      const T *row = A + (long long)rr * ld + k0;
-      T sacc = T(0);
-      if (sizeof(T) == 4) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const float4 l4 = reinterpret_cast<const float4 *>(row)[q];
-          sacc += (T)l4.x * y[k0 + 4 * q] + (T)l4.y * y[k0 + 4 * q + 1] + (T)l4.z * y[k0 + 4 * q + 2] +
-                  (T)l4.w * y[k0 + 4 * q + 3];
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < 32; ++q) {
-          const double2 l2 = reinterpret_cast<const double2 *>(row)[q];
-          sacc += (T)l2.x * y[k0 + 2 * q] + (T)l2.y * y[k0 + 2 * q + 1];
-        }
-      }
-      y[rr] -= sacc;
-    }
+    diag_matvec<T>(y, k0, Dinv + dinv_off(p, kb) + 4096, tmp);
+    for (int rr = threadIdx.x; rr < k0; rr += SOLVE_THREADS)
+      y[rr] -= row_dot64<T>(A + (long long)rr * ld + k0, y + k0);
     __syncthreads();
   }
   double *dd = delta + (long long)p.vec_off;
-  for (int i = threadIdx.x; i < p.ndof; i += 256) dd[i] = accumulate ? dd[i] + (double)y[i] : (double)y[i];
+  for (int i = threadIdx.x; i < p.ndof; i += SOLVE_THREADS) dd[i] = accumulate ? dd[i] + (double)y[i] : (double)y[i];
 }
 
 size_t lu_solve_lds_bytes(int npad, int elem_size) { return (size_t)npad * elem_size; }
 
-void lu_prepare_attributes(size_t max_lds) {
-  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lu_solve<float, true>),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
-  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lu_solve<float, false>),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
-  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lu_solve<double, true>),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
-  hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lu_solve<double, false>),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+// Dynamic LDS available to k_lu_solve next to its own static LDS.
+size_t lu_solve_max_dynamic(size_t max_lds) { return max_lds > 1024 ? max_lds - 1024 : 0; }
+
+hipError_t lu_prepare_attributes(size_t max_lds) {
+  const int dyn = (int)lu_solve_max_dynamic(max_lds);
+  const void *fns[4] = {reinterpret_cast<const void *>(&k_lu_solve<float, true>),
+                        reinterpret_cast<const void *>(&k_lu_solve<float, false>),
+                        reinterpret_cast<const void *>(&k_lu_solve<double, true>),
+                        reinterpret_cast<const void *>(&k_lu_solve<double, false>)};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template <class T>
 void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
-                     const T *K, const int *ipiv, int pivoting, const double *r, double *delta, int max_npad,
-                     int accumulate) {
+                     const T *K, const T *Dinv, const int *ipiv, int pivoting, const double *r, double *delta,
+                     int max_npad, int accumulate) {
   if (ncell <= 0) return;
   const size_t lds = lu_solve_lds_bytes(max_npad, sizeof(T));
   if (pivoting)
-    hipLaunchKernelGGL((k_lu_solve<T, true>), dim3(ncell), dim3(256), lds, s, cp, cs, cells, K, ipiv, r, delta,
-                       accumulate);
+    hipLaunchKernelGGL((k_lu_solve<T, true>), dim3(ncell), dim3(SOLVE_THREADS), lds, s, cp, cs, cells, K, Dinv, ipiv,
+                       r, delta, accumulate);
   else
-    hipLaunchKernelGGL((k_lu_solve<T, false>), dim3(ncell), dim3(256), lds, s, cp, cs, cells, K, ipiv, r, delta,
-                       accumulate);
+    hipLaunchKernelGGL((k_lu_solve<T, false>), dim3(ncell), dim3(SOLVE_THREADS), lds, s, cp, cs, cells, K, Dinv,
+                       ipiv, r, delta, accumulate);
 }
 
-#define WCFE_INST(T)                                                                                              \
-  template void launch_lu_diag<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *);       \
-  template void launch_lu_panel_pp<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *,    \
-                                      int *);                                                                      \
-  template void launch_lu_laswp<T>(hipStream_t, const CellParams *, const WorkList &, int, T *, const int *);      \
-  template void launch_lu_trsm<T>(hipStream_t, const CellParams *, const WorkList &, int, T *, int);               \
-  template void launch_lu_gemm<T>(hipStream_t, const CellParams *, const WorkList &, int, T *);                    \
-  template void launch_lu_solve<T>(hipStream_t, const CellParams *, const CellState *, const int *, int, const T *, \
-                                   const int *, int, const double *, double *, int, int);
+#define WCFE_INST(T)                                                                                                 \
+  template void launch_lu_diag_inv<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *); \
+  template void launch_lu_panel_pp<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *,       \
+                                      int *, T *);                                                                    \
+  template void launch_lu_laswp<T>(hipStream_t, const CellParams *, const WorkList &, int, T *, const int *);         \
+  template void launch_lu_trsm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *,           \
+                                  const T *);                                                                         \
+  template void launch_lu_gemm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *,      \
+                                  int);                                                                               \
+  template void launch_lu_solve<T>(hipStream_t, const CellParams *, const CellState *, const int *, int, const T *,   \
+                                   const T *, const int *, int, const double *, double *, int, int);
 WCFE_INST(float)
 WCFE_INST(double)
 

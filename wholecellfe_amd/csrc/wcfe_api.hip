@@ -103,7 +103,7 @@ struct LuPlan {
       if (k >= p.nblk) return 0;
       const int m = p.nblk - k - 1;
       switch (kind) {
-        case WL_DIAG: return pivoting ? 0 : 1;
+        case WL_DIAG: return (!pivoting && k == 0) ? 1 : 0;
         case WL_TRSM: return pivoting ? m : 2 * m;
         case WL_GEMM: return m * m;
         case WL_PANEL: return pivoting ? 1 : 0;
@@ -162,6 +162,7 @@ struct Batch {
   int *node_cell = nullptr, *adj_start = nullptr, *adj = nullptr;
   double *Ke64 = nullptr, *G = nullptr, *r = nullptr, *delta = nullptr, *Rea = nullptr, *MM = nullptr;
   void *K = nullptr;
+  void *Dinv = nullptr;
   int *ipiv = nullptr;
   int *cells = nullptr;
   int *replica = nullptr;
@@ -170,7 +171,7 @@ struct Batch {
 
   void release() {
     void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
-                    sp_local, cp, cs, node_cell, adj_start, adj, Ke64, G, r, delta, Rea, MM, K, ipiv, cells,
+                    sp_local, cp, cs, node_cell, adj_start, adj, Ke64, G, r, delta, Rea, MM, K, Dinv, ipiv, cells,
                     replica, transitions, plan.d_prefix};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -462,8 +463,9 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   b.ipiv = dalloc<int>(std::max(sys, 1));
   HIPCHECK(hipMemset(b.ipiv, 0, sizeof(int) * std::max(sys, 1)));
   b.K = (void *)dalloc<unsigned char>((size_t)kp * elem_size);
+  b.Dinv = (void *)dalloc<unsigned char>((size_t)std::max(sys, 1) * 128 * elem_size);
   b.plan.build(b.cp_h, ctx->opt.pivoting);
-  if (lu_solve_lds_bytes(b.max_npad, elem_size) > ctx->max_lds)
+  if (lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
          ctx->max_lds);
 }
@@ -524,26 +526,27 @@ template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc)
     PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
     launch_diag_tau<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K);
   }
+  T *Dinv = static_cast<T *>(b.Dinv);
   for (int k = 0; k < nsteps; ++k) {
     bool any;
     {
       PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
       if (piv) {
         WorkList wp = worklist(b, sc, k, WL_PANEL, &any);
-        if (any) launch_lu_panel_pp<T>(s, b.cp, b.cs, wp, k, K, b.ipiv);
+        if (any) launch_lu_panel_pp<T>(s, b.cp, b.cs, wp, k, K, b.ipiv, Dinv);
         WorkList ws = worklist(b, sc, k, WL_SWP, &any);
         if (any) launch_lu_laswp<T>(s, b.cp, ws, k, K, b.ipiv);
       } else {
         WorkList wd = worklist(b, sc, k, WL_DIAG, &any);
-        if (any) launch_lu_diag<T>(s, b.cp, b.cs, wd, k, K);
+        if (any) launch_lu_diag_inv<T>(s, b.cp, b.cs, wd, k, K, Dinv);
       }
       WorkList wt = worklist(b, sc, k, WL_TRSM, &any);
-      if (any) launch_lu_trsm<T>(s, b.cp, wt, k, K, piv ? 0 : 1);
+      if (any) launch_lu_trsm<T>(s, b.cp, b.cs, wt, k, K, Dinv);
     }
     WorkList wg = worklist(b, sc, k, WL_GEMM, &any);
     if (any) {
       PhaseScope ph(ctx, WCFE_PH_LU_GEMM, wg.total);
-      launch_lu_gemm<T>(s, b.cp, wg, k, K);
+      launch_lu_gemm<T>(s, b.cp, b.cs, wg, k, K, Dinv, piv ? 0 : 1);
     }
   }
 }
@@ -565,7 +568,8 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
   for (int it = 0; it < std::max(1, ctx->opt.max_refine); ++it) {
     {
       PhaseScope ph(ctx, WCFE_PH_SOLVE);
-      launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, b.ipiv, piv, b.r, b.delta, npad, it > 0);
+      launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.ipiv, piv, b.r,
+                         b.delta, npad, it > 0);
     }
     PhaseScope ph(ctx, WCFE_PH_REFINE);
     launch_matvec_ir(s, v, b.node_cell, b.adj_start, b.adj, b.Ke64, b.G, b.delta, b.r, sc.node_begin, sc.node_end,
@@ -726,7 +730,7 @@ int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
     int lds = 0;
     HIPCHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->opt.device));
     ctx->max_lds = (size_t)lds;
-    lu_prepare_attributes(ctx->max_lds);
+    HIPCHECK(lu_prepare_attributes(ctx->max_lds));
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   });
   if (rc != WCFE_OK) {
@@ -1157,7 +1161,7 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
     sys += p.npad;
     max_npad = std::max(max_npad, p.npad);
   }
-  if (lu_solve_lds_bytes(max_npad, sizeof(T)) > ctx->max_lds)
+  if (lu_solve_lds_bytes(max_npad, sizeof(T)) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve", max_npad);
   std::vector<T> Kh((size_t)kp, T(0));
   for (int i = 0; i < n_sys; ++i) {
@@ -1182,6 +1186,7 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
   CellState *dcs = sc.up(cs.data(), cs.size());
   int *dcells = sc.up(ident.data(), ident.size());
   int *dipiv = sc.zeros<int>((size_t)std::max(sys, 1));
+  T *dDinv = sc.zeros<T>((size_t)std::max(sys, 1) * 128);
   LuPlan plan;
   plan.build(cp, ctx->opt.pivoting);
   sc.ptrs.push_back(plan.d_prefix);
@@ -1191,20 +1196,20 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
     const StepWL &w = plan.batched[k];
     auto wl = [&](int kind) { return WorkList{dcells, plan.d_prefix + w.off[kind], n_sys, w.tot[kind]}; };
     if (piv) {
-      launch_lu_panel_pp<T>(s, dcp, dcs, wl(WL_PANEL), k, dK, dipiv);
+      launch_lu_panel_pp<T>(s, dcp, dcs, wl(WL_PANEL), k, dK, dipiv, dDinv);
       launch_lu_laswp<T>(s, dcp, wl(WL_SWP), k, dK, dipiv);
     } else {
-      launch_lu_diag<T>(s, dcp, dcs, wl(WL_DIAG), k, dK);
+      launch_lu_diag_inv<T>(s, dcp, dcs, wl(WL_DIAG), k, dK, dDinv);
     }
-    launch_lu_trsm<T>(s, dcp, wl(WL_TRSM), k, dK, piv ? 0 : 1);
-    launch_lu_gemm<T>(s, dcp, wl(WL_GEMM), k, dK);
+    launch_lu_trsm<T>(s, dcp, dcs, wl(WL_TRSM), k, dK, dDinv);
+    launch_lu_gemm<T>(s, dcp, dcs, wl(WL_GEMM), k, dK, dDinv, piv ? 0 : 1);
   }
   // refinement: r = b; repeat { d = LU^-1 r; x += d; r = b - A x (FP64) }
   launch_cell_norm(s, dcp, dcs, dcells, n_sys, db, 0, 0);
   launch_ir_control(s, dcs, dcells, n_sys, ctx->opt.lin_rtol, 1);
   HIPCHECK(hipMemcpyAsync(dr, db, sizeof(double) * vo, hipMemcpyDeviceToDevice, s));
   for (int it = 0; it < std::max(1, ctx->opt.max_refine); ++it) {
-    launch_lu_solve<T>(s, dcp, dcs, dcells, n_sys, dK, dipiv, piv, dr, dx, max_npad, it > 0);
+    launch_lu_solve<T>(s, dcp, dcs, dcells, n_sys, dK, dDinv, dipiv, piv, dr, dx, max_npad, it > 0);
     launch_dense_residual(s, dcp, dcs, dcells, n_sys, dA, dAoff, db, dx, dr);
     launch_cell_norm(s, dcp, dcs, dcells, n_sys, dr, 1, 1);
     launch_ir_control(s, dcs, dcells, n_sys, ctx->opt.lin_rtol, 0);

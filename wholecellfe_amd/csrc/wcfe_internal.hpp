@@ -46,23 +46,28 @@ template <class T>
 void launch_diag_tau(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell, const T *K);
 
 // ---- dense LU and triangular solves (k_lu.hip) ----
+// Dinv holds, per cell and diagonal block, the inverses of the block's unit
+// lower and upper triangles (2 x 64 x 64 per block, npad * 128 per cell).
 template <class T>
-void launch_lu_diag(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K);
+void launch_lu_diag_inv(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv);
 template <class T>
 void launch_lu_panel_pp(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K,
-                        int *ipiv);
+                        int *ipiv, T *Dinv);
 template <class T>
 void launch_lu_laswp(hipStream_t s, const CellParams *cp, const WorkList &wl, int k, T *K, const int *ipiv);
 template <class T>
-void launch_lu_trsm(hipStream_t s, const CellParams *cp, const WorkList &wl, int k, T *K, int with_lower);
+void launch_lu_trsm(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K,
+                    const T *Dinv);
 template <class T>
-void launch_lu_gemm(hipStream_t s, const CellParams *cp, const WorkList &wl, int k, T *K);
+void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
+                    int next_diag);
 template <class T>
 void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
-                     const T *K, const int *ipiv, int pivoting, const double *r, double *delta, int max_npad,
-                     int accumulate);
+                     const T *K, const T *Dinv, const int *ipiv, int pivoting, const double *r, double *delta,
+                     int max_npad, int accumulate);
 size_t lu_solve_lds_bytes(int npad, int elem_size);
-void lu_prepare_attributes(size_t max_lds);
+size_t lu_solve_max_dynamic(size_t max_lds);
+hipError_t lu_prepare_attributes(size_t max_lds);
 
 // ---- reference-shaped stateless kernels (k_reference.hip) ----
 void launch_ref_force_fa(hipStream_t s, int n_nodes, const int *ring_start, const int *ring, const int *tris,
