@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pivoting", choices=["static", "partial"], default="static")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only time a plain device copy of MIB MiB (HBM probe / counter reference) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (independent cells would be "
@@ -89,6 +91,13 @@ def main():
 
     import wholecellfe_amd as W
     from wholecellfe_amd import configs
+
+    if a.copy_probe:
+        with W.Context() as c:
+            nbytes = a.copy_probe << 20
+            ms = c.copy_probe(nbytes, reps=20)
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms, "hbm_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return
 
     log = lambda *m: print("[bench]", *m, file=sys.stderr, flush=True)
     t_setup = time.perf_counter()

@@ -139,6 +139,11 @@ int wcfe_newton_step(wcfe_ctx *ctx, wcfe_step_stats *stats);
 int wcfe_load_step(wcfe_ctx *ctx, int32_t *iterations, wcfe_step_stats *stats);
 int wcfe_synchronize(wcfe_ctx *ctx);
 
+/* Diagnostics: time `reps` plain device copies of `bytes` bytes (one read and
+ * one write of every byte, 16 bytes per thread) on the context's stream; the
+ * HBM bandwidth probe and the known-size reference for byte counters. */
+int wcfe_copy_probe(wcfe_ctx *ctx, int64_t bytes, int32_t reps, double *ms_per_copy);
+
 /* Phases of a Newton step, for wcfe_profile_step. */
 #define WCFE_PH_FILL 0     /* zero the dense systems and residuals */
 #define WCFE_PH_ASSEMBLE 1 /* colour-ordered element assembly of K and G (work = elements) */

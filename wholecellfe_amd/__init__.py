@@ -31,6 +31,7 @@ EXPORTS = [
     "wcfe_get_positions", "wcfe_set_positions", "wcfe_get_ul_state", "wcfe_get_springs", "wcfe_get_cell_info",
     "wcfe_residual_fp64", "wcfe_element_systems", "wcfe_assembled_system", "wcfe_get_colors", "wcfe_dense_solve",
     "wcfe_calculate_res", "wcfe_force_fa", "wcfe_update_fa", "wcfe_profile_step",
+    "wcfe_copy_probe",
 ]
 PHASES = ["fill", "assemble", "lu_gemm", "lu_panel", "solve", "refine", "residual", "commit"]
 
@@ -222,6 +223,12 @@ class Context:
         work = np.zeros(n)
         self._check(self._L.wcfe_profile_step(self._h, _dp(ms), _ip(launches), _dp(work)))
         return {p: dict(ms=float(ms[i]), launches=int(launches[i]), work=float(work[i])) for i, p in enumerate(PHASES)}
+
+    def copy_probe(self, nbytes, reps=10):
+        """Milliseconds per plain device copy of nbytes (read + write)."""
+        ms = C.c_double(0)
+        self._check(self._L.wcfe_copy_probe(self._h, C.c_int64(nbytes), reps, C.byref(ms)))
+        return ms.value
 
     def synchronize(self):
         self._check(self._L.wcfe_synchronize(self._h))

@@ -464,4 +464,17 @@ void launch_diag_tau(hipStream_t s, const CellParams *cp, CellState *cs, const i
 template void launch_diag_tau<float>(hipStream_t, const CellParams *, CellState *, const int *, int, const float *);
 template void launch_diag_tau<double>(hipStream_t, const CellParams *, CellState *, const int *, int, const double *);
 
+// plain device copy, 16 bytes per thread per iteration (bandwidth probe and
+// the known-size reference for the FETCH_SIZE / WRITE_SIZE counters)
+__global__ __launch_bounds__(256) void k_copy(const float4 *src, float4 *dst, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+void launch_copy(hipStream_t s, const void *src, void *dst, long long bytes) {
+  const long long n = bytes / 16;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, static_cast<const float4 *>(src), static_cast<float4 *>(dst),
+                     n);
+}
+
 }  // namespace wcfe

@@ -8,12 +8,17 @@
 // sweeps into small matrix-vector products.
 //
 // Per block step k, over every cell of the work list at once:
-//   static pivoting : [k = 0: k_lu_diag_inv]
+//   static pivoting, two block steps k, k+1 at a time (delayed update):
+//                     [k = 0: k_lu_diag_inv]
 //                     k_lu_trsm_mm   U_kj = L_kk^-1 A_kj, L_ik = A_ik U_kk^-1
-//                     k_lu_gemm      A_ij -= L_ik U_kj (i, j > k); the
-//                                    workgroup that finishes block (k+1, k+1)
-//                                    goes on to factor it and invert its
-//                                    triangles for step k+1.
+//                     k_lu_gemm      strip: block column and row k+1 updated
+//                                    with panel k only
+//                     k_lu_trsm_mm   panel k+1
+//                     k_lu_gemm      A_ij -= L_ik U_kj + L_i,k+1 U_k+1,j on
+//                                    i, j >= k+2 (each C block read and
+//                                    written once per two steps)
+//                     The workgroup that finishes a diagonal block goes on
+//                     to factor it and invert its triangles.
 //                     Pivots below tau (64 eps max|K_ii|) are replaced by
 //                     +-tau (static pivoting, Li & Demmel); FP64 iterative
 //                     refinement removes the perturbation.
@@ -35,13 +40,13 @@ namespace wcfe {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ int find_item_cell(const int *prefix, int ncell, int item) {
-  int lo = 0, hi = ncell;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (prefix[mid] <= item) lo = mid; else hi = mid;
-  }
-  return lo;
+// Work items: map[item] = (cell << 20) | index of the item within the cell.
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Remap the workgroup id so that every XCD receives one contiguous
+// range of work items: the tiles of one cell and block row then share an L2.
+__device__ __forceinline__ int xcd_item(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 // Dinv layout: per cell, per diagonal block k: L_kk^-1 (64 x 64 row-major,
@@ -164,13 +169,12 @@ __device__ __forceinline__ void invert_triangles(const T *a, T *Li, T *Ui) {
 
 // k = 0 of static pivoting: factor block (0, 0) and invert its triangles.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_diag_inv(const CellParams *cp, CellState *cs, const int *cells,
-                                                     const int *prefix, int ncell, int k, T *K, T *Dinv) {
+__global__ __launch_bounds__(256) void k_lu_diag_inv(const CellParams *cp, CellState *cs, const int *map, int k, T *K, T *Dinv) {
   constexpr int P = 65;
   __shared__ T a[64 * P];
   __shared__ T prow[128];
-  const int ci = find_item_cell(prefix, ncell, blockIdx.x);
-  const int c = cells[ci];
+  const int v = map[blockIdx.x];
+  const int c = v >> 20;
   const CellParams p = cp[c];
   const long long ld = p.npad;
   T *blk = K + p.k_off + (long long)(k * NB) * ld + k * NB;
@@ -191,7 +195,7 @@ template <class T>
 void launch_lu_diag_inv(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K,
                         T *Dinv) {
   if (wl.total <= 0) return;
-  hipLaunchKernelGGL(k_lu_diag_inv<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.cells, wl.prefix, wl.ncell, k, K,
+  hipLaunchKernelGGL(k_lu_diag_inv<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K,
                      Dinv);
 }
 
@@ -200,16 +204,15 @@ void launch_lu_diag_inv(hipStream_t s, const CellParams *cp, CellState *cs, cons
 // inverses of the diagonal triangles
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(512) void k_lu_panel_pp(const CellParams *cp, CellState *cs, const int *cells,
-                                                     const int *prefix, int ncell, int k, T *K, int *ipiv, T *Dinv) {
+__global__ __launch_bounds__(512) void k_lu_panel_pp(const CellParams *cp, CellState *cs, const int *map, int k, T *K, int *ipiv, T *Dinv) {
   constexpr int P = 65;
   __shared__ T prow[64];
   __shared__ double rv[16];
   __shared__ int ri[16];
   __shared__ int s_piv;
   __shared__ T a[64 * P];
-  const int ci = find_item_cell(prefix, ncell, blockIdx.x);
-  const int c = cells[ci];
+  const int v = map[blockIdx.x];
+  const int c = v >> 20;
   const CellParams p = cp[c];
   const long long ld = p.npad;
   T *A = K + p.k_off;
@@ -276,18 +279,18 @@ template <class T>
 void launch_lu_panel_pp(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K,
                         int *ipiv, T *Dinv) {
   if (wl.total <= 0) return;
-  hipLaunchKernelGGL(k_lu_panel_pp<T>, dim3(wl.total), dim3(512), 0, s, cp, cs, wl.cells, wl.prefix, wl.ncell, k, K,
+  hipLaunchKernelGGL(k_lu_panel_pp<T>, dim3(wl.total), dim3(512), 0, s, cp, cs, wl.map, k, K,
                      ipiv, Dinv);
 }
 
 // row interchanges of block k applied to the trailing columns [k0+64, npad)
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_laswp(const CellParams *cp, const int *cells, const int *prefix, int ncell,
+__global__ __launch_bounds__(256) void k_lu_laswp(const CellParams *cp, const int *map,
                                                   int k, T *K, const int *ipiv) {
-  const int ci = find_item_cell(prefix, ncell, blockIdx.x);
-  const int c = cells[ci];
+  const int v = map[blockIdx.x];
+  const int c = v >> 20;
   const CellParams p = cp[c];
-  const int chunk = blockIdx.x - prefix[ci];
+  const int chunk = v & 0xFFFFF;
   const int k0 = k * NB;
   const int col = k0 + NB + chunk * 256 + threadIdx.x;
   if (col >= p.npad) return;
@@ -306,7 +309,7 @@ __global__ __launch_bounds__(256) void k_lu_laswp(const CellParams *cp, const in
 template <class T>
 void launch_lu_laswp(hipStream_t s, const CellParams *cp, const WorkList &wl, int k, T *K, const int *ipiv) {
   if (wl.total <= 0) return;
-  hipLaunchKernelGGL(k_lu_laswp<T>, dim3(wl.total), dim3(256), 0, s, cp, wl.cells, wl.prefix, wl.ncell, k, K, ipiv);
+  hipLaunchKernelGGL(k_lu_laswp<T>, dim3(wl.total), dim3(256), 0, s, cp, wl.map, k, K, ipiv);
 }
 
 // ---------------------------------------------------------------------------
@@ -352,16 +355,17 @@ __device__ __forceinline__ void mma_64(const T *xs, const T *ys, typename Acc4<T
 //   items [0, m):  U_kj = L_kk^-1 A_kj
 //   items [m, 2m): L_ik = A_ik U_kk^-1   (static pivoting only)
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_trsm_mm(const CellParams *cp, const int *cells, const int *prefix,
-                                                    int ncell, int k, T *K, const T *Dinv) {
+__global__ __launch_bounds__(256) void k_lu_trsm_mm(const CellParams *cp, const int *map, int k, T *K,
+                                                    const T *Dinv) {
   constexpr int P = Pitch<T>::v;
   typedef typename Acc4<T>::type acc_t;
   __shared__ __attribute__((aligned(16))) T xs[64 * P];
   __shared__ __attribute__((aligned(16))) T ys[64 * P];
-  const int ci = find_item_cell(prefix, ncell, blockIdx.x);
-  const int c = cells[ci];
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int v = map[item];
+  const int c = v >> 20;
   const CellParams p = cp[c];
-  const int t = blockIdx.x - prefix[ci];
+  const int t = v & 0xFFFFF;
   const int m = p.nblk - k - 1;
   const bool upper = t < m;
   const int other = k + 1 + (upper ? t : t - m);
@@ -399,30 +403,44 @@ template <class T>
 void launch_lu_trsm(hipStream_t s, const CellParams *cp, CellState *, const WorkList &wl, int k, T *K,
                     const T *Dinv) {
   if (wl.total <= 0) return;
-  hipLaunchKernelGGL(k_lu_trsm_mm<T>, dim3(wl.total), dim3(256), 0, s, cp, wl.cells, wl.prefix, wl.ncell, k, K, Dinv);
+  hipLaunchKernelGGL(k_lu_trsm_mm<T>, dim3(wl.total), dim3(256), 0, s, cp, wl.map, k, K, Dinv);
 }
 
-// trailing update C = A_ij - L_ik U_kj; with `next_diag` (static pivoting)
-// the workgroup of block (k+1, k+1) then factors that block and stores the
-// inverses of its triangles for step k+1
-template <class T>
-__global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState *cs, const int *cells,
-                                                 const int *prefix, int ncell, int k, T *K, T *Dinv, int next_diag) {
+// Trailing update C = A_ij - sum_p L_{i,k+p} U_{k+p,j} over NP consecutive
+// panels k .. k+NP-1 (C is read and written once for NP panels).
+//   square mode: (i, j) over [s0, nblk)^2 with s0 = k + NP
+//   strip mode (NP = 1): block column k+1 from row k+1 down, then block row
+//                        k+1 from column k+2 on (the blocks panel k+1 needs)
+// With `next_diag` (static pivoting) the workgroup of the first diagonal
+// block of the updated region (s0, s0) / (k+1, k+1) then factors that block
+// and stores the inverses of its triangles.
+template <class T, int NP>
+__global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState *cs, const int *map, int k, T *K, T *Dinv, int next_diag,
+                                                 int strip) {
   constexpr int P = Pitch<T>::v;
   typedef typename Acc4<T>::type acc_t;
   __shared__ __attribute__((aligned(16))) T ls[64 * P];
   __shared__ __attribute__((aligned(16))) T us[64 * P];
   __shared__ T prow[128];
-  const int ci = find_item_cell(prefix, ncell, blockIdx.x);
-  const int c = cells[ci];
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int v = map[item];
+  const int c = v >> 20;
   const CellParams p = cp[c];
-  const int t = blockIdx.x - prefix[ci];
-  const int m = p.nblk - k - 1;
-  const int bi = k + 1 + t / m, bj = k + 1 + t % m;
+  const int t = v & 0xFFFFF;
+  int bi, bj, sd;
+  if (strip) {
+    const int m = p.nblk - k - 1;
+    if (t < m) { bi = k + 1 + t; bj = k + 1; }
+    else { bi = k + 1; bj = k + 2 + (t - m); }
+    sd = k + 1;
+  } else {
+    const int s0 = k + NP, mt = p.nblk - s0;
+    bi = s0 + t / mt;
+    bj = s0 + t % mt;
+    sd = s0;
+  }
   T *A = K + p.k_off;
   const long long ld = p.npad;
-  load_block<T, P>(ls, A + (long long)(bi * NB) * ld + k * NB, ld);
-  load_block<T, P>(us, A + (long long)(k * NB) * ld + bj * NB, ld);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
@@ -435,9 +453,15 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         acc[ti][tj][r] = C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr];
-  __syncthreads();
-  mma_64<T, P, P>(ls, us, acc, T(-1));
-  const bool diag_next = next_diag && bi == k + 1 && bj == k + 1;
+#pragma unroll
+  for (int pn = 0; pn < NP; ++pn) {
+    if (pn) __syncthreads();
+    load_block<T, P>(ls, A + (long long)(bi * NB) * ld + (k + pn) * NB, ld);
+    load_block<T, P>(us, A + (long long)((k + pn) * NB) * ld + bj * NB, ld);
+    __syncthreads();
+    mma_64<T, P, P>(ls, us, acc, T(-1));
+  }
+  const bool diag_next = next_diag && bi == sd && bj == sd;
   if (!diag_next) {
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
@@ -448,7 +472,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
           C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
     return;
   }
-  // block (k+1, k+1) is final for step k+1: factor it here
+  // diagonal block sd is final: factor it here
   __syncthreads();
   constexpr int PA = 65;
   T *a = ls;  // reuse: 64 x 65 fits in 64 x P
@@ -462,7 +486,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
   const int fixed = factor_diag<T, PA>(a, prow, (T)cs[c].tau);
   for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
     C[(idx >> 6) * ld + (idx & 63)] = a[(idx >> 6) * PA + (idx & 63)];
-  T *D = Dinv + dinv_off(p, k + 1);
+  T *D = Dinv + dinv_off(p, sd);
   invert_triangles<T, PA>(a, D, D + 4096);
   if (threadIdx.x < 64) {
     const int cnt = __popcll(__ballot(fixed != 0));
@@ -471,10 +495,13 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
 }
 template <class T>
 void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
-                    int next_diag) {
+                    int next_diag, int npanel, int strip) {
   if (wl.total <= 0) return;
-  hipLaunchKernelGGL(k_lu_gemm<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.cells, wl.prefix, wl.ncell, k, K, Dinv,
-                     next_diag);
+  if (npanel == 2)
+    hipLaunchKernelGGL((k_lu_gemm<T, 2>), dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag, 0);
+  else
+    hipLaunchKernelGGL((k_lu_gemm<T, 1>), dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag,
+                       strip);
 }
 
 // ---------------------------------------------------------------------------
@@ -612,7 +639,7 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
   template void launch_lu_trsm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *,           \
                                   const T *);                                                                         \
   template void launch_lu_gemm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *,      \
-                                  int);                                                                               \
+                                  int, int, int);                                                                     \
   template void launch_lu_solve<T>(hipStream_t, const CellParams *, const CellState *, const int *, int, const T *,   \
                                    const T *, const int *, int, const double *, double *, int, int);
 WCFE_INST(float)

@@ -82,60 +82,93 @@ int color_elements(int n_nodes, int n_tris, const int *tris, std::vector<int> &c
   return nc;
 }
 
-// LU work lists: for every block step, the exclusive prefix sums of work items
-// over the cells of a list (all cells batched, or one cell at a time).
-struct StepWL {
-  int off[5];
-  int tot[5];
+// LU schedule: a list of stages (kernel kind, block step k), each with the
+// exclusive prefix sums of its work items over the cells of a list (all cells
+// batched, or one cell at a time).
+enum StageKind { ST_DIAG, ST_TRSM, ST_STRIP, ST_GEMM1, ST_GEMM2, ST_PANEL, ST_SWP };
+struct Stage {
+  int kind, k, off, tot;
+  long long work;  // 64 x 64 x 64 block products (GEMM stages)
 };
-enum { WL_DIAG = 0, WL_TRSM = 1, WL_GEMM = 2, WL_PANEL = 3, WL_SWP = 4 };
 
 struct LuPlan {
   int max_nblk = 0;
+  int pivoting = 0;
   int *d_prefix = nullptr;
-  std::vector<StepWL> batched;
-  std::vector<std::vector<StepWL>> single;
+  std::vector<Stage> batched;
+  std::vector<std::vector<Stage>> single;
 
-  void build(const std::vector<CellParams> &cp, int pivoting) {
+  static long long block_products(const CellParams &p, int kind, int k) {
+    const int nb = p.nblk, m = nb - k - 1;
+    if (k >= nb) return 0;
+    if (kind == ST_STRIP) return m >= 1 ? 2 * m - 1 : 0;
+    if (kind == ST_GEMM1) return (long long)m * m;
+    if (kind == ST_GEMM2) return m >= 2 ? 2LL * (m - 1) * (m - 1) : 0;
+    return 0;
+  }
+  static int items(const CellParams &p, int kind, int k, int pivoting) {
+    const int nb = p.nblk;
+    const int m = nb - k - 1;
+    switch (kind) {
+      case ST_DIAG: return (k == 0 && nb > 0) ? 1 : 0;
+      case ST_TRSM: return k < nb ? (pivoting ? m : 2 * m) : 0;
+      case ST_STRIP: return (k < nb && m >= 1) ? 2 * m - 1 : 0;
+      case ST_GEMM1: return k < nb ? m * m : 0;
+      case ST_GEMM2: return (nb - k - 2 >= 1) ? (nb - k - 2) * (nb - k - 2) : 0;
+      case ST_PANEL: return k < nb ? 1 : 0;
+      default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
+    }
+  }
+  // static pivoting: two block steps per round (delayed update); partial
+  // pivoting: one block step per round
+  static std::vector<std::pair<int, int>> schedule(int nblk, int pivoting) {
+    std::vector<std::pair<int, int>> s;
+    if (!pivoting) {
+      s.push_back({ST_DIAG, 0});
+      for (int k = 0; k < nblk; k += 2) {
+        s.push_back({ST_TRSM, k});
+        s.push_back({ST_STRIP, k});
+        s.push_back({ST_TRSM, k + 1});
+        s.push_back({ST_GEMM2, k});
+      }
+    } else {
+      for (int k = 0; k < nblk; ++k) {
+        s.push_back({ST_PANEL, k});
+        s.push_back({ST_SWP, k});
+        s.push_back({ST_TRSM, k});
+        s.push_back({ST_GEMM1, k});
+      }
+    }
+    return s;
+  }
+
+  void build(const std::vector<CellParams> &cp, int piv) {
+    pivoting = piv;
     const int n = (int)cp.size();
     std::vector<int> buf;
-    auto items = [&](const CellParams &p, int k, int kind) -> int {
-      if (k >= p.nblk) return 0;
-      const int m = p.nblk - k - 1;
-      switch (kind) {
-        case WL_DIAG: return (!pivoting && k == 0) ? 1 : 0;
-        case WL_TRSM: return pivoting ? m : 2 * m;
-        case WL_GEMM: return m * m;
-        case WL_PANEL: return pivoting ? 1 : 0;
-        default: return (pivoting && m > 0) ? (m * NB + 255) / 256 : 0;
-      }
-    };
     max_nblk = 0;
     for (auto &p : cp) max_nblk = std::max(max_nblk, p.nblk);
-    batched.assign(max_nblk, StepWL{});
-    for (int k = 0; k < max_nblk; ++k)
-      for (int kind = 0; kind < 5; ++kind) {
-        batched[k].off[kind] = (int)buf.size();
-        int acc = 0;
-        for (int c = 0; c < n; ++c) {
-          buf.push_back(acc);
-          acc += items(cp[c], k, kind);
-        }
-        buf.push_back(acc);
-        batched[k].tot[kind] = acc;
+    if (n >= (1 << 11)) throw ApiError{WCFE_ERR_ARG, "at most 2047 systems per batch"};
+    batched.clear();
+    for (auto st : schedule(max_nblk, pivoting)) {
+      Stage g{st.first, st.second, (int)buf.size(), 0, 0};
+      for (int c = 0; c < n; ++c) {
+        const int t = items(cp[c], st.first, st.second, pivoting);
+        if (t >= (1 << 20)) throw ApiError{WCFE_ERR_ARG, "too many LU work items per cell"};
+        for (int i = 0; i < t; ++i) buf.push_back((c << 20) | i);
+        g.tot += t;
+        g.work += block_products(cp[c], st.first, st.second);
       }
-    single.assign(n, {});
-    for (int c = 0; c < n; ++c) {
-      single[c].assign(cp[c].nblk, StepWL{});
-      for (int k = 0; k < cp[c].nblk; ++k)
-        for (int kind = 0; kind < 5; ++kind) {
-          single[c][k].off[kind] = (int)buf.size();
-          const int t = items(cp[c], k, kind);
-          buf.push_back(0);
-          buf.push_back(t);
-          single[c][k].tot[kind] = t;
-        }
+      if (g.tot > 0) batched.push_back(g);
     }
+    single.assign(n, {});
+    for (int c = 0; c < n; ++c)
+      for (auto st : schedule(cp[c].nblk, pivoting)) {
+        const int t = items(cp[c], st.first, st.second, pivoting);
+        if (t == 0) continue;
+        single[c].push_back(Stage{st.first, st.second, (int)buf.size(), t, block_products(cp[c], st.first, st.second)});
+        for (int i = 0; i < t; ++i) buf.push_back((c << 20) | i);
+      }
     d_prefix = dupload(buf);
   }
 };
@@ -493,19 +526,6 @@ void color_range(const Batch &b, const Scope &sc, int k, int &e0, int &e1) {
   }
 }
 
-WorkList worklist(const Batch &b, const Scope &sc, int k, int kind, bool *any) {
-  const StepWL *w;
-  if (sc.cell < 0) {
-    if (k >= (int)b.plan.batched.size()) { *any = false; return WorkList{}; }
-    w = &b.plan.batched[k];
-  } else {
-    if (k >= (int)b.plan.single[sc.cell].size()) { *any = false; return WorkList{}; }
-    w = &b.plan.single[sc.cell][k];
-  }
-  *any = w->tot[kind] > 0;
-  return WorkList{sc.cells, b.plan.d_prefix + w->off[kind], sc.ncell, w->tot[kind]};
-}
-
 template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &sc, int mode) {
   hipStream_t s = ctx->stream;
   View v = make_view(ctx, b);
@@ -517,38 +537,62 @@ template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &s
   }
 }
 
-template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc) {
+// Launch the LU stages of a plan for a list of cells.
+template <class T>
+void enqueue_lu(wcfe_ctx *ctx, const LuPlan &plan, const std::vector<Stage> &stages, const int *cells, int ncell,
+                const CellParams *cp, CellState *cs, T *K, T *Dinv, int *ipiv) {
   hipStream_t s = ctx->stream;
-  T *K = static_cast<T *>(b.K);
-  const int nsteps = sc.cell < 0 ? b.plan.max_nblk : b.cp_h[sc.cell].nblk;
-  const bool piv = ctx->opt.pivoting == WCFE_PIVOT_PARTIAL;
-  if (!piv) {
+  const bool piv = plan.pivoting != 0;
+  {
     PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
-    launch_diag_tau<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K);
+    if (!piv) launch_diag_tau<T>(s, cp, cs, cells, ncell, K);
   }
-  T *Dinv = static_cast<T *>(b.Dinv);
-  for (int k = 0; k < nsteps; ++k) {
-    bool any;
-    {
-      PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
-      if (piv) {
-        WorkList wp = worklist(b, sc, k, WL_PANEL, &any);
-        if (any) launch_lu_panel_pp<T>(s, b.cp, b.cs, wp, k, K, b.ipiv, Dinv);
-        WorkList ws = worklist(b, sc, k, WL_SWP, &any);
-        if (any) launch_lu_laswp<T>(s, b.cp, ws, k, K, b.ipiv);
-      } else {
-        WorkList wd = worklist(b, sc, k, WL_DIAG, &any);
-        if (any) launch_lu_diag_inv<T>(s, b.cp, b.cs, wd, k, K, Dinv);
+  for (const Stage &g : stages) {
+    const WorkList wl{plan.d_prefix + g.off, g.tot};
+    switch (g.kind) {
+      case ST_DIAG: {
+        PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
+        launch_lu_diag_inv<T>(s, cp, cs, wl, g.k, K, Dinv);
+        break;
       }
-      WorkList wt = worklist(b, sc, k, WL_TRSM, &any);
-      if (any) launch_lu_trsm<T>(s, b.cp, b.cs, wt, k, K, Dinv);
-    }
-    WorkList wg = worklist(b, sc, k, WL_GEMM, &any);
-    if (any) {
-      PhaseScope ph(ctx, WCFE_PH_LU_GEMM, wg.total);
-      launch_lu_gemm<T>(s, b.cp, b.cs, wg, k, K, Dinv, piv ? 0 : 1);
+      case ST_PANEL: {
+        PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
+        launch_lu_panel_pp<T>(s, cp, cs, wl, g.k, K, ipiv, Dinv);
+        break;
+      }
+      case ST_SWP: {
+        PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
+        launch_lu_laswp<T>(s, cp, wl, g.k, K, ipiv);
+        break;
+      }
+      case ST_TRSM: {
+        PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
+        launch_lu_trsm<T>(s, cp, cs, wl, g.k, K, Dinv);
+        break;
+      }
+      case ST_STRIP: {
+        PhaseScope ph(ctx, WCFE_PH_LU_GEMM, g.work);
+        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, 1, 1);
+        break;
+      }
+      case ST_GEMM1: {
+        PhaseScope ph(ctx, WCFE_PH_LU_GEMM, g.work);
+        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, piv ? 0 : 1, 1, 0);
+        break;
+      }
+      case ST_GEMM2: {
+        PhaseScope ph(ctx, WCFE_PH_LU_GEMM, g.work);
+        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, 2, 0);
+        break;
+      }
     }
   }
+}
+
+template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc) {
+  const std::vector<Stage> &stages = sc.cell < 0 ? b.plan.batched : b.plan.single[sc.cell];
+  enqueue_lu<T>(ctx, b.plan, stages, sc.cells, sc.ncell, b.cp, b.cs, static_cast<T *>(b.K),
+                static_cast<T *>(b.Dinv), b.ipiv);
 }
 
 // The refinement residual for a mesh batch: r = -G - K d from the FP64
@@ -946,6 +990,38 @@ int wcfe_profile_step(wcfe_ctx *ctx, double *phase_ms, int32_t *phase_launches, 
   return rc;
 }
 
+int wcfe_copy_probe(wcfe_ctx *ctx, int64_t bytes, int32_t reps, double *ms_per_copy) {
+  if (!ctx || bytes < 16 || reps < 1 || !ms_per_copy) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    bytes &= ~15LL;
+    void *src = dalloc<unsigned char>((size_t)bytes);
+    void *dst = nullptr;
+    try {
+      dst = dalloc<unsigned char>((size_t)bytes);
+    } catch (...) {
+      (void)hipFree(src);
+      throw;
+    }
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    HIPCHECK(hipMemsetAsync(src, 1, (size_t)bytes, ctx->stream));
+    launch_copy(ctx->stream, src, dst, bytes);  // warm up
+    HIPCHECK(hipEventRecord(a, ctx->stream));
+    for (int i = 0; i < reps; ++i) launch_copy(ctx->stream, src, dst, bytes);
+    HIPCHECK(hipEventRecord(b, ctx->stream));
+    HIPCHECK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, a, b));
+    *ms_per_copy = ms / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+  });
+}
+
 int wcfe_synchronize(wcfe_ctx *ctx) {
   if (!ctx) return WCFE_ERR_ARG;
   return guarded(ctx, [&] { HIPCHECK(hipStreamSynchronize(ctx->stream)); });
@@ -1191,19 +1267,7 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
   plan.build(cp, ctx->opt.pivoting);
   sc.ptrs.push_back(plan.d_prefix);
   const bool piv = ctx->opt.pivoting == WCFE_PIVOT_PARTIAL;
-  if (!piv) launch_diag_tau<T>(s, dcp, dcs, dcells, n_sys, dK);
-  for (int k = 0; k < plan.max_nblk; ++k) {
-    const StepWL &w = plan.batched[k];
-    auto wl = [&](int kind) { return WorkList{dcells, plan.d_prefix + w.off[kind], n_sys, w.tot[kind]}; };
-    if (piv) {
-      launch_lu_panel_pp<T>(s, dcp, dcs, wl(WL_PANEL), k, dK, dipiv, dDinv);
-      launch_lu_laswp<T>(s, dcp, wl(WL_SWP), k, dK, dipiv);
-    } else {
-      launch_lu_diag_inv<T>(s, dcp, dcs, wl(WL_DIAG), k, dK, dDinv);
-    }
-    launch_lu_trsm<T>(s, dcp, dcs, wl(WL_TRSM), k, dK, dDinv);
-    launch_lu_gemm<T>(s, dcp, dcs, wl(WL_GEMM), k, dK, dDinv, piv ? 0 : 1);
-  }
+  enqueue_lu<T>(ctx, plan, plan.batched, dcells, n_sys, dcp, dcs, dK, dDinv, dipiv);
   // refinement: r = b; repeat { d = LU^-1 r; x += d; r = b - A x (FP64) }
   launch_cell_norm(s, dcp, dcs, dcells, n_sys, db, 0, 0);
   launch_ir_control(s, dcs, dcells, n_sys, ctx->opt.lin_rtol, 1);

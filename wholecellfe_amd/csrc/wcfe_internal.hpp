@@ -6,12 +6,10 @@
 
 namespace wcfe {
 
-// Work list of one launch over a set of cells: `cells` holds cell ids and
-// `prefix` the exclusive prefix sum (length ncell + 1) of work items per cell.
+// Work list of one launch: map[item] = (cell << 20) | (item index within the
+// cell), for item in [0, total).
 struct WorkList {
-  const int *cells;
-  const int *prefix;
-  int ncell;
+  const int *map;
   int total;
 };
 
@@ -21,6 +19,7 @@ enum AsmMode : int { ASM_FULL = 0, ASM_RESIDUAL = 1, ASM_STRAIN = 2 };
 template <class T>
 void launch_fill_systems(hipStream_t s, const CellParams *cp, const int *cells, int ncell, int max_npad, T *K);
 void launch_fill(hipStream_t s, double *p, long long n, double value);
+void launch_copy(hipStream_t s, const void *src, void *dst, long long bytes);
 template <class T>
 void launch_assemble(hipStream_t s, const View &v, int e_begin, int e_end, int mode, T *K, double *G, double *Ke64,
                      double *Rea, double *MM, int *cs_degenerate_base);
@@ -60,7 +59,7 @@ void launch_lu_trsm(hipStream_t s, const CellParams *cp, CellState *cs, const Wo
                     const T *Dinv);
 template <class T>
 void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
-                    int next_diag);
+                    int next_diag, int npanel, int strip);
 template <class T>
 void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                      const T *K, const T *Dinv, const int *ipiv, int pivoting, const double *r, double *delta,
