@@ -71,6 +71,21 @@ def cpu_baseline(cells, budget_s=20.0):
                       (len(rec), len(cells), max(n for n, _ in rec), spent, coef)}
 
 
+def measured_traffic(n_cells, pivoting):
+    """HBM bytes per launch of the trailing-update kernel, from the rocprofv3
+    FETCH_SIZE and WRITE_SIZE passes summarised in profiles/ (scaled by the
+    same counters' reading for a plain copy of known size); None when no
+    summary exists for this workload."""
+    path = os.path.join(REPO, "profiles", "k_lu_gemm_batched%d_%s_traffic.txt" % (n_cells, pivoting))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,14 +147,17 @@ def main():
     g = prof["lu_gemm"]
     tile_flops = 2.0 * 64 ** 3
     achieved = g["work"] * tile_flops / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
-    roofline = {"kernel": "k_lu_gemm<float> (LU trailing update, v_mfma_f32_16x16x4f32)", "bound": "mfma",
+    launches = max(g["launches"], 1)
+    traffic = measured_traffic(a.cells, a.pivoting)
+    roofline = {"kernel": "k_lu_gemm<float,%d> (LU trailing update, v_mfma_f32_16x16x4f32)" %
+                          (2 if a.pivoting == "static" else 1), "bound": "mfma",
                 "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / MFMA_FP32_PEAK, "traffic": None,
-                "launches_per_step": g["launches"], "tiles_per_step": g["work"],
-                "avg_launch_us": g["ms"] * 1e3 / max(g["launches"], 1),
-                "flops_per_tile": tile_flops}
+                "frac": achieved / MFMA_FP32_PEAK, "traffic": traffic,
+                "launches_per_step": g["launches"], "block_products_per_step": g["work"],
+                "flops_per_block_product": tile_flops, "avg_launch_us": g["ms"] * 1e3 / launches,
+                "algorithmic_bytes_per_launch": g["work"] / launches * 3 * 64 * 64 * 4}
     lu_alg = configs.lu_flops(cells)
-    lu_ms = prof["lu_gemm"]["ms"] + prof["lu_panel"]["ms"]
+    lu_ms = prof["lu_gemm"]["ms"] + prof["lu_strip"]["ms"] + prof["lu_panel"]["ms"]
 
     # ---- plain FP64 GPU form: same kernels in FP64, one cell at a time, no graph ----
     fp64 = None

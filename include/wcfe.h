@@ -147,13 +147,14 @@ int wcfe_copy_probe(wcfe_ctx *ctx, int64_t bytes, int32_t reps, double *ms_per_c
 /* Phases of a Newton step, for wcfe_profile_step. */
 #define WCFE_PH_FILL 0     /* zero the dense systems and residuals */
 #define WCFE_PH_ASSEMBLE 1 /* colour-ordered element assembly of K and G (work = elements) */
-#define WCFE_PH_LU_GEMM 2  /* LU trailing updates on MFMA (work = 64x64 tiles) */
+#define WCFE_PH_LU_GEMM 2  /* LU trailing updates on MFMA (work = 64x64x64 block products) */
 #define WCFE_PH_LU_PANEL 3 /* LU diagonal/panel factorisation, interchanges, block solves */
 #define WCFE_PH_SOLVE 4    /* triangular solves */
 #define WCFE_PH_REFINE 5   /* FP64 refinement residuals, norms, control */
 #define WCFE_PH_RESIDUAL 6 /* update x, FP64 residual at the new x, convergence test */
 #define WCFE_PH_COMMIT 7   /* strain update, bond kinetics, time advance */
-#define WCFE_NPHASES 8
+#define WCFE_PH_LU_STRIP 8 /* LU next-panel strip updates on MFMA (work = block products) */
+#define WCFE_NPHASES 9
 /* Run one Newton step eagerly with HIP events around every phase; per phase:
  * summed kernel milliseconds, launches, and work units (see above).  The
  * state advances exactly as in wcfe_newton_step.  Diagnostics: the extra

@@ -33,7 +33,7 @@ EXPORTS = [
     "wcfe_calculate_res", "wcfe_force_fa", "wcfe_update_fa", "wcfe_profile_step",
     "wcfe_copy_probe",
 ]
-PHASES = ["fill", "assemble", "lu_gemm", "lu_panel", "solve", "refine", "residual", "commit"]
+PHASES = ["fill", "assemble", "lu_gemm", "lu_panel", "solve", "refine", "residual", "commit", "lu_strip"]
 
 
 class Options(C.Structure):

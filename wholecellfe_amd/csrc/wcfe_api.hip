@@ -571,7 +571,7 @@ void enqueue_lu(wcfe_ctx *ctx, const LuPlan &plan, const std::vector<Stage> &sta
         break;
       }
       case ST_STRIP: {
-        PhaseScope ph(ctx, WCFE_PH_LU_GEMM, g.work);
+        PhaseScope ph(ctx, WCFE_PH_LU_STRIP, g.work);
         launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, 1, 1);
         break;
       }
