@@ -70,6 +70,10 @@ typedef struct wcfe_options {
   uint32_t seed_lo, seed_hi; /* Philox key for Bell kinetics */
   double k_on, k_off0, f_bell; /* Bell kinetics: binding rate, unloaded off-rate, force scale */
   double kappa;          /* active-stress isotropy (reference: 0.5, UL:872) */
+  int32_t symmetric;     /* 1: cells whose tangent is symmetric by construction (no node-attached
+                            springs) are factored LDL^T-style with static pivoting: only the lower
+                            triangle is updated and U_kj = D_k L_jk^T (half the flops);
+                            0: always the full LU */
 } wcfe_options;
 
 typedef struct wcfe_cell_desc {

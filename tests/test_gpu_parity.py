@@ -165,6 +165,39 @@ def test_dense_solve_batched_uneven(prec, piv):
     assert it.max() <= 10
 
 
+@pytest.mark.parametrize("prec", [W.PREC_MIXED, W.PREC_FP64])
+def test_dense_solve_symmetric_path(prec):
+    """Exactly symmetric systems (SPD and indefinite) take the lower-triangle
+    factorisation with U = D L^T; both it and the full LU reach 1e-8."""
+    print("case symmetric prec", prec, flush=True)
+    rng = np.random.default_rng(11)
+    mats, rhss = [], []
+    for n in [5, 64, 65, 130, 300, 611]:
+        T = rng.standard_normal((n, n))
+        S = T + T.T  # exactly symmetric
+        if n % 2:
+            A = S / np.sqrt(n) + 3.0 * np.eye(n) * rng.choice([-1, 1], n)  # indefinite
+        else:
+            A = S @ S / n + np.eye(n)
+            A = 0.5 * (A + A.T)
+        assert np.array_equal(A, A.T)
+        mats.append(A)
+        rhss.append(rng.standard_normal(n))
+    out = {}
+    for sym in (1, 0):
+        c = W.Context(precision=prec, pivoting=W.PIVOT_STATIC, max_refine=10, symmetric=sym)
+        xs, rel, it, _ = c.dense_solve(mats, rhss)
+        c.close()
+        for A, b, x, r in zip(mats, rhss, xs, rel):
+            res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+            assert res <= 1e-8, (sym, A.shape, res)
+            assert r <= 1e-8
+            assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-6, atol=1e-9)
+        out[sym] = xs
+    # multi-block systems went through a different factorisation
+    assert any(not np.array_equal(a, b) for a, b in zip(out[0][3:], out[1][3:]))
+
+
 def test_partial_pivots_match_oracle_fp32():
     """Pivot rows of the GPU panel factorisation equal the FP32 oracle LU's on
     a random matrix (well-separated column maxima)."""
@@ -204,6 +237,26 @@ def test_newton_load_step_vs_oracle(prec, table):
     G, _ = O.assemble(O.make_cell(dd), want_K=False)
     G0, _ = O.assemble(O.make_cell(d), want_K=False)
     assert np.linalg.norm(G) <= 1e-8 * np.linalg.norm(G0)
+
+
+def test_symmetric_factorisation_matches_full_lu():
+    """A batch of uneven cells (up to 15 LU blocks) advanced one load step with
+    the symmetric (lower-triangle) factorisation and with the full LU: the
+    same converged positions to well below the Newton tolerance."""
+    ds = [cells.cst_cell(K=k, n_spr=30 + 10 * k, law_table=(k % 2 == 1), seed=k, time=1.0) for k in (5, 9, 13)]
+    outs = []
+    for sym in (1, 0):
+        g = W.Context(precision=W.PREC_MIXED, t0=1.0, newton_rtol=1e-11, newton_atol=1e-11, max_refine=12,
+                      symmetric=sym)
+        for d in ds:
+            cells.add_to_context(g, d)
+        g.finalize()
+        st = g.load_step()
+        assert st["max_lin_relres"] <= 1e-8
+        outs.append([g.positions(i) for i in range(len(ds))])
+        g.close()
+    for d, a, b in zip(ds, outs[0], outs[1]):
+        assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b - d["X"]))
 
 
 def test_ul_md_table_one_newton_iteration_vs_oracle():

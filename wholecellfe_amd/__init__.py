@@ -42,7 +42,7 @@ class Options(C.Structure):
         ("use_graph", C.c_int32), ("max_refine", C.c_int32), ("lin_rtol", C.c_double),
         ("newton_rtol", C.c_double), ("newton_atol", C.c_double), ("max_newton", C.c_int32), ("dt", C.c_double),
         ("t0", C.c_double), ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32), ("k_on", C.c_double),
-        ("k_off0", C.c_double), ("f_bell", C.c_double), ("kappa", C.c_double),
+        ("k_off0", C.c_double), ("f_bell", C.c_double), ("kappa", C.c_double), ("symmetric", C.c_int32),
     ]
 
 

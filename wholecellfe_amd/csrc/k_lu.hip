@@ -354,6 +354,9 @@ __device__ __forceinline__ void mma_64(const T *xs, const T *ys, typename Acc4<T
 // block triangular solves as products with the stored inverses:
 //   items [0, m):  U_kj = L_kk^-1 A_kj
 //   items [m, 2m): L_ik = A_ik U_kk^-1   (static pivoting only)
+// Symmetric cells (p.sym) have only the L items, [0, m): with A = L U and
+// A symmetric, U = D L^T (D = diag U), so each L_ik tile also yields
+// U_ki = D_k L_ik^T, stored by transposition through LDS.
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_trsm_mm(const CellParams *cp, const int *map, int k, T *K,
                                                     const T *Dinv) {
@@ -367,8 +370,8 @@ __global__ __launch_bounds__(256) void k_lu_trsm_mm(const CellParams *cp, const 
   const CellParams p = cp[c];
   const int t = v & 0xFFFFF;
   const int m = p.nblk - k - 1;
-  const bool upper = t < m;
-  const int other = k + 1 + (upper ? t : t - m);
+  const bool upper = !p.sym && t < m;
+  const int other = k + 1 + ((upper || p.sym) ? t : t - m);
   const long long ld = p.npad;
   T *A = K + p.k_off;
   const T *D = Dinv + dinv_off(p, k);
@@ -398,6 +401,20 @@ __global__ __launch_bounds__(256) void k_lu_trsm_mm(const CellParams *cp, const 
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         blk[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
+  if (!p.sym) return;
+  // U_k,other[col][row] = L_other,k[row][col] * U_kk[col][col]
+  __syncthreads();  // every wave is done reading xs
+#pragma unroll
+  for (int tj = 0; tj < 2; ++tj) {
+    const int col = wc + tj * 16 + lr;
+    const T d = A[(long long)(k * NB + col) * ld + k * NB + col];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xs[col * P + wr + ti * 16 + acc_row(T(0), lk, r)] = acc[ti][tj][r] * d;
+  }
+  __syncthreads();
+  store_block<T, P>(A + (long long)(k * NB) * ld + other * NB, xs, ld);
 }
 template <class T>
 void launch_lu_trsm(hipStream_t s, const CellParams *cp, CellState *, const WorkList &wl, int k, T *K,
@@ -429,10 +446,20 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
   const int t = v & 0xFFFFF;
   int bi, bj, sd;
   if (strip) {
+    // symmetric cells: block column only (t < m)
     const int m = p.nblk - k - 1;
     if (t < m) { bi = k + 1 + t; bj = k + 1; }
     else { bi = k + 1; bj = k + 2 + (t - m); }
     sd = k + 1;
+  } else if (p.sym) {
+    // lower triangle (i >= j) of the trailing matrix, row by row
+    const int s0 = k + NP;
+    int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while (i * (i + 1) / 2 > t) --i;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    bi = s0 + i;
+    bj = s0 + (t - i * (i + 1) / 2);
+    sd = s0;
   } else {
     const int s0 = k + NP, mt = p.nblk - s0;
     bi = s0 + t / mt;
@@ -529,24 +556,55 @@ __device__ __forceinline__ void diag_matvec(T *y, int k0, const T *Minv, T *tmp)
   __syncthreads();
 }
 
-// A[rr, k0 .. k0+63] . y[k0 .. k0+63] for one row
+// y[rr] -= A[rr, k0 .. k0+63] . y[k0 .. k0+63] for rr in [r0, r1): a group of
+// G lanes reads one row segment (G x 16 bytes, one coalesced 256 / 512-byte
+// run), the partial dots are reduced across the group with shuffles.  Four
+// rows per group are loaded before any arithmetic so the loads overlap.
 template <class T>
-__device__ __forceinline__ T row_dot64(const T *row, const T *yk) {
-  T sacc = T(0);
-  if (sizeof(T) == 4) {
+__device__ __forceinline__ void block_column_update(T *y, const T *A, long long ld, int k0, int r0, int r1) {
+  constexpr int V = 16 / sizeof(T);   // elements per 16-byte load
+  constexpr int G = 64 / V;           // lanes per row segment
+  constexpr int ROWS = SOLVE_THREADS / G;
+  const int q = threadIdx.x % G, g = threadIdx.x / G;
+  T yk[V];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const float4 l4 = reinterpret_cast<const float4 *>(row)[q];
-      sacc += (T)l4.x * yk[4 * q] + (T)l4.y * yk[4 * q + 1] + (T)l4.z * yk[4 * q + 2] + (T)l4.w * yk[4 * q + 3];
-    }
-  } else {
+  for (int e = 0; e < V; ++e) yk[e] = y[k0 + q * V + e];
+  for (int base = r0; base < r1; base += 4 * ROWS) {
+    T a[4][V];
 #pragma unroll
-    for (int q = 0; q < 32; ++q) {
-      const double2 l2 = reinterpret_cast<const double2 *>(row)[q];
-      sacc += (T)l2.x * yk[2 * q] + (T)l2.y * yk[2 * q + 1];
+    for (int u = 0; u < 4; ++u) {
+      const int rr = base + u * ROWS + g;
+      if (rr < r1) {
+        const T *src = A + (long long)rr * ld + k0 + q * V;
+        if constexpr (V == 4) {
+          const float4 v = *reinterpret_cast<const float4 *>(src);
+          a[u][0] = v.x; a[u][1] = v.y; a[u][2] = v.z; a[u][3] = v.w;
+        } else {
+          const double2 v = *reinterpret_cast<const double2 *>(src);
+          a[u][0] = (T)v.x; a[u][1] = (T)v.y;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; ++e) a[u][e] = T(0);
+      }
     }
+    T sacc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      T t = T(0);
+#pragma unroll
+      for (int e = 0; e < V; ++e) t += a[u][e] * yk[e];
+#pragma unroll
+      for (int m = 1; m < G; m <<= 1) t += __shfl_xor(t, m, 64);
+      sacc[u] = t;
+    }
+    if (q == 0)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int rr = base + u * ROWS + g;
+        if (rr < r1) y[rr] -= sacc[u];
+      }
   }
-  return sacc;
 }
 
 template <class T, bool PIVOT>
@@ -583,16 +641,14 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_lu_solve(const CellParams *cp
       __syncthreads();
     }
     diag_matvec<T>(y, k0, Dinv + dinv_off(p, kb), tmp);
-    for (int rr = k0 + NB + threadIdx.x; rr < p.npad; rr += SOLVE_THREADS)
-      y[rr] -= row_dot64<T>(A + (long long)rr * ld + k0, y + k0);
+    block_column_update<T>(y, A, ld, k0, k0 + NB, p.npad);
     __syncthreads();
   }
   // backward: U with its diagonal
   for (int kb = p.nblk - 1; kb >= 0; --kb) {
     const int k0 = kb * NB;
     diag_matvec<T>(y, k0, Dinv + dinv_off(p, kb) + 4096, tmp);
-    for (int rr = threadIdx.x; rr < k0; rr += SOLVE_THREADS)
-      y[rr] -= row_dot64<T>(A + (long long)rr * ld + k0, y + k0);
+    block_column_update<T>(y, A, ld, k0, 0, k0);
     __syncthreads();
   }
   double *dd = delta + (long long)p.vec_off;

@@ -98,12 +98,14 @@ struct LuPlan {
   std::vector<Stage> batched;
   std::vector<std::vector<Stage>> single;
 
+  // trailing tiles of an mt x mt block matrix: all, or the lower triangle
+  static long long tiles(const CellParams &p, long long mt) { return p.sym ? mt * (mt + 1) / 2 : mt * mt; }
   static long long block_products(const CellParams &p, int kind, int k) {
     const int nb = p.nblk, m = nb - k - 1;
     if (k >= nb) return 0;
-    if (kind == ST_STRIP) return m >= 1 ? 2 * m - 1 : 0;
-    if (kind == ST_GEMM1) return (long long)m * m;
-    if (kind == ST_GEMM2) return m >= 2 ? 2LL * (m - 1) * (m - 1) : 0;
+    if (kind == ST_STRIP) return m >= 1 ? (p.sym ? m : 2 * m - 1) : 0;
+    if (kind == ST_GEMM1) return tiles(p, m);
+    if (kind == ST_GEMM2) return m >= 2 ? 2 * tiles(p, m - 1) : 0;
     return 0;
   }
   static int items(const CellParams &p, int kind, int k, int pivoting) {
@@ -111,10 +113,10 @@ struct LuPlan {
     const int m = nb - k - 1;
     switch (kind) {
       case ST_DIAG: return (k == 0 && nb > 0) ? 1 : 0;
-      case ST_TRSM: return k < nb ? (pivoting ? m : 2 * m) : 0;
-      case ST_STRIP: return (k < nb && m >= 1) ? 2 * m - 1 : 0;
-      case ST_GEMM1: return k < nb ? m * m : 0;
-      case ST_GEMM2: return (nb - k - 2 >= 1) ? (nb - k - 2) * (nb - k - 2) : 0;
+      case ST_TRSM: return k < nb ? ((pivoting || p.sym) ? m : 2 * m) : 0;
+      case ST_STRIP: return (k < nb && m >= 1) ? (p.sym ? m : 2 * m - 1) : 0;
+      case ST_GEMM1: return k < nb ? (int)tiles(p, m) : 0;
+      case ST_GEMM2: return (nb - k - 2 >= 1) ? (int)tiles(p, nb - k - 2) : 0;
       case ST_PANEL: return k < nb ? 1 : 0;
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
     }
@@ -350,6 +352,8 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     p.nblk = p.npad / NB;
     p.sys_off = sys;
     p.vec_off = 2 * N;
+    // the tangent is symmetric unless node-attached springs (area derivative) are present
+    p.sym = ctx->opt.symmetric && ctx->opt.pivoting == WCFE_PIVOT_STATIC && !h.node_springs;
     p.k_off = kp;
     p.mu = h.mu; p.E = h.E; p.nu = h.nu; p.thick = h.thick; p.k_lin = h.k_lin; p.rho_max = h.rho_max;
     N += h.n_nodes;
@@ -749,6 +753,7 @@ void wcfe_default_options(wcfe_options *o) {
   o->k_off0 = 0.1;
   o->f_bell = 4.0;
   o->kappa = 0.5;
+  o->symmetric = 1;
 }
 
 int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
@@ -1231,6 +1236,15 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
     p.sys_off = sys;
     p.vec_off = vo;
     a_off[i] = ap;
+    // exactly symmetric systems take the symmetric (lower-triangle) factorisation
+    if (ctx->opt.symmetric && ctx->opt.pivoting == WCFE_PIVOT_STATIC) {
+      const double *Ai = A + ap;
+      bool sym = true;
+      for (int r = 0; r < n[i] && sym; ++r)
+        for (int c2 = 0; c2 < r; ++c2)
+          if (Ai[(size_t)r * n[i] + c2] != Ai[(size_t)c2 * n[i] + r]) { sym = false; break; }
+      p.sym = sym;
+    }
     kp += (long long)p.npad * p.npad;
     ap += (long long)n[i] * n[i];
     vo += n[i];

@@ -25,7 +25,8 @@ struct CellParams {
   int node_off, n_nodes, elem_off, n_elems, spr_off, n_spr;
   int ndof, npad, nblk, sys_off;  // sys_off: offset of the per-cell pivot vector
   int vec_off;                    // offset of the cell's dofs in the batch vectors (2 node_off)
-  int pad_;
+  int sym;                        // 1: symmetric tangent, only the lower triangle is updated
+                                  //    and U_kj = D_k L_jk^T is written by transposition
   long long k_off;                // offset of the dense matrix (elements)
   double mu, E, nu, thick, k_lin, rho_max;
 };
