@@ -655,6 +655,248 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_lu_solve(const CellParams *cp
   for (int i = threadIdx.x; i < p.ndof; i += SOLVE_THREADS) dd[i] = accumulate ? dd[i] + (double)y[i] : (double)y[i];
 }
 
+// ---------------------------------------------------------------------------
+// Triangular solves for static pivoting across many workgroups: one workgroup
+// per 64-row block row, every block row of every cell in one launch.
+//   forward   y_k = L_kk^-1 (b_k - sum_{j<k} L_kj y_j)
+//   backward  x_k = U_kk^-1 (y_k - sum_{j>k} U_kj x_j)      (x overwrites y)
+// A workgroup streams its block row while the blocks it depends on are still
+// being solved.  The solution entries are exchanged as self-validating 64-bit
+// words (32-bit payload, 32-bit epoch; two words per FP64 entry) written and
+// read with agent-scope atomics: a reader spins on the words it needs until
+// they carry the current epoch, so no flag, fence or cache maintenance sits
+// on the dependency chain.  The epoch advances by 2 per solve (forward E,
+// backward E + 1), so stale words are never mistaken for new ones.
+// Workgroups take their work item from a ticket counter, so every item a
+// workgroup waits for belongs to a workgroup that started earlier; every wait
+// is bounded (SPIN_LIMIT polls), after which the cell is flagged
+// (CellState::solve_stall) and the workgroup finishes without waiting again.
+// sync: [0] forward ticket, [1] backward ticket, [2] epoch, [3] any cell of
+//       the list still refining (0: both sweeps return at once).
+// ---------------------------------------------------------------------------
+constexpr int MW_THREADS = 256;
+constexpr int SPIN_LIMIT = 1 << 18;
+
+__device__ __forceinline__ int mw_ticket(int *ticket, int *sh) {
+  if (threadIdx.x == 0) *sh = atomicAdd(ticket, 1);
+  __syncthreads();
+  return *sh;
+}
+
+template <class T> struct MwShape {
+  static constexpr int V = 16 / sizeof(T);  // elements per 16-byte load
+  static constexpr int G = 64 / V;          // lanes per 64-element row segment
+  static constexpr int RP = MW_THREADS / G; // rows per pass
+  static constexpr int NP = 64 / RP;        // passes per 64-row block
+  static constexpr int WPE = sizeof(T) / 4; // 64-bit words per exchanged entry
+};
+
+// part[u] += M[row_u, 0..63] . v[0..63] for this thread's slice, M a 64 x 64
+// block (leading dimension ld), rows row_u = g + RP u, columns q V .. q V + V-1
+template <class T>
+__device__ __forceinline__ void mw_load_block(T (&a)[MwShape<T>::NP][MwShape<T>::V], const T *M, long long ld) {
+  typedef MwShape<T> S;
+  const int q = threadIdx.x % S::G, g = threadIdx.x / S::G;
+#pragma unroll
+  for (int u = 0; u < S::NP; ++u) {
+    const T *src = M + (long long)(g + S::RP * u) * ld + q * S::V;
+    if constexpr (S::V == 4) {
+      const float4 v = *reinterpret_cast<const float4 *>(src);
+      a[u][0] = v.x; a[u][1] = v.y; a[u][2] = v.z; a[u][3] = v.w;
+    } else {
+      const double2 v = *reinterpret_cast<const double2 *>(src);
+      a[u][0] = v.x; a[u][1] = v.y;
+    }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T mw_group_sum(T t) {
+#pragma unroll
+  for (int m = 1; m < MwShape<T>::G; m <<= 1) t += __shfl_xor(t, m, 64);
+  return t;
+}
+
+__device__ __forceinline__ unsigned long long mw_word_load(const unsigned long long *w) {
+  return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void mw_word_store(unsigned long long *w, unsigned payload, unsigned epoch) {
+  __hip_atomic_store(w, ((unsigned long long)epoch << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Word w of the exchange buffer, already loaded into `v`: spin until it
+// carries `epoch` (bounded).
+__device__ __forceinline__ void mw_settle(unsigned long long &v, const unsigned long long *w, unsigned epoch,
+                                          bool &gave_up, int *stall) {
+  for (int spins = 0; (unsigned)(v >> 32) != epoch && !gave_up; ++spins) {
+    if (spins >= SPIN_LIMIT) {
+      gave_up = true;
+      *stall = 1;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+    v = mw_word_load(w);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T mw_decode(const unsigned long long *w) {
+  if constexpr (MwShape<T>::WPE == 1) {
+    return __uint_as_float((unsigned)w[0]);
+  } else {
+    return __hiloint2double((int)(unsigned)w[1], (int)(unsigned)w[0]);
+  }
+}
+
+// Entry e of the exchange buffer, waiting until it carries `epoch`.
+template <class T>
+__device__ __forceinline__ T mw_get(const unsigned long long *yw, long long e, unsigned epoch, bool &gave_up,
+                                    int *stall) {
+  constexpr int WPE = MwShape<T>::WPE;
+  unsigned long long w[WPE];
+#pragma unroll
+  for (int k = 0; k < WPE; ++k) w[k] = mw_word_load(yw + e * WPE + k);
+#pragma unroll
+  for (int k = 0; k < WPE; ++k) mw_settle(w[k], yw + e * WPE + k, epoch, gave_up, stall);
+  return mw_decode<T>(w);
+}
+
+template <class T>
+__device__ __forceinline__ void mw_put(unsigned long long *yw, long long e, T v, unsigned epoch) {
+  if constexpr (MwShape<T>::WPE == 1) {
+    mw_word_store(yw + e, __float_as_uint(v), epoch);
+  } else {
+    mw_word_store(yw + 2 * e, (unsigned)__double2loint(v), epoch);
+    mw_word_store(yw + 2 * e + 1, (unsigned)__double2hiint(v), epoch);
+  }
+}
+
+template <class T, bool FORWARD>
+__global__ __launch_bounds__(MW_THREADS) void k_lu_solve_mw(const CellParams *cp, CellState *cs, const int *map,
+                                                            const T *K, const T *Dinv, const double *rhs,
+                                                            unsigned long long *yw, double *delta, const int *sync_c,
+                                                            int *tickets, int accumulate) {
+  typedef MwShape<T> S;
+  constexpr int UNR = sizeof(T) == 4 ? 4 : 2;  // blocks in flight per thread
+  __shared__ int sh_ticket;
+  __shared__ T vin[64];
+  if (sync_c[3] == 0) return;  // no cell of the list is refining
+  const unsigned epoch_f = (unsigned)sync_c[2], epoch = epoch_f + (FORWARD ? 0u : 1u);
+  const int item = map[mw_ticket(tickets + (FORWARD ? 0 : 1), &sh_ticket)];
+  const int c = item >> 20, kb = item & 0xFFFFF;
+  if (cs[c].ir_done) return;
+  const CellParams p = cp[c];
+  const T *Arow = K + p.k_off + (long long)(kb * NB) * p.npad;
+  const long long ld = p.npad;
+  const int k0 = kb * NB;
+  unsigned long long *ycw = yw + (long long)p.sys_off * S::WPE;
+  int *stall = &cs[c].solve_stall;
+  const int q = threadIdx.x % S::G, g = threadIdx.x / S::G;
+  bool gave_up = false;
+  // the block's diagonal inverse, loaded before any wait
+  T dinv[S::NP][S::V];
+  mw_load_block<T>(dinv, Dinv + dinv_off(p, kb) + (FORWARD ? 0 : 4096), 64);
+  T part[S::NP];
+#pragma unroll
+  for (int u = 0; u < S::NP; ++u) part[u] = T(0);
+  // off-diagonal blocks in the order they are solved: step s is block
+  // j = s (forward) or nblk - 1 - s (backward)
+  const int nj = FORWARD ? kb : p.nblk - 1 - kb;
+  for (int s = 0; s < nj; s += UNR) {
+    T a[UNR][S::NP][S::V];
+    T yv[UNR][S::V];
+#pragma unroll
+    for (int i = 0; i < UNR; ++i)
+      if (s + i < nj) mw_load_block<T>(a[i], Arow + (FORWARD ? s + i : p.nblk - 1 - s - i) * NB, ld);
+    // every word of the step is requested before any is checked
+    constexpr int NW = S::V * S::WPE;
+    unsigned long long w[UNR][NW];
+#pragma unroll
+    for (int i = 0; i < UNR; ++i) {
+      const long long base = (long long)(FORWARD ? s + i : p.nblk - 1 - s - i) * NB * S::WPE + q * NW;
+#pragma unroll
+      for (int k = 0; k < NW; ++k)
+        w[i][k] = (s + i < nj) ? mw_word_load(ycw + base + k) : ((unsigned long long)epoch << 32);
+    }
+#pragma unroll
+    for (int i = 0; i < UNR; ++i) {
+      const long long base = (long long)(FORWARD ? s + i : p.nblk - 1 - s - i) * NB * S::WPE + q * NW;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) mw_settle(w[i][k], ycw + base + k, epoch, gave_up, stall);
+#pragma unroll
+      for (int e = 0; e < S::V; ++e) yv[i][e] = mw_decode<T>(&w[i][e * S::WPE]);
+    }
+#pragma unroll
+    for (int i = 0; i < UNR; ++i)
+      if (s + i < nj)
+#pragma unroll
+        for (int u = 0; u < S::NP; ++u)
+#pragma unroll
+          for (int e = 0; e < S::V; ++e) part[u] += a[i][u][e] * yv[i][e];
+  }
+  // right-hand side of the block minus the off-diagonal sum
+#pragma unroll
+  for (int u = 0; u < S::NP; ++u) {
+    const T sum = mw_group_sum<T>(part[u]);
+    const int row = g + S::RP * u;
+    if (q == 0) {
+      T b;
+      if (FORWARD) b = (k0 + row < p.ndof) ? (T)rhs[(long long)p.vec_off + k0 + row] : T(0);
+      else b = mw_get<T>(ycw, k0 + row, epoch_f, gave_up, stall);
+      vin[row] = b - sum;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < S::NP; ++u) {
+    T t = T(0);
+#pragma unroll
+    for (int e = 0; e < S::V; ++e) t += dinv[u][e] * vin[q * S::V + e];
+    t = mw_group_sum<T>(t);
+    const int row = g + S::RP * u;
+    if (q == 0) {
+      mw_put<T>(ycw, k0 + row, t, epoch);
+      if (!FORWARD && k0 + row < p.ndof) {
+        double *dd = delta + (long long)p.vec_off + k0 + row;
+        *dd = accumulate ? *dd + (double)t : (double)t;
+      }
+    }
+  }
+}
+
+// Reset the tickets, advance the epoch and record whether any cell of the
+// list is still refining (one workgroup).
+__global__ __launch_bounds__(256) void k_solve_prepare(int *sync, const CellState *cs, const int *cells, int ncell) {
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  int a = 0;
+  for (int i = threadIdx.x; i < ncell; i += 256) a |= cs[cells[i]].ir_done ? 0 : 1;
+  if (a) any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sync[0] = 0;
+    sync[1] = 0;
+    sync[2] += 2;
+    sync[3] = any;
+  }
+}
+
+template <class T>
+void launch_lu_solve_mw(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell,
+                        const WorkList &fwd, const WorkList &bwd, const T *K, const T *Dinv, const double *r,
+                        double *delta, void *yw, int *sync, int accumulate) {
+  if (fwd.total <= 0) return;
+  unsigned long long *w = static_cast<unsigned long long *>(yw);
+  hipLaunchKernelGGL(k_solve_prepare, dim3(1), dim3(256), 0, s, sync, cs, cells, ncell);
+  hipLaunchKernelGGL((k_lu_solve_mw<T, true>), dim3(fwd.total), dim3(MW_THREADS), 0, s, cp, cs, fwd.map, K, Dinv, r,
+                     w, delta, sync, sync, accumulate);
+  hipLaunchKernelGGL((k_lu_solve_mw<T, false>), dim3(bwd.total), dim3(MW_THREADS), 0, s, cp, cs, bwd.map, K, Dinv,
+                     r, w, delta, sync, sync, accumulate);
+}
+
+size_t lu_solve_exchange_bytes(long long rows, int elem_size) { return (size_t)rows * 2 * elem_size; }
+
 size_t lu_solve_lds_bytes(int npad, int elem_size) { return (size_t)npad * elem_size; }
 
 // Dynamic LDS available to k_lu_solve next to its own static LDS.
@@ -697,7 +939,10 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
   template void launch_lu_gemm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *,      \
                                   int, int, int);                                                                     \
   template void launch_lu_solve<T>(hipStream_t, const CellParams *, const CellState *, const int *, int, const T *,   \
-                                   const T *, const int *, int, const double *, double *, int, int);
+                                   const T *, const int *, int, const double *, double *, int, int);                  \
+  template void launch_lu_solve_mw<T>(hipStream_t, const CellParams *, CellState *, const int *, int,               \
+                                      const WorkList &, const WorkList &, const T *, const T *, const double *,      \
+                                      double *, void *, int *, int);
 WCFE_INST(float)
 WCFE_INST(double)
 

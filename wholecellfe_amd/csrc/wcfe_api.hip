@@ -97,6 +97,12 @@ struct LuPlan {
   int *d_prefix = nullptr;
   std::vector<Stage> batched;
   std::vector<std::vector<Stage>> single;
+  // triangular-solve item lists (one item per block row), static pivoting
+  struct SolveLists {
+    int fwd_off = 0, bwd_off = 0, tot = 0;
+  };
+  SolveLists solve_batched;
+  std::vector<SolveLists> solve_single;
 
   // trailing tiles of an mt x mt block matrix: all, or the lower triangle
   static long long tiles(const CellParams &p, long long mt) { return p.sym ? mt * (mt + 1) / 2 : mt * mt; }
@@ -171,6 +177,25 @@ struct LuPlan {
         single[c].push_back(Stage{st.first, st.second, (int)buf.size(), t, block_products(cp[c], st.first, st.second)});
         for (int i = 0; i < t; ++i) buf.push_back((c << 20) | i);
       }
+    // solve lists: largest systems first (longest dependency chains), block
+    // rows ascending (forward) / descending (backward) within a system
+    auto add_lists = [&](const std::vector<int> &cl) {
+      SolveLists L;
+      L.fwd_off = (int)buf.size();
+      for (int c : cl)
+        for (int kb = 0; kb < cp[c].nblk; ++kb) buf.push_back((c << 20) | kb);
+      L.tot = (int)buf.size() - L.fwd_off;
+      L.bwd_off = (int)buf.size();
+      for (int c : cl)
+        for (int kb = cp[c].nblk - 1; kb >= 0; --kb) buf.push_back((c << 20) | kb);
+      return L;
+    };
+    std::vector<int> order(n);
+    for (int c = 0; c < n; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cp[a].nblk > cp[b].nblk; });
+    solve_batched = add_lists(order);
+    solve_single.assign(n, SolveLists{});
+    for (int c = 0; c < n; ++c) solve_single[c] = add_lists(std::vector<int>{c});
     d_prefix = dupload(buf);
   }
 };
@@ -198,6 +223,9 @@ struct Batch {
   double *Ke64 = nullptr, *G = nullptr, *r = nullptr, *delta = nullptr, *Rea = nullptr, *MM = nullptr;
   void *K = nullptr;
   void *Dinv = nullptr;
+  void *ysol = nullptr;  // triangular-solve intermediate, one per system row
+  int *sync = nullptr;   // triangular-solve tickets and per-cell counters
+  int nsync = 0;
   int *ipiv = nullptr;
   int *cells = nullptr;
   int *replica = nullptr;
@@ -206,7 +234,7 @@ struct Batch {
 
   void release() {
     void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
-                    sp_local, cp, cs, node_cell, adj_start, adj, Ke64, G, r, delta, Rea, MM, K, Dinv, ipiv, cells,
+                    sp_local, cp, cs, node_cell, adj_start, adj, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
                     replica, transitions, plan.d_prefix};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -501,6 +529,12 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   HIPCHECK(hipMemset(b.ipiv, 0, sizeof(int) * std::max(sys, 1)));
   b.K = (void *)dalloc<unsigned char>((size_t)kp * elem_size);
   b.Dinv = (void *)dalloc<unsigned char>((size_t)std::max(sys, 1) * 128 * elem_size);
+  const size_t yb = lu_solve_exchange_bytes(std::max(sys, 1), elem_size);
+  b.ysol = (void *)dalloc<unsigned char>(yb);
+  HIPCHECK(hipMemset(b.ysol, 0, yb));
+  b.nsync = SOLVE_SYNC_INTS;
+  b.sync = dalloc<int>(b.nsync);
+  HIPCHECK(hipMemset(b.sync, 0, sizeof(int) * b.nsync));
   b.plan.build(b.cp_h, ctx->opt.pivoting);
   if (lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
@@ -616,8 +650,15 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
   for (int it = 0; it < std::max(1, ctx->opt.max_refine); ++it) {
     {
       PhaseScope ph(ctx, WCFE_PH_SOLVE);
-      launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.ipiv, piv, b.r,
-                         b.delta, npad, it > 0);
+      if (piv) {
+        launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.ipiv, piv, b.r,
+                           b.delta, npad, it > 0);
+      } else {
+        const LuPlan::SolveLists &L = sc.cell < 0 ? b.plan.solve_batched : b.plan.solve_single[sc.cell];
+        launch_lu_solve_mw<T>(s, b.cp, b.cs, sc.cells, sc.ncell, WorkList{b.plan.d_prefix + L.fwd_off, L.tot},
+                              WorkList{b.plan.d_prefix + L.bwd_off, L.tot}, K, static_cast<const T *>(b.Dinv), b.r,
+                              b.delta, b.ysol, b.sync, it > 0);
+      }
     }
     PhaseScope ph(ctx, WCFE_PH_REFINE);
     launch_matvec_ir(s, v, b.node_cell, b.adj_start, b.adj, b.Ke64, b.G, b.delta, b.r, sc.node_begin, sc.node_end,
@@ -702,7 +743,7 @@ void collect_stats(wcfe_ctx *ctx, wcfe_step_stats *st, int *err_code) {
   wcfe_step_stats s;
   memset(&s, 0, sizeof s);
   s.n_cells = b.ncell;
-  int degenerate = -1, zero = -1;
+  int degenerate = -1, zero = -1, stall = -1;
   for (int c = 0; c < b.ncell; ++c) {
     const CellState &q = cs[c];
     s.n_converged += q.converged;
@@ -714,6 +755,7 @@ void collect_stats(wcfe_ctx *ctx, wcfe_step_stats *st, int *err_code) {
     s.max_newton_rel = std::max(s.max_newton_rel, rel);
     if (q.degenerate && degenerate < 0) degenerate = c;
     if (q.zero_pivot && zero < 0) zero = c;
+    if (q.solve_stall && stall < 0) stall = c;
   }
   if (st) *st = s;
   *err_code = WCFE_OK;
@@ -723,6 +765,9 @@ void collect_stats(wcfe_ctx *ctx, wcfe_step_stats *st, int *err_code) {
   } else if (zero >= 0) {
     *err_code = WCFE_ERR_ZERO_PIVOT;
     ctx->err = "cell " + std::to_string(zero) + ": exactly singular column in partial-pivot LU";
+  } else if (stall >= 0) {
+    *err_code = WCFE_ERR_HIP;
+    ctx->err = "cell " + std::to_string(stall) + ": triangular-solve wait exceeded its bound";
   }
 }
 
@@ -1286,8 +1331,16 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
   launch_cell_norm(s, dcp, dcs, dcells, n_sys, db, 0, 0);
   launch_ir_control(s, dcs, dcells, n_sys, ctx->opt.lin_rtol, 1);
   HIPCHECK(hipMemcpyAsync(dr, db, sizeof(double) * vo, hipMemcpyDeviceToDevice, s));
+  unsigned char *dy = sc.zeros<unsigned char>(lu_solve_exchange_bytes(std::max(sys, 1), sizeof(T)));
+  int *dsync = sc.zeros<int>((size_t)SOLVE_SYNC_INTS);
   for (int it = 0; it < std::max(1, ctx->opt.max_refine); ++it) {
-    launch_lu_solve<T>(s, dcp, dcs, dcells, n_sys, dK, dDinv, dipiv, piv, dr, dx, max_npad, it > 0);
+    if (piv)
+      launch_lu_solve<T>(s, dcp, dcs, dcells, n_sys, dK, dDinv, dipiv, piv, dr, dx, max_npad, it > 0);
+    else
+      launch_lu_solve_mw<T>(s, dcp, dcs, dcells, n_sys,
+                            WorkList{plan.d_prefix + plan.solve_batched.fwd_off, plan.solve_batched.tot},
+                            WorkList{plan.d_prefix + plan.solve_batched.bwd_off, plan.solve_batched.tot}, dK, dDinv,
+                            dr, dx, dy, dsync, it > 0);
     launch_dense_residual(s, dcp, dcs, dcells, n_sys, dA, dAoff, db, dx, dr);
     launch_cell_norm(s, dcp, dcs, dcells, n_sys, dr, 1, 1);
     launch_ir_control(s, dcs, dcells, n_sys, ctx->opt.lin_rtol, 0);
@@ -1300,6 +1353,7 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
     if (relres) relres[i] = cs[i].lin_relres;
     if (iters) iters[i] = cs[i].ir_iters;
     if (cs[i].zero_pivot) fail(WCFE_ERR_ZERO_PIVOT, "system %d is exactly singular", i);
+    if (cs[i].solve_stall) fail(WCFE_ERR_HIP, "system %d: triangular-solve wait exceeded its bound", i);
   }
   if (ipiv_out) {
     std::vector<int> pv(std::max(sys, 1));

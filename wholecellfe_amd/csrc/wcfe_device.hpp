@@ -49,7 +49,7 @@ struct CellState {
   int zero_pivot;    // partial pivoting met an exactly zero column
   int degenerate;    // an element had zero area or was inverted
   int lin_failed;    // refinement missed lin_rtol
-  int pad_;
+  int solve_stall;   // a triangular-solve wait hit its bound (results invalid)
 };
 
 // Everything an element kernel needs, passed by value.

@@ -64,6 +64,17 @@ template <class T>
 void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                      const T *K, const T *Dinv, const int *ipiv, int pivoting, const double *r, double *delta,
                      int max_npad, int accumulate);
+// static pivoting: forward and backward sweeps with one workgroup per block
+// row (fwd: items (cell << 20 | block) by cell, block ascending; bwd: block
+// descending) over the cells `cells` (for the any-refining test); yw: the
+// exchange buffer, lu_solve_exchange_bytes(system rows) bytes, zeroed once;
+// sync: SOLVE_SYNC_INTS ints, zeroed once
+constexpr int SOLVE_SYNC_INTS = 4;
+template <class T>
+void launch_lu_solve_mw(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell,
+                        const WorkList &fwd, const WorkList &bwd, const T *K, const T *Dinv, const double *r,
+                        double *delta, void *yw, int *sync, int accumulate);
+size_t lu_solve_exchange_bytes(long long rows, int elem_size);
 size_t lu_solve_lds_bytes(int npad, int elem_size);
 size_t lu_solve_max_dynamic(size_t max_lds);
 hipError_t lu_prepare_attributes(size_t max_lds);
