@@ -51,6 +51,10 @@ enum wcfe_spring_law { WCFE_SPRING_LINEAR = 0, WCFE_SPRING_TABLE = 1 };
 enum wcfe_kinetics { WCFE_KIN_NONE = 0, WCFE_KIN_MEANFIELD = 1, WCFE_KIN_BELL = 2 };
 enum wcfe_precision { WCFE_PREC_MIXED = 0, WCFE_PREC_FP64 = 1 };
 enum wcfe_pivoting { WCFE_PIVOT_STATIC = 0, WCFE_PIVOT_PARTIAL = 1 };
+/* linear solver of a cell: dense blocked LU, or block-CSR + block-Jacobi PCG
+ * (both inside the FP64 iterative refinement); AUTO picks CSR above
+ * wcfe_options.dense_max_dof unknowns */
+enum wcfe_solver { WCFE_SOLVER_AUTO = 0, WCFE_SOLVER_DENSE = 1, WCFE_SOLVER_CSR = 2 };
 
 typedef struct wcfe_ctx wcfe_ctx;
 
@@ -74,6 +78,9 @@ typedef struct wcfe_options {
                             springs) are factored LDL^T-style with static pivoting: only the lower
                             triangle is updated and U_kj = D_k L_jk^T (half the flops);
                             0: always the full LU */
+  int32_t dense_max_dof; /* WCFE_SOLVER_AUTO: cells with more unknowns use the CSR solver */
+  double cg_rtol;        /* PCG (working precision) stops at ||r|| <= cg_rtol ||r0|| per refinement sweep */
+  int32_t cg_max_iter;   /* PCG iteration cap per refinement sweep */
 } wcfe_options;
 
 typedef struct wcfe_cell_desc {
@@ -102,6 +109,7 @@ typedef struct wcfe_cell_desc {
   double k_lin;           /* [pN/um] linear law */
   int32_t table_id;       /* table law: id given to wcfe_set_table */
   int32_t kinetics;       /* wcfe_kinetics */
+  int32_t solver;         /* wcfe_solver */
 } wcfe_cell_desc;
 
 typedef struct wcfe_step_stats {
@@ -112,6 +120,8 @@ typedef struct wcfe_step_stats {
   int32_t n_failed;       /* cells whose refinement missed lin_rtol */
   double max_lin_relres;  /* max over cells of the final ||b - K d|| / ||b|| */
   double max_newton_rel;  /* max over cells of ||G(x_new)|| / ||G(x0)|| */
+  int32_t max_cg_iters;   /* CSR cells: most PCG iterations of one refinement sweep */
+  int32_t n_cg_breakdown; /* CSR cells whose PCG met a non-positive curvature p.Kp <= 0 */
 } wcfe_step_stats;
 
 /* ---- lifecycle ---- */
@@ -137,7 +147,9 @@ int wcfe_finalize(wcfe_ctx *ctx);
  * solve K d = -G with refinement to lin_rtol; x += d; FP64 residual at the new
  * x; cells meeting the convergence test commit their load step (kinetics,
  * strain update, time += dt).  stats may be NULL (then nothing is copied back
- * and the call does not synchronise). */
+ * and the call does not synchronise, unless the batch has CSR cells, whose
+ * PCG checks convergence on the host every few iterations; such batches are
+ * not captured in a graph). */
 int wcfe_newton_step(wcfe_ctx *ctx, wcfe_step_stats *stats);
 /* Newton iterations until every cell has committed one load step. */
 int wcfe_load_step(wcfe_ctx *ctx, int32_t *iterations, wcfe_step_stats *stats);

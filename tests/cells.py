@@ -41,7 +41,7 @@ def ul_cell(K=5, seed=2, table=True, time=1.3, deform=0.01):
     return d
 
 
-def add_to_context(ctx, d, table_id=0, kinetics=0):
+def add_to_context(ctx, d, table_id=0, kinetics=0, solver=0):
     """Register cell dict d on a wholecellfe_amd.Context (tables must be set
     before finalize)."""
     if "tab_px" in d:
@@ -52,4 +52,4 @@ def add_to_context(ctx, d, table_id=0, kinetics=0):
                         rho_max=d.get("rho_max", 100.0), spr_tri=d.get("spr_tri"), spr_bary=d.get("spr_bary"),
                         spr_anchor=d.get("spr_anchor"), spr_state=d.get("spr_state"),
                         spring_law=d.get("spring_law", 0), k_lin=d.get("k_lin", 1000.0), table_id=table_id,
-                        kinetics=kinetics)
+                        kinetics=kinetics, solver=solver)

@@ -20,6 +20,7 @@ SPRING_LINEAR, SPRING_TABLE = 0, 1
 KIN_NONE, KIN_MEANFIELD, KIN_BELL = 0, 1, 2
 PREC_MIXED, PREC_FP64 = 0, 1
 PIVOT_STATIC, PIVOT_PARTIAL = 0, 1
+SOLVER_AUTO, SOLVER_DENSE, SOLVER_CSR = 0, 1, 2
 
 D = C.POINTER(C.c_double)
 I32 = C.POINTER(C.c_int32)
@@ -43,6 +44,7 @@ class Options(C.Structure):
         ("newton_rtol", C.c_double), ("newton_atol", C.c_double), ("max_newton", C.c_int32), ("dt", C.c_double),
         ("t0", C.c_double), ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32), ("k_on", C.c_double),
         ("k_off0", C.c_double), ("f_bell", C.c_double), ("kappa", C.c_double), ("symmetric", C.c_int32),
+        ("dense_max_dof", C.c_int32), ("cg_rtol", C.c_double), ("cg_max_iter", C.c_int32),
     ]
 
 
@@ -52,7 +54,7 @@ class CellDesc(C.Structure):
         ("mu", C.c_double), ("E", C.c_double), ("nu", C.c_double), ("thick", C.c_double), ("ea", D),
         ("node_springs", C.c_int32), ("cC", D), ("fau", D), ("rho_max", C.c_double), ("n_spr", C.c_int32),
         ("spr_tri", I32), ("spr_bary", D), ("spr_anchor", D), ("spr_state", D), ("spring_law", C.c_int32),
-        ("k_lin", C.c_double), ("table_id", C.c_int32), ("kinetics", C.c_int32),
+        ("k_lin", C.c_double), ("table_id", C.c_int32), ("kinetics", C.c_int32), ("solver", C.c_int32),
     ]
 
 
@@ -60,7 +62,7 @@ class StepStats(C.Structure):
     _fields_ = [
         ("n_cells", C.c_int32), ("n_converged", C.c_int32), ("max_refine_used", C.c_int32),
         ("n_static_pivots", C.c_int32), ("n_failed", C.c_int32), ("max_lin_relres", C.c_double),
-        ("max_newton_rel", C.c_double),
+        ("max_newton_rel", C.c_double), ("max_cg_iters", C.c_int32), ("n_cg_breakdown", C.c_int32),
     ]
 
     def as_dict(self):
@@ -168,7 +170,8 @@ class Context:
 
     def add_cell(self, X, tris, law=LAW_CST, x=None, mu=1000.0, E=1000.0, nu=0.3, thick=0.2, ea=None,
                  node_springs=0, cC=None, fau=None, rho_max=100.0, spr_tri=None, spr_bary=None, spr_anchor=None,
-                 spr_state=None, spring_law=SPRING_LINEAR, k_lin=1000.0, table_id=0, kinetics=KIN_NONE):
+                 spr_state=None, spring_law=SPRING_LINEAR, k_lin=1000.0, table_id=0, kinetics=KIN_NONE,
+                 solver=SOLVER_AUTO):
         X = _f64(X).reshape(-1)
         tris = _i32(tris).reshape(-1)
         arrs = dict(X=X, x=_f64(x), tris=tris, ea=_f64(ea), cC=_f64(cC), fau=_f64(fau), spr_tri=_i32(spr_tri),
@@ -190,6 +193,7 @@ class Context:
         d.spr_tri, d.spr_bary, d.spr_anchor, d.spr_state = (_ip(arrs["spr_tri"]), _dp(arrs["spr_bary"]),
                                                             _dp(arrs["spr_anchor"]), _dp(arrs["spr_state"]))
         d.spring_law, d.k_lin, d.table_id, d.kinetics = spring_law, k_lin, table_id, kinetics
+        d.solver = solver
         cid = C.c_int32(-1)
         self._check(self._L.wcfe_add_cell(self._h, C.byref(d), C.byref(cid)))
         self.cells.append(dict(n_nodes=d.n_nodes, n_tris=d.n_tris, n_spr=d.n_spr))

@@ -73,8 +73,8 @@ void launch_neg_copy(hipStream_t s, const double *src, double *dst, long long n)
 // element assembly (one colour per launch)
 // ---------------------------------------------------------------------------
 template <class T, int MODE>
-__global__ __launch_bounds__(128) void k_assemble(View v, int e_begin, int e_end, T *K, double *G, double *Ke64,
-                                                  double *Rea, double *MM, CellState *cs) {
+__global__ __launch_bounds__(128) void k_assemble(View v, int e_begin, int e_end, T *K, T *Kb, double *G,
+                                                  double *Ke64, double *Rea, double *MM, CellState *cs) {
   const int e = e_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= e_end) return;
   const int c = v.el_cell[e];
@@ -96,7 +96,20 @@ __global__ __launch_bounds__(128) void k_assemble(View v, int e_begin, int e_end
     G[2 * n[a]] += Ge[2 * a];
     G[2 * n[a] + 1] += Ge[2 * a + 1];
   }
-  if (MODE == ASM_FULL) {
+  if (MODE == ASM_FULL && v.cp[c].csr) {
+    // block-CSR: the 3 x 3 node blocks of the element
+    const int *bp = v.el_bpos + 9LL * e;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        T *B = Kb + 4LL * bp[3 * a + b];
+        B[0] += T(Ke[(2 * a) * 6 + 2 * b]);
+        B[1] += T(Ke[(2 * a) * 6 + 2 * b + 1]);
+        B[2] += T(Ke[(2 * a + 1) * 6 + 2 * b]);
+        B[3] += T(Ke[(2 * a + 1) * 6 + 2 * b + 1]);
+      }
+  } else if (MODE == ASM_FULL) {
     const CellParams &p = v.cp[c];
     T *A = K + p.k_off;
 #pragma unroll
@@ -111,32 +124,35 @@ __global__ __launch_bounds__(128) void k_assemble(View v, int e_begin, int e_end
           Ar[col + 1] += T(Ke[(2 * a + pp) * 6 + 2 * b + 1]);
         }
       }
-    if (Ke64) {
-      double2 *dst = reinterpret_cast<double2 *>(Ke64 + 36 * (long long)e);
+  }
+  if (MODE == ASM_FULL && Ke64) {
+    double2 *dst = reinterpret_cast<double2 *>(Ke64 + 36 * (long long)e);
 #pragma unroll
-      for (int k = 0; k < 18; ++k) dst[k] = make_double2(Ke[2 * k], Ke[2 * k + 1]);
-    }
+    for (int k = 0; k < 18; ++k) dst[k] = make_double2(Ke[2 * k], Ke[2 * k + 1]);
   }
 }
 
 template <class T>
-void launch_assemble(hipStream_t s, const View &v, int e_begin, int e_end, int mode, T *K, double *G, double *Ke64,
-                     double *Rea, double *MM, int *) {
+void launch_assemble(hipStream_t s, const View &v, int e_begin, int e_end, int mode, T *K, T *Kb, double *G,
+                     double *Ke64, double *Rea, double *MM) {
   const int n = e_end - e_begin;
   if (n <= 0) return;
   dim3 grid((n + 127) / 128), block(128);
   CellState *cs = const_cast<CellState *>(v.cs);
   if (mode == ASM_FULL)
-    hipLaunchKernelGGL((k_assemble<T, ASM_FULL>), grid, block, 0, s, v, e_begin, e_end, K, G, Ke64, Rea, MM, cs);
+    hipLaunchKernelGGL((k_assemble<T, ASM_FULL>), grid, block, 0, s, v, e_begin, e_end, K, Kb, G, Ke64, Rea, MM,
+                       cs);
   else if (mode == ASM_RESIDUAL)
-    hipLaunchKernelGGL((k_assemble<T, ASM_RESIDUAL>), grid, block, 0, s, v, e_begin, e_end, K, G, Ke64, Rea, MM, cs);
+    hipLaunchKernelGGL((k_assemble<T, ASM_RESIDUAL>), grid, block, 0, s, v, e_begin, e_end, K, Kb, G, Ke64, Rea, MM,
+                       cs);
   else
-    hipLaunchKernelGGL((k_assemble<T, ASM_STRAIN>), grid, block, 0, s, v, e_begin, e_end, K, G, Ke64, Rea, MM, cs);
+    hipLaunchKernelGGL((k_assemble<T, ASM_STRAIN>), grid, block, 0, s, v, e_begin, e_end, K, Kb, G, Ke64, Rea, MM,
+                       cs);
 }
-template void launch_assemble<float>(hipStream_t, const View &, int, int, int, float *, double *, double *, double *,
-                                     double *, int *);
+template void launch_assemble<float>(hipStream_t, const View &, int, int, int, float *, float *, double *, double *,
+                                     double *, double *);
 template void launch_assemble<double>(hipStream_t, const View &, int, int, int, double *, double *, double *,
-                                      double *, double *, int *);
+                                      double *, double *, double *);
 
 __global__ __launch_bounds__(128) void k_element_dump(View v, int e_begin, int e_end, double *Ge, double *Ke) {
   const int e = e_begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -439,6 +455,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_diag_tau(const CellParams *cp, CellState *cs, const int *cells, const T *K) {
   const int c = cells[blockIdx.x];
   const CellParams &p = cp[c];
+  if (p.npad == 0) return;  // sparse cell: no dense system
   const T *A = K + p.k_off;
   double m = 0.0;
   for (int i = threadIdx.x; i < p.ndof; i += 256) m = fmax(m, fabs((double)A[(long long)i * p.npad + i]));

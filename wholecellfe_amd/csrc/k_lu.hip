@@ -618,6 +618,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_lu_solve(const CellParams *cp
   const int c = cells[blockIdx.x];
   if (cs[c].ir_done) return;
   const CellParams p = cp[c];
+  if (p.npad == 0) return;  // sparse cell: solved by PCG
   const T *A = K + p.k_off;
   const long long ld = p.npad;
   const double *r = rhs + (long long)p.vec_off;

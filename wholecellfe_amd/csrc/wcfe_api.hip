@@ -57,6 +57,7 @@ template <class T> T *dupload(const std::vector<T> &h) {
 
 struct HostCell {
   int n_nodes = 0, n_tris = 0, law = 0, node_springs = 0, spring_law = 0, table_id = 0, kinetics = 0, n_spr = 0;
+  int solver = 0;
   double mu = 1000, E = 1000, nu = 0.3, thick = 1, rho_max = 100, k_lin = 1000;
   std::vector<double> X, x, ea, cC, fau, spr_bary, spr_anchor, spr_state;
   std::vector<int> tris, spr_tri;
@@ -202,7 +203,7 @@ struct LuPlan {
 
 // A batch of cells laid out in HBM.
 struct Batch {
-  int ncell = 0, N = 0, T = 0, S = 0, ncolor = 0, max_npad = 0, any_ul = 0, any_bell = 0;
+  int ncell = 0, N = 0, T = 0, S = 0, ncolor = 0, max_npad = 0, any_ul = 0, any_bell = 0, any_csr = 0;
   long long kpool = 0;
   std::vector<CellParams> cp_h;
   std::vector<int> color_start;                 // global colour-major ranges (ncolor + 1)
@@ -231,11 +232,23 @@ struct Batch {
   int *replica = nullptr;
   int *transitions = nullptr;
   LuPlan plan;
+  // sparse (block-CSR + PCG) cells
+  std::vector<int> csr_cells_h;                 // ids of the sparse cells
+  std::vector<int> chunk_off_h, nchunk_h;       // per cell: its chunks (sparse cells)
+  std::vector<long long> blk_begin_h, blk_end_h;// per cell: its block range (sparse cells)
+  int nchunk = 0;
+  long long nnzb = 0;
+  int *row_ptr = nullptr, *bcol = nullptr, *diag_pos = nullptr, *el_bpos = nullptr, *csr_cells = nullptr;
+  int4 *chunks = nullptr;
+  void *Kb = nullptr, *cgvec = nullptr;         // block values; x, r, z, p, q (2N each) and dinv (4N)
+  CgState *cg = nullptr;
+  double *partials = nullptr;
 
   void release() {
     void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
                     sp_local, cp, cs, node_cell, adj_start, adj, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
-                    replica, transitions, plan.d_prefix};
+                    replica, transitions, plan.d_prefix, row_ptr, bcol, diag_pos, el_bpos, csr_cells, chunks, Kb,
+                    cgvec, cg, partials};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     *this = Batch();
@@ -284,6 +297,7 @@ struct wcfe_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   Profiler *prof = nullptr;
+  int cg_iters_max = 0, cg_breakdowns = 0;  // sparse cells, since the last step began
 };
 
 namespace {
@@ -330,7 +344,7 @@ struct PhaseScope {
 View make_view(const wcfe_ctx *ctx, const Batch &b) {
   View v;
   v.X = b.X; v.x = b.x; v.ea = b.ea; v.cC = b.cC; v.fau = b.fau;
-  v.el_tri = b.el_tri; v.el_cell = b.el_cell; v.el_spr = b.el_spr;
+  v.el_tri = b.el_tri; v.el_cell = b.el_cell; v.el_spr = b.el_spr; v.el_bpos = b.el_bpos;
   v.sp_bary = b.sp_bary; v.sp_anchor = b.sp_anchor; v.sp_state = b.sp_state;
   v.cp = b.cp; v.cs = b.cs;
   v.tab_px = ctx->tab_px; v.tab_py = ctx->tab_py; v.tab_k = ctx->tab_k;
@@ -376,7 +390,8 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     p.spr_off = S;
     p.n_spr = h.n_spr;
     p.ndof = 2 * h.n_nodes;
-    p.npad = ((p.ndof + NB - 1) / NB) * NB;
+    p.csr = h.solver == WCFE_SOLVER_CSR || (h.solver == WCFE_SOLVER_AUTO && p.ndof > ctx->opt.dense_max_dof);
+    p.npad = p.csr ? 0 : ((p.ndof + NB - 1) / NB) * NB;
     p.nblk = p.npad / NB;
     p.sys_off = sys;
     p.vec_off = 2 * N;
@@ -391,6 +406,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.max_npad = std::max(b.max_npad, p.npad);
     b.any_ul |= (h.law == WCFE_LAW_UL);
     b.any_bell |= (h.kinetics == WCFE_KIN_BELL);
+    b.any_csr |= p.csr;
   }
   b.N = N;
   b.S = S;
@@ -535,8 +551,76 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   b.nsync = SOLVE_SYNC_INTS;
   b.sync = dalloc<int>(b.nsync);
   HIPCHECK(hipMemset(b.sync, 0, sizeof(int) * b.nsync));
+  // sparse cells: block-CSR pattern (each node's neighbours and itself,
+  // columns ascending), the element block positions and the PCG chunks
+  b.csr_cells_h.clear();
+  b.chunk_off_h.assign(b.ncell, 0);
+  b.nchunk_h.assign(b.ncell, 0);
+  b.blk_begin_h.assign(b.ncell, 0);
+  b.blk_end_h.assign(b.ncell, 0);
+  if (b.any_csr) {
+    std::vector<std::vector<int>> nbr(N);
+    for (int e = 0; e < T; ++e) {
+      if (!b.cp_h[el_cell[e]].csr) continue;
+      for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < 3; ++q) nbr[el_tri[3 * (size_t)e + a]].push_back(el_tri[3 * (size_t)e + q]);
+    }
+    std::vector<int> rp(N + 1, 0), bcol, dpos(N, 0);
+    for (int i = 0; i < N; ++i) {
+      std::sort(nbr[i].begin(), nbr[i].end());
+      nbr[i].erase(std::unique(nbr[i].begin(), nbr[i].end()), nbr[i].end());
+      rp[i + 1] = rp[i] + (int)nbr[i].size();
+      for (size_t k = 0; k < nbr[i].size(); ++k)
+        if (nbr[i][k] == i) dpos[i] = rp[i] + (int)k;
+      bcol.insert(bcol.end(), nbr[i].begin(), nbr[i].end());
+    }
+    std::vector<int> bpos(9 * (size_t)T, 0);
+    for (int e = 0; e < T; ++e) {
+      if (!b.cp_h[el_cell[e]].csr) continue;
+      for (int a = 0; a < 3; ++a) {
+        const int na = el_tri[3 * (size_t)e + a];
+        for (int q = 0; q < 3; ++q) {
+          const int nq = el_tri[3 * (size_t)e + q];
+          bpos[9 * (size_t)e + 3 * a + q] =
+              rp[na] + (int)(std::lower_bound(nbr[na].begin(), nbr[na].end(), nq) - nbr[na].begin());
+        }
+      }
+    }
+    std::vector<int4> ch;
+    for (int c = 0; c < b.ncell; ++c) {
+      const CellParams &p = b.cp_h[c];
+      if (!p.csr) continue;
+      b.csr_cells_h.push_back(c);
+      b.chunk_off_h[c] = (int)ch.size();
+      for (int i = p.node_off; i < p.node_off + p.n_nodes; i += CG_ROWS)
+        ch.push_back(make_int4(c, i, std::min(i + CG_ROWS, p.node_off + p.n_nodes), 0));
+      b.nchunk_h[c] = (int)ch.size() - b.chunk_off_h[c];
+      b.blk_begin_h[c] = rp[p.node_off];
+      b.blk_end_h[c] = rp[p.node_off + p.n_nodes];
+    }
+    b.nchunk = (int)ch.size();
+    b.nnzb = rp[N];
+    b.row_ptr = dupload(rp);
+    b.bcol = dupload(bcol);
+    b.diag_pos = dupload(dpos);
+    b.el_bpos = dupload(bpos);
+    b.chunks = dupload(ch);
+    b.csr_cells = dupload(b.csr_cells_h);
+    b.Kb = (void *)dalloc<unsigned char>((size_t)std::max<long long>(b.nnzb, 1) * 4 * elem_size);
+    b.cgvec = (void *)dalloc<unsigned char>((size_t)std::max(N, 1) * 14 * elem_size);
+    std::vector<CgState> cg(b.ncell);
+    for (int c = 0; c < b.ncell; ++c) {
+      memset(&cg[c], 0, sizeof cg[c]);
+      cg[c].chunk_off = b.chunk_off_h[c];
+      cg[c].nchunk = b.nchunk_h[c];
+      cg[c].done = 1;
+    }
+    b.cg = dupload(cg);
+    b.partials = dalloc<double>(2 * (size_t)std::max(b.nchunk, 1));
+  }
   b.plan.build(b.cp_h, ctx->opt.pivoting);
-  if (lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
+  if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
+      lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
          ctx->max_lds);
 }
@@ -564,6 +648,30 @@ void color_range(const Batch &b, const Scope &sc, int k, int &e0, int &e1) {
   }
 }
 
+// The sparse cells of a scope: their chunks and their id list (device).
+bool scope_sparse(const Batch &b, const Scope &sc, int &cbase, int &nch, const int *&cl, int &ncl) {
+  if (!b.any_csr) return false;
+  if (sc.cell < 0) {
+    cbase = 0;
+    nch = b.nchunk;
+    cl = b.csr_cells;
+    ncl = (int)b.csr_cells_h.size();
+  } else {
+    if (!b.cp_h[sc.cell].csr) return false;
+    cbase = b.chunk_off_h[sc.cell];
+    nch = b.nchunk_h[sc.cell];
+    cl = sc.cells;
+    ncl = 1;
+  }
+  return nch > 0;
+}
+
+template <class T> CgVectors<T> cg_vectors(const Batch &b) {
+  T *base = static_cast<T *>(b.cgvec);
+  const long long n2 = 2LL * b.N;
+  return CgVectors<T>{base, base + n2, base + 2 * n2, base + 3 * n2, base + 4 * n2, base + 5 * n2};
+}
+
 template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &sc, int mode) {
   hipStream_t s = ctx->stream;
   View v = make_view(ctx, b);
@@ -571,7 +679,7 @@ template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &s
   for (int k = 0; k < b.ncolor; ++k) {
     int e0, e1;
     color_range(b, sc, k, e0, e1);
-    launch_assemble<T>(s, v, e0, e1, mode, K, b.G, b.Ke64, b.Rea, b.MM, nullptr);
+    launch_assemble<T>(s, v, e0, e1, mode, K, static_cast<T *>(b.Kb), b.G, b.Ke64, b.Rea, b.MM);
   }
 }
 
@@ -631,6 +739,46 @@ template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc)
   const std::vector<Stage> &stages = sc.cell < 0 ? b.plan.batched : b.plan.single[sc.cell];
   enqueue_lu<T>(ctx, b.plan, stages, sc.cells, sc.ncell, b.cp, b.cs, static_cast<T *>(b.K),
                 static_cast<T *>(b.Dinv), b.ipiv);
+  const int *cl;
+  int cb, nch, ncl;
+  if (scope_sparse(b, sc, cb, nch, cl, ncl)) {
+    PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
+    launch_bsr_diag_inv<T>(ctx->stream, b.chunks + cb, nch, b.diag_pos, static_cast<const T *>(b.Kb),
+                           cg_vectors<T>(b).dinv);
+  }
+}
+
+// Block-Jacobi PCG on the sparse cells of the scope for the refinement
+// right-hand side b.r; the correction goes to (or is added to) b.delta.
+// Convergence is checked on the host every CG_CHECK iterations.
+template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, int accumulate) {
+  const int *cl;
+  int cb, nch, ncl;
+  if (!scope_sparse(b, sc, cb, nch, cl, ncl)) return;
+  constexpr int CG_CHECK = 16;
+  hipStream_t s = ctx->stream;
+  const CgVectors<T> v = cg_vectors<T>(b);
+  const double rtol = ctx->opt.cg_rtol;
+  const int maxit = std::max(1, ctx->opt.cg_max_iter);
+  launch_pcg_init<T>(s, b.chunks, cb, nch, b.cs, b.r, v, b.partials);
+  launch_pcg_scalars(s, cl, ncl, b.cs, b.cg, b.partials, CG_INIT, rtol, maxit);
+  std::vector<int> ids = sc.cell < 0 ? b.csr_cells_h : std::vector<int>{sc.cell};
+  std::vector<CgState> h(b.ncell);
+  for (int it = 0; it < maxit; it += CG_CHECK) {
+    for (int k = 0; k < CG_CHECK; ++k)
+      launch_pcg_iteration<T>(s, b.chunks, cb, nch, cl, ncl, b.cs, b.cg, b.row_ptr, b.bcol,
+                              static_cast<const T *>(b.Kb), v, b.partials, rtol, maxit);
+    HIPCHECK(hipMemcpyAsync(h.data(), b.cg, sizeof(CgState) * b.ncell, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    bool all = true;
+    for (int c : ids) all = all && h[c].done;
+    if (all) break;
+  }
+  launch_pcg_finish<T>(s, b.chunks + cb, nch, b.cs, v.x, b.delta, accumulate);
+  for (int c : ids) {
+    ctx->cg_iters_max = std::max(ctx->cg_iters_max, h[c].iters);
+    ctx->cg_breakdowns += h[c].breakdown;
+  }
 }
 
 // The refinement residual for a mesh batch: r = -G - K d from the FP64
@@ -653,12 +801,13 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
       if (piv) {
         launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.ipiv, piv, b.r,
                            b.delta, npad, it > 0);
-      } else {
+      } else if (npad > 0) {
         const LuPlan::SolveLists &L = sc.cell < 0 ? b.plan.solve_batched : b.plan.solve_single[sc.cell];
         launch_lu_solve_mw<T>(s, b.cp, b.cs, sc.cells, sc.ncell, WorkList{b.plan.d_prefix + L.fwd_off, L.tot},
                               WorkList{b.plan.d_prefix + L.bwd_off, L.tot}, K, static_cast<const T *>(b.Dinv), b.r,
                               b.delta, b.ysol, b.sync, it > 0);
       }
+      enqueue_pcg<T>(ctx, b, sc, it > 0);
     }
     PhaseScope ph(ctx, WCFE_PH_REFINE);
     launch_matvec_ir(s, v, b.node_cell, b.adj_start, b.adj, b.Ke64, b.G, b.delta, b.r, sc.node_begin, sc.node_end,
@@ -677,6 +826,10 @@ template <class T> void enqueue_newton(wcfe_ctx *ctx, Batch &b, const Scope &sc)
   {
     PhaseScope ph(ctx, WCFE_PH_FILL);
     launch_fill_systems<T>(s, b.cp, sc.cells, sc.ncell, npad, K);
+    if (b.any_csr) {
+      const long long bb = sc.cell < 0 ? 0 : b.blk_begin_h[sc.cell], be = sc.cell < 0 ? b.nnzb : b.blk_end_h[sc.cell];
+      launch_zero<T>(s, static_cast<T *>(b.Kb) + 4 * bb, 4 * (be - bb));
+    }
     launch_fill(s, b.G + v0, vn, 0.0);
   }
   {
@@ -718,6 +871,8 @@ template <class T> void enqueue_newton(wcfe_ctx *ctx, Batch &b, const Scope &sc)
 
 void enqueue_step(wcfe_ctx *ctx) {
   Batch &b = ctx->b;
+  ctx->cg_iters_max = 0;
+  ctx->cg_breakdowns = 0;
   if (ctx->opt.batched) {
     if (mixed(ctx)) enqueue_newton<float>(ctx, b, scope_all(b));
     else enqueue_newton<double>(ctx, b, scope_all(b));
@@ -757,6 +912,8 @@ void collect_stats(wcfe_ctx *ctx, wcfe_step_stats *st, int *err_code) {
     if (q.zero_pivot && zero < 0) zero = c;
     if (q.solve_stall && stall < 0) stall = c;
   }
+  s.max_cg_iters = ctx->cg_iters_max;
+  s.n_cg_breakdown = ctx->cg_breakdowns;
   if (st) *st = s;
   *err_code = WCFE_OK;
   if (degenerate >= 0) {
@@ -799,6 +956,9 @@ void wcfe_default_options(wcfe_options *o) {
   o->f_bell = 4.0;
   o->kappa = 0.5;
   o->symmetric = 1;
+  o->dense_max_dof = 8192;
+  o->cg_rtol = 1e-5;
+  o->cg_max_iter = 20000;
 }
 
 int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
@@ -900,6 +1060,8 @@ int wcfe_add_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d, int32_t *cell_id) {
     h.k_lin = d->k_lin;
     h.table_id = d->table_id;
     h.kinetics = d->kinetics;
+    if (d->solver < WCFE_SOLVER_AUTO || d->solver > WCFE_SOLVER_CSR) fail(WCFE_ERR_ARG, "unknown solver %d", d->solver);
+    h.solver = d->solver;
     h.n_spr = d->n_spr;
     h.X.assign(d->X, d->X + 2 * d->n_nodes);
     if (d->x) h.x.assign(d->x, d->x + 2 * d->n_nodes);
@@ -950,7 +1112,7 @@ int wcfe_newton_step(wcfe_ctx *ctx, wcfe_step_stats *stats) {
   int rc = guarded(ctx, [&] {
     if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
     HIPCHECK(hipSetDevice(ctx->opt.device));
-    if (ctx->opt.use_graph && ctx->opt.batched) {
+    if (ctx->opt.use_graph && ctx->opt.batched && !ctx->b.any_csr) {
       if (!ctx->gexec) {
         HIPCHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         try {
@@ -1205,17 +1367,40 @@ int wcfe_assembled_system(wcfe_ctx *ctx, int32_t cell, double *K, double *G) {
     const CellParams &p = b.cp_h[cell];
     launch_fill(ctx->stream, b.G + 2LL * p.node_off, 2LL * p.n_nodes, 0.0);
     const size_t es = mixed(ctx) ? 4 : 8;
+    const long long b0 = p.csr ? b.blk_begin_h[cell] : 0, b1 = p.csr ? b.blk_end_h[cell] : 0;
     if (mixed(ctx)) {
       launch_fill_systems<float>(ctx->stream, b.cp, sc.cells, 1, p.npad, static_cast<float *>(b.K));
+      if (p.csr) launch_zero<float>(ctx->stream, static_cast<float *>(b.Kb) + 4 * b0, 4 * (b1 - b0));
       enqueue_assemble<float>(ctx, b, sc, ASM_FULL);
     } else {
       launch_fill_systems<double>(ctx->stream, b.cp, sc.cells, 1, p.npad, static_cast<double *>(b.K));
+      if (p.csr) launch_zero<double>(ctx->stream, static_cast<double *>(b.Kb) + 4 * b0, 4 * (b1 - b0));
       enqueue_assemble<double>(ctx, b, sc, ASM_FULL);
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     if (G) HIPCHECK(hipMemcpy(G, b.G + 2LL * p.node_off, sizeof(double) * p.ndof, hipMemcpyDeviceToHost));
-    if (K) {
+    if (K && p.csr) {
+      // densify the cell's block-CSR rows
+      const long long b0 = b.blk_begin_h[cell], nb = b.blk_end_h[cell] - b0;
+      std::vector<int> rp(p.n_nodes + 1), col((size_t)nb);
+      std::vector<unsigned char> raw((size_t)nb * 4 * es);
+      HIPCHECK(hipMemcpy(rp.data(), b.row_ptr + p.node_off, sizeof(int) * rp.size(), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(col.data(), b.bcol + b0, sizeof(int) * col.size(), hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(raw.data(), static_cast<unsigned char *>(b.Kb) + (size_t)b0 * 4 * es, raw.size(),
+                         hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < (size_t)p.ndof * p.ndof; ++i) K[i] = 0.0;
+      for (int i = 0; i < p.n_nodes; ++i)
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+          const int j = col[k - b0] - p.node_off;
+          for (int q = 0; q < 4; ++q) {
+            const size_t o = (size_t)(k - b0) * 4 + q;
+            const double v = es == 4 ? (double)reinterpret_cast<float *>(raw.data())[o]
+                                     : reinterpret_cast<double *>(raw.data())[o];
+            K[(size_t)(2 * i + q / 2) * p.ndof + 2 * j + q % 2] = v;
+          }
+        }
+    } else if (K) {
       std::vector<unsigned char> raw((size_t)p.npad * p.npad * es);
       HIPCHECK(hipMemcpy(raw.data(), static_cast<unsigned char *>(b.K) + (size_t)p.k_off * es, raw.size(),
                          hipMemcpyDeviceToHost));
@@ -1296,7 +1481,8 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
     sys += p.npad;
     max_npad = std::max(max_npad, p.npad);
   }
-  if (lu_solve_lds_bytes(max_npad, sizeof(T)) > lu_solve_max_dynamic(ctx->max_lds))
+  if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
+      lu_solve_lds_bytes(max_npad, sizeof(T)) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve", max_npad);
   std::vector<T> Kh((size_t)kp, T(0));
   for (int i = 0; i < n_sys; ++i) {

@@ -27,6 +27,8 @@ struct CellParams {
   int vec_off;                    // offset of the cell's dofs in the batch vectors (2 node_off)
   int sym;                        // 1: symmetric tangent, only the lower triangle is updated
                                   //    and U_kj = D_k L_jk^T is written by transposition
+  int csr;                        // 1: block-CSR system + PCG (no dense storage: npad = nblk = 0)
+  int pad2_;
   long long k_off;                // offset of the dense matrix (elements)
   double mu, E, nu, thick, k_lin, rho_max;
 };
@@ -64,6 +66,7 @@ struct View {
   const int *el_tri;  // 3 per element, global node ids
   const int *el_cell; // owning cell
   const int *el_spr;  // 2 per element: [begin, end) of its hosted barycentric springs
+  const int *el_bpos; // 9 per element of a sparse cell: block-CSR index of block (a, b)
   // barycentric springs (grouped by host element)
   const double *sp_bary;   // 3 per spring
   const double *sp_anchor; // 2 per spring

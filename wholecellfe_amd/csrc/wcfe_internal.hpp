@@ -20,9 +20,10 @@ template <class T>
 void launch_fill_systems(hipStream_t s, const CellParams *cp, const int *cells, int ncell, int max_npad, T *K);
 void launch_fill(hipStream_t s, double *p, long long n, double value);
 void launch_copy(hipStream_t s, const void *src, void *dst, long long bytes);
+// K: dense systems; Kb: block-CSR values of sparse cells (View::el_bpos)
 template <class T>
-void launch_assemble(hipStream_t s, const View &v, int e_begin, int e_end, int mode, T *K, double *G, double *Ke64,
-                     double *Rea, double *MM, int *cs_degenerate_base);
+void launch_assemble(hipStream_t s, const View &v, int e_begin, int e_end, int mode, T *K, T *Kb, double *G,
+                     double *Ke64, double *Rea, double *MM);
 void launch_element_dump(hipStream_t s, const View &v, int e_begin, int e_end, double *Ge, double *Ke);
 void launch_cell_norm(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell,
                       const double *vec, int which, int only_active);
@@ -78,6 +79,36 @@ size_t lu_solve_exchange_bytes(long long rows, int elem_size);
 size_t lu_solve_lds_bytes(int npad, int elem_size);
 size_t lu_solve_max_dynamic(size_t max_lds);
 hipError_t lu_prepare_attributes(size_t max_lds);
+
+// ---- sparse path: block-CSR systems and batched PCG (k_csr.hip) ----
+constexpr int CG_ROWS = 256;  // nodes per chunk (one thread per node)
+enum CgMode : int { CG_INIT = 0, CG_ALPHA = 1, CG_BETA = 2 };
+// per-cell conjugate-gradient state (one entry per cell; used by sparse cells)
+struct CgState {
+  double rz, rr, rr0, pq, alpha, beta;
+  int iters, done, breakdown, total_iters;
+  int chunk_off, nchunk;  // the cell's chunks in the chunk list / partials
+  int pad_[2];
+};
+template <class T> struct CgVectors {
+  T *x, *r, *z, *p, *q, *dinv;  // 2 N each (dinv: 4 N)
+};
+template <class T> void launch_zero(hipStream_t s, T *p, long long n);
+template <class T>
+void launch_bsr_diag_inv(hipStream_t s, const int4 *chunks, int nchunk, const int *diag_pos, const T *val, T *dinv);
+// chunks [cbase, cbase + nchunk) of the batch's chunk list
+template <class T>
+void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const CellState *cs,
+                     const double *rhs, const CgVectors<T> &v, double *partials);
+template <class T>
+void launch_pcg_iteration(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const int *cells, int ncell,
+                          const CellState *cs, CgState *cg, const int *row_ptr, const int *col, const T *val,
+                          const CgVectors<T> &v, double *partials, double rtol, int max_iter);
+void launch_pcg_scalars(hipStream_t s, const int *cells, int ncell, const CellState *cs, CgState *cg,
+                        const double *partials, int mode, double rtol, int max_iter);
+template <class T>
+void launch_pcg_finish(hipStream_t s, const int4 *chunks, int nchunk, const CellState *cs, const T *x, double *delta,
+                       int accumulate);
 
 // ---- reference-shaped stateless kernels (k_reference.hip) ----
 void launch_ref_force_fa(hipStream_t s, int n_nodes, const int *ring_start, const int *ring, const int *tris,
