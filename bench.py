@@ -26,6 +26,17 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 METRIC = "Newton steps/s (batched mixed-prec vs plain FP64 GPU) at 1e-8 rel residual"
+WORKLOAD_TEXT = {
+    "batched256": "batched %(n)d disk cells, T log-uniform 200-5000 (total %(t)d triangles), E 0.5-5 kPa, k in {1, "
+                  "31 pN/nm, 64-knot table}, T/3 springs/cell, dense FP32 LU (MFMA, %(piv)s pivoting) + FP64 "
+                  "refinement to 1e-8, hipGraph",
+    "single20k": "one disk cell, %(t)d triangles, 5000 springs on the 64-knot stiffening table, dense FP32 LU "
+                 "(MFMA, %(piv)s pivoting) + FP64 refinement to 1e-8",
+    "single500k": "one disk cell, %(t)d triangles, 100000 springs on the 64-knot stiffening table, block-CSR FP32 "
+                  "block-Jacobi PCG + FP64 refinement to 1e-8",
+    "replicas1024": "%(n)d replicas, T uniform 1000-10000 (total %(t)d triangles), Bell kinetics, solver per cell "
+                    "by size (dense FP32 LU / block-CSR PCG) + FP64 refinement to 1e-8",
+}
 MFMA_FP32_PEAK = 157.3e12
 HBM_PEAK = 8.0e12
 
@@ -86,12 +97,55 @@ def measured_traffic(n_cells, pivoting):
     return None
 
 
+def gemm_roofline(prof, a):
+    """The dense LU's trailing-update kernel: MFMA-bound; achieved = block
+    products x 2 64^3 flop / its summed launch time (HIP events)."""
+    g = prof["lu_gemm"]
+    tile_flops = 2.0 * 64 ** 3
+    achieved = g["work"] * tile_flops / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
+    launches = max(g["launches"], 1)
+    traffic = measured_traffic(a.cells, a.pivoting) if a.workload == "batched256" else None
+    return {"kernel": "k_lu_gemm<float,%d> (LU trailing update, v_mfma_f32_16x16x4f32)" %
+                      (2 if a.pivoting == "static" else 1), "bound": "mfma",
+            "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / MFMA_FP32_PEAK, "traffic": traffic,
+            "launches_per_step": g["launches"], "block_products_per_step": g["work"],
+            "flops_per_block_product": tile_flops, "avg_launch_us": g["ms"] * 1e3 / launches,
+            "algorithmic_bytes_per_launch": g["work"] / launches * 3 * 64 * 64 * 4}
+
+
+def pcg_roofline(cells, prof, st):
+    """Block-CSR PCG (FP32): HBM-bound; algorithmic bytes per iteration
+    20 nnzb (2 x 2 FP32 values + column index per block) + 116 N (row
+    pointers, the five PCG vectors, the 2 x 2 diagonal inverses) over the PCG
+    iterations of the solve phase."""
+    nnzb, nodes = 0, 0
+    for c in cells:
+        if c.get("solver", 0) != 2:
+            continue
+        T = np.asarray(c["tris"]).reshape(-1, 3).astype(np.int64)
+        n = len(c["X"])
+        pairs = np.concatenate([T[:, i] * n + T[:, j] for i in range(3) for j in range(3)])
+        nnzb += len(np.unique(pairs))
+        nodes += n
+    iters = max(st.get("max_cg_iters", 0), 1) * max(st.get("max_refine_used", 1), 1)
+    bytes_iter = 20.0 * nnzb + 116.0 * nodes
+    ms = prof["solve"]["ms"]
+    achieved = bytes_iter * iters / (ms * 1e-3) if ms > 0 else 0.0
+    return {"kernel": "block-CSR PCG (k_pcg_spmv_dot + k_pcg_update + k_pcg_paxpy)", "bound": "hbm",
+            "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+            "traffic": None, "nnz_blocks": nnzb, "nodes": nodes, "pcg_iterations": iters,
+            "bytes_per_iteration": bytes_iter}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cells", type=int, default=256)
+    ap.add_argument("--cells", type=int, default=256, help="cells of the batched256 workload")
+    ap.add_argument("--workload", default="batched256",
+                    help="batched256 (default: the headline batch), single20k, single500k or replicas1024")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -116,7 +170,11 @@ def main():
 
     log = lambda *m: print("[bench]", *m, file=sys.stderr, flush=True)
     t_setup = time.perf_counter()
-    cells, tables = configs.batched_cells(n_cells=a.cells)
+    if a.workload == "batched256":
+        cells, tables = configs.batched_cells(n_cells=a.cells)
+    else:
+        cells, tables = configs.workload(a.workload)
+        a.no_cpu = True  # the CPU baseline samples the dense batch only
     n_tri = sum(len(c["tris"]) for c in cells)
     ndofs = [2 * len(c["X"]) for c in cells]
     log("built %d cells: %d triangles, dof %d..%d in %.1fs" % (len(cells), n_tri, min(ndofs), max(ndofs),
@@ -144,19 +202,9 @@ def main():
     ctx.close()
     log("check step:", st)
     log("phases:", {k: round(v["ms"], 3) for k, v in prof.items()})
-    g = prof["lu_gemm"]
-    tile_flops = 2.0 * 64 ** 3
-    achieved = g["work"] * tile_flops / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
-    launches = max(g["launches"], 1)
-    traffic = measured_traffic(a.cells, a.pivoting)
-    roofline = {"kernel": "k_lu_gemm<float,%d> (LU trailing update, v_mfma_f32_16x16x4f32)" %
-                          (2 if a.pivoting == "static" else 1), "bound": "mfma",
-                "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / MFMA_FP32_PEAK, "traffic": traffic,
-                "launches_per_step": g["launches"], "block_products_per_step": g["work"],
-                "flops_per_block_product": tile_flops, "avg_launch_us": g["ms"] * 1e3 / launches,
-                "algorithmic_bytes_per_launch": g["work"] / launches * 3 * 64 * 64 * 4}
-    lu_alg = configs.lu_flops(cells)
+    roofline = gemm_roofline(prof, a) if prof["lu_gemm"]["launches"] > 0 else pcg_roofline(cells, prof, st)
+    dense = [c for c in cells if c.get("solver", 0) != 2]
+    lu_alg = configs.lu_flops(dense, symmetric=a.pivoting == "static")
     lu_ms = prof["lu_gemm"]["ms"] + prof["lu_strip"]["ms"] + prof["lu_panel"]["ms"]
 
     # ---- plain FP64 GPU form: same kernels in FP64, one cell at a time, no graph ----
@@ -186,9 +234,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "Newton steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
-        "config": {"workload": "batched %d disk cells, T log-uniform 200-5000 (total %d triangles), E 0.5-5 kPa, "
-                               "k in {1, 31 pN/nm, 64-knot table}, T/3 springs/cell, dense FP32 LU (MFMA, %s "
-                               "pivoting) + FP64 refinement to 1e-8, hipGraph" % (len(cells), n_tri, a.pivoting),
+        "config": {"workload": WORKLOAD_TEXT.get(a.workload, a.workload) % dict(n=len(cells), t=n_tri, piv=a.pivoting),
                    "cells": len(cells), "triangles": n_tri, "dof_min": min(ndofs), "dof_max": max(ndofs),
                    "parallelism": "single GPU"},
         "roofline": roofline,

@@ -22,15 +22,34 @@ def _disk_cell(T_target, rng, radius=10.0, jitter=0.05):
 
 
 def single_cell(T_target=905, n_spr=300, E=1000.0, nu=0.3, thick=0.2, spring="linear", k_lin=1000.0, seed=1,
-                kinetics=0):
-    """One disk cell (config family 1/2): linear plane-stress CST, springs at
-    uniform random points with rest length 0."""
+                kinetics=0, solver=0):
+    """One disk cell (config families 1, 2, 4): linear plane-stress CST,
+    springs at uniform random points with rest length 0."""
     rng = np.random.default_rng(seed)
     X, T = _disk_cell(T_target, rng)
     tri, bary, anchor = mesh.random_springs(X, T, n_spr, 10.0, rng)
     cell = dict(X=X, tris=T, law=1, E=E, nu=nu, thick=thick, spr_tri=tri, spr_bary=bary, spr_anchor=anchor,
-                spring_law=0 if spring == "linear" else 1, k_lin=k_lin, table_id=TABLE_ID, kinetics=kinetics)
+                spring_law=0 if spring == "linear" else 1, k_lin=k_lin, table_id=TABLE_ID, kinetics=kinetics,
+                solver=solver)
     return [cell], {TABLE_ID: mesh.stiffening_table()}
+
+
+# the named workloads: family 2 (one 20k-triangle cell, dense FP32 LU),
+# family 3 (256 uneven cells, the headline batch), family 4 (one 500k-triangle
+# cell, block-CSR PCG), family 5 (1024 Bell replicas)
+WORKLOADS = ("batched256", "single20k", "single500k", "replicas1024")
+
+
+def workload(name):
+    if name == "batched256":
+        return batched_cells(256)
+    if name == "single20k":
+        return single_cell(T_target=20000, n_spr=5000, spring="table", seed=2, solver=1)
+    if name == "single500k":
+        return single_cell(T_target=500000, n_spr=100000, spring="table", seed=4, solver=2)
+    if name == "replicas1024":
+        return replicas(1024)
+    raise KeyError(name)
 
 
 def batched_cells(n_cells=256, t_min=200, t_max=5000, seed=3, spr_per_tri=1.0 / 3.0, kinetics=0):
@@ -75,10 +94,11 @@ def load(ctx, cells, tables):
     ctx.finalize()
 
 
-def lu_flops(cells):
-    """(2/3) n^3 + 2 n^2 per cell, n = 2 N (dense LU of one Newton step)."""
+def lu_flops(cells, symmetric=False):
+    """Dense factorisation + one solve per cell, n = 2 N: (2/3) n^3 + 2 n^2
+    for LU, (1/3) n^3 + 2 n^2 for a symmetric (LDL^T-shaped) factorisation."""
     f = 0.0
     for c in cells:
         n = 2 * len(c["X"])
-        f += 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2
+        f += (1.0 if symmetric else 2.0) / 3.0 * n ** 3 + 2.0 * n ** 2
     return f
