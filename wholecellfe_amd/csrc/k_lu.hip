@@ -33,6 +33,8 @@
 // MFMA: v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64.
 // k_lu_solve applies the factors to one right-hand side per cell with the
 // whole vector resident in LDS.
+#include <type_traits>
+
 #include "wcfe_internal.hpp"
 
 namespace wcfe {
@@ -423,17 +425,56 @@ void launch_lu_trsm(hipStream_t s, const CellParams *cp, CellState *, const Work
   hipLaunchKernelGGL(k_lu_trsm_mm<T>, dim3(wl.total), dim3(256), 0, s, cp, wl.map, k, K, Dinv);
 }
 
-// Trailing update C = A_ij - sum_p L_{i,k+p} U_{k+p,j} over NP consecutive
-// panels k .. k+NP-1 (C is read and written once for NP panels).
-//   square mode: (i, j) over [s0, nblk)^2 with s0 = k + NP
-//   strip mode (NP = 1): block column k+1 from row k+1 down, then block row
-//                        k+1 from column k+2 on (the blocks panel k+1 needs)
-// With `next_diag` (static pivoting) the workgroup of the first diagonal
-// block of the updated region (s0, s0) / (k+1, k+1) then factors that block
-// and stores the inverses of its triangles.
-template <class T, int NP>
-__global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState *cs, const int *map, int k, T *K, T *Dinv, int next_diag,
-                                                 int strip) {
+// A 64 x 64 tile in registers: each thread holds TileShape::E elements,
+// moved as 16-byte vectors.
+template <class T> struct TileShape {
+  static constexpr int VN = 16 / sizeof(T);     // elements per vector
+  static constexpr int PER_ROW = 64 / VN;       // vectors per row
+  static constexpr int N = 64 * PER_ROW / 256;  // vectors per thread
+  static constexpr int E = N * VN;              // elements per thread
+};
+template <class T>
+__device__ __forceinline__ void tile_load(T (&t)[TileShape<T>::E], const T *g, long long ld) {
+  typedef TileShape<T> R;
+#pragma unroll
+  for (int j = 0; j < R::N; ++j) {
+    const int idx = threadIdx.x + 256 * j, r = idx / R::PER_ROW, c = (idx % R::PER_ROW) * R::VN;
+    if constexpr (R::VN == 4) {
+      const float4 v = *reinterpret_cast<const float4 *>(g + r * ld + c);
+      t[4 * j] = v.x; t[4 * j + 1] = v.y; t[4 * j + 2] = v.z; t[4 * j + 3] = v.w;
+    } else {
+      const double2 v = *reinterpret_cast<const double2 *>(g + r * ld + c);
+      t[2 * j] = v.x; t[2 * j + 1] = v.y;
+    }
+  }
+}
+template <class T, int P>
+__device__ __forceinline__ void tile_to_lds(const T (&t)[TileShape<T>::E], T *s) {
+  typedef TileShape<T> R;
+#pragma unroll
+  for (int j = 0; j < R::N; ++j) {
+    const int idx = threadIdx.x + 256 * j, r = idx / R::PER_ROW, c = (idx % R::PER_ROW) * R::VN;
+    if constexpr (R::VN == 4) {
+      *reinterpret_cast<float4 *>(s + r * P + c) = make_float4(t[4 * j], t[4 * j + 1], t[4 * j + 2], t[4 * j + 3]);
+    } else {
+      *reinterpret_cast<double2 *>(s + r * P + c) = make_double2(t[2 * j], t[2 * j + 1]);
+    }
+  }
+}
+
+// Trailing update C = A_ij - sum_p L_{i,k+p} U_{k+p,j} over np consecutive
+// panels k .. k+np-1 (C is read and written once for np panels); the next
+// panel's tiles are loaded into registers while the current one is on MFMA.
+//   square mode: (i, j) over [s0, nblk)^2 with s0 = k + np (lower triangle
+//                i >= j for symmetric cells)
+//   strip mode:  target s0 = k + np: block column s0 from row s0 down, then
+//                (non-symmetric cells) block row s0 from column s0 + 1 on --
+//                the blocks panel s0 needs
+// With `next_diag` (static pivoting) the workgroup of the diagonal block
+// (s0, s0) then factors that block and stores the inverses of its triangles.
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState *cs, const int *map, int k, T *K,
+                                                 T *Dinv, int next_diag, int np, int strip) {
   constexpr int P = Pitch<T>::v;
   typedef typename Acc4<T>::type acc_t;
   __shared__ __attribute__((aligned(16))) T ls[64 * P];
@@ -444,34 +485,33 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
   const int c = v >> 20;
   const CellParams p = cp[c];
   const int t = v & 0xFFFFF;
-  int bi, bj, sd;
+  const int s0 = k + np;
+  int bi, bj;
   if (strip) {
-    // symmetric cells: block column only (t < m)
-    const int m = p.nblk - k - 1;
-    if (t < m) { bi = k + 1 + t; bj = k + 1; }
-    else { bi = k + 1; bj = k + 2 + (t - m); }
-    sd = k + 1;
+    const int m = p.nblk - s0;
+    if (t < m) { bi = s0 + t; bj = s0; }
+    else { bi = s0; bj = s0 + 1 + (t - m); }
   } else if (p.sym) {
     // lower triangle (i >= j) of the trailing matrix, row by row
-    const int s0 = k + NP;
     int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
     while (i * (i + 1) / 2 > t) --i;
     while ((i + 1) * (i + 2) / 2 <= t) ++i;
     bi = s0 + i;
     bj = s0 + (t - i * (i + 1) / 2);
-    sd = s0;
   } else {
-    const int s0 = k + NP, mt = p.nblk - s0;
+    const int mt = p.nblk - s0;
     bi = s0 + t / mt;
     bj = s0 + t % mt;
-    sd = s0;
   }
   T *A = K + p.k_off;
   const long long ld = p.npad;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
-  const int lr = lane & 15, lk = lane >> 4;
+  const int lk = lane >> 4, lr = lane & 15;
   T *C = A + (long long)(bi * NB) * ld + bj * NB;
+  T lt[TileShape<T>::E], ut[TileShape<T>::E];
+  tile_load<T>(lt, A + (long long)(bi * NB) * ld + k * NB, ld);
+  tile_load<T>(ut, A + (long long)(k * NB) * ld + bj * NB, ld);
   acc_t acc[2][2];
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
@@ -480,15 +520,18 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
 #pragma unroll
       for (int r = 0; r < 4; ++r)
         acc[ti][tj][r] = C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr];
-#pragma unroll
-  for (int pn = 0; pn < NP; ++pn) {
-    if (pn) __syncthreads();
-    load_block<T, P>(ls, A + (long long)(bi * NB) * ld + (k + pn) * NB, ld);
-    load_block<T, P>(us, A + (long long)((k + pn) * NB) * ld + bj * NB, ld);
+  for (int pn = 0; pn < np; ++pn) {
+    if (pn) __syncthreads();  // the previous panel's products are done with the LDS tiles
+    tile_to_lds<T, P>(lt, ls);
+    tile_to_lds<T, P>(ut, us);
     __syncthreads();
+    if (pn + 1 < np) {
+      tile_load<T>(lt, A + (long long)(bi * NB) * ld + (k + pn + 1) * NB, ld);
+      tile_load<T>(ut, A + (long long)((k + pn + 1) * NB) * ld + bj * NB, ld);
+    }
     mma_64<T, P, P>(ls, us, acc, T(-1));
   }
-  const bool diag_next = next_diag && bi == sd && bj == sd;
+  const bool diag_next = next_diag && bi == s0 && bj == s0;
   if (!diag_next) {
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
@@ -499,7 +542,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
           C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
     return;
   }
-  // diagonal block sd is final: factor it here
+  // diagonal block s0 is final: factor it here
   __syncthreads();
   constexpr int PA = 65;
   T *a = ls;  // reuse: 64 x 65 fits in 64 x P
@@ -513,7 +556,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
   const int fixed = factor_diag<T, PA>(a, prow, (T)cs[c].tau);
   for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
     C[(idx >> 6) * ld + (idx & 63)] = a[(idx >> 6) * PA + (idx & 63)];
-  T *D = Dinv + dinv_off(p, sd);
+  T *D = Dinv + dinv_off(p, s0);
   invert_triangles<T, PA>(a, D, D + 4096);
   if (threadIdx.x < 64) {
     const int cnt = __popcll(__ballot(fixed != 0));
@@ -524,11 +567,8 @@ template <class T>
 void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
                     int next_diag, int npanel, int strip) {
   if (wl.total <= 0) return;
-  if (npanel == 2)
-    hipLaunchKernelGGL((k_lu_gemm<T, 2>), dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag, 0);
-  else
-    hipLaunchKernelGGL((k_lu_gemm<T, 1>), dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag,
-                       strip);
+  hipLaunchKernelGGL(k_lu_gemm<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag, npanel,
+                     strip);
 }
 
 // ---------------------------------------------------------------------------

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -83,14 +84,18 @@ int color_elements(int n_nodes, int n_tris, const int *tris, std::vector<int> &c
   return nc;
 }
 
-// LU schedule: a list of stages (kernel kind, block step k), each with the
-// exclusive prefix sums of its work items over the cells of a list (all cells
-// batched, or one cell at a time).
-enum StageKind { ST_DIAG, ST_TRSM, ST_STRIP, ST_GEMM1, ST_GEMM2, ST_PANEL, ST_SWP };
+// LU schedule: a list of stages (kernel kind, block step k, panel count),
+// each with the exclusive prefix sums of its work items over the cells of a
+// list (all cells batched, or one cell at a time).
+enum StageKind { ST_DIAG, ST_TRSM, ST_STRIP, ST_GEMM1, ST_GEMM, ST_PANEL, ST_SWP };
 struct Stage {
-  int kind, k, off, tot;
-  long long work;  // 64 x 64 x 64 block products (GEMM stages)
+  int kind, k, np, off, tot;
+  long long work;  // 64 x 64 x 64 block products (GEMM and strip stages)
 };
+
+// Panels per round of the static-pivoting schedule (delayed update depth):
+// the trailing matrix is read and written once per LU_DEPTH block steps.
+constexpr int LU_DEPTH = 4;
 
 struct LuPlan {
   int max_nblk = 0;
@@ -107,45 +112,50 @@ struct LuPlan {
 
   // trailing tiles of an mt x mt block matrix: all, or the lower triangle
   static long long tiles(const CellParams &p, long long mt) { return p.sym ? mt * (mt + 1) / 2 : mt * mt; }
-  static long long block_products(const CellParams &p, int kind, int k) {
-    const int nb = p.nblk, m = nb - k - 1;
-    if (k >= nb) return 0;
-    if (kind == ST_STRIP) return m >= 1 ? (p.sym ? m : 2 * m - 1) : 0;
-    if (kind == ST_GEMM1) return tiles(p, m);
-    if (kind == ST_GEMM2) return m >= 2 ? 2 * tiles(p, m - 1) : 0;
-    return 0;
-  }
-  static int items(const CellParams &p, int kind, int k, int pivoting) {
+  static long long items(const CellParams &p, int kind, int k, int np, int pivoting) {
     const int nb = p.nblk;
     const int m = nb - k - 1;
     switch (kind) {
       case ST_DIAG: return (k == 0 && nb > 0) ? 1 : 0;
       case ST_TRSM: return k < nb ? ((pivoting || p.sym) ? m : 2 * m) : 0;
-      case ST_STRIP: return (k < nb && m >= 1) ? (p.sym ? m : 2 * m - 1) : 0;
-      case ST_GEMM1: return k < nb ? (int)tiles(p, m) : 0;
-      case ST_GEMM2: return (nb - k - 2 >= 1) ? (int)tiles(p, nb - k - 2) : 0;
+      case ST_STRIP: {  // block column (and row) k + np, after panels k .. k+np-1
+        const int ms = nb - k - np;
+        return ms >= 1 ? (p.sym ? ms : 2 * ms - 1) : 0;
+      }
+      case ST_GEMM1: return k < nb ? tiles(p, m) : 0;
+      case ST_GEMM: return nb - k - np >= 1 ? tiles(p, nb - k - np) : 0;
       case ST_PANEL: return k < nb ? 1 : 0;
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
     }
   }
-  // static pivoting: two block steps per round (delayed update); partial
-  // pivoting: one block step per round
-  static std::vector<std::pair<int, int>> schedule(int nblk, int pivoting) {
-    std::vector<std::pair<int, int>> s;
+  static long long block_products(const CellParams &p, int kind, int k, int np, int pivoting) {
+    if (kind == ST_STRIP || kind == ST_GEMM || kind == ST_GEMM1) return (long long)np * items(p, kind, k, np, pivoting);
+    return 0;
+  }
+  // static pivoting: LU_DEPTH block steps per round -- panel k, then the
+  // left-looking strips that bring block column k + j up to date with panels
+  // k .. k+j-1 (factoring diagonal block k + j) and its block solve, then one
+  // trailing update with all the round's panels; partial pivoting: one block
+  // step per round
+  static std::vector<std::array<int, 3>> schedule(int nblk, int pivoting) {
+    std::vector<std::array<int, 3>> s;
     if (!pivoting) {
-      s.push_back({ST_DIAG, 0});
-      for (int k = 0; k < nblk; k += 2) {
-        s.push_back({ST_TRSM, k});
-        s.push_back({ST_STRIP, k});
-        s.push_back({ST_TRSM, k + 1});
-        s.push_back({ST_GEMM2, k});
+      s.push_back({ST_DIAG, 0, 0});
+      for (int k = 0; k < nblk; k += LU_DEPTH) {
+        const int R = std::min(LU_DEPTH, nblk - k);
+        s.push_back({ST_TRSM, k, 0});
+        for (int j = 1; j < R; ++j) {
+          s.push_back({ST_STRIP, k, j});
+          s.push_back({ST_TRSM, k + j, 0});
+        }
+        s.push_back({ST_GEMM, k, R});
       }
     } else {
       for (int k = 0; k < nblk; ++k) {
-        s.push_back({ST_PANEL, k});
-        s.push_back({ST_SWP, k});
-        s.push_back({ST_TRSM, k});
-        s.push_back({ST_GEMM1, k});
+        s.push_back({ST_PANEL, k, 0});
+        s.push_back({ST_SWP, k, 0});
+        s.push_back({ST_TRSM, k, 0});
+        s.push_back({ST_GEMM1, k, 1});
       }
     }
     return s;
@@ -160,22 +170,24 @@ struct LuPlan {
     if (n >= (1 << 11)) throw ApiError{WCFE_ERR_ARG, "at most 2047 systems per batch"};
     batched.clear();
     for (auto st : schedule(max_nblk, pivoting)) {
-      Stage g{st.first, st.second, (int)buf.size(), 0, 0};
+      Stage g{st[0], st[1], st[2], (int)buf.size(), 0, 0};
       for (int c = 0; c < n; ++c) {
-        const int t = items(cp[c], st.first, st.second, pivoting);
+        const long long t = items(cp[c], st[0], st[1], st[2], pivoting);
         if (t >= (1 << 20)) throw ApiError{WCFE_ERR_ARG, "too many LU work items per cell"};
         for (int i = 0; i < t; ++i) buf.push_back((c << 20) | i);
-        g.tot += t;
-        g.work += block_products(cp[c], st.first, st.second);
+        g.tot += (int)t;
+        g.work += block_products(cp[c], st[0], st[1], st[2], pivoting);
       }
       if (g.tot > 0) batched.push_back(g);
     }
     single.assign(n, {});
     for (int c = 0; c < n; ++c)
       for (auto st : schedule(cp[c].nblk, pivoting)) {
-        const int t = items(cp[c], st.first, st.second, pivoting);
+        const long long t = items(cp[c], st[0], st[1], st[2], pivoting);
         if (t == 0) continue;
-        single[c].push_back(Stage{st.first, st.second, (int)buf.size(), t, block_products(cp[c], st.first, st.second)});
+        if (t >= (1 << 20)) throw ApiError{WCFE_ERR_ARG, "too many LU work items per cell"};
+        single[c].push_back(Stage{st[0], st[1], st[2], (int)buf.size(), (int)t,
+                                  block_products(cp[c], st[0], st[1], st[2], pivoting)});
         for (int i = 0; i < t; ++i) buf.push_back((c << 20) | i);
       }
     // solve lists: largest systems first (longest dependency chains), block
@@ -718,7 +730,7 @@ void enqueue_lu(wcfe_ctx *ctx, const LuPlan &plan, const std::vector<Stage> &sta
       }
       case ST_STRIP: {
         PhaseScope ph(ctx, WCFE_PH_LU_STRIP, g.work);
-        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, 1, 1);
+        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, g.np, 1);
         break;
       }
       case ST_GEMM1: {
@@ -726,9 +738,9 @@ void enqueue_lu(wcfe_ctx *ctx, const LuPlan &plan, const std::vector<Stage> &sta
         launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, piv ? 0 : 1, 1, 0);
         break;
       }
-      case ST_GEMM2: {
+      case ST_GEMM: {
         PhaseScope ph(ctx, WCFE_PH_LU_GEMM, g.work);
-        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, 2, 0);
+        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, g.np, 0);
         break;
       }
     }
