@@ -81,6 +81,8 @@ typedef struct wcfe_options {
   int32_t dense_max_dof; /* WCFE_SOLVER_AUTO: cells with more unknowns use the CSR solver */
   double cg_rtol;        /* PCG (working precision) stops at ||r|| <= cg_rtol ||r0|| per refinement sweep */
   int32_t cg_max_iter;   /* PCG iteration cap per refinement sweep */
+  int32_t lu_groups;     /* batched dense LU: cells split in this many groups factored on concurrent
+                            streams (1..16; 0 = 4) */
 } wcfe_options;
 
 typedef struct wcfe_cell_desc {

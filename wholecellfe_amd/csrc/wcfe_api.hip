@@ -109,6 +109,11 @@ struct LuPlan {
   };
   SolveLists solve_batched;
   std::vector<SolveLists> solve_single;
+  // the batch split in groups of cells factored on concurrent streams (cells
+  // dealt out by decreasing size so that every group holds big and small
+  // ones); each group runs the same schedule over its own cells
+  std::vector<std::vector<Stage>> grp;
+  std::vector<int> grp_cells_off, grp_ncell;  // each group's cell list in d_prefix
 
   // trailing tiles of an mt x mt block matrix: all, or the lower triangle
   static long long tiles(const CellParams &p, long long mt) { return p.sym ? mt * (mt + 1) / 2 : mt * mt; }
@@ -161,7 +166,7 @@ struct LuPlan {
     return s;
   }
 
-  void build(const std::vector<CellParams> &cp, int piv) {
+  void build(const std::vector<CellParams> &cp, int piv, int groups) {
     pivoting = piv;
     const int n = (int)cp.size();
     std::vector<int> buf;
@@ -207,6 +212,34 @@ struct LuPlan {
     for (int c = 0; c < n; ++c) order[c] = c;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cp[a].nblk > cp[b].nblk; });
     solve_batched = add_lists(order);
+    // concurrent groups
+    grp.clear();
+    grp_cells_off.clear();
+    grp_ncell.clear();
+    const int G = std::min(std::max(groups, 1), n);
+    if (G > 1) {
+      for (int g = 0; g < G; ++g) {
+        std::vector<int> mine;
+        for (int q = g; q < n; q += G) mine.push_back(order[q]);
+        int gmax = 0;
+        for (int c : mine) gmax = std::max(gmax, cp[c].nblk);
+        std::vector<Stage> st;
+        for (auto sk : schedule(gmax, pivoting)) {
+          Stage gs{sk[0], sk[1], sk[2], (int)buf.size(), 0, 0};
+          for (int c : mine) {
+            const long long t = items(cp[c], sk[0], sk[1], sk[2], pivoting);
+            for (int i = 0; i < t; ++i) buf.push_back((c << 20) | i);
+            gs.tot += (int)t;
+            gs.work += block_products(cp[c], sk[0], sk[1], sk[2], pivoting);
+          }
+          if (gs.tot > 0) st.push_back(gs);
+        }
+        grp.push_back(st);
+        grp_cells_off.push_back((int)buf.size());
+        grp_ncell.push_back((int)mine.size());
+        buf.insert(buf.end(), mine.begin(), mine.end());
+      }
+    }
     solve_single.assign(n, SolveLists{});
     for (int c = 0; c < n; ++c) solve_single[c] = add_lists(std::vector<int>{c});
     d_prefix = dupload(buf);
@@ -310,6 +343,9 @@ struct wcfe_ctx {
   hipGraphExec_t gexec = nullptr;
   Profiler *prof = nullptr;
   int cg_iters_max = 0, cg_breakdowns = 0;  // sparse cells, since the last step began
+  std::vector<hipStream_t> side;            // concurrent LU groups
+  hipEvent_t fork_ev = nullptr;
+  std::vector<hipEvent_t> join_ev;
 };
 
 namespace {
@@ -630,7 +666,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.cg = dupload(cg);
     b.partials = dalloc<double>(2 * (size_t)std::max(b.nchunk, 1));
   }
-  b.plan.build(b.cp_h, ctx->opt.pivoting);
+  b.plan.build(b.cp_h, ctx->opt.pivoting, ctx->opt.lu_groups);
   if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
       lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
@@ -697,9 +733,8 @@ template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &s
 
 // Launch the LU stages of a plan for a list of cells.
 template <class T>
-void enqueue_lu(wcfe_ctx *ctx, const LuPlan &plan, const std::vector<Stage> &stages, const int *cells, int ncell,
-                const CellParams *cp, CellState *cs, T *K, T *Dinv, int *ipiv) {
-  hipStream_t s = ctx->stream;
+void enqueue_lu(wcfe_ctx *ctx, hipStream_t s, const LuPlan &plan, const std::vector<Stage> &stages, const int *cells,
+                int ncell, const CellParams *cp, CellState *cs, T *K, T *Dinv, int *ipiv) {
   const bool piv = plan.pivoting != 0;
   {
     PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
@@ -749,8 +784,23 @@ void enqueue_lu(wcfe_ctx *ctx, const LuPlan &plan, const std::vector<Stage> &sta
 
 template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc) {
   const std::vector<Stage> &stages = sc.cell < 0 ? b.plan.batched : b.plan.single[sc.cell];
-  enqueue_lu<T>(ctx, b.plan, stages, sc.cells, sc.ncell, b.cp, b.cs, static_cast<T *>(b.K),
-                static_cast<T *>(b.Dinv), b.ipiv);
+  if (sc.cell < 0 && !ctx->prof && b.plan.grp.size() > 1 && ctx->side.size() >= b.plan.grp.size()) {
+    // independent cell groups on concurrent streams: one group's latency-bound
+    // stages (diagonal blocks, block solves, strips) overlap another's
+    // trailing updates
+    HIPCHECK(hipEventRecord(ctx->fork_ev, ctx->stream));
+    for (size_t g = 0; g < b.plan.grp.size(); ++g) {
+      hipStream_t gs = ctx->side[g];
+      HIPCHECK(hipStreamWaitEvent(gs, ctx->fork_ev, 0));
+      enqueue_lu<T>(ctx, gs, b.plan, b.plan.grp[g], b.plan.d_prefix + b.plan.grp_cells_off[g], b.plan.grp_ncell[g],
+                    b.cp, b.cs, static_cast<T *>(b.K), static_cast<T *>(b.Dinv), b.ipiv);
+      HIPCHECK(hipEventRecord(ctx->join_ev[g], gs));
+      HIPCHECK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[g], 0));
+    }
+  } else {
+    enqueue_lu<T>(ctx, ctx->stream, b.plan, stages, sc.cells, sc.ncell, b.cp, b.cs, static_cast<T *>(b.K),
+                  static_cast<T *>(b.Dinv), b.ipiv);
+  }
   const int *cl;
   int cb, nch, ncl;
   if (scope_sparse(b, sc, cb, nch, cl, ncl)) {
@@ -971,6 +1021,7 @@ void wcfe_default_options(wcfe_options *o) {
   o->dense_max_dof = 8192;
   o->cg_rtol = 1e-5;
   o->cg_max_iter = 20000;
+  o->lu_groups = 4;
 }
 
 int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
@@ -998,6 +1049,17 @@ int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
     ctx->max_lds = (size_t)lds;
     HIPCHECK(lu_prepare_attributes(ctx->max_lds));
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    if (ctx->opt.lu_groups < 0 || ctx->opt.lu_groups > 16) fail(WCFE_ERR_ARG, "lu_groups must be in [0, 16]");
+    if (ctx->opt.lu_groups == 0) ctx->opt.lu_groups = 4;
+    HIPCHECK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    for (int g = 0; g < ctx->opt.lu_groups; ++g) {
+      hipStream_t st;
+      hipEvent_t ev;
+      HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      ctx->side.push_back(st);
+      HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      ctx->join_ev.push_back(ev);
+    }
   });
   if (rc != WCFE_OK) {
     // keep the context so the caller can read the message, but report the code
@@ -1018,6 +1080,9 @@ int wcfe_destroy(wcfe_ctx *ctx) {
   if (ctx->tab_py) (void)hipFree(ctx->tab_py);
   if (ctx->tab_k) (void)hipFree(ctx->tab_k);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  for (hipStream_t st : ctx->side) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : ctx->join_ev) (void)hipEventDestroy(ev);
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   delete ctx;
   return WCFE_OK;
 }
@@ -1521,10 +1586,10 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
   int *dipiv = sc.zeros<int>((size_t)std::max(sys, 1));
   T *dDinv = sc.zeros<T>((size_t)std::max(sys, 1) * 128);
   LuPlan plan;
-  plan.build(cp, ctx->opt.pivoting);
+  plan.build(cp, ctx->opt.pivoting, 1);
   sc.ptrs.push_back(plan.d_prefix);
   const bool piv = ctx->opt.pivoting == WCFE_PIVOT_PARTIAL;
-  enqueue_lu<T>(ctx, plan, plan.batched, dcells, n_sys, dcp, dcs, dK, dDinv, dipiv);
+  enqueue_lu<T>(ctx, ctx->stream, plan, plan.batched, dcells, n_sys, dcp, dcs, dK, dDinv, dipiv);
   // refinement: r = b; repeat { d = LU^-1 r; x += d; r = b - A x (FP64) }
   launch_cell_norm(s, dcp, dcs, dcells, n_sys, db, 0, 0);
   launch_ir_control(s, dcs, dcells, n_sys, ctx->opt.lin_rtol, 1);
