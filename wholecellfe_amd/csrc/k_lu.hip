@@ -812,78 +812,100 @@ __device__ __forceinline__ void mw_put(unsigned long long *yw, long long e, T v,
   }
 }
 
+// One workgroup solves two consecutive block rows of the sweep: `rf`, then
+// `rs` (rs = rf + 1 forward, rf - 1 backward; absent past the end), so the
+// dependency chain has one cross-workgroup hop per two blocks.
+template <class T>
+__device__ __forceinline__ void mw_fetch_step(unsigned long long (&w)[MwShape<T>::V * MwShape<T>::WPE],
+                                              const unsigned long long *ycw, int j, unsigned epoch, bool valid) {
+  typedef MwShape<T> S;
+  constexpr int NW = S::V * S::WPE;
+  const int q = threadIdx.x % S::G;
+  const long long base = (long long)j * NB * S::WPE + q * NW;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) w[k] = valid ? mw_word_load(ycw + base + k) : ((unsigned long long)epoch << 32);
+}
+
 template <class T, bool FORWARD>
 __global__ __launch_bounds__(MW_THREADS) void k_lu_solve_mw(const CellParams *cp, CellState *cs, const int *map,
                                                             const T *K, const T *Dinv, const double *rhs,
                                                             unsigned long long *yw, double *delta, const int *sync_c,
                                                             int *tickets, int accumulate) {
   typedef MwShape<T> S;
-  constexpr int UNR = sizeof(T) == 4 ? 4 : 2;  // blocks in flight per thread
+  constexpr int NW = S::V * S::WPE;
+  constexpr int UNR = sizeof(T) == 4 ? 2 : 1;  // steps in flight per thread
   __shared__ int sh_ticket;
   __shared__ T vin[64];
+  __shared__ T yf[64];
   if (sync_c[3] == 0) return;  // no cell of the list is refining
   const unsigned epoch_f = (unsigned)sync_c[2], epoch = epoch_f + (FORWARD ? 0u : 1u);
   const int item = map[mw_ticket(tickets + (FORWARD ? 0 : 1), &sh_ticket)];
-  const int c = item >> 20, kb = item & 0xFFFFF;
+  const int c = item >> 20, pi = item & 0xFFFFF;
   if (cs[c].ir_done) return;
   const CellParams p = cp[c];
-  const T *Arow = K + p.k_off + (long long)(kb * NB) * p.npad;
   const long long ld = p.npad;
-  const int k0 = kb * NB;
+  // processing position of the first row; block of step s
+  const int pf = 2 * pi;
+  auto blk = [&](int st) { return FORWARD ? st : p.nblk - 1 - st; };
+  const int rf = blk(pf);
+  const bool two = pf + 1 < p.nblk;
+  const int rs = two ? blk(pf + 1) : rf;
+  const T *Af = K + p.k_off + (long long)(rf * NB) * ld;
+  const T *As = K + p.k_off + (long long)(rs * NB) * ld;
   unsigned long long *ycw = yw + (long long)p.sys_off * S::WPE;
   int *stall = &cs[c].solve_stall;
   const int q = threadIdx.x % S::G, g = threadIdx.x / S::G;
   bool gave_up = false;
-  // the block's diagonal inverse, loaded before any wait
-  T dinv[S::NP][S::V];
-  mw_load_block<T>(dinv, Dinv + dinv_off(p, kb) + (FORWARD ? 0 : 4096), 64);
-  T part[S::NP];
+  // loaded before any wait: both diagonal inverses and the block (rs, rf)
+  T dinf[S::NP][S::V], dins[S::NP][S::V], xs[S::NP][S::V];
+  mw_load_block<T>(dinf, Dinv + dinv_off(p, rf) + (FORWARD ? 0 : 4096), 64);
+  mw_load_block<T>(dins, Dinv + dinv_off(p, rs) + (FORWARD ? 0 : 4096), 64);
+  mw_load_block<T>(xs, As + rf * NB, ld);
+  T pf_[S::NP], ps_[S::NP];
 #pragma unroll
-  for (int u = 0; u < S::NP; ++u) part[u] = T(0);
-  // off-diagonal blocks in the order they are solved: step s is block
-  // j = s (forward) or nblk - 1 - s (backward)
-  const int nj = FORWARD ? kb : p.nblk - 1 - kb;
-  for (int s = 0; s < nj; s += UNR) {
-    T a[UNR][S::NP][S::V];
-    T yv[UNR][S::V];
-#pragma unroll
-    for (int i = 0; i < UNR; ++i)
-      if (s + i < nj) mw_load_block<T>(a[i], Arow + (FORWARD ? s + i : p.nblk - 1 - s - i) * NB, ld);
-    // every word of the step is requested before any is checked
-    constexpr int NW = S::V * S::WPE;
+  for (int u = 0; u < S::NP; ++u) pf_[u] = ps_[u] = T(0);
+  // the blocks before rf, in the order they are solved
+  for (int st = 0; st < pf; st += UNR) {
+    T a0[UNR][S::NP][S::V], a1[UNR][S::NP][S::V];
     unsigned long long w[UNR][NW];
 #pragma unroll
     for (int i = 0; i < UNR; ++i) {
-      const long long base = (long long)(FORWARD ? s + i : p.nblk - 1 - s - i) * NB * S::WPE + q * NW;
-#pragma unroll
-      for (int k = 0; k < NW; ++k)
-        w[i][k] = (s + i < nj) ? mw_word_load(ycw + base + k) : ((unsigned long long)epoch << 32);
+      const bool ok = st + i < pf;
+      const int j = blk(ok ? st + i : st);
+      if (ok) {
+        mw_load_block<T>(a0[i], Af + j * NB, ld);
+        if (two) mw_load_block<T>(a1[i], As + j * NB, ld);
+      }
+      mw_fetch_step<T>(w[i], ycw, j, epoch, ok);
     }
 #pragma unroll
     for (int i = 0; i < UNR; ++i) {
-      const long long base = (long long)(FORWARD ? s + i : p.nblk - 1 - s - i) * NB * S::WPE + q * NW;
+      if (st + i >= pf) continue;
+      const long long base = (long long)blk(st + i) * NB * S::WPE + q * NW;
 #pragma unroll
       for (int k = 0; k < NW; ++k) mw_settle(w[i][k], ycw + base + k, epoch, gave_up, stall);
+      T yv[S::V];
 #pragma unroll
-      for (int e = 0; e < S::V; ++e) yv[i][e] = mw_decode<T>(&w[i][e * S::WPE]);
+      for (int e = 0; e < S::V; ++e) yv[e] = mw_decode<T>(&w[i][e * S::WPE]);
+#pragma unroll
+      for (int u = 0; u < S::NP; ++u)
+#pragma unroll
+        for (int e = 0; e < S::V; ++e) {
+          pf_[u] += a0[i][u][e] * yv[e];
+          if (two) ps_[u] += a1[i][u][e] * yv[e];
+        }
     }
-#pragma unroll
-    for (int i = 0; i < UNR; ++i)
-      if (s + i < nj)
-#pragma unroll
-        for (int u = 0; u < S::NP; ++u)
-#pragma unroll
-          for (int e = 0; e < S::V; ++e) part[u] += a[i][u][e] * yv[i][e];
   }
-  // right-hand side of the block minus the off-diagonal sum
+  // first row: y_rf = D_rf^-1 (b_rf - sum)
+  const int k0f = rf * NB, k0s = rs * NB;
 #pragma unroll
   for (int u = 0; u < S::NP; ++u) {
-    const T sum = mw_group_sum<T>(part[u]);
+    const T sum = mw_group_sum<T>(pf_[u]);
     const int row = g + S::RP * u;
     if (q == 0) {
       T b;
-      if (FORWARD) b = (k0 + row < p.ndof) ? (T)rhs[(long long)p.vec_off + k0 + row] : T(0);
-      else b = mw_get<T>(ycw, k0 + row, epoch_f, gave_up, stall);
+      if (FORWARD) b = (k0f + row < p.ndof) ? (T)rhs[(long long)p.vec_off + k0f + row] : T(0);
+      else b = mw_get<T>(ycw, k0f + row, epoch_f, gave_up, stall);
       vin[row] = b - sum;
     }
   }
@@ -892,13 +914,46 @@ __global__ __launch_bounds__(MW_THREADS) void k_lu_solve_mw(const CellParams *cp
   for (int u = 0; u < S::NP; ++u) {
     T t = T(0);
 #pragma unroll
-    for (int e = 0; e < S::V; ++e) t += dinv[u][e] * vin[q * S::V + e];
+    for (int e = 0; e < S::V; ++e) t += dinf[u][e] * vin[q * S::V + e];
     t = mw_group_sum<T>(t);
     const int row = g + S::RP * u;
     if (q == 0) {
-      mw_put<T>(ycw, k0 + row, t, epoch);
-      if (!FORWARD && k0 + row < p.ndof) {
-        double *dd = delta + (long long)p.vec_off + k0 + row;
+      mw_put<T>(ycw, k0f + row, t, epoch);
+      yf[row] = t;
+      if (!FORWARD && k0f + row < p.ndof) {
+        double *dd = delta + (long long)p.vec_off + k0f + row;
+        *dd = accumulate ? *dd + (double)t : (double)t;
+      }
+    }
+  }
+  if (!two) return;
+  __syncthreads();
+  // second row: y_rs = D_rs^-1 (b_rs - sum - A(rs, rf) y_rf)
+#pragma unroll
+  for (int u = 0; u < S::NP; ++u) {
+#pragma unroll
+    for (int e = 0; e < S::V; ++e) ps_[u] += xs[u][e] * yf[q * S::V + e];
+    const T sum = mw_group_sum<T>(ps_[u]);
+    const int row = g + S::RP * u;
+    if (q == 0) {
+      T b;
+      if (FORWARD) b = (k0s + row < p.ndof) ? (T)rhs[(long long)p.vec_off + k0s + row] : T(0);
+      else b = mw_get<T>(ycw, k0s + row, epoch_f, gave_up, stall);
+      vin[row] = b - sum;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < S::NP; ++u) {
+    T t = T(0);
+#pragma unroll
+    for (int e = 0; e < S::V; ++e) t += dins[u][e] * vin[q * S::V + e];
+    t = mw_group_sum<T>(t);
+    const int row = g + S::RP * u;
+    if (q == 0) {
+      mw_put<T>(ycw, k0s + row, t, epoch);
+      if (!FORWARD && k0s + row < p.ndof) {
+        double *dd = delta + (long long)p.vec_off + k0s + row;
         *dd = accumulate ? *dd + (double)t : (double)t;
       }
     }

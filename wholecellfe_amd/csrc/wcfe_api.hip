@@ -197,15 +197,16 @@ struct LuPlan {
       }
     // solve lists: largest systems first (longest dependency chains), block
     // rows ascending (forward) / descending (backward) within a system
+    // one item per pair of block rows (in the order the sweep solves them)
     auto add_lists = [&](const std::vector<int> &cl) {
       SolveLists L;
       L.fwd_off = (int)buf.size();
       for (int c : cl)
-        for (int kb = 0; kb < cp[c].nblk; ++kb) buf.push_back((c << 20) | kb);
+        for (int pi = 0; 2 * pi < cp[c].nblk; ++pi) buf.push_back((c << 20) | pi);
       L.tot = (int)buf.size() - L.fwd_off;
       L.bwd_off = (int)buf.size();
       for (int c : cl)
-        for (int kb = cp[c].nblk - 1; kb >= 0; --kb) buf.push_back((c << 20) | kb);
+        for (int pi = 0; 2 * pi < cp[c].nblk; ++pi) buf.push_back((c << 20) | pi);
       return L;
     };
     std::vector<int> order(n);
