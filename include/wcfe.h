@@ -83,6 +83,7 @@ typedef struct wcfe_options {
   int32_t cg_max_iter;   /* PCG iteration cap per refinement sweep */
   int32_t lu_groups;     /* batched dense LU: cells split in this many groups factored on concurrent
                             streams (1..16; 0 = 4) */
+  int32_t lu_depth;      /* static pivoting: block steps per delayed trailing update (1..8; 0 = 4) */
 } wcfe_options;
 
 typedef struct wcfe_cell_desc {
@@ -157,6 +158,18 @@ int wcfe_newton_step(wcfe_ctx *ctx, wcfe_step_stats *stats);
 int wcfe_load_step(wcfe_ctx *ctx, int32_t *iterations, wcfe_step_stats *stats);
 int wcfe_synchronize(wcfe_ctx *ctx);
 
+/* Autotuner: times `steps` Newton steps of the finalized batch for each
+ * candidate setting and keeps the fastest.  WCFE_TUNE_SOLVER: dense / CSR
+ * threshold (dense_max_dof, cells with solver AUTO; candidates that would not
+ * fit in free device memory are skipped); WCFE_TUNE_LU: delayed-update depth
+ * (2, 4, 6) x concurrent cell groups (1, 2, 4).  Every trial starts from the
+ * state the cells were added with, and so does the batch afterwards: call it
+ * after wcfe_finalize and before any Newton step.  chosen (optional, 3):
+ * lu_depth, lu_groups, dense_max_dof; ms_best (optional): ms per step. */
+#define WCFE_TUNE_SOLVER 1
+#define WCFE_TUNE_LU 2
+int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, double *ms_best);
+
 /* Diagnostics: time `reps` plain device copies of `bytes` bytes (one read and
  * one write of every byte, 16 bytes per thread) on the context's stream; the
  * HBM bandwidth probe and the known-size reference for byte counters. */
@@ -186,6 +199,12 @@ int wcfe_get_positions(wcfe_ctx *ctx, int32_t cell, double *x /* 2N */);
 int wcfe_set_positions(wcfe_ctx *ctx, int32_t cell, const double *x /* 2N */);
 int wcfe_get_ul_state(wcfe_ctx *ctx, int32_t cell, double *X /* 2N */, double *ea /* 3N */, double *cC /* N */,
                       double *fau /* 2N */);
+/* Restore a cell's start-of-step state (any pointer may be NULL; time NaN
+ * keeps the current load time), e.g. to redo a load step at a smaller dt after
+ * the host's collision handling shortened it; the next Newton iteration
+ * starts a new load step. */
+int wcfe_set_ul_state(wcfe_ctx *ctx, int32_t cell, const double *X /* 2N */, const double *ea /* 3N */,
+                      const double *cC /* N */, const double *fau /* 2N */, double time);
 int wcfe_get_springs(wcfe_ctx *ctx, int32_t cell, double *state /* n_spr */, double *anchor /* 2 n_spr */);
 int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step, int32_t *newton_iter,
                        int32_t *n_colors, int32_t *ndof);

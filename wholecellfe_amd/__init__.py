@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwcfe.so")
 
 OK = 0
+ERR_ARG, ERR_HIP, ERR_OOM, ERR_DEGENERATE, ERR_ZERO_PIVOT, ERR_NOT_CONVERGED, ERR_STATE, ERR_NO_DEVICE = range(1, 9)
 ERR_NAMES = {1: "ARG", 2: "HIP", 3: "OOM", 4: "DEGENERATE", 5: "ZERO_PIVOT", 6: "NOT_CONVERGED", 7: "STATE",
              8: "NO_DEVICE"}
 LAW_UL, LAW_CST = 0, 1
@@ -32,8 +33,9 @@ EXPORTS = [
     "wcfe_get_positions", "wcfe_set_positions", "wcfe_get_ul_state", "wcfe_get_springs", "wcfe_get_cell_info",
     "wcfe_residual_fp64", "wcfe_element_systems", "wcfe_assembled_system", "wcfe_get_colors", "wcfe_dense_solve",
     "wcfe_calculate_res", "wcfe_force_fa", "wcfe_update_fa", "wcfe_profile_step",
-    "wcfe_copy_probe",
+    "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state",
 ]
+TUNE_SOLVER, TUNE_LU = 1, 2
 PHASES = ["fill", "assemble", "lu_gemm", "lu_panel", "solve", "refine", "residual", "commit", "lu_strip"]
 
 
@@ -45,7 +47,7 @@ class Options(C.Structure):
         ("t0", C.c_double), ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32), ("k_on", C.c_double),
         ("k_off0", C.c_double), ("f_bell", C.c_double), ("kappa", C.c_double), ("symmetric", C.c_int32),
         ("dense_max_dof", C.c_int32), ("cg_rtol", C.c_double), ("cg_max_iter", C.c_int32),
-        ("lu_groups", C.c_int32),
+        ("lu_groups", C.c_int32), ("lu_depth", C.c_int32),
     ]
 
 
@@ -219,6 +221,19 @@ class Context:
         d = st.as_dict()
         d["iterations"] = it.value
         return d
+
+    def autotune(self, flags=TUNE_SOLVER | TUNE_LU, steps=2):
+        """Time candidate settings on this batch and keep the fastest (call
+        after finalize, before any Newton step)."""
+        chosen = np.zeros(3, dtype=np.int32)
+        ms = C.c_double(0.0)
+        self._check(self._L.wcfe_autotune(self._h, flags, steps, _ip(chosen), C.byref(ms)))
+        return dict(lu_depth=int(chosen[0]), lu_groups=int(chosen[1]), dense_max_dof=int(chosen[2]),
+                    ms_per_step=ms.value)
+
+    def set_ul_state(self, cell, X=None, ea=None, cC=None, fau=None, time=float("nan")):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (X, ea, cC, fau)]
+        self._check(self._L.wcfe_set_ul_state(self._h, cell, *[_dp(a) for a in arrs], C.c_double(time)))
 
     def profile_step(self):
         """One eager Newton step with per-phase HIP-event timing."""

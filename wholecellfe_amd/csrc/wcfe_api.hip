@@ -93,9 +93,6 @@ struct Stage {
   long long work;  // 64 x 64 x 64 block products (GEMM and strip stages)
 };
 
-// Panels per round of the static-pivoting schedule (delayed update depth):
-// the trailing matrix is read and written once per LU_DEPTH block steps.
-constexpr int LU_DEPTH = 4;
 
 struct LuPlan {
   int max_nblk = 0;
@@ -142,12 +139,16 @@ struct LuPlan {
   // k .. k+j-1 (factoring diagonal block k + j) and its block solve, then one
   // trailing update with all the round's panels; partial pivoting: one block
   // step per round
-  static std::vector<std::array<int, 3>> schedule(int nblk, int pivoting) {
+  // `depth` = panels per round of the static-pivoting schedule (delayed
+  // update depth): the trailing matrix is read and written once per `depth`
+  // block steps
+  int depth = 4;
+  std::vector<std::array<int, 3>> schedule(int nblk, int pivoting) const {
     std::vector<std::array<int, 3>> s;
     if (!pivoting) {
       s.push_back({ST_DIAG, 0, 0});
-      for (int k = 0; k < nblk; k += LU_DEPTH) {
-        const int R = std::min(LU_DEPTH, nblk - k);
+      for (int k = 0; k < nblk; k += depth) {
+        const int R = std::min(depth, nblk - k);
         s.push_back({ST_TRSM, k, 0});
         for (int j = 1; j < R; ++j) {
           s.push_back({ST_STRIP, k, j});
@@ -166,8 +167,9 @@ struct LuPlan {
     return s;
   }
 
-  void build(const std::vector<CellParams> &cp, int piv, int groups) {
+  void build(const std::vector<CellParams> &cp, int piv, int groups, int lu_depth) {
     pivoting = piv;
+    depth = lu_depth;
     const int n = (int)cp.size();
     std::vector<int> buf;
     max_nblk = 0;
@@ -344,6 +346,7 @@ struct wcfe_ctx {
   hipGraphExec_t gexec = nullptr;
   Profiler *prof = nullptr;
   int cg_iters_max = 0, cg_breakdowns = 0;  // sparse cells, since the last step began
+  bool stepped = false;                      // a Newton step has run since the batch was built
   std::vector<hipStream_t> side;            // concurrent LU groups
   hipEvent_t fork_ev = nullptr;
   std::vector<hipEvent_t> join_ev;
@@ -667,7 +670,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.cg = dupload(cg);
     b.partials = dalloc<double>(2 * (size_t)std::max(b.nchunk, 1));
   }
-  b.plan.build(b.cp_h, ctx->opt.pivoting, ctx->opt.lu_groups);
+  b.plan.build(b.cp_h, ctx->opt.pivoting, ctx->opt.lu_groups, ctx->opt.lu_depth);
   if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
       lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
@@ -1023,6 +1026,7 @@ void wcfe_default_options(wcfe_options *o) {
   o->cg_rtol = 1e-5;
   o->cg_max_iter = 20000;
   o->lu_groups = 4;
+  o->lu_depth = 4;
 }
 
 int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
@@ -1052,6 +1056,8 @@ int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     if (ctx->opt.lu_groups < 0 || ctx->opt.lu_groups > 16) fail(WCFE_ERR_ARG, "lu_groups must be in [0, 16]");
     if (ctx->opt.lu_groups == 0) ctx->opt.lu_groups = 4;
+    if (ctx->opt.lu_depth < 0 || ctx->opt.lu_depth > 8) fail(WCFE_ERR_ARG, "lu_depth must be in [0, 8]");
+    if (ctx->opt.lu_depth == 0) ctx->opt.lu_depth = 4;
     HIPCHECK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     for (int g = 0; g < ctx->opt.lu_groups; ++g) {
       hipStream_t st;
@@ -1184,33 +1190,135 @@ int wcfe_finalize(wcfe_ctx *ctx) {
   });
 }
 
+namespace {
+// One batched Newton step on the context's stream (graph replay when enabled).
+void step_impl(wcfe_ctx *ctx) {
+  HIPCHECK(hipSetDevice(ctx->opt.device));
+  ctx->stepped = true;
+  if (ctx->opt.use_graph && ctx->opt.batched && !ctx->b.any_csr) {
+    if (!ctx->gexec) {
+      HIPCHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue_step(ctx);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIPCHECK(hipStreamEndCapture(ctx->stream, &ctx->graph));
+      HIPCHECK(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
+    }
+    HIPCHECK(hipGraphLaunch(ctx->gexec, ctx->stream));
+  } else {
+    enqueue_step(ctx);
+  }
+}
+
+// Drop the captured graph and the device batch and lay the batch out again
+// from the host copies of the cells (the state the cells were added with).
+void rebuild_batch(wcfe_ctx *ctx) {
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->gexec = nullptr;
+  ctx->graph = nullptr;
+  ctx->b.release();
+  build_batch(ctx, ctx->b, ctx->cells, mixed(ctx) ? 4 : 8);
+  ctx->stepped = false;
+}
+
+// Device bytes of the dense systems the cells would need at a threshold.
+double dense_bytes(const wcfe_ctx *ctx, int dense_max_dof) {
+  double s = 0.0;
+  for (const HostCell &h : ctx->cells) {
+    const int n = 2 * h.n_nodes;
+    const bool csr = h.solver == WCFE_SOLVER_CSR || (h.solver == WCFE_SOLVER_AUTO && n > dense_max_dof);
+    if (csr) continue;
+    const double np = std::ceil(n / 64.0) * 64.0;
+    s += np * np * (mixed(ctx) ? 4.0 : 8.0);
+  }
+  return s;
+}
+}  // namespace
+
 int wcfe_newton_step(wcfe_ctx *ctx, wcfe_step_stats *stats) {
   if (!ctx) return WCFE_ERR_ARG;
   int code = WCFE_OK;
   int rc = guarded(ctx, [&] {
     if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
-    HIPCHECK(hipSetDevice(ctx->opt.device));
-    if (ctx->opt.use_graph && ctx->opt.batched && !ctx->b.any_csr) {
-      if (!ctx->gexec) {
-        HIPCHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        try {
-          enqueue_step(ctx);
-        } catch (...) {
-          hipGraph_t g = nullptr;
-          (void)hipStreamEndCapture(ctx->stream, &g);
-          if (g) (void)hipGraphDestroy(g);
-          throw;
-        }
-        HIPCHECK(hipStreamEndCapture(ctx->stream, &ctx->graph));
-        HIPCHECK(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
-      }
-      HIPCHECK(hipGraphLaunch(ctx->gexec, ctx->stream));
-    } else {
-      enqueue_step(ctx);
-    }
+    step_impl(ctx);
     if (stats) collect_stats(ctx, stats, &code);
   });
   return rc != WCFE_OK ? rc : code;
+}
+
+int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, double *ms_best) {
+  if (!ctx || steps < 1) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
+    if (ctx->stepped) fail(WCFE_ERR_STATE, "autotune runs on the state the cells were added with: call it before "
+                                           "any Newton step");
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    size_t free_b = 0, total_b = 0;
+    HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+    const double current_dense = dense_bytes(ctx, ctx->opt.dense_max_dof);
+    auto trial = [&]() {
+      rebuild_batch(ctx);
+      step_impl(ctx);  // warm-up (captures the graph)
+      HIPCHECK(hipEventRecord(e0, ctx->stream));
+      for (int i = 0; i < steps; ++i) step_impl(ctx);
+      HIPCHECK(hipEventRecord(e1, ctx->stream));
+      HIPCHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+      return (double)ms / steps;
+    };
+    wcfe_options best = ctx->opt;
+    double best_ms = 1e300;
+    // 1. dense / CSR threshold (cells with solver AUTO)
+    if (flags & WCFE_TUNE_SOLVER) {
+      const int cands[] = {0, 2048, 4096, 8192, 16384, 1 << 30};
+      for (int dm : cands) {
+        if (dense_bytes(ctx, dm) > current_dense + 0.8 * (double)free_b) continue;  // would not fit
+        ctx->opt = best;
+        ctx->opt.dense_max_dof = dm;
+        const double ms = trial();
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = ctx->opt;
+        }
+      }
+    }
+    // 2. delayed-update depth and concurrent cell groups of the dense LU
+    if (flags & WCFE_TUNE_LU) {
+      const int depths[] = {2, 4, 6}, groups[] = {1, 2, 4};
+      const wcfe_options base = best;
+      for (int d : depths)
+        for (int g : groups) {
+          ctx->opt = base;
+          ctx->opt.lu_depth = d;
+          ctx->opt.lu_groups = g;
+          const double ms = trial();
+          if (ms < best_ms) {
+            best_ms = ms;
+            best = ctx->opt;
+          }
+        }
+    }
+    ctx->opt = best;
+    rebuild_batch(ctx);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (chosen) {
+      chosen[0] = ctx->opt.lu_depth;
+      chosen[1] = ctx->opt.lu_groups;
+      chosen[2] = ctx->opt.dense_max_dof;
+    }
+    if (ms_best) *ms_best = best_ms < 1e300 ? best_ms : 0.0;
+  });
 }
 
 int wcfe_load_step(wcfe_ctx *ctx, int32_t *iterations, wcfe_step_stats *stats) {
@@ -1255,6 +1363,7 @@ int wcfe_profile_step(wcfe_ctx *ctx, double *phase_ms, int32_t *phase_launches, 
     HIPCHECK(hipSetDevice(ctx->opt.device));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     ctx->prof = &prof;
+    ctx->stepped = true;
     try {
       enqueue_step(ctx);
     } catch (...) {
@@ -1347,6 +1456,28 @@ int wcfe_get_ul_state(wcfe_ctx *ctx, int32_t cell, double *X, double *ea, double
     if (ea) HIPCHECK(hipMemcpy(ea, ctx->b.ea + 3LL * p.node_off, sizeof(double) * 3 * p.n_nodes, hipMemcpyDeviceToHost));
     if (cC) HIPCHECK(hipMemcpy(cC, ctx->b.cC + p.node_off, sizeof(double) * p.n_nodes, hipMemcpyDeviceToHost));
     if (fau) HIPCHECK(hipMemcpy(fau, ctx->b.fau + p.node_off, sizeof(double2) * p.n_nodes, hipMemcpyDeviceToHost));
+  });
+}
+
+int wcfe_set_ul_state(wcfe_ctx *ctx, int32_t cell, const double *X, const double *ea, const double *cC,
+                      const double *fau, double time) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    const CellParams &p = ctx->b.cp_h[cell];
+    Batch &b = ctx->b;
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if (X) HIPCHECK(hipMemcpy(b.X + p.node_off, X, sizeof(double2) * p.n_nodes, hipMemcpyHostToDevice));
+    if (ea) HIPCHECK(hipMemcpy(b.ea + 3LL * p.node_off, ea, sizeof(double) * 3 * p.n_nodes, hipMemcpyHostToDevice));
+    if (cC) HIPCHECK(hipMemcpy(b.cC + p.node_off, cC, sizeof(double) * p.n_nodes, hipMemcpyHostToDevice));
+    if (fau) HIPCHECK(hipMemcpy(b.fau + p.node_off, fau, sizeof(double2) * p.n_nodes, hipMemcpyHostToDevice));
+    // the next Newton iteration starts a new load step from this state
+    CellState st;
+    HIPCHECK(hipMemcpy(&st, b.cs + cell, sizeof st, hipMemcpyDeviceToHost));
+    if (time == time) st.time = time;
+    st.newton_it = 0;
+    st.converged = 0;
+    HIPCHECK(hipMemcpy(b.cs + cell, &st, sizeof st, hipMemcpyHostToDevice));
   });
 }
 
@@ -1587,7 +1718,7 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
   int *dipiv = sc.zeros<int>((size_t)std::max(sys, 1));
   T *dDinv = sc.zeros<T>((size_t)std::max(sys, 1) * 128);
   LuPlan plan;
-  plan.build(cp, ctx->opt.pivoting, 1);
+  plan.build(cp, ctx->opt.pivoting, 1, ctx->opt.lu_depth);
   sc.ptrs.push_back(plan.d_prefix);
   const bool piv = ctx->opt.pivoting == WCFE_PIVOT_PARTIAL;
   enqueue_lu<T>(ctx, ctx->stream, plan, plan.batched, dcells, n_sys, dcp, dcs, dK, dDinv, dipiv);
