@@ -97,7 +97,7 @@ def measured_traffic(n_cells, pivoting):
     return None
 
 
-def gemm_roofline(prof, a):
+def gemm_roofline(prof, a, depth):
     """The dense LU's trailing-update kernel: MFMA-bound; achieved = block
     products x 2 64^3 flop / its summed launch time (HIP events)."""
     g = prof["lu_gemm"]
@@ -105,13 +105,13 @@ def gemm_roofline(prof, a):
     achieved = g["work"] * tile_flops / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
     launches = max(g["launches"], 1)
     traffic = measured_traffic(a.cells, a.pivoting) if a.workload == "batched256" else None
-    return {"kernel": "k_lu_gemm<float,%d> (LU trailing update, v_mfma_f32_16x16x4f32)" %
-                      (2 if a.pivoting == "static" else 1), "bound": "mfma",
+    return {"kernel": "k_lu_gemm<float,false> (LU trailing update, v_mfma_f32_16x16x4f32)", "bound": "mfma",
             "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
             "frac": achieved / MFMA_FP32_PEAK, "traffic": traffic,
             "launches_per_step": g["launches"], "block_products_per_step": g["work"],
             "flops_per_block_product": tile_flops, "avg_launch_us": g["ms"] * 1e3 / launches,
-            "algorithmic_bytes_per_launch": g["work"] / launches * 3 * 64 * 64 * 4}
+            "c_tile_bytes_per_launch": g["work"] / launches / depth * 2 * 64 * 64 * 4,
+            "panels_per_update": depth}
 
 
 def pcg_roofline(cells, prof, st):
@@ -150,6 +150,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pivoting", choices=["static", "partial"], default="static")
+    ap.add_argument("--autotune", choices=["none", "solver", "lu", "all"], default=None,
+                    help="run wcfe_autotune on the batch before timing (default: solver for replicas1024, "
+                         "none otherwise)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (HBM probe / counter reference) and exit")
     a = ap.parse_args()
@@ -185,6 +188,13 @@ def main():
     # ---- batched mixed precision, one hipGraph replay per Newton step ----
     ctx = W.Context(precision=W.PREC_MIXED, batched=1, use_graph=1, max_refine=8, **common)
     configs.load(ctx, cells, tables)
+    tune = a.autotune or ("solver" if a.workload == "replicas1024" else "none")
+    tuned = None
+    if tune != "none":
+        flags = {"solver": W.TUNE_SOLVER, "lu": W.TUNE_LU, "all": W.TUNE_SOLVER | W.TUNE_LU}[tune]
+        t_tune = time.perf_counter()
+        tuned = ctx.autotune(flags=flags, steps=2)
+        log("autotune (%s): %s in %.1fs" % (tune, tuned, time.perf_counter() - t_tune))
     log("mixed context ready")
     for _ in range(a.warmup):
         ctx.newton_step(stats=False)
@@ -199,10 +209,12 @@ def main():
     log("mixed: %.2f ms/step, %.0f Newton steps/s" % (ms_step, value))
     st = ctx.newton_step(stats=True)
     prof = ctx.profile_step()
+    ctx_depth = ctx.opt.lu_depth
     ctx.close()
     log("check step:", st)
     log("phases:", {k: round(v["ms"], 3) for k, v in prof.items()})
-    roofline = gemm_roofline(prof, a) if prof["lu_gemm"]["launches"] > 0 else pcg_roofline(cells, prof, st)
+    depth = tuned["lu_depth"] if tuned else ctx_depth
+    roofline = gemm_roofline(prof, a, depth) if prof["lu_gemm"]["launches"] > 0 else pcg_roofline(cells, prof, st)
     dense = [c for c in cells if c.get("solver", 0) != 2]
     lu_alg = configs.lu_flops(dense, symmetric=a.pivoting == "static")
     lu_ms = prof["lu_gemm"]["ms"] + prof["lu_strip"]["ms"] + prof["lu_panel"]["ms"]
@@ -247,6 +259,7 @@ def main():
         "static_pivots": st["n_static_pivots"],
         "lu_fraction_of_mfma_peak": (lu_alg / (lu_ms * 1e-3)) / MFMA_FP32_PEAK if lu_ms > 0 else None,
         "phases_ms": {k: round(v["ms"], 4) for k, v in prof.items()},
+        "autotuned": tuned,
     }
     print(json.dumps(out), flush=True)
 

@@ -472,9 +472,9 @@ __device__ __forceinline__ void tile_to_lds(const T (&t)[TileShape<T>::E], T *s)
 //                the blocks panel s0 needs
 // With `next_diag` (static pivoting) the workgroup of the diagonal block
 // (s0, s0) then factors that block and stores the inverses of its triangles.
-template <class T>
+template <class T, bool STRIP>
 __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState *cs, const int *map, int k, T *K,
-                                                 T *Dinv, int next_diag, int np, int strip) {
+                                                 T *Dinv, int next_diag, int np) {
   constexpr int P = Pitch<T>::v;
   typedef typename Acc4<T>::type acc_t;
   __shared__ __attribute__((aligned(16))) T ls[64 * P];
@@ -487,7 +487,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
   const int t = v & 0xFFFFF;
   const int s0 = k + np;
   int bi, bj;
-  if (strip) {
+  if (STRIP) {
     const int m = p.nblk - s0;
     if (t < m) { bi = s0 + t; bj = s0; }
     else { bi = s0; bj = s0 + 1 + (t - m); }
@@ -567,8 +567,12 @@ template <class T>
 void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
                     int next_diag, int npanel, int strip) {
   if (wl.total <= 0) return;
-  hipLaunchKernelGGL(k_lu_gemm<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag, npanel,
-                     strip);
+  if (strip)
+    hipLaunchKernelGGL((k_lu_gemm<T, true>), dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag,
+                       npanel);
+  else
+    hipLaunchKernelGGL((k_lu_gemm<T, false>), dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag,
+                       npanel);
 }
 
 // ---------------------------------------------------------------------------

@@ -1266,7 +1266,14 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
     HIPCHECK(hipMemGetInfo(&free_b, &total_b));
     const double current_dense = dense_bytes(ctx, ctx->opt.dense_max_dof);
     auto trial = [&]() {
-      rebuild_batch(ctx);
+      try {
+        rebuild_batch(ctx);
+      } catch (const ApiError &e) {
+        if (e.code != WCFE_ERR_OOM) throw;
+        ctx->b.release();  // this candidate does not fit: skip it
+        (void)hipGetLastError();
+        return 1e300;
+      }
       step_impl(ctx);  // warm-up (captures the graph)
       HIPCHECK(hipEventRecord(e0, ctx->stream));
       for (int i = 0; i < steps; ++i) step_impl(ctx);
@@ -1282,7 +1289,8 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
     if (flags & WCFE_TUNE_SOLVER) {
       const int cands[] = {0, 2048, 4096, 8192, 16384, 1 << 30};
       for (int dm : cands) {
-        if (dense_bytes(ctx, dm) > current_dense + 0.8 * (double)free_b) continue;  // would not fit
+        if (dense_bytes(ctx, dm) > std::min(current_dense + 0.8 * (double)free_b, 0.6 * (double)total_b))
+          continue;  // would not fit
         ctx->opt = best;
         ctx->opt.dense_max_dof = dm;
         const double ms = trial();
