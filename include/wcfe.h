@@ -145,6 +145,20 @@ int wcfe_add_cell(wcfe_ctx *ctx, const wcfe_cell_desc *desc, int32_t *cell_id);
 /* Colour elements, lay the batch out in HBM, allocate the dense systems. */
 int wcfe_finalize(wcfe_ctx *ctx);
 
+/* ---- topology changes and checkpoints (after wcfe_finalize) ---- */
+/* Replace cell `cell` by a new description -- e.g. after remeshing changed its
+ * connectivity (edge flips, splits, defragmentation); the caller maps its
+ * state onto the new nodes.  The cell keeps its load time and step; every
+ * other cell keeps its current state; the batch is coloured and laid out
+ * again. */
+int wcfe_replace_cell(wcfe_ctx *ctx, int32_t cell, const wcfe_cell_desc *desc);
+/* Binary checkpoint of every cell (mesh, material, positions, strains, node
+ * springs, bond states and anchors, load time and step -- the Bell stream
+ * position) and the force tables.  Options are not stored: load into a
+ * context created with the options wanted, then call wcfe_finalize. */
+int wcfe_checkpoint_save(wcfe_ctx *ctx, const char *path);
+int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path);
+
 /* ---- Newton path (after wcfe_finalize) ---- */
 /* One batched Newton iteration on every cell: assemble K, G at x; factor;
  * solve K d = -G with refinement to lin_rtol; x += d; FP64 residual at the new

@@ -81,3 +81,76 @@ def test_set_ul_state_redoes_a_load_step_bitwise():
     for a, b in zip(c.ul_state(0), s2):
         assert np.array_equal(a, b)
     c.close()
+
+
+def _kw(d, kinetics=0):
+    return dict(x=d.get("x"), law=d["law"], mu=d.get("mu", 1000.0), E=d.get("E", 1000.0), nu=d.get("nu", 0.3),
+                thick=d.get("thick", 0.2), ea=d.get("ea"), node_springs=d.get("node_springs", 0), cC=d.get("cC"),
+                fau=d.get("fau0"), rho_max=d.get("rho_max", 100.0), spr_tri=d.get("spr_tri"),
+                spr_bary=d.get("spr_bary"), spr_anchor=d.get("spr_anchor"), spr_state=d.get("spr_state"),
+                spring_law=d.get("spring_law", 0), k_lin=d.get("k_lin", 1000.0), kinetics=kinetics)
+
+
+def test_replace_cell_keeps_the_other_cells():
+    """Remesh one cell of a batch mid-run: the other cells go on exactly as in
+    a batch without the replaced cell (bitwise, after the same number of
+    Newton iterations), and the new cell converges."""
+    ds = _batch()[:3]
+    new = cells.cst_cell(K=9, n_spr=70, seed=42, time=1.0)
+    opt = dict(precision=W.PREC_MIXED, t0=1.0, newton_rtol=1e-10, newton_atol=1e-9)
+    c = W.Context(**opt)
+    for d in ds:
+        cells.add_to_context(c, d)
+    c.finalize()
+    for _ in range(4):
+        c.newton_step()
+    step1 = c.cell_info(1)["step"]
+    c.replace_cell(1, new["X"], new["tris"], **_kw(new))
+    for _ in range(5):
+        st = c.newton_step()
+    assert st["max_lin_relres"] <= 1e-8
+    assert c.cell_info(1)["step"] > step1  # the new mesh committed load steps
+    got = [c.positions(i) for i in (0, 2)]
+    c.close()
+    r = W.Context(**opt)
+    for d in (ds[0], ds[2]):
+        cells.add_to_context(r, d)
+    r.finalize()
+    for _ in range(9):
+        r.newton_step()
+    for i, g in enumerate(got):
+        assert np.array_equal(r.positions(i), g)
+    r.close()
+
+
+def test_checkpoint_round_trip_bitwise(tmp_path):
+    """Save after one load step, continue; reload into a fresh context and
+    continue: identical positions, strains and bond states (Bell stream)."""
+    d0 = cells.cst_cell(K=6, n_spr=60, seed=3, time=1.0)
+    d1 = cells.ul_cell(K=5, table=False, time=1.3, deform=0.0)
+    d1["x"] = d1["X"].copy()
+    path = tmp_path / "state.ckp"
+    c = W.Context(precision=W.PREC_MIXED, t0=1.0, newton_rtol=1e-10, newton_atol=1e-9, seed_lo=11)
+    cells.add_to_context(c, d0, kinetics=W.KIN_BELL)
+    cells.add_to_context(c, d1, kinetics=W.KIN_MEANFIELD)
+    c.finalize()
+    c.load_step()
+    c.checkpoint_save(path)
+    steps = [c.cell_info(i)["step"] for i in range(2)]
+    sizes = [(x["n_nodes"], x["n_tris"], x["n_spr"]) for x in c.cells]
+    c.load_step()
+    ref = [(c.positions(i), c.ul_state(i), c.springs(i)) for i in range(2)]
+    c.close()
+    r = W.Context(precision=W.PREC_MIXED, t0=1.0, newton_rtol=1e-10, newton_atol=1e-9, seed_lo=11)
+    r.checkpoint_load(path, sizes)
+    r.finalize()
+    assert [r.cell_info(i)["step"] for i in range(2)] == steps
+    r.load_step()
+    for i in range(2):
+        x, ul, sp = r.positions(i), r.ul_state(i), r.springs(i)
+        assert np.array_equal(x, ref[i][0])
+        for a, b in zip(ul, ref[i][1]):
+            assert np.array_equal(a, b)
+        for a, b in zip(sp, ref[i][2]):
+            assert np.array_equal(a, b)
+    r.close()

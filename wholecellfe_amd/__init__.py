@@ -33,7 +33,8 @@ EXPORTS = [
     "wcfe_get_positions", "wcfe_set_positions", "wcfe_get_ul_state", "wcfe_get_springs", "wcfe_get_cell_info",
     "wcfe_residual_fp64", "wcfe_element_systems", "wcfe_assembled_system", "wcfe_get_colors", "wcfe_dense_solve",
     "wcfe_calculate_res", "wcfe_force_fa", "wcfe_update_fa", "wcfe_profile_step",
-    "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state",
+    "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state", "wcfe_replace_cell", "wcfe_checkpoint_save",
+    "wcfe_checkpoint_load",
 ]
 TUNE_SOLVER, TUNE_LU = 1, 2
 PHASES = ["fill", "assemble", "lu_gemm", "lu_panel", "solve", "refine", "residual", "commit", "lu_strip"]
@@ -171,10 +172,34 @@ class Context:
         px, py, k = _f64(px), _f64(py), _f64(k)
         self._check(self._L.wcfe_set_table(self._h, table_id, len(px), _dp(px), _dp(py), _dp(k)))
 
-    def add_cell(self, X, tris, law=LAW_CST, x=None, mu=1000.0, E=1000.0, nu=0.3, thick=0.2, ea=None,
-                 node_springs=0, cC=None, fau=None, rho_max=100.0, spr_tri=None, spr_bary=None, spr_anchor=None,
-                 spr_state=None, spring_law=SPRING_LINEAR, k_lin=1000.0, table_id=0, kinetics=KIN_NONE,
-                 solver=SOLVER_AUTO):
+    def add_cell(self, X, tris, **kw):
+        d = self._desc(X, tris, **kw)
+        cid = C.c_int32(-1)
+        self._check(self._L.wcfe_add_cell(self._h, C.byref(d), C.byref(cid)))
+        self.cells.append(dict(n_nodes=d.n_nodes, n_tris=d.n_tris, n_spr=d.n_spr))
+        return cid.value
+
+    def replace_cell(self, cell, X, tris, **kw):
+        """New mesh and state for one cell after finalize (e.g. after
+        remeshing); the other cells keep their state."""
+        d = self._desc(X, tris, **kw)
+        self._check(self._L.wcfe_replace_cell(self._h, cell, C.byref(d)))
+        self.cells[cell] = dict(n_nodes=d.n_nodes, n_tris=d.n_tris, n_spr=d.n_spr)
+        self._keep = []
+
+    def checkpoint_save(self, path):
+        self._check(self._L.wcfe_checkpoint_save(self._h, str(path).encode()))
+
+    def checkpoint_load(self, path, sizes):
+        """Load a checkpoint into this (empty) context; `sizes` lists
+        (n_nodes, n_tris, n_spr) per cell for the host-side bookkeeping."""
+        self._check(self._L.wcfe_checkpoint_load(self._h, str(path).encode()))
+        self.cells = [dict(n_nodes=a, n_tris=b, n_spr=c) for a, b, c in sizes]
+
+    def _desc(self, X, tris, law=LAW_CST, x=None, mu=1000.0, E=1000.0, nu=0.3, thick=0.2, ea=None,
+              node_springs=0, cC=None, fau=None, rho_max=100.0, spr_tri=None, spr_bary=None, spr_anchor=None,
+              spr_state=None, spring_law=SPRING_LINEAR, k_lin=1000.0, table_id=0, kinetics=KIN_NONE,
+              solver=SOLVER_AUTO):
         X = _f64(X).reshape(-1)
         tris = _i32(tris).reshape(-1)
         arrs = dict(X=X, x=_f64(x), tris=tris, ea=_f64(ea), cC=_f64(cC), fau=_f64(fau), spr_tri=_i32(spr_tri),
@@ -197,10 +222,7 @@ class Context:
                                                             _dp(arrs["spr_anchor"]), _dp(arrs["spr_state"]))
         d.spring_law, d.k_lin, d.table_id, d.kinetics = spring_law, k_lin, table_id, kinetics
         d.solver = solver
-        cid = C.c_int32(-1)
-        self._check(self._L.wcfe_add_cell(self._h, C.byref(d), C.byref(cid)))
-        self.cells.append(dict(n_nodes=d.n_nodes, n_tris=d.n_tris, n_spr=d.n_spr))
-        return cid.value
+        return d
 
     def finalize(self):
         self._check(self._L.wcfe_finalize(self._h))

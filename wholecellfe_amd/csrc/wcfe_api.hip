@@ -59,6 +59,10 @@ template <class T> T *dupload(const std::vector<T> &h) {
 struct HostCell {
   int n_nodes = 0, n_tris = 0, law = 0, node_springs = 0, spring_law = 0, table_id = 0, kinetics = 0, n_spr = 0;
   int solver = 0;
+  int step = 0;       // committed load steps (also the Bell stream's step counter)
+  double time = NAN;  // load time; NaN = wcfe_options.t0
+  bool has_state = false;  // `state` holds the full solver state (Newton iterate, norms)
+  CellState state;
   double mu = 1000, E = 1000, nu = 0.3, thick = 1, rho_max = 100, k_lin = 1000;
   std::vector<double> X, x, ea, cC, fau, spr_bary, spr_anchor, spr_state;
   std::vector<int> tris, spr_tri;
@@ -557,9 +561,12 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     }
   }
   std::vector<CellState> cs(b.ncell);
-  for (auto &s : cs) {
+  for (int c = 0; c < b.ncell; ++c) {
+    CellState &s = cs[c];
     memset(&s, 0, sizeof s);
-    s.time = ctx->opt.t0;
+    if (hc[c].has_state) s = hc[c].state;
+    s.time = std::isnan(hc[c].time) ? ctx->opt.t0 : hc[c].time;
+    s.step = hc[c].step;
   }
   std::vector<int> ident(b.ncell), replica(b.ncell);
   for (int c = 0; c < b.ncell; ++c) ident[c] = replica[c] = c;
@@ -1117,10 +1124,11 @@ int wcfe_set_table(wcfe_ctx *ctx, int32_t id, int32_t n, const double *px, const
   });
 }
 
-int wcfe_add_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d, int32_t *cell_id) {
-  if (!ctx || !d) return WCFE_ERR_ARG;
-  return guarded(ctx, [&] {
-    if (ctx->finalized) fail(WCFE_ERR_STATE, "cells must be added before wcfe_finalize");
+}  // extern "C"
+
+namespace {
+// Validate a cell description and copy it to the host side of the context.
+HostCell host_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d) {
     if (d->n_nodes <= 0 || d->n_tris <= 0 || !d->X || !d->tris) fail(WCFE_ERR_ARG, "empty mesh");
     if (d->law != WCFE_LAW_UL && d->law != WCFE_LAW_CST) fail(WCFE_ERR_ARG, "unknown law %d", d->law);
     if (d->node_springs && d->law != WCFE_LAW_UL)
@@ -1163,7 +1171,57 @@ int wcfe_add_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d, int32_t *cell_id) {
       h.spr_anchor.assign(d->spr_anchor, d->spr_anchor + 2 * d->n_spr);
       if (d->spr_state) h.spr_state.assign(d->spr_state, d->spr_state + d->n_spr);
     }
-    ctx->cells.push_back(std::move(h));
+    return h;
+}
+
+// Copy every cell's current device state (positions, strains, bond state,
+// load time and step) back into the host copies of the cells.
+void state_to_host(wcfe_ctx *ctx) {
+  Batch &b = ctx->b;
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  std::vector<CellState> cs(b.ncell);
+  HIPCHECK(hipMemcpy(cs.data(), b.cs, sizeof(CellState) * b.ncell, hipMemcpyDeviceToHost));
+  for (int c = 0; c < b.ncell; ++c) {
+    HostCell &h = ctx->cells[c];
+    const CellParams &p = b.cp_h[c];
+    h.X.resize(2 * (size_t)h.n_nodes);
+    h.x.resize(2 * (size_t)h.n_nodes);
+    h.ea.resize(3 * (size_t)h.n_nodes);
+    h.cC.resize(h.n_nodes);
+    h.fau.resize(2 * (size_t)h.n_nodes);
+    HIPCHECK(hipMemcpy(h.X.data(), b.X + p.node_off, sizeof(double2) * h.n_nodes, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(h.x.data(), b.x + p.node_off, sizeof(double2) * h.n_nodes, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(h.ea.data(), b.ea + 3LL * p.node_off, sizeof(double) * 3 * h.n_nodes, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(h.cC.data(), b.cC + p.node_off, sizeof(double) * h.n_nodes, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(h.fau.data(), b.fau + p.node_off, sizeof(double2) * h.n_nodes, hipMemcpyDeviceToHost));
+    if (h.n_spr > 0) {
+      std::vector<double> st(h.n_spr), an(2 * (size_t)h.n_spr);
+      HIPCHECK(hipMemcpy(st.data(), b.sp_state + p.spr_off, sizeof(double) * h.n_spr, hipMemcpyDeviceToHost));
+      HIPCHECK(hipMemcpy(an.data(), b.sp_anchor + 2LL * p.spr_off, sizeof(double) * 2 * h.n_spr,
+                         hipMemcpyDeviceToHost));
+      h.spr_state.resize(h.n_spr);
+      for (int k = 0; k < h.n_spr; ++k) {
+        const int g = b.spr_pos[c][k] - p.spr_off;
+        h.spr_state[k] = st[g];
+        h.spr_anchor[2 * k] = an[2 * g];
+        h.spr_anchor[2 * k + 1] = an[2 * g + 1];
+      }
+    }
+    h.time = cs[c].time;
+    h.step = cs[c].step;
+    h.state = cs[c];
+    h.has_state = true;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int wcfe_add_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d, int32_t *cell_id) {
+  if (!ctx || !d) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (ctx->finalized) fail(WCFE_ERR_STATE, "cells must be added before wcfe_finalize");
+    ctx->cells.push_back(host_cell(ctx, d));
     if (cell_id) *cell_id = (int32_t)ctx->cells.size() - 1;
   });
 }
@@ -1909,6 +1967,167 @@ int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double
     int un = 0;
     HIPCHECK(hipMemcpy(&un, dun, sizeof(int), hipMemcpyDeviceToHost));
     if (n_unlocated) *n_unlocated = un;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+const char CKP_MAGIC[8] = {'W', 'C', 'F', 'E', 'C', 'K', 'P', '1'};
+struct CkpWriter {
+  FILE *f;
+  void raw(const void *p, size_t n) {
+    if (n && fwrite(p, 1, n, f) != n) fail(WCFE_ERR_ARG, "checkpoint write failed");
+  }
+  void i32(int v) { raw(&v, 4); }
+  void f64(double v) { raw(&v, 8); }
+  template <class T> void vec(const std::vector<T> &v, size_t n) {
+    if (v.size() == n) raw(v.data(), n * sizeof(T));
+    else {
+      std::vector<T> z(n, T(0));
+      raw(z.data(), n * sizeof(T));
+    }
+  }
+};
+struct CkpReader {
+  FILE *f;
+  void raw(void *p, size_t n) {
+    if (n && fread(p, 1, n, f) != n) fail(WCFE_ERR_ARG, "checkpoint file is truncated");
+  }
+  int i32() { int v; raw(&v, 4); return v; }
+  double f64() { double v; raw(&v, 8); return v; }
+  template <class T> void vec(std::vector<T> &v, long long n) {
+    if (n < 0 || n > (1LL << 34)) fail(WCFE_ERR_ARG, "checkpoint array size %lld out of range", n);
+    v.resize((size_t)n);
+    raw(v.data(), (size_t)n * sizeof(T));
+  }
+};
+}  // namespace
+
+extern "C" {
+
+int wcfe_replace_cell(wcfe_ctx *ctx, int32_t cell, const wcfe_cell_desc *d) {
+  if (!ctx || !d) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    HostCell h = host_cell(ctx, d);
+    state_to_host(ctx);  // the other cells keep their current state
+    h.time = ctx->cells[cell].time;
+    h.step = ctx->cells[cell].step;
+    ctx->cells[cell] = std::move(h);
+    rebuild_batch(ctx);
+    ctx->stepped = true;
+  });
+}
+
+
+
+int wcfe_checkpoint_save(wcfe_ctx *ctx, const char *path) {
+  if (!ctx || !path) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
+    state_to_host(ctx);
+    FILE *f = fopen(path, "wb");
+    if (!f) fail(WCFE_ERR_ARG, "cannot open %s for writing", path);
+    CkpWriter w{f};
+    try {
+      w.raw(CKP_MAGIC, 8);
+      w.i32(1);
+      w.i32((int)ctx->cells.size());
+      w.i32((int)ctx->tables.size());
+      for (auto &kv : ctx->tables) {
+        const int n = (int)kv.second.px.size();
+        w.i32(kv.first);
+        w.i32(n);
+        w.vec(kv.second.px, n);
+        w.vec(kv.second.py, n);
+        w.vec(kv.second.k, n);
+      }
+      for (const HostCell &h : ctx->cells) {
+        const size_t N = h.n_nodes, T = h.n_tris, S = h.n_spr;
+        for (int v : {h.n_nodes, h.n_tris, h.law, h.node_springs, h.spring_law, h.table_id, h.kinetics, h.n_spr,
+                      h.solver, h.step})
+          w.i32(v);
+        for (double v : {h.mu, h.E, h.nu, h.thick, h.rho_max, h.k_lin, h.time}) w.f64(v);
+        w.vec(h.X, 2 * N);
+        w.vec(h.x, 2 * N);
+        w.vec(h.ea, 3 * N);
+        w.vec(h.cC, N);
+        w.vec(h.fau, 2 * N);
+        w.vec(h.tris, 3 * T);
+        w.vec(h.spr_tri, S);
+        w.vec(h.spr_bary, 3 * S);
+        w.vec(h.spr_anchor, 2 * S);
+        w.vec(h.spr_state, S);
+        w.raw(&h.state, sizeof(CellState));  // Newton iterate state (mid-load-step cells resume exactly)
+      }
+    } catch (...) {
+      fclose(f);
+      throw;
+    }
+    if (fclose(f) != 0) fail(WCFE_ERR_ARG, "closing %s failed", path);
+  });
+}
+
+int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path) {
+  if (!ctx || !path) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (ctx->finalized || !ctx->cells.empty()) fail(WCFE_ERR_STATE, "load a checkpoint into an empty context");
+    FILE *f = fopen(path, "rb");
+    if (!f) fail(WCFE_ERR_ARG, "cannot open %s", path);
+    CkpReader r{f};
+    std::map<int, Table> tables;
+    std::vector<HostCell> cells;
+    try {
+      char magic[8];
+      r.raw(magic, 8);
+      if (memcmp(magic, CKP_MAGIC, 8) != 0) fail(WCFE_ERR_ARG, "%s is not a wcfe checkpoint", path);
+      if (r.i32() != 1) fail(WCFE_ERR_ARG, "unsupported checkpoint version");
+      const int ncell = r.i32(), ntab = r.i32();
+      if (ncell < 1 || ncell >= (1 << 11) || ntab < 0 || ntab > 1024) fail(WCFE_ERR_ARG, "bad checkpoint header");
+      for (int t = 0; t < ntab; ++t) {
+        const int id = r.i32(), n = r.i32();
+        if (n < 1 || n > MAX_TABLE) fail(WCFE_ERR_ARG, "bad table size in checkpoint");
+        Table tb;
+        r.vec(tb.px, n);
+        r.vec(tb.py, n);
+        r.vec(tb.k, n);
+        tables[id] = tb;
+      }
+      for (int c = 0; c < ncell; ++c) {
+        HostCell h;
+        int *iv[] = {&h.n_nodes, &h.n_tris, &h.law, &h.node_springs, &h.spring_law, &h.table_id, &h.kinetics,
+                     &h.n_spr, &h.solver, &h.step};
+        for (int *v : iv) *v = r.i32();
+        double *dv[] = {&h.mu, &h.E, &h.nu, &h.thick, &h.rho_max, &h.k_lin, &h.time};
+        for (double *v : dv) *v = r.f64();
+        if (h.n_nodes <= 0 || h.n_tris <= 0 || h.n_spr < 0) fail(WCFE_ERR_ARG, "bad cell sizes in checkpoint");
+        const long long N = h.n_nodes, T = h.n_tris, S = h.n_spr;
+        r.vec(h.X, 2 * N);
+        r.vec(h.x, 2 * N);
+        r.vec(h.ea, 3 * N);
+        r.vec(h.cC, N);
+        r.vec(h.fau, 2 * N);
+        r.vec(h.tris, 3 * T);
+        r.vec(h.spr_tri, S);
+        r.vec(h.spr_bary, 3 * S);
+        r.vec(h.spr_anchor, 2 * S);
+        r.vec(h.spr_state, S);
+        r.raw(&h.state, sizeof(CellState));
+        h.has_state = true;
+        for (int v : h.tris)
+          if (v < 0 || v >= h.n_nodes) fail(WCFE_ERR_ARG, "checkpoint triangle index out of range");
+        for (int t : h.spr_tri)
+          if (t < 0 || t >= h.n_tris) fail(WCFE_ERR_ARG, "checkpoint spring triangle out of range");
+        cells.push_back(std::move(h));
+      }
+    } catch (...) {
+      fclose(f);
+      throw;
+    }
+    fclose(f);
+    ctx->tables = tables;
+    ctx->cells = std::move(cells);
   });
 }
 
