@@ -7,7 +7,7 @@
 // in and out of the reference functions; it contains no mechanics of its own.
 //
 // Usage:  refharness_<variant> <mode> <case_dir>
-//   modes: elem | calcres | forcefa | updatefa | areatot
+//   modes: elem | calcres | forcefa | updatefa | areatot | bend | areacon | ap
 // Inputs are read from <case_dir>/<name>.bin (float64 or int64, row-major),
 // outputs written to <case_dir>/out_<name>.bin.  Scalars come from
 // <case_dir>/meta.txt ("key value" lines).
@@ -18,6 +18,9 @@
 //   calculateForce_FA                  UL:1198 (kMD) / UL1:1196 (k1, k31)
 //   updateFA (11-argument form)        UL:2026 (kMD) / UL1:2005 (k1, k31)
 //   calculateAreaTot                   UL:1662
+//   calculateBendingForce2D            UL:1571
+//   calculateAreaConstraintForce       UL:1691 (tot_area from calculateAreaTot at x)
+//   calculateForce_AP                  UL:1477 (its own srand(time(NULL)) / rand() draws)
 #include "vec.h"
 #include "nondestructivetrimesh.h"
 #include <Eigen/Dense>
@@ -47,6 +50,12 @@ void updateFA(std::vector<double> &c_A, std::vector<double> &c_C, std::vector<Ve
               const NonDestructiveTriMesh &mesh_cell, const std::vector<Vec3d> &X_t_subs,
               const std::vector<Vec3d> &x_t_subs, const NonDestructiveTriMesh &mesh_subs, double kint);
 Eigen::Vector4d calculateAreaTot(const std::vector<Vec3d> &x_t, const NonDestructiveTriMesh &mesh);
+void calculateForce_AP(Eigen::VectorXd &F_constraint, const std::vector<Vec3d> &X_t, const std::vector<Vec3d> &x_t,
+                       const std::vector<double> &c_A, const NonDestructiveTriMesh &mesh, double dt_real);
+void calculateBendingForce2D(Eigen::VectorXd &F_bending, const std::vector<Vec3d> &X_t, const std::vector<Vec3d> &x_t,
+                             const NonDestructiveTriMesh &mesh);
+void calculateAreaConstraintForce(Eigen::VectorXd &F_areaConstraint, const std::vector<Vec3d> &X_t,
+                                  const std::vector<Vec3d> &x_t, const NonDestructiveTriMesh &mesh, double tot_area);
 
 namespace {
 
@@ -108,6 +117,22 @@ int main(int argc, char **argv) {
   if (mode == "areatot") {
     Eigen::Vector4d a = calculateAreaTot(x, mesh);
     store("areatot", a.data(), 4);
+    return 0;
+  }
+  if (mode == "bend" || mode == "areacon" || mode == "ap") {
+    Eigen::VectorXd F(3 * n);
+    F.setZero();  // the caller zeroes the force vector (UL:561, :578, :585)
+    if (mode == "bend") {
+      calculateBendingForce2D(F, X, x, mesh);
+    } else if (mode == "areacon") {
+      Eigen::Vector4d a = calculateAreaTot(x, mesh);
+      calculateAreaConstraintForce(F, X, x, mesh, a(0));
+      store("areatot", a.data(), 4);
+    } else {
+      std::vector<double> cA(n, 100.0);
+      calculateForce_AP(F, X, x, cA, mesh, meta["dt"]);
+    }
+    store("F", F.data(), 3 * n);
     return 0;
   }
   if (mode == "elem" || mode == "calcres") {

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <vector>
 
 namespace {
@@ -472,6 +473,178 @@ extern "C" void orc_update_fa(int n_nodes, int n_tris, const int32_t *tris, cons
   if (n_unlocated) *n_unlocated = unloc;
 }
 
+// --------------------------------------------------------------------------
+// Boundary forces (UL:1477-1568 calculateForce_AP, UL:1571-1660
+// calculateBendingForce2D, UL:1691-1784 calculateAreaConstraintForce)
+// --------------------------------------------------------------------------
+extern "C" int orc_boundary_candidates(int n_nodes, int n_tris, const int32_t *tris, int32_t *is_bnd,
+                                       int32_t *cand_start, int32_t *cand) {
+  // El Topo's rule (nondestructivetrimesh.cpp:527-543): an edge with an odd
+  // number of incident triangles is a boundary edge, its ends are boundary
+  // vertices.  Triangles with t0 == t1 are deleted ones and skipped (:497).
+  std::map<std::pair<int, int>, int> edge_count;
+  for (int e = 0; e < n_tris; ++e) {
+    const int32_t *t = tris + 3 * e;
+    if (t[0] == t[1]) continue;
+    for (int j = 0; j < 3; ++j) {
+      int a = t[j], b = t[(j + 1) % 3];
+      edge_count[std::make_pair(std::min(a, b), std::max(a, b))]++;
+    }
+  }
+  for (int i = 0; i < n_nodes; ++i) is_bnd[i] = 0;
+  for (const auto &kv : edge_count)
+    if (kv.second % 2 == 1) is_bnd[kv.first.first] = is_bnd[kv.first.second] = 1;
+  // Candidates in the order the reference's loops visit them (UL:1497-1528):
+  // one-ring triangles by increasing index (the vertex-to-triangle map is
+  // filled in triangle order, nondestructivetrimesh.cpp:503-506), corners in
+  // slot order; nint is the first corner that is neither ni nor nj.
+  std::vector<int> st, li;
+  one_rings(n_nodes, n_tris, tris, st, li);
+  int m = 0;
+  for (int ni = 0; ni < n_nodes; ++ni) {
+    cand_start[ni] = m;
+    if (!is_bnd[ni]) continue;
+    for (int k = st[ni]; k < st[ni + 1]; ++k) {
+      const int32_t *t = tris + 3 * li[k];
+      if (t[0] == t[1]) continue;
+      for (int enj = 0; enj < 3; ++enj) {
+        int nj = t[enj];
+        if (!is_bnd[nj] || nj == ni) continue;
+        int nint = -1;
+        for (int enk = 0; enk < 3; ++enk)
+          if (t[enk] != nj && t[enk] != ni) { nint = t[enk]; break; }
+        cand[2 * m] = nj;
+        cand[2 * m + 1] = nint;
+        ++m;
+      }
+    }
+  }
+  cand_start[n_nodes] = m;
+  return m;
+}
+
+namespace {
+
+struct BFrame {
+  V3<double> xi, x1, x2;  // the vertex, the start and the end of its boundary curve segment
+  bool ok;
+};
+
+// UL:1497-1533: j1 is the neighbour with (x_i - x_j) x (x_int - x_i) . e3 > 0,
+// j2 any other; later candidates overwrite earlier ones.
+BFrame boundary_frame(const double *x, int ni, const int32_t *cs, const int32_t *cand) {
+  BFrame f;
+  bool h1 = false, h2 = false;
+  for (int c = 0; c < 3; ++c) f.xi.a[c] = x[3 * ni + c];
+  for (int k = cs[ni]; k < cs[ni + 1]; ++k) {
+    const int nj = cand[2 * k], nint = cand[2 * k + 1];
+    V3<double> a, b;
+    for (int c = 0; c < 3; ++c) {
+      a.a[c] = f.xi.a[c] - x[3 * nj + c];
+      b.a[c] = x[3 * nint + c] - f.xi.a[c];
+    }
+    if (cross(a, b).a[2] > 0) {
+      for (int c = 0; c < 3; ++c) f.x1.a[c] = x[3 * nj + c];
+      h1 = true;
+    } else {
+      for (int c = 0; c < 3; ++c) f.x2.a[c] = x[3 * nj + c];
+      h2 = true;
+    }
+  }
+  f.ok = h1 && h2;
+  return f;
+}
+
+// normal_i and the segment lengths of UL:1537-1545 (also :1634-1641, :1764-1771)
+void boundary_normal(const BFrame &f, double n[3], double *len1, double *len2) {
+  V3<double> d1, d2;
+  for (int c = 0; c < 3; ++c) {
+    d1.a[c] = f.xi.a[c] - f.x1.a[c];
+    d2.a[c] = f.x2.a[c] - f.xi.a[c];
+  }
+  // -e3 x v = (v_y, -v_x, 0)
+  V3<double> n1{{d1.a[1], -d1.a[0], 0.0}}, n2{{d2.a[1], -d2.a[0], 0.0}};
+  const double nn1 = norm(n1), nn2 = norm(n2);
+  for (int c = 0; c < 3; ++c) { n1.a[c] /= nn1; n2.a[c] /= nn2; }
+  const double l1 = norm(d1), l2 = norm(d2), L = l1 + l2;
+  for (int c = 0; c < 3; ++c) n[c] = l1 / L * n1.a[c] + l2 / L * n2.a[c];
+  *len1 = l1;
+  *len2 = l2;
+}
+
+struct BTable {
+  std::vector<int32_t> is_bnd, cs, cand;
+  BTable(int n_nodes, int n_tris, const int32_t *tris)
+      : is_bnd(n_nodes), cs(n_nodes + 1), cand(12 * (size_t)std::max(n_tris, 1)) {
+    orc_boundary_candidates(n_nodes, n_tris, tris, is_bnd.data(), cs.data(), cand.data());
+  }
+};
+
+}  // namespace
+
+extern "C" int orc_bending_force_2d(int n_nodes, int n_tris, const int32_t *tris, const double *x, double k_bend,
+                                    double *F) {
+  BTable bt(n_nodes, n_tris, tris);
+  int open = 0;
+  for (int ni = 0; ni < n_nodes; ++ni) {
+    if (!bt.is_bnd[ni]) continue;
+    BFrame f = boundary_frame(x, ni, bt.cs.data(), bt.cand.data());
+    if (!f.ok) { ++open; for (int c = 0; c < 3; ++c) F[3 * ni + c] = 0.0; continue; }
+    // UL:1609-1624 turning angle and curvature
+    V3<double> e1, e2;
+    for (int c = 0; c < 3; ++c) {
+      e1.a[c] = f.xi.a[c] - f.x1.a[c];
+      e2.a[c] = f.x2.a[c] - f.xi.a[c];
+    }
+    const double ne1 = norm(e1), ne2 = norm(e2);
+    for (int c = 0; c < 3; ++c) { e1.a[c] /= ne1; e2.a[c] /= ne2; }
+    const double costurn = e1.a[0] * e2.a[0] + e1.a[1] * e2.a[1] + e1.a[2] * e2.a[2];
+    const double turna = cross(e1, e2).a[2] > 0 ? costurn - 1 : 1 - costurn;
+    double n[3], l1, l2;
+    boundary_normal(f, n, &l1, &l2);
+    const double kappa = turna * 2 / (l1 + l2);
+    for (int c = 0; c < 3; ++c) F[3 * ni + c] = k_bend * kappa * n[c];  // UL:1644-1647
+  }
+  return open;
+}
+
+extern "C" int orc_area_constraint_force(int n_nodes, int n_tris, const int32_t *tris, const double *x,
+                                         double tot_area, double k_area, double area_0, double *F) {
+  BTable bt(n_nodes, n_tris, tris);
+  int open = 0;
+  for (int ni = 0; ni < n_nodes; ++ni) {
+    if (!bt.is_bnd[ni]) continue;
+    BFrame f = boundary_frame(x, ni, bt.cs.data(), bt.cand.data());
+    if (!f.ok) { ++open; for (int c = 0; c < 3; ++c) F[3 * ni + c] = 0.0; continue; }
+    double n[3], l1, l2;
+    boundary_normal(f, n, &l1, &l2);
+    const double L = l1 + l2;
+    for (int c = 0; c < 3; ++c) F[3 * ni + c] = -k_area * (tot_area - area_0) / 2 * L * n[c];  // UL:1776-1781
+  }
+  return open;
+}
+
+extern "C" int orc_force_ap(int n_nodes, int n_tris, const int32_t *tris, const double *x, double dt, double rate,
+                            double quantum, double density, const double *u, double *F) {
+  BTable bt(n_nodes, n_tris, tris);
+  int open = 0;
+  for (int ni = 0; ni < n_nodes; ++ni) {
+    if (!bt.is_bnd[ni]) continue;
+    BFrame f = boundary_frame(x, ni, bt.cs.data(), bt.cand.data());
+    if (!f.ok) { ++open; for (int c = 0; c < 3; ++c) F[3 * ni + c] = 0.0; continue; }
+    double n[3], l1, l2;
+    boundary_normal(f, n, &l1, &l2);
+    const double L = l1 + l2;
+    // UL:1549-1562: one draw per vertex; every one of the density L / 2 trials
+    // adds a quantum when the draw is below rate dt exp(-rate dt)
+    double factin = 0;
+    for (int ii = 0; ii < density * L / 2; ii++)
+      if (u[ni] < rate * dt * ::exp(-rate * dt)) factin += quantum;
+    for (int c = 0; c < 3; ++c) F[3 * ni + c] = factin * n[c];
+  }
+  return open;
+}
+
 // ==========================================================================
 // Part 2: planar Newton model
 // ==========================================================================
@@ -651,6 +824,126 @@ extern "C" void orc_element_system(const orc_cell *c, int e, const int32_t *spr_
   element_system_impl(c, e, sst, sli, Ge, Ke);
 }
 
+namespace {
+
+// j1 / j2 of boundary vertex ni by the rule of boundary_frame (indices)
+bool frame_indices(const double *x3, int ni, const int32_t *cs, const int32_t *cand, int *j1, int *j2) {
+  bool h1 = false, h2 = false;
+  for (int k = cs[ni]; k < cs[ni + 1]; ++k) {
+    const int nj = cand[2 * k], nint = cand[2 * k + 1];
+    V3<double> a, b;
+    for (int q = 0; q < 3; ++q) {
+      a.a[q] = x3[3 * ni + q] - x3[3 * nj + q];
+      b.a[q] = x3[3 * nint + q] - x3[3 * ni + q];
+    }
+    if (cross(a, b).a[2] > 0) { *j1 = nj; h1 = true; }
+    else { *j2 = nj; h2 = true; }
+  }
+  return h1 && h2;
+}
+
+std::vector<double> planar3(const double *x2, int n) {
+  std::vector<double> x3(3 * (size_t)n, 0.0);
+  for (int i = 0; i < n; ++i) { x3[3 * i] = x2[2 * i]; x3[3 * i + 1] = x2[2 * i + 1]; }
+  return x3;
+}
+
+// Node terms of the implicit residual and their tangent (rows of node i only,
+// plus the area term's rank-one coupling of boundary vertices).
+void add_node_terms(const orc_cell *c, double *G, double *K) {
+  const int N = c->n_nodes, n = 2 * N;
+  const bool bnd = c->k_bend != 0 || c->k_area != 0 || c->ap_u;
+  if (c->drag != 0) {
+    const double cd = c->drag / c->dt;
+    for (int i = 0; i < n; ++i) {
+      G[i] += cd * (c->x[i] - c->X[i]);
+      if (K) K[(size_t)i * n + i] += cd;
+    }
+  }
+  if (!bnd) return;
+  std::vector<double> x3 = planar3(c->x, N), X3 = planar3(c->X, N), F(3 * (size_t)N, 0.0);
+  double A = 0;
+  if (c->k_bend != 0) {
+    orc_bending_force_2d(N, c->n_tris, c->tris, x3.data(), c->k_bend, F.data());
+    for (int i = 0; i < N; ++i) for (int q = 0; q < 2; ++q) G[2 * i + q] -= F[3 * i + q];
+  }
+  if (c->k_area != 0) {
+    double a4[4];
+    orc_area_tot(c->n_tris, c->tris, x3.data(), a4);
+    A = a4[0];
+    std::fill(F.begin(), F.end(), 0.0);
+    orc_area_constraint_force(N, c->n_tris, c->tris, x3.data(), A, c->k_area, c->area_0, F.data());
+    for (int i = 0; i < N; ++i) for (int q = 0; q < 2; ++q) G[2 * i + q] -= F[3 * i + q];
+  }
+  if (c->ap_u) {
+    std::fill(F.begin(), F.end(), 0.0);
+    orc_force_ap(N, c->n_tris, c->tris, X3.data(), c->dt, c->ap_rate, c->ap_quantum, c->ap_density, c->ap_u,
+                 F.data());
+    for (int i = 0; i < N; ++i) for (int q = 0; q < 2; ++q) G[2 * i + q] -= F[3 * i + q];
+  }
+  if (!K || (c->k_bend == 0 && c->k_area == 0)) return;
+  BTable bt(N, c->n_tris, c->tris);
+  std::vector<int> J1(N, -1), J2(N, -1);
+  std::vector<double> g(2 * (size_t)N, 0.0);  // dA/dx at boundary vertices
+  for (int i = 0; i < N; ++i) {
+    if (!bt.is_bnd[i]) continue;
+    int j1 = -1, j2 = -1;
+    if (!frame_indices(x3.data(), i, bt.cs.data(), bt.cand.data(), &j1, &j2)) continue;
+    J1[i] = j1;
+    J2[i] = j2;
+    g[2 * i] = 0.5 * (c->x[2 * j2 + 1] - c->x[2 * j1 + 1]);
+    g[2 * i + 1] = -0.5 * (c->x[2 * j2] - c->x[2 * j1]);
+  }
+  // -dF/dx added to K; R = [[0, 1], [-1, 0]]
+  const double R[2][2] = {{0, 1}, {-1, 0}};
+  for (int i = 0; i < N; ++i) {
+    if (J1[i] < 0) continue;
+    const int j1 = J1[i], j2 = J2[i];
+    const int nodes[3] = {i, j1, j2};
+    const double dsm[3] = {0, -1, 1};
+    double Rs[2] = {c->x[2 * j2 + 1] - c->x[2 * j1 + 1], -(c->x[2 * j2] - c->x[2 * j1])};
+    double dF[3][2][2] = {};
+    if (c->k_bend != 0) {
+      const double u[2] = {c->x[2 * i] - c->x[2 * j1], c->x[2 * i + 1] - c->x[2 * j1 + 1]};
+      const double w[2] = {c->x[2 * j2] - c->x[2 * i], c->x[2 * j2 + 1] - c->x[2 * i + 1]};
+      const double lu = ::sqrt(u[0] * u[0] + u[1] * u[1]), lw = ::sqrt(w[0] * w[0] + w[1] * w[1]);
+      const double e1[2] = {u[0] / lu, u[1] / lu}, e2[2] = {w[0] / lw, w[1] / lw};
+      const double cth = e1[0] * e2[0] + e1[1] * e2[1];
+      const double sg = (e1[0] * e2[1] - e1[1] * e2[0]) > 0 ? 1.0 : -1.0;
+      const double t = sg * (cth - 1), L = lu + lw;
+      // d cos / du, d cos / dw
+      const double cu[2] = {(e2[0] - cth * e1[0]) / lu, (e2[1] - cth * e1[1]) / lu};
+      const double cw[2] = {(e1[0] - cth * e2[0]) / lw, (e1[1] - cth * e2[1]) / lw};
+      const double dc[3][2] = {{cu[0] - cw[0], cu[1] - cw[1]}, {-cu[0], -cu[1]}, {cw[0], cw[1]}};
+      const double dL[3][2] = {{e1[0] - e2[0], e1[1] - e2[1]}, {-e1[0], -e1[1]}, {e2[0], e2[1]}};
+      const double k2 = 2 * c->k_bend;
+      for (int m = 0; m < 3; ++m)
+        for (int r = 0; r < 2; ++r)
+          for (int q = 0; q < 2; ++q)
+            dF[m][r][q] += k2 * (Rs[r] * (sg * dc[m][q] / (L * L) - 2 * t * dL[m][q] / (L * L * L)) +
+                                 t / (L * L) * dsm[m] * R[r][q]);
+    }
+    if (c->k_area != 0) {
+      const double h = -0.5 * c->k_area * (A - c->area_0);
+      for (int m = 0; m < 3; ++m)
+        for (int r = 0; r < 2; ++r)
+          for (int q = 0; q < 2; ++q) dF[m][r][q] += h * dsm[m] * R[r][q];
+      // rank one: dF_i/dx_m = -k g_i g_m^T over boundary vertices m
+      for (int m = 0; m < N; ++m) {
+        if (J1[m] < 0) continue;
+        for (int r = 0; r < 2; ++r)
+          for (int q = 0; q < 2; ++q)
+            K[(size_t)(2 * i + r) * n + 2 * m + q] += c->k_area * g[2 * i + r] * g[2 * m + q];
+      }
+    }
+    for (int m = 0; m < 3; ++m)
+      for (int r = 0; r < 2; ++r)
+        for (int q = 0; q < 2; ++q) K[(size_t)(2 * i + r) * n + 2 * nodes[m] + q] -= dF[m][r][q];
+  }
+}
+
+}  // namespace
+
 extern "C" void orc_assemble(const orc_cell *c, const int32_t *color, int n_colors, double *G, double *K) {
   const int n = 2 * c->n_nodes;
   memset(G, 0, sizeof(double) * n);
@@ -672,6 +965,7 @@ extern "C" void orc_assemble(const orc_cell *c, const int32_t *color, int n_colo
       }
     }
   }
+  add_node_terms(c, G, K);
 }
 
 template <class T> static int lu_factor_t(int n, T *A, int32_t *ipiv, int pivoting) {

@@ -49,6 +49,27 @@ void orc_update_fa(int n_nodes, int n_tris, const int32_t *tris, const double *X
                    double *cC, double *fau, double dt, int n_snodes, int n_stris, const int32_t *stris,
                    const double *SX, const double *Sx, int law, double kint, int *n_unlocated);
 
+/* Boundary vertices and the neighbour candidates the reference's boundary
+ * loops visit (UL:1497-1528 and its copies at :1594-1607, :1714-1746), by El
+ * Topo's rule (odd edge valence, nondestructivetrimesh.cpp:527-543).
+ * is_bnd: N; cand_start: N+1; cand: 2 ints (nj, nint) per candidate,
+ * capacity 12 T.  Returns the number of candidates. */
+int orc_boundary_candidates(int n_nodes, int n_tris, const int32_t *tris, int32_t *is_bnd, int32_t *cand_start,
+                            int32_t *cand);
+/* UL:1571-1660 calculateBendingForce2D (k_bend: reference 20 pN/um).  x, F:
+ * 3N; only boundary entries of F are written.  Returns the number of boundary
+ * vertices missing a curve neighbour (their force is set to 0; the reference
+ * reads an uninitialised vector there). */
+int orc_bending_force_2d(int n_nodes, int n_tris, const int32_t *tris, const double *x, double k_bend, double *F);
+/* UL:1691-1784 calculateAreaConstraintForce (k_area 1, area_0 10*10*3.1416). */
+int orc_area_constraint_force(int n_nodes, int n_tris, const int32_t *tris, const double *x, double tot_area,
+                              double k_area, double area_0, double *F);
+/* UL:1477-1568 calculateForce_AP with the per-vertex draw u[N] in [0,1)
+ * supplied (the reference: rand()/(RAND_MAX+1.)); rate 10 1/s, quantum 5 pN,
+ * density 100 per um in the reference. */
+int orc_force_ap(int n_nodes, int n_tris, const int32_t *tris, const double *x, double dt, double rate,
+                 double quantum, double density, const double *u, double *F);
+
 /* ---------------- Part 2: planar Newton model (2 DOF per node) ----------- */
 
 enum { ORC_LAW_UL = 0, ORC_LAW_CST = 1 };
@@ -78,6 +99,13 @@ typedef struct orc_cell {
   double k_lin;
   int32_t tab_n;
   const double *tab_px, *tab_py, *tab_k;
+  /* boundary and body forces of the reference cell model, implicit form
+   * (UL law): G += drag (x - X)/dt (UL:568-574 with v = (x - X)/dt)
+   *             - F_bend(x) (UL:1571-1660) - F_area(x) (UL:1691-1784, area at x)
+   *             - F_AP(X)  (UL:1477-1568 at the start-of-step positions, draws ap_u) */
+  double k_bend, k_area, area_0, drag, dt;
+  const double *ap_u;    /* N draws in (0,1), NULL = no actin polymerisation force */
+  double ap_rate, ap_quantum, ap_density;
 } orc_cell;
 
 /* Spring force magnitude f(D) and slope f'(D) (UL:1258-1264 table form). */
@@ -87,9 +115,13 @@ void orc_spring_law(const orc_cell *c, double D, double *f, double *df);
  * number of colours (or -1 if more than 64 are needed). */
 int orc_color_elements(int n_nodes, int n_tris, const int32_t *tris, int32_t *color);
 
-/* Equilibrium residual G(x) = R_int(x) - F_spring(x) (2N) and, if K != NULL,
- * the dense tangent dG/dx (2N x 2N row-major), assembled colour by colour in
- * element-index order within a colour (color may be NULL: element order). */
+/* Equilibrium residual G(x) = R_int(x) - F_spring(x) - boundary/body terms
+ * (2N) and, if K != NULL, the dense tangent dG/dx (2N x 2N row-major),
+ * assembled colour by colour in element-index order within a colour (color
+ * may be NULL: element order), then the node terms.  The area term's
+ * rank-one part uses dA/dx_m = R (x_j2 - x_j1) / 2 at boundary vertices
+ * (R v = (v_y, -v_x)), the exact area gradient when every boundary vertex has
+ * two boundary neighbours. */
 void orc_assemble(const orc_cell *c, const int32_t *color, int n_colors, double *G, double *K);
 
 /* Element-level pieces for parity tests: G_e[6], K_e[36] (row-major) of one

@@ -8,7 +8,7 @@ small compressed .npz fixtures next to this script.  The fixtures are data
 files, seeded inputs, and the reference's outputs); nothing of the reference's
 source travels.
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [--only boundary]
 """
 import os
 import shutil
@@ -65,6 +65,47 @@ def deform(X, rng, amp):
     return x
 
 
+def boundary_cases(disk296, t296, disk483, t483):
+    """calculateBendingForce2D, calculateAreaConstraintForce, calculateForce_AP.
+
+    calculateForce_AP seeds rand() from the clock, so its fixture records one
+    realisation: every boundary vertex's force is either 0 or the fired value.
+    dt = 0.1 puts the firing probability at rate dt exp(-rate dt) = 1/e.
+    """
+    rng = np.random.default_rng(20250917)
+    for name, X, T in [("bound_disk296", disk296, t296), ("bound_disk483", disk483, t483)]:
+        x = deform(X, rng, 0.02)
+        inp = dict(X=X, x=x, tris=T)
+        ob = run("kMD", "bend", inp, {})
+        oa = run("kMD", "areacon", inp, {})
+        op = run("kMD", "ap", inp, dict(dt=0.1))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **inp, ap_dt=0.1, F_bend=ob["F"],
+                            F_area=oa["F"], areatot=oa["areatot"], F_ap=op["F"])
+        print("wrote", name)
+    # notched disk: two passes that drop every 5th triangle with a boundary
+    # edge, so that some boundary vertices see up to 6 boundary neighbours
+    # (the reference keeps the last one it meets)
+    X, T = disk296, t296
+    for r in range(2):
+        edges = {}
+        for t in T:
+            for j in range(3):
+                k = tuple(sorted((int(t[j]), int(t[(j + 1) % 3]))))
+                edges[k] = edges.get(k, 0) + 1
+        bt = [e for e, t in enumerate(T)
+              if any(edges[tuple(sorted((int(t[j]), int(t[(j + 1) % 3]))))] == 1 for j in range(3))]
+        keep = np.ones(len(T), bool)
+        keep[bt[r::5]] = False
+        T = T[keep]
+    x = deform(X, rng, 0.02)
+    inp = dict(X=X, x=x, tris=T)
+    ob = run("kMD", "bend", inp, {})
+    oa = run("kMD", "areacon", inp, {})
+    np.savez_compressed(os.path.join(HERE, "bound_disk296_notch.npz"), **inp, F_bend=ob["F"], F_area=oa["F"],
+                        areatot=oa["areatot"])
+    print("wrote bound_disk296_notch")
+
+
 def main():
     if not os.path.exists(HARNESS.format("kMD")):
         sys.exit("build the reference harness first: make -C oracle ref")
@@ -72,6 +113,9 @@ def main():
     disk296, t296 = read_ecomsol(os.path.join(SCRIPTS, "disk_20um_diam_edited.txt"))
     disk483, t483 = read_ecomsol(os.path.join(SCRIPTS, "disk_20um_diam_N1025.txt"))
     subs, tsubs = read_ecomsol(os.path.join(SCRIPTS, "sq_subs_100um.txt"))
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "boundary":
+        boundary_cases(disk296, t296, disk483, t483)
+        return
 
     # --- element residual / calculateRes (law identical in all variants) ---
     for name, X, T, time, ea3d in [("ul_disk296_t1p3", disk296, t296, 1.3, False),
@@ -146,6 +190,7 @@ def main():
     # config-1 template mesh (483 nodes / 905 triangles) for the bench and tests
     np.savez_compressed(os.path.join(HERE, "mesh_disk483.npz"), X=disk483[:, :2], tris=t483.astype(np.int32))
     print("wrote mesh_disk483")
+    boundary_cases(disk296, t296, disk483, t483)
 
 
 if __name__ == "__main__":

@@ -54,6 +54,9 @@ class Cell(C.Structure):
         ("n_spr", C.c_int32), ("spr_tri", I32), ("spr_bary", D), ("spr_anchor", D), ("spr_state", D),
         ("spring_law", C.c_int32), ("k_lin", C.c_double),
         ("tab_n", C.c_int32), ("tab_px", D), ("tab_py", D), ("tab_k", D),
+        ("k_bend", C.c_double), ("k_area", C.c_double), ("area_0", C.c_double), ("drag", C.c_double),
+        ("dt", C.c_double), ("ap_u", D), ("ap_rate", C.c_double), ("ap_quantum", C.c_double),
+        ("ap_density", C.c_double),
     ]
 
 
@@ -103,6 +106,15 @@ def make_cell(model):
     px = arr("tab_px", f64)
     c.tab_n = 0 if px is None else len(px)
     c.tab_px, c.tab_py, c.tab_k = dp(px), dp(arr("tab_py", f64)), dp(arr("tab_k", f64))
+    c.k_bend = model.get("k_bend", 0.0)
+    c.k_area = model.get("k_area", 0.0)
+    c.area_0 = model.get("area_0", 10 * 10 * 3.1416)
+    c.drag = model.get("drag", 0.0)
+    c.dt = model.get("dt", 0.01)
+    c.ap_u = dp(arr("ap_u", f64))
+    c.ap_rate = model.get("ap_rate", 10.0)
+    c.ap_quantum = model.get("ap_quantum", 5.0)
+    c.ap_density = model.get("ap_density", 100.0)
     c._keep = keep
     return c
 
@@ -147,6 +159,40 @@ def update_fa(tris, X, x, cA, cC, fau, dt, stris, SX, Sx, law, kint):
                         dp(fau), C.c_double(dt), C.c_int(len(SX)), C.c_int(len(stris)), ip(i32(stris)),
                         dp(f64(SX)), dp(f64(Sx)), C.c_int(law), C.c_double(kint), C.byref(unloc))
     return cA, cC, fau, unloc.value
+
+
+def boundary_candidates(n_nodes, tris):
+    tris = i32(tris).reshape(-1)
+    T = len(tris) // 3
+    isb = np.zeros(n_nodes, dtype=np.int32)
+    st = np.zeros(n_nodes + 1, dtype=np.int32)
+    cand = np.zeros(12 * max(T, 1), dtype=np.int32)
+    m = lib().orc_boundary_candidates(C.c_int(n_nodes), C.c_int(T), ip(tris), ip(isb), ip(st), ip(cand))
+    return isb, st, cand[:2 * m].reshape(-1, 2)
+
+
+def bending_force_2d(tris, x, k_bend=20.0):
+    n = len(x)
+    F = np.zeros(3 * n)
+    nopen = lib().orc_bending_force_2d(C.c_int(n), C.c_int(len(tris)), ip(i32(tris)), dp(f64(x)),
+                                       C.c_double(k_bend), dp(F))
+    return F, nopen
+
+
+def area_constraint_force(tris, x, tot_area, k_area=1.0, area_0=10 * 10 * 3.1416):
+    n = len(x)
+    F = np.zeros(3 * n)
+    nopen = lib().orc_area_constraint_force(C.c_int(n), C.c_int(len(tris)), ip(i32(tris)), dp(f64(x)),
+                                            C.c_double(tot_area), C.c_double(k_area), C.c_double(area_0), dp(F))
+    return F, nopen
+
+
+def force_ap(tris, x, dt, u, rate=10.0, quantum=5.0, density=100.0):
+    n = len(x)
+    F = np.zeros(3 * n)
+    nopen = lib().orc_force_ap(C.c_int(n), C.c_int(len(tris)), ip(i32(tris)), dp(f64(x)), C.c_double(dt),
+                               C.c_double(rate), C.c_double(quantum), C.c_double(density), dp(f64(u)), dp(F))
+    return F, nopen
 
 
 # ---------------- Part 2 wrappers ----------------
