@@ -113,6 +113,20 @@ typedef struct wcfe_cell_desc {
   int32_t table_id;       /* table law: id given to wcfe_set_table */
   int32_t kinetics;       /* wcfe_kinetics */
   int32_t solver;         /* wcfe_solver */
+  /* boundary and body forces of the reference cell model (UL law), in the
+   * implicit residual G(x) = ... + drag (x - X)/dt - F_bend(x) - F_area(x) - F_AP(X):
+   * bending and area terms at the current positions with their exact tangent,
+   * actin polymerisation at the start-of-step positions X with one Philox draw
+   * per boundary vertex and load step (key wcfe_options.seed_lo/hi, counter
+   * (cell, vertex, step, 1)).  Zero = off. */
+  double k_bend;          /* calculateBendingForce2D (UL:1571-1660), reference 20 pN/um */
+  double k_area;          /* calculateAreaConstraintForce (UL:1691-1784), reference 1 pN/um^3 */
+  double area_0;          /* target area [um^2], reference 10*10*3.1416 (UL:1777) */
+  double drag;            /* viscous drag (UL:568-574), reference 0.001 pN s/um; velocity (x - X)/dt */
+  int32_t ap;             /* 1: calculateForce_AP (UL:1477-1568) */
+  double ap_rate;         /* [1/s], reference 10 (UL:1552) */
+  double ap_quantum;      /* [pN] per firing trial, reference 5 (UL:1561) */
+  double ap_density;      /* trials per um of boundary (x L/2), reference 100 (UL:1555) */
 } wcfe_cell_desc;
 
 typedef struct wcfe_step_stats {
@@ -259,6 +273,31 @@ int wcfe_force_fa(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t 
 int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double *x, double *cA, double *cC,
                    double *fau, double dt, int32_t n_snodes, int32_t n_stris, const int32_t *stris, const double *SX,
                    const double *Sx, int32_t law, double kint, int32_t *n_unlocated);
+/* calculateAreaTot (UL:1662-1688): out[4] = {area, centroid x, y, z}; x: 3N. */
+int wcfe_area_tot(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
+                  double *out);
+/* Boundary forces, one GPU thread per boundary vertex (El Topo's boundary
+ * rule, nondestructivetrimesh.cpp:527-543; the reference's neighbour search
+ * and "last candidate wins" rule, UL:1497-1533).  x, F: 3N; F is overwritten
+ * (0 at interior vertices).  n_open (optional) receives the number of boundary
+ * vertices without both curve neighbours: their force is 0 (the reference
+ * reads an uninitialised vector there).
+ * calculateBendingForce2D (UL:1571-1660); reference k_bend = 20 pN/um. */
+int wcfe_bending_force_2d(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
+                          double k_bend, double *F, int32_t *n_open);
+/* calculateAreaConstraintForce (UL:1691-1784) with the caller's tot_area
+ * (calculateAreaTot at x, UL:527); reference k_area = 1, area_0 = 10*10*3.1416. */
+int wcfe_area_constraint_force(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
+                               double tot_area, double k_area, double area_0, double *F, int32_t *n_open);
+/* calculateForce_AP (UL:1477-1568): per boundary vertex one draw u in [0,1)
+ * (u: N values, the reference's rand()/(RAND_MAX+1.)); the force is
+ * quantum * ceil(density L / 2) along the vertex normal when
+ * u < rate dt exp(-rate dt), else 0.  u == NULL draws from the context's
+ * Philox key with counter (draw, vertex, 0, 2).  Reference: rate 10 1/s,
+ * quantum 5 pN, density 100 per um. */
+int wcfe_force_ap(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x, double dt,
+                  double rate, double quantum, double density, const double *u, uint32_t draw, double *F,
+                  int32_t *n_open);
 
 #ifdef __cplusplus
 }

@@ -2,7 +2,7 @@
 //
 // Drop-in for the hot-path call sites of wholeCellFE's advance_sim
 // (wholeCellFE_kMD/cellBP/main.cpp:356-363) and for updateForces_Explicit's
-// inner calls (UL_growth_explicit.cpp:543, :562).  Each adapter has the
+// inner calls (UL_growth_explicit.cpp:527, :543, :562, :565, :582, :587).  Each adapter has the
 // signature and argument meaning of the reference function it replaces and
 // forwards to the C ABI in wcfe.h (GPU kernels for gfx950); El Topo meshes,
 // VTK output and everything else stay in the host application.
@@ -144,6 +144,74 @@ void updateFA(std::vector<double> &c_A, std::vector<double> &c_C, std::vector<Ve
                        force_law(), kint, &unlocated));
   for (int i = 0; i < n; ++i)
     for (int c = 0; c < 3; ++c) fa_u[i][c] = fau[3 * i + c];
+}
+
+// calculateAreaTot (UL_growth_explicit.cpp:1662): {area, centroid} as the
+// caller's 4-vector type, e.g. calculateAreaTot<Eigen::Vector4d>(x_t, mesh).
+template <class Vec4, class Vec3, class Mesh> Vec4 calculateAreaTot(const std::vector<Vec3> &x_t, const Mesh &mesh) {
+  std::vector<double> x = flat3(x_t);
+  std::vector<int32_t> tris = flat_tris(mesh);
+  double out[4];
+  check(wcfe_area_tot(context(), (int32_t)x_t.size(), (int32_t)(tris.size() / 3), tris.data(), x.data(), out));
+  Vec4 v;
+  for (int k = 0; k < 4; ++k) v(k) = out[k];
+  return v;
+}
+
+// Boundary vertices whose neighbour search found only one side of the curve
+// (the reference reads an uninitialised vector there; the adapters apply no
+// force and count them here).
+inline int &open_boundary_vertices() {
+  thread_local int n = 0;
+  return n;
+}
+
+// calculateBendingForce2D (UL_growth_explicit.cpp:1571), k_bend = 20 pN/um
+// (UL:1653).  The boundary is found from the triangles (El Topo's rule).
+template <class VecX, class Vec3, class Mesh>
+void calculateBendingForce2D(VecX &F_bending, const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t,
+                             const Mesh &mesh) {
+  (void)X_t;
+  const int n = (int)x_t.size();
+  std::vector<double> x = flat3(x_t), F(3 * (size_t)n);
+  std::vector<int32_t> tris = flat_tris(mesh);
+  check(wcfe_bending_force_2d(context(), n, (int32_t)(tris.size() / 3), tris.data(), x.data(), 20.0, F.data(),
+                              &open_boundary_vertices()));
+  F_bending.resize(3 * n);
+  for (int i = 0; i < 3 * n; ++i) F_bending(i) = F[i];
+}
+
+// calculateAreaConstraintForce (UL_growth_explicit.cpp:1691), k_area = 1,
+// area_0 = 10*10*3.1416 (UL:1776-1777).
+template <class VecX, class Vec3, class Mesh>
+void calculateAreaConstraintForce(VecX &F_areaConstraint, const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t,
+                                  const Mesh &mesh, double tot_area) {
+  (void)X_t;
+  const int n = (int)x_t.size();
+  std::vector<double> x = flat3(x_t), F(3 * (size_t)n);
+  std::vector<int32_t> tris = flat_tris(mesh);
+  check(wcfe_area_constraint_force(context(), n, (int32_t)(tris.size() / 3), tris.data(), x.data(), tot_area, 1.0,
+                                   10 * 10 * 3.1416, F.data(), &open_boundary_vertices()));
+  F_areaConstraint.resize(3 * n);
+  for (int i = 0; i < 3 * n; ++i) F_areaConstraint(i) = F[i];
+}
+
+// calculateForce_AP (UL_growth_explicit.cpp:1477): rate 10 1/s, 5 pN per
+// trial, 100 trials per um (UL:1552-1561).  The reference draws from rand()
+// reseeded with the clock on every call; the adapter draws from the
+// library's Philox stream, a new draw index per call.
+template <class VecX, class Vec3, class Mesh>
+void calculateForce_AP(VecX &F_constraint, const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t,
+                       const std::vector<double> &c_A, const Mesh &mesh, double dt_real) {
+  (void)X_t; (void)c_A;
+  thread_local uint32_t draw = 0;
+  const int n = (int)x_t.size();
+  std::vector<double> x = flat3(x_t), F(3 * (size_t)n);
+  std::vector<int32_t> tris = flat_tris(mesh);
+  check(wcfe_force_ap(context(), n, (int32_t)(tris.size() / 3), tris.data(), x.data(), dt_real, 10.0, 5.0, 100.0,
+                      nullptr, draw++, F.data(), &open_boundary_vertices()));
+  F_constraint.resize(3 * n);
+  for (int i = 0; i < 3 * n; ++i) F_constraint(i) = F[i];
 }
 
 }  // namespace wcfe_compat

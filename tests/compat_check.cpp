@@ -6,7 +6,7 @@
 // NonDestructiveTriMesh num_triangles/get_triangle) keep this file free of
 // the reference's headers.
 //
-// Usage: compat_check <case_dir> <mode>   (mode: calcres | forcefa | updatefa)
+// Usage: compat_check <case_dir> <mode>   (mode: calcres | forcefa | updatefa | boundary)
 // Reads <name>.bin arrays written by tests/test_gpu_compat.py and writes
 // out_<name>.bin results.
 #include <array>
@@ -27,6 +27,10 @@ struct VecX {
   void resize(size_t n) { d.resize(n); }
   double &operator()(size_t i) { return d[i]; }
   const double &operator()(size_t i) const { return d[i]; }
+};
+struct Vec4 {
+  double v[4];
+  double &operator()(int i) { return v[i]; }
 };
 struct Mesh {
   std::vector<std::array<size_t, 3>> tris;
@@ -80,6 +84,18 @@ int main(int argc, char **argv) {
       store("Res", Res.d);
       store("Res_ea", Rea.d);
       store("MM", MM.d);
+    } else if (mode == "boundary") {
+      // updateForces_Explicit's boundary calls (UL:527, :565, :582, :587)
+      Vec4 area = wcfe_compat::calculateAreaTot<Vec4>(x, m);
+      VecX Fb, Fa, Fp;
+      wcfe_compat::calculateBendingForce2D(Fb, X, x, m);
+      wcfe_compat::calculateAreaConstraintForce(Fa, X, x, m, area(0));
+      std::vector<double> cA(X.size(), 100.0);
+      wcfe_compat::calculateForce_AP(Fp, X, x, cA, m, meta[1]);
+      store("areatot", std::vector<double>(area.v, area.v + 4));
+      store("F_bend", Fb.d);
+      store("F_area", Fa.d);
+      store("F_ap", Fp.d);
     } else if (mode == "forcefa") {
       std::vector<double> cC = load<double>("cC");
       std::vector<Vec3> fau = vec3(load<double>("fau"));

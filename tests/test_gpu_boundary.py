@@ -1,0 +1,266 @@
+"""GPU parity of the boundary and body forces of the reference cell model.
+
+Reference (wholeCellFE_kMD/cellBP/UL_growth_explicit.cpp): calculateForce_AP
+:1477-1568, calculateBendingForce2D :1571-1660, calculateAreaTot :1662-1688,
+calculateAreaConstraintForce :1691-1784, viscous drag :568-574.
+
+* Reference-shaped calls (3 components per node) against the compiled
+  reference's golden vectors and the oracle: 1e-13 of the array's largest
+  entry (same operation order, FP64, no contraction).
+* Implicit node terms (planar Newton residual and tangent, FP64 kernels)
+  against the oracle's planar model: 1e-10 relative; the mixed form stores K
+  in FP32: 1e-6.  Converged positions: 1e-8 of the displacement.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+W = pytest.importorskip("wholecellfe_amd")
+
+CASES = ["bound_disk296", "bound_disk483", "bound_disk296_notch"]
+SEED = (5, 0)  # wcfe_default_options Philox key
+
+
+def relerr(a, b):
+    a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def philox_u(ctr, key=SEED):
+    return (O.philox(ctr, key)[0] + 0.5) / 4294967296.0
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = W.Context()
+    yield c
+    c.close()
+
+
+# ---------------------------------------------------------------------------
+# reference-shaped calls
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", CASES)
+def test_area_tot_bending_area_vs_reference(ctx, case):
+    print("case", case, flush=True)
+    z = golden(case)
+    T, x = z["tris"], z["x"]
+    A = ctx.area_tot(T, x)
+    assert relerr(A, z["areatot"]) < 1e-13
+    Fb, nopen = ctx.bending_force_2d(T, x, 20.0)
+    assert nopen == 0
+    assert relerr(Fb, z["F_bend"]) < 1e-13
+    Fa, _ = ctx.area_constraint_force(T, x, z["areatot"][0], 1.0, W.AREA_0_REF)
+    assert relerr(Fa, z["F_area"]) < 1e-13
+
+
+@pytest.mark.parametrize("case", CASES[:2])
+def test_force_ap_vs_reference(ctx, case):
+    # the fixture is one clock-seeded realisation: every fired vertex must
+    # equal the always-firing evaluation, every other vertex 0
+    print("case", case, flush=True)
+    z = golden(case)
+    T, x, dt = z["tris"], z["x"], float(z["ap_dt"])
+    n = len(x)
+    fired, _ = ctx.force_ap(T, x, dt, u=np.zeros(n))
+    idle, _ = ctx.force_ap(T, x, dt, u=np.full(n, 0.999999))
+    g = z["F_ap"].reshape(-1, 3)
+    on = np.abs(g).max(axis=1) > 0
+    assert 0 < on.sum()
+    assert relerr(fired.reshape(-1, 3)[on], g[on]) < 1e-13
+    assert not np.any(idle)
+    # arbitrary draws against the oracle
+    u = np.random.default_rng(4).uniform(size=n)
+    Fg, _ = ctx.force_ap(T, x, dt, u=u)
+    Fo, _ = O.force_ap(T, x, dt, u)
+    assert np.array_equal(Fg == 0, Fo == 0)
+    assert relerr(Fg, Fo) < 1e-13
+
+
+def test_force_ap_philox_draws(ctx):
+    z = golden("bound_disk483")
+    T, x = z["tris"], z["x"]
+    n = len(x)
+    draw, dt = 7, 0.1
+    Fg, _ = ctx.force_ap(T, x, dt, u=None, draw=draw)
+    u = np.array([philox_u((draw, i, 0, 2)) for i in range(n)])
+    Fo, _ = O.force_ap(T, x, dt, u)
+    assert np.count_nonzero(Fo) > 0
+    assert np.array_equal(Fg == 0, Fo == 0)
+    assert relerr(Fg, Fo) < 1e-13
+
+
+def test_open_boundary_vertices_are_counted(ctx):
+    # a lone triangle: every vertex is a boundary vertex whose two neighbours
+    # share one triangle; both are found, so nothing is open.  A vertex whose
+    # neighbour search finds only one side is reported and gets no force.
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float)
+    F, nopen = ctx.bending_force_2d(np.array([[0, 1, 2]]), X, 20.0)
+    Fo, no = O.bending_force_2d(np.array([[0, 1, 2]]), X, 20.0)
+    assert nopen == no
+    assert relerr(F, Fo) < 1e-13 if np.any(Fo) else not np.any(F)
+
+
+# ---------------------------------------------------------------------------
+# implicit node terms
+# ---------------------------------------------------------------------------
+def bterm_cell(case="bound_disk296", deform=0.02, seed=3, dt=0.1, ap=True, k_bend=20.0, k_area=1.0, drag=0.001):
+    """Reference cell model: UL membrane, node-attached integrin springs
+    (linear 1 pN/nm, the k1 variant) anchoring it, and the boundary/body
+    forces."""
+    z = golden(case)
+    X = z["X"][:, :2].copy()
+    N = len(X)
+    rng = np.random.default_rng(seed)
+    x = X @ np.array([[1.02, 0.01], [-0.01, 0.99]]).T + deform * rng.standard_normal(X.shape)
+    ea = 0.01 * rng.standard_normal((N, 3))
+    d = dict(X=X.reshape(-1), x=x.reshape(-1), tris=z["tris"].reshape(-1).astype(np.int32), law=0, mu=1000.0,
+             thick=1.0, time=1.0, ea=ea.reshape(-1), node_springs=1, cC=rng.uniform(0.2, 1.0, N),
+             fau0=(0.004 * rng.standard_normal((N, 2))).reshape(-1), rho_max=100.0, spring_law=0, k_lin=1000.0,
+             k_bend=k_bend, k_area=k_area, area_0=W.AREA_0_REF, drag=drag, dt=dt, ap_rate=10.0, ap_quantum=5.0,
+             ap_density=100.0)
+    if ap:
+        d["ap"] = 1
+    return d
+
+
+def oracle_model(d, replica=0, step=0):
+    m = dict(d)
+    if m.pop("ap", 0):
+        N = len(d["X"]) // 2
+        m["ap_u"] = np.array([philox_u((replica, i, step, 1)) for i in range(N)])
+    return m
+
+
+def add_bterm_cell(g, d, solver=0):
+    return g.add_cell(X=d["X"], x=d["x"], tris=d["tris"], law=0, mu=d["mu"], thick=d["thick"], ea=d["ea"],
+                      node_springs=d["node_springs"], cC=d["cC"], fau=d["fau0"], rho_max=d["rho_max"],
+                      spring_law=d["spring_law"], k_lin=d["k_lin"], solver=solver, k_bend=d["k_bend"], k_area=d["k_area"], area_0=d["area_0"], drag=d["drag"],
+                      ap=d.get("ap", 0), ap_rate=d["ap_rate"], ap_quantum=d["ap_quantum"],
+                      ap_density=d["ap_density"])
+
+
+@pytest.mark.parametrize("prec,solver", [(W.PREC_FP64, W.SOLVER_DENSE), (W.PREC_MIXED, W.SOLVER_DENSE),
+                                         (W.PREC_FP64, W.SOLVER_CSR), (W.PREC_MIXED, W.SOLVER_CSR)])
+def test_node_terms_system_vs_oracle(prec, solver):
+    # block-CSR cells take drag and actin polymerisation only (bending and
+    # area tangents need the dense solver)
+    print("case prec", prec, "solver", solver, flush=True)
+    dense = solver == W.SOLVER_DENSE
+    d = bterm_cell(k_bend=20.0 if dense else 0.0, k_area=1.0 if dense else 0.0)
+    g = W.Context(precision=prec, t0=d["time"], dt=d["dt"])
+    add_bterm_cell(g, d, solver)
+    g.finalize()
+    K, G = g.assembled_system(0)
+    g.close()
+    Go, Ko = O.assemble(O.make_cell(oracle_model(d)))
+    assert relerr(G, Go) < 1e-10
+    assert relerr(K, Ko) < (1e-10 if prec == W.PREC_FP64 else 1e-6)
+    # the node terms are present: without them the system differs
+    G0, K0 = O.assemble(O.make_cell(dict(d, k_bend=0.0, k_area=0.0, drag=0.0, ap=0)))
+    assert np.max(np.abs(K0)) > 0
+    assert relerr(G0, Go) > 1e-6 and relerr(K0, Ko) > 1e-8
+
+
+@pytest.mark.parametrize("prec,solver", [(W.PREC_FP64, W.SOLVER_DENSE), (W.PREC_MIXED, W.SOLVER_DENSE),
+                                         (W.PREC_MIXED, W.SOLVER_CSR)])
+def test_node_terms_load_step_vs_oracle(prec, solver):
+    print("case prec", prec, "solver", solver, flush=True)
+    dense = solver == W.SOLVER_DENSE
+    d = bterm_cell(k_bend=200.0 if dense else 0.0, k_area=5.0 if dense else 0.0, drag=0.001 if dense else 5.0)
+    g = W.Context(precision=prec, t0=d["time"], dt=d["dt"], newton_rtol=1e-11, newton_atol=5e-8, max_refine=16)
+    add_bterm_cell(g, d, solver)
+    g.finalize()
+    st = g.load_step()
+    x_gpu = g.positions(0)
+    g.close()
+    assert st["max_lin_relres"] <= 1e-8
+    m = oracle_model(d)
+    x_o, it, _ = O.newton_solve(O.make_cell(m), d["x"], rtol=1e-12, atol=5e-8, maxit=25)
+    assert it >= 0
+    assert np.max(np.abs(x_gpu - x_o)) <= 1e-8 * np.max(np.abs(x_o - d["X"]))
+
+
+def test_node_terms_batched_graph_vs_per_cell():
+    """Cells of two meshes, with and without node terms, advanced one load
+    step as one batch (captured graph) and one cell at a time: the same
+    positions bit for bit; each cell's positions match the oracle."""
+    ds = [bterm_cell("bound_disk296", seed=5), bterm_cell("bound_disk483", seed=6, ap=False),
+          bterm_cell("bound_disk296", seed=7, k_bend=0.0, k_area=0.0, drag=0.0, ap=False)]
+    outs = []
+    for batched in (1, 0):
+        g = W.Context(precision=W.PREC_MIXED, t0=1.0, dt=0.1, newton_rtol=1e-11, newton_atol=5e-8, max_refine=16,
+                      batched=batched, use_graph=batched)
+        for d in ds:
+            add_bterm_cell(g, d)
+        g.finalize()
+        g.load_step()
+        outs.append([g.positions(i) for i in range(len(ds))])
+        g.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+    for c, d in enumerate(ds):
+        x_o, it, _ = O.newton_solve(O.make_cell(oracle_model(d, replica=c)), d["x"], rtol=1e-12, atol=5e-8,
+                                    maxit=25)
+        assert it >= 0
+        assert np.max(np.abs(outs[0][c] - x_o)) <= 1e-8 * np.max(np.abs(x_o - d["X"]))
+
+
+def test_node_terms_second_load_step_draws_new_ap_stream(tmp_path):
+    """The AP draws advance with the load step (counter (cell, vertex, step,
+    1)); a checkpoint taken after the first step resumes the second bit for
+    bit, boundary-force parameters included."""
+    d = bterm_cell(dt=0.1)
+    g = W.Context(precision=W.PREC_FP64, t0=1.0, dt=0.1, newton_rtol=1e-11, newton_atol=5e-8, max_refine=16)
+    add_bterm_cell(g, d)
+    g.finalize()
+    g.load_step()
+    X1, ea1, cC1, fau1 = g.ul_state(0)
+    path = str(tmp_path / "b.ckp")
+    g.checkpoint_save(path)
+    g.load_step()
+    x2 = g.positions(0)
+    g.close()
+    # oracle: second step from the committed state, draws of step 1
+    m = oracle_model(dict(d, X=X1, x=X1, ea=ea1, cC=cC1, fau0=fau1, time=1.1), step=1)
+    x_o, it, _ = O.newton_solve(O.make_cell(m), X1, rtol=1e-12, atol=5e-8, maxit=25)
+    assert it >= 0
+    assert np.max(np.abs(x2 - x_o)) <= 1e-8 * np.max(np.abs(x_o - X1))
+    h = W.Context(precision=W.PREC_FP64, t0=1.0, dt=0.1, newton_rtol=1e-11, newton_atol=5e-8, max_refine=16)
+    h.checkpoint_load(path, [(len(d["X"]) // 2, len(d["tris"]) // 3, 0)])
+    h.finalize()
+    h.load_step()
+    assert np.array_equal(h.positions(0), x2)
+    h.close()
+
+
+@pytest.mark.parametrize("term", ["k_bend", "k_area"])
+def test_bending_and_area_on_block_csr_are_refused_and_auto_goes_dense(term):
+    d = bterm_cell(k_bend=0.0, k_area=0.0)
+    d[term] = 1.0
+    g = W.Context()
+    add_bterm_cell(g, d, W.SOLVER_CSR)
+    with pytest.raises(W.WcfeError):
+        g.finalize()
+    g.close()
+    # AUTO above dense_max_dof: dense for this cell, CSR for one with drag and
+    # actin polymerisation only
+    g = W.Context(dense_max_dof=64, t0=1.0, dt=0.1, max_refine=16)
+    add_bterm_cell(g, d, W.SOLVER_AUTO)
+    add_bterm_cell(g, dict(d, k_bend=0.0, k_area=0.0), W.SOLVER_AUTO)
+    g.finalize()
+    st = g.newton_step()
+    assert st["max_lin_relres"] <= 1e-8
+    g.close()
+
+
+def test_node_terms_need_ul_law():
+    d = bterm_cell()
+    g = W.Context()
+    with pytest.raises(W.WcfeError):
+        g.add_cell(X=d["X"], tris=d["tris"], law=W.LAW_CST, k_bend=20.0)
+    g.close()

@@ -62,3 +62,24 @@ def test_update_fa_adapter(case, law):
                              tris=g["tris"], Stris=g["Stris"]), [0.0, float(g["dt"]), float(g["kint"]), law])
     assert relerr(o["cC"], g["out_cC"]) < 1e-12
     assert relerr(o["fau"], g["out_fau"]) < 1e-11
+
+
+@pytest.mark.parametrize("case", ["bound_disk296", "bound_disk296_notch"])
+def test_boundary_force_adapters(case):
+    import oracle_py as O
+    g = golden(case)
+    dt = 0.1
+    o = run("boundary", dict(X=g["X"], x=g["x"], tris=g["tris"]), [0.0, dt, 0.0, 1])
+    assert relerr(o["areatot"], g["areatot"]) < 1e-13
+    assert relerr(o["F_bend"], g["F_bend"]) < 1e-13
+    # tot_area comes from the GPU's tree reduction here (1e-16 apart from the
+    # reference's sequential sum); the force scales with tot_area - area_0,
+    # which cancels A ~ 310 against area_0 ~ 314 (about 80x amplification)
+    assert relerr(o["F_area"], g["F_area"]) < 1e-11
+    # actin polymerisation: each vertex either idle or the fired value
+    n = len(g["x"])
+    fired, _ = O.force_ap(g["tris"], g["x"], dt, np.zeros(n))
+    a, f = o["F_ap"].reshape(-1, 3), fired.reshape(-1, 3)
+    on = np.abs(a).max(axis=1) > 0
+    assert on.sum() > 0
+    assert relerr(a[on], f[on]) < 1e-13

@@ -34,9 +34,15 @@ EXPORTS = [
     "wcfe_residual_fp64", "wcfe_element_systems", "wcfe_assembled_system", "wcfe_get_colors", "wcfe_dense_solve",
     "wcfe_calculate_res", "wcfe_force_fa", "wcfe_update_fa", "wcfe_profile_step",
     "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state", "wcfe_replace_cell", "wcfe_checkpoint_save",
-    "wcfe_checkpoint_load",
+    "wcfe_checkpoint_load", "wcfe_area_tot", "wcfe_bending_force_2d", "wcfe_area_constraint_force",
+    "wcfe_force_ap",
 ]
 TUNE_SOLVER, TUNE_LU = 1, 2
+# reference constants of the boundary and body forces (UL_growth_explicit.cpp)
+K_BEND_REF = 20.0              # pN/um, :1653
+K_AREA_REF = 1.0               # pN/um^3, :1776
+AREA_0_REF = 10 * 10 * 3.1416  # um^2, :1777
+DRAG_REF = 0.001               # pN s/um, :569
 PHASES = ["fill", "assemble", "lu_gemm", "lu_panel", "solve", "refine", "residual", "commit", "lu_strip"]
 
 
@@ -59,6 +65,8 @@ class CellDesc(C.Structure):
         ("node_springs", C.c_int32), ("cC", D), ("fau", D), ("rho_max", C.c_double), ("n_spr", C.c_int32),
         ("spr_tri", I32), ("spr_bary", D), ("spr_anchor", D), ("spr_state", D), ("spring_law", C.c_int32),
         ("k_lin", C.c_double), ("table_id", C.c_int32), ("kinetics", C.c_int32), ("solver", C.c_int32),
+        ("k_bend", C.c_double), ("k_area", C.c_double), ("area_0", C.c_double), ("drag", C.c_double),
+        ("ap", C.c_int32), ("ap_rate", C.c_double), ("ap_quantum", C.c_double), ("ap_density", C.c_double),
     ]
 
 
@@ -199,7 +207,8 @@ class Context:
     def _desc(self, X, tris, law=LAW_CST, x=None, mu=1000.0, E=1000.0, nu=0.3, thick=0.2, ea=None,
               node_springs=0, cC=None, fau=None, rho_max=100.0, spr_tri=None, spr_bary=None, spr_anchor=None,
               spr_state=None, spring_law=SPRING_LINEAR, k_lin=1000.0, table_id=0, kinetics=KIN_NONE,
-              solver=SOLVER_AUTO):
+              solver=SOLVER_AUTO, k_bend=0.0, k_area=0.0, area_0=AREA_0_REF, drag=0.0, ap=0, ap_rate=10.0,
+              ap_quantum=5.0, ap_density=100.0):
         X = _f64(X).reshape(-1)
         tris = _i32(tris).reshape(-1)
         arrs = dict(X=X, x=_f64(x), tris=tris, ea=_f64(ea), cC=_f64(cC), fau=_f64(fau), spr_tri=_i32(spr_tri),
@@ -222,6 +231,8 @@ class Context:
                                                             _dp(arrs["spr_anchor"]), _dp(arrs["spr_state"]))
         d.spring_law, d.k_lin, d.table_id, d.kinetics = spring_law, k_lin, table_id, kinetics
         d.solver = solver
+        d.k_bend, d.k_area, d.area_0, d.drag = k_bend, k_area, area_0, drag
+        d.ap, d.ap_rate, d.ap_quantum, d.ap_density = int(ap), ap_rate, ap_quantum, ap_density
         return d
 
     def finalize(self):
@@ -362,6 +373,44 @@ class Context:
         self._check(self._L.wcfe_force_fa(self._h, n, len(tris) // 3, _ip(tris), _dp(x), _dp(cC), _dp(fau), law,
                                           C.c_double(kint), _dp(F)))
         return F
+
+    def area_tot(self, tris, x):
+        """GPU counterpart of calculateAreaTot (UL:1662): [area, cx, cy, cz]."""
+        tris, x = _i32(tris).reshape(-1), _f64(x).reshape(-1)
+        out = np.zeros(4)
+        self._check(self._L.wcfe_area_tot(self._h, len(x) // 3, len(tris) // 3, _ip(tris), _dp(x), _dp(out)))
+        return out
+
+    def bending_force_2d(self, tris, x, k_bend=K_BEND_REF):
+        """GPU counterpart of calculateBendingForce2D (UL:1571); returns (F 3N, n_open)."""
+        tris, x = _i32(tris).reshape(-1), _f64(x).reshape(-1)
+        n = len(x) // 3
+        F, op = np.zeros(3 * n), C.c_int32(0)
+        self._check(self._L.wcfe_bending_force_2d(self._h, n, len(tris) // 3, _ip(tris), _dp(x), C.c_double(k_bend),
+                                                  _dp(F), C.byref(op)))
+        return F, op.value
+
+    def area_constraint_force(self, tris, x, tot_area, k_area=K_AREA_REF, area_0=AREA_0_REF):
+        """GPU counterpart of calculateAreaConstraintForce (UL:1691); returns (F 3N, n_open)."""
+        tris, x = _i32(tris).reshape(-1), _f64(x).reshape(-1)
+        n = len(x) // 3
+        F, op = np.zeros(3 * n), C.c_int32(0)
+        self._check(self._L.wcfe_area_constraint_force(self._h, n, len(tris) // 3, _ip(tris), _dp(x),
+                                                       C.c_double(tot_area), C.c_double(k_area),
+                                                       C.c_double(area_0), _dp(F), C.byref(op)))
+        return F, op.value
+
+    def force_ap(self, tris, x, dt, u=None, draw=0, rate=10.0, quantum=5.0, density=100.0):
+        """GPU counterpart of calculateForce_AP (UL:1477) with per-vertex draws
+        u (None: the context's Philox stream, counter (draw, vertex, 0, 2))."""
+        tris, x = _i32(tris).reshape(-1), _f64(x).reshape(-1)
+        n = len(x) // 3
+        u = _f64(u)
+        F, op = np.zeros(3 * n), C.c_int32(0)
+        self._check(self._L.wcfe_force_ap(self._h, n, len(tris) // 3, _ip(tris), _dp(x), C.c_double(dt),
+                                          C.c_double(rate), C.c_double(quantum), C.c_double(density), _dp(u),
+                                          C.c_uint32(draw), _dp(F), C.byref(op)))
+        return F, op.value
 
     def update_fa(self, X, x, cA, cC, fau, dt, stris, SX, Sx, law, kint):
         """GPU counterpart of updateFA, 11-argument form (UL:2026 / UL1:2005)."""

@@ -21,7 +21,7 @@ COMPAT_BIN = os.path.join(BUILD, "compat_check")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["k_assemble.hip", "k_lu.hip", "k_csr.hip", "k_reference.hip", "wcfe_api.hip"]
+SOURCES = ["k_assemble.hip", "k_lu.hip", "k_csr.hip", "k_reference.hip", "k_boundary.hip", "wcfe_api.hip"]
 HEADERS = ["wcfe_device.hpp", "wcfe_internal.hpp"]
 
 FLAGS = [
@@ -58,7 +58,7 @@ def build(force=True, verbose=False):
     if not force and not stale():
         return LIB
     os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(5, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs

@@ -218,10 +218,11 @@ void launch_cell_norm(hipStream_t s, const CellParams *cp, CellState *cs, const 
 __global__ __launch_bounds__(256) void k_matvec_ir(const int *el_tri, const int *node_cell, const int *adj_start,
                                                    const int *adj, const double *Ke64, const double *G,
                                                    const double *delta, double *r, int node_begin, int node_end,
-                                                   const CellState *cs) {
+                                                   const CellState *cs, const CellParams *cp, BndView bv) {
   const int i = node_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= node_end) return;
-  if (cs[node_cell[i]].ir_done) return;
+  const int c = node_cell[i];
+  if (cs[c].ir_done) return;
   const double2 *d2 = reinterpret_cast<const double2 *>(delta);
   double y0 = 0.0, y1 = 0.0;
   for (int k = adj_start[i]; k < adj_start[i + 1]; ++k) {
@@ -235,16 +236,42 @@ __global__ __launch_bounds__(256) void k_matvec_ir(const int *el_tri, const int 
       y1 += Ke[6 + 2 * b] * d.x + Ke[6 + 2 * b + 1] * d.y;
     }
   }
+  if (bv.bidx && cp[c].bterm) {
+    // node terms: drag diagonal, the boundary vertex's 2 x 6 block and the
+    // area term's rank-one part k g_i (sum_m g_m . d_m)
+    const CellParams &p = cp[c];
+    const double2 di = d2[i];
+    if (p.bterm & BT_DRAG) {
+      const double cd = p.drag / bv.dt;
+      y0 += cd * di.x;
+      y1 += cd * di.y;
+    }
+    const int b = bv.bidx[i];
+    if (b >= 0 && (p.bterm & (BT_BEND | BT_AREA))) {
+      const int2 fr = bv.frame[b];
+      if (fr.x >= 0) {
+        const double *J = bv.J + 12LL * b;
+        const double2 d1 = d2[fr.x], dd2 = d2[fr.y];
+        y0 += J[0] * di.x + J[1] * di.y + J[2] * d1.x + J[3] * d1.y + J[4] * dd2.x + J[5] * dd2.y;
+        y1 += J[6] * di.x + J[7] * di.y + J[8] * d1.x + J[9] * d1.y + J[10] * dd2.x + J[11] * dd2.y;
+        if (p.bterm & BT_AREA) {
+          const double2 g = bv.g[b];
+          y0 += p.k_area * g.x * bv.gd[c];
+          y1 += p.k_area * g.y * bv.gd[c];
+        }
+      }
+    }
+  }
   r[2 * i] = -G[2 * i] - y0;
   r[2 * i + 1] = -G[2 * i + 1] - y1;
 }
 void launch_matvec_ir(hipStream_t s, const View &v, const int *node_cell, const int *adj_start, const int *adj,
                       const double *Ke64, const double *G, const double *delta, double *r, int node_begin,
-                      int node_end, const CellState *cs) {
+                      int node_end, const CellState *cs, const BndView &bv) {
   const int n = node_end - node_begin;
   if (n <= 0) return;
   hipLaunchKernelGGL(k_matvec_ir, dim3((n + 255) / 256), dim3(256), 0, s, v.el_tri, node_cell, adj_start, adj, Ke64,
-                     G, delta, r, node_begin, node_end, cs);
+                     G, delta, r, node_begin, node_end, cs, v.cp, bv);
 }
 
 __global__ void k_ir_control(CellState *cs, const int *cells, int ncell, double lin_rtol, int reset) {
@@ -352,23 +379,6 @@ void launch_commit_nodes(hipStream_t s, const View &v, const int *node_cell, dou
   if (n <= 0) return;
   hipLaunchKernelGGL(k_commit_nodes, dim3((n + 255) / 256), dim3(256), 0, s, v, node_cell, X, x, ea, cC, fau, Rea, MM,
                      node_begin, node_end, dt);
-}
-
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
-__device__ __forceinline__ void philox4x32_10(unsigned c[4], unsigned k0, unsigned k1) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c[0];
-    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c[2];
-    const unsigned n0 = (unsigned)(p1 >> 32) ^ c[1] ^ k0;
-    const unsigned n2 = (unsigned)(p0 >> 32) ^ c[3] ^ k1;
-    c[1] = (unsigned)p1;
-    c[3] = (unsigned)p0;
-    c[0] = n0;
-    c[2] = n2;
-    k0 += 0x9E3779B9u;
-    k1 += 0xBB67AE85u;
-  }
 }
 
 // Bell kinetics per barycentric spring after a converged load step:

@@ -64,6 +64,9 @@ struct HostCell {
   bool has_state = false;  // `state` holds the full solver state (Newton iterate, norms)
   CellState state;
   double mu = 1000, E = 1000, nu = 0.3, thick = 1, rho_max = 100, k_lin = 1000;
+  // boundary and body forces of the reference cell model
+  int ap = 0;
+  double k_bend = 0, k_area = 0, area_0 = 0, drag = 0, ap_rate = 0, ap_quantum = 0, ap_density = 0;
   std::vector<double> X, x, ea, cC, fau, spr_bary, spr_anchor, spr_state;
   std::vector<int> tris, spr_tri;
 };
@@ -86,6 +89,67 @@ int color_elements(int n_nodes, int n_tris, const int *tris, std::vector<int> &c
     nc = std::max(nc, col + 1);
   }
   return nc;
+}
+
+// Boundary vertices by El Topo's rule (an edge with an odd number of incident
+// triangles is a boundary edge, eltopo3d/nondestructivetrimesh.cpp:527-543;
+// triangles with t0 == t1 are deleted ones, :497) and, per boundary vertex ni,
+// the (nj, nint) pairs the reference's boundary-force loops visit
+// (UL_growth_explicit.cpp:1497-1528): one-ring triangles by increasing index,
+// corners nj in slot order that are boundary vertices other than ni, nint the
+// first corner that is neither.  cst: N+1 offsets into cand (an empty range
+// marks an interior vertex).
+void boundary_candidates(int n_nodes, int n_tris, const int *tris, std::vector<int> &cst, std::vector<int2> &cand) {
+  std::vector<unsigned long long> edges;
+  edges.reserve(3 * (size_t)n_tris);
+  for (int e = 0; e < n_tris; ++e) {
+    const int *t = tris + 3 * e;
+    if (t[0] == t[1]) continue;
+    for (int j = 0; j < 3; ++j) {
+      const unsigned a = (unsigned)std::min(t[j], t[(j + 1) % 3]), b = (unsigned)std::max(t[j], t[(j + 1) % 3]);
+      edges.push_back(((unsigned long long)a << 32) | b);
+    }
+  }
+  std::sort(edges.begin(), edges.end());
+  std::vector<char> bnd(n_nodes, 0);
+  for (size_t i = 0; i < edges.size();) {
+    size_t j = i;
+    while (j < edges.size() && edges[j] == edges[i]) ++j;
+    if ((j - i) % 2 == 1) bnd[edges[i] >> 32] = bnd[edges[i] & 0xFFFFFFFFull] = 1;
+    i = j;
+  }
+  std::vector<int> rs(n_nodes + 1, 0), rl;
+  for (int e = 0; e < n_tris; ++e)
+    for (int a = 0; a < 3; ++a) rs[tris[3 * e + a] + 1]++;
+  for (int i = 0; i < n_nodes; ++i) rs[i + 1] += rs[i];
+  rl.assign(rs[n_nodes], 0);
+  {
+    std::vector<int> fill(rs.begin(), rs.end() - 1);
+    for (int e = 0; e < n_tris; ++e)
+      for (int a = 0; a < 3; ++a) rl[fill[tris[3 * e + a]]++] = e;
+  }
+  cst.assign(n_nodes + 1, 0);
+  cand.clear();
+  for (int ni = 0; ni < n_nodes; ++ni) {
+    cst[ni] = (int)cand.size();
+    if (!bnd[ni]) continue;
+    for (int k = rs[ni]; k < rs[ni + 1]; ++k) {
+      const int *t = tris + 3 * rl[k];
+      if (t[0] == t[1]) continue;
+      for (int enj = 0; enj < 3; ++enj) {
+        const int nj = t[enj];
+        if (!bnd[nj] || nj == ni) continue;
+        int nint = -1;
+        for (int enk = 0; enk < 3; ++enk)
+          if (t[enk] != nj && t[enk] != ni) {
+            nint = t[enk];
+            break;
+          }
+        if (nint >= 0) cand.push_back(make_int2(nj, nint));
+      }
+    }
+  }
+  cst[n_nodes] = (int)cand.size();
 }
 
 // LU schedule: a list of stages (kernel kind, block step k, panel count),
@@ -295,12 +359,19 @@ struct Batch {
   void *Kb = nullptr, *cgvec = nullptr;         // block values; x, r, z, p, q (2N each) and dinv (4N)
   CgState *cg = nullptr;
   double *partials = nullptr;
+  // boundary and body forces (cells with bterm != 0)
+  int any_bterm = 0, any_area = 0, nbnd = 0;
+  int *bt_bidx = nullptr, *bt_node = nullptr, *bt_cst = nullptr, *crange = nullptr;
+  int2 *bt_cand = nullptr, *bt_frame = nullptr;
+  double2 *bt_g = nullptr;
+  double *bt_J = nullptr, *bt_area = nullptr, *bt_gd = nullptr;
 
   void release() {
     void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
                     sp_local, cp, cs, node_cell, adj_start, adj, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
                     replica, transitions, plan.d_prefix, row_ptr, bcol, diag_pos, el_bpos, csr_cells, chunks, Kb,
-                    cgvec, cg, partials};
+                    cgvec, cg, partials, bt_bidx, bt_node, bt_cst, crange, bt_cand, bt_frame, bt_g, bt_J, bt_area,
+                    bt_gd};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     *this = Batch();
@@ -446,13 +517,29 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     p.spr_off = S;
     p.n_spr = h.n_spr;
     p.ndof = 2 * h.n_nodes;
-    p.csr = h.solver == WCFE_SOLVER_CSR || (h.solver == WCFE_SOLVER_AUTO && p.ndof > ctx->opt.dense_max_dof);
+    // the block-CSR solver is block-Jacobi conjugate gradients on the sparse
+    // pattern: the bending tangent is not symmetric and the area term's has a
+    // dense rank-one part over the boundary, so those cells take the dense LU
+    const bool dense_only = h.k_bend != 0 || h.k_area != 0;
+    p.csr = h.solver == WCFE_SOLVER_CSR ||
+            (h.solver == WCFE_SOLVER_AUTO && p.ndof > ctx->opt.dense_max_dof && !dense_only);
+    if (p.csr && dense_only)
+      fail(WCFE_ERR_ARG,
+           "cell %d: the bending and area-constraint tangents need the dense solver (block-CSR runs conjugate "
+           "gradients on the sparse pattern); use WCFE_SOLVER_DENSE or AUTO for cells with k_bend or k_area > 0",
+           c);
     p.npad = p.csr ? 0 : ((p.ndof + NB - 1) / NB) * NB;
     p.nblk = p.npad / NB;
     p.sys_off = sys;
     p.vec_off = 2 * N;
-    // the tangent is symmetric unless node-attached springs (area derivative) are present
-    p.sym = ctx->opt.symmetric && ctx->opt.pivoting == WCFE_PIVOT_STATIC && !h.node_springs;
+    p.bterm = (h.k_bend != 0 ? BT_BEND : 0) | (h.k_area != 0 ? BT_AREA : 0) | (h.ap ? BT_AP : 0) |
+              (h.drag != 0 ? BT_DRAG : 0);
+    p.k_bend = h.k_bend; p.k_area = h.k_area; p.area_0 = h.area_0; p.drag = h.drag;
+    p.ap_rate = h.ap_rate; p.ap_quantum = h.ap_quantum; p.ap_density = h.ap_density;
+    // the tangent is symmetric unless node-attached springs (area derivative)
+    // or the boundary bending / area terms are present
+    p.sym = ctx->opt.symmetric && ctx->opt.pivoting == WCFE_PIVOT_STATIC && !h.node_springs &&
+            !(p.bterm & (BT_BEND | BT_AREA));
     p.k_off = kp;
     p.mu = h.mu; p.E = h.E; p.nu = h.nu; p.thick = h.thick; p.k_lin = h.k_lin; p.rho_max = h.rho_max;
     N += h.n_nodes;
@@ -461,6 +548,8 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     kp += (long long)p.npad * p.npad;
     b.max_npad = std::max(b.max_npad, p.npad);
     b.any_ul |= (h.law == WCFE_LAW_UL);
+    b.any_bterm |= p.bterm != 0;
+    b.any_area |= (p.bterm & BT_AREA) != 0;
     b.any_bell |= (h.kinetics == WCFE_KIN_BELL);
     b.any_csr |= p.csr;
   }
@@ -570,6 +659,33 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   }
   std::vector<int> ident(b.ncell), replica(b.ncell);
   for (int c = 0; c < b.ncell; ++c) ident[c] = replica[c] = c;
+  // boundary tables of the cells with boundary forces (global node ids)
+  std::vector<int> bidx, bnode, bcst, crange;
+  std::vector<int2> bcand;
+  if (b.any_bterm) {
+    bidx.assign(N, -1);
+    bcst.push_back(0);
+    for (int c = 0; c < b.ncell; ++c) {
+      CellParams &p = b.cp_h[c];
+      p.bnd_off = (int)bnode.size();
+      if (!(p.bterm & (BT_BEND | BT_AREA | BT_AP))) continue;
+      std::vector<int> cst;
+      std::vector<int2> cand;
+      boundary_candidates(hc[c].n_nodes, hc[c].n_tris, hc[c].tris.data(), cst, cand);
+      for (int i = 0; i < hc[c].n_nodes; ++i) {
+        if (cst[i + 1] == cst[i]) continue;
+        bidx[p.node_off + i] = (int)bnode.size();
+        bnode.push_back(p.node_off + i);
+        for (int k = cst[i]; k < cst[i + 1]; ++k)
+          bcand.push_back(make_int2(p.node_off + cand[k].x, p.node_off + cand[k].y));
+        bcst.push_back((int)bcand.size());
+      }
+      p.n_bnd = (int)bnode.size() - p.bnd_off;
+    }
+    b.nbnd = (int)bnode.size();
+    crange.reserve((size_t)b.ncell * 2 * b.ncolor);
+    for (int c = 0; c < b.ncell; ++c) crange.insert(crange.end(), b.cell_color[c].begin(), b.cell_color[c].end());
+  }
   // upload
   b.X = dupload(X);
   b.x = dupload(x);
@@ -592,6 +708,23 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   b.adj = dupload(adj);
   b.cells = dupload(ident);
   b.replica = dupload(replica);
+  if (b.any_bterm) {
+    b.bt_bidx = dupload(bidx);
+    b.bt_node = dupload(bnode);
+    b.bt_cst = dupload(bcst);
+    b.bt_cand = dupload(bcand);
+    b.crange = dupload(crange);
+    b.bt_frame = dalloc<int2>(std::max(b.nbnd, 1));
+    b.bt_g = dalloc<double2>(std::max(b.nbnd, 1));
+    b.bt_J = dalloc<double>(12 * (size_t)std::max(b.nbnd, 1));
+    HIPCHECK(hipMemset(b.bt_frame, 0xFF, sizeof(int2) * std::max(b.nbnd, 1)));
+    HIPCHECK(hipMemset(b.bt_g, 0, sizeof(double2) * std::max(b.nbnd, 1)));
+    HIPCHECK(hipMemset(b.bt_J, 0, sizeof(double) * 12 * std::max(b.nbnd, 1)));
+    b.bt_area = dalloc<double>(b.ncell);
+    b.bt_gd = dalloc<double>(b.ncell);
+    HIPCHECK(hipMemset(b.bt_area, 0, sizeof(double) * b.ncell));
+    HIPCHECK(hipMemset(b.bt_gd, 0, sizeof(double) * b.ncell));
+  }
   b.transitions = dalloc<int>(1);
   HIPCHECK(hipMemset(b.transitions, 0, sizeof(int)));
   b.Ke64 = dalloc<double>(36 * (size_t)T);
@@ -731,6 +864,19 @@ template <class T> CgVectors<T> cg_vectors(const Batch &b) {
   return CgVectors<T>{base, base + n2, base + 2 * n2, base + 3 * n2, base + 4 * n2, base + 5 * n2};
 }
 
+BndView bnd_view(const wcfe_ctx *ctx, const Batch &b) {
+  BndView bv;
+  bv.bidx = b.bt_bidx; bv.node = b.bt_node; bv.cst = b.bt_cst; bv.cand = b.bt_cand;
+  bv.frame = b.bt_frame; bv.g = b.bt_g; bv.J = b.bt_J; bv.area = b.bt_area; bv.gd = b.bt_gd;
+  bv.replica = b.replica;
+  bv.dt = ctx->opt.dt;
+  bv.key_lo = ctx->opt.seed_lo;
+  bv.key_hi = ctx->opt.seed_hi;
+  return bv;
+}
+
+// Element assembly colour by colour; in FULL and RESIDUAL mode followed by
+// the node terms (boundary and body forces) of the cells that have them.
 template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &sc, int mode) {
   hipStream_t s = ctx->stream;
   View v = make_view(ctx, b);
@@ -740,6 +886,14 @@ template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &s
     color_range(b, sc, k, e0, e1);
     launch_assemble<T>(s, v, e0, e1, mode, K, static_cast<T *>(b.Kb), b.G, b.Ke64, b.Rea, b.MM);
   }
+  if (!b.any_bterm || mode == ASM_STRAIN) return;
+  const BndView bv = bnd_view(ctx, b);
+  const int b0 = sc.cell < 0 ? 0 : b.cp_h[sc.cell].bnd_off;
+  const int b1 = sc.cell < 0 ? b.nbnd : b0 + b.cp_h[sc.cell].n_bnd;
+  launch_bnd_frames(s, bv, b.x, b0, b1);
+  if (b.any_area) launch_cell_area(s, b.cp, sc.cells, sc.ncell, b.crange, b.ncolor, b.el_tri, b.x, b.bt_area);
+  launch_node_terms<T>(s, mode == ASM_FULL, v, bv, b.node_cell, sc.node_begin, sc.node_end, K,
+                       static_cast<T *>(b.Kb), b.row_ptr, b.bcol, b.diag_pos, b.G);
 }
 
 // Launch the LU stages of a plan for a list of cells.
@@ -883,8 +1037,10 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
       enqueue_pcg<T>(ctx, b, sc, it > 0);
     }
     PhaseScope ph(ctx, WCFE_PH_REFINE);
+    const BndView bv = bnd_view(ctx, b);
+    if (b.any_area) launch_cell_gdot(s, b.cp, sc.cells, sc.ncell, bv, b.delta);
     launch_matvec_ir(s, v, b.node_cell, b.adj_start, b.adj, b.Ke64, b.G, b.delta, b.r, sc.node_begin, sc.node_end,
-                     b.cs);
+                     b.cs, bv);
     launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.r, 1, 1);
     launch_ir_control(s, b.cs, sc.cells, sc.ncell, ctx->opt.lin_rtol, 0);
   }
@@ -1154,6 +1310,18 @@ HostCell host_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d) {
     h.kinetics = d->kinetics;
     if (d->solver < WCFE_SOLVER_AUTO || d->solver > WCFE_SOLVER_CSR) fail(WCFE_ERR_ARG, "unknown solver %d", d->solver);
     h.solver = d->solver;
+    const bool bterm = d->k_bend != 0 || d->k_area != 0 || d->drag != 0 || d->ap;
+    if (bterm && d->law != WCFE_LAW_UL)
+      fail(WCFE_ERR_ARG, "boundary and body forces follow the reference cell model and need the UL law");
+    if (!(d->k_bend >= 0) || !(d->k_area >= 0) || !(d->drag >= 0))
+      fail(WCFE_ERR_ARG, "k_bend, k_area and drag must be >= 0");
+    if (d->ap && (!(d->ap_rate >= 0) || !(d->ap_quantum >= 0) || !(d->ap_density >= 0)))
+      fail(WCFE_ERR_ARG, "actin polymerisation rate, quantum and density must be >= 0");
+    if ((d->drag != 0 || d->ap) && !(ctx->opt.dt > 0))
+      fail(WCFE_ERR_ARG, "drag and actin polymerisation need wcfe_options.dt > 0");
+    h.k_bend = d->k_bend; h.k_area = d->k_area; h.area_0 = d->area_0; h.drag = d->drag;
+    h.ap = d->ap ? 1 : 0;
+    h.ap_rate = d->ap_rate; h.ap_quantum = d->ap_quantum; h.ap_density = d->ap_density;
     h.n_spr = d->n_spr;
     h.X.assign(d->X, d->X + 2 * d->n_nodes);
     if (d->x) h.x.assign(d->x, d->x + 2 * d->n_nodes);
@@ -1970,10 +2138,94 @@ int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double
   });
 }
 
+int wcfe_area_tot(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
+                  double *out) {
+  if (!ctx || !x || !out) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_mesh(n_nodes, n_tris, tris);
+    if (n_tris <= 0) fail(WCFE_ERR_ARG, "no triangles");
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    Scratch sc;
+    int *dtr = sc.up(tris, 3 * (size_t)n_tris);
+    double *dx = sc.up(x, 3 * (size_t)n_nodes), *dout = sc.zeros<double>(4);
+    launch_ref_area_tot(ctx->stream, n_tris, dtr, dx, dout);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(out, dout, sizeof(double) * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"
+
+namespace {
+// One reference-shaped boundary-force call: candidate tables on the host, one
+// GPU thread per vertex.
+int ref_boundary_call(wcfe_ctx *ctx, int which, int n_nodes, int n_tris, const int32_t *tris, const double *x,
+                      const RefBoundaryArgs &a, const double *u, double *F, int32_t *n_open) {
+  if (!ctx || !x || !F) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_mesh(n_nodes, n_tris, tris);
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    std::vector<int> cst;
+    std::vector<int2> cand;
+    boundary_candidates(n_nodes, n_tris, tris, cst, cand);
+    Scratch sc;
+    int *dcs = sc.up(cst.data(), cst.size());
+    int2 *dca = sc.up(cand.data(), cand.size());
+    double *dx = sc.up(x, 3 * (size_t)n_nodes), *dF = sc.zeros<double>(3 * (size_t)n_nodes);
+    double *du = u ? sc.up(u, (size_t)n_nodes) : nullptr;
+    int *dopen = sc.zeros<int>(1);
+    launch_ref_boundary(ctx->stream, which, n_nodes, dcs, dca, dx, a, du, dF, dopen);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(F, dF, sizeof(double) * 3 * n_nodes, hipMemcpyDeviceToHost));
+    int op = 0;
+    HIPCHECK(hipMemcpy(&op, dopen, sizeof(int), hipMemcpyDeviceToHost));
+    if (n_open) *n_open = op;
+  });
+}
+}  // namespace
+
+extern "C" {
+
+int wcfe_bending_force_2d(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
+                          double k_bend, double *F, int32_t *n_open) {
+  RefBoundaryArgs a{};
+  a.k_bend = k_bend;
+  return ref_boundary_call(ctx, REF_BEND, n_nodes, n_tris, tris, x, a, nullptr, F, n_open);
+}
+
+int wcfe_area_constraint_force(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
+                               double tot_area, double k_area, double area_0, double *F, int32_t *n_open) {
+  RefBoundaryArgs a{};
+  a.tot_area = tot_area;
+  a.k_area = k_area;
+  a.area_0 = area_0;
+  return ref_boundary_call(ctx, REF_AREA, n_nodes, n_tris, tris, x, a, nullptr, F, n_open);
+}
+
+int wcfe_force_ap(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x, double dt,
+                  double rate, double quantum, double density, const double *u, uint32_t draw, double *F,
+                  int32_t *n_open) {
+  RefBoundaryArgs a{};
+  a.dt = dt;
+  a.rate = rate;
+  a.quantum = quantum;
+  a.density = density;
+  if (ctx) {
+    a.key_lo = ctx->opt.seed_lo;
+    a.key_hi = ctx->opt.seed_hi;
+  }
+  a.draw = draw;
+  return ref_boundary_call(ctx, REF_AP, n_nodes, n_tris, tris, x, a, u, F, n_open);
+}
+
 }  // extern "C"
 
 namespace {
 const char CKP_MAGIC[8] = {'W', 'C', 'F', 'E', 'C', 'K', 'P', '1'};
+// version 2 adds the boundary and body force parameters after the scalars
+const int CKP_VERSION = 2;
 struct CkpWriter {
   FILE *f;
   void raw(const void *p, size_t n) {
@@ -2032,7 +2284,7 @@ int wcfe_checkpoint_save(wcfe_ctx *ctx, const char *path) {
     CkpWriter w{f};
     try {
       w.raw(CKP_MAGIC, 8);
-      w.i32(1);
+      w.i32(CKP_VERSION);
       w.i32((int)ctx->cells.size());
       w.i32((int)ctx->tables.size());
       for (auto &kv : ctx->tables) {
@@ -2049,6 +2301,8 @@ int wcfe_checkpoint_save(wcfe_ctx *ctx, const char *path) {
                       h.solver, h.step})
           w.i32(v);
         for (double v : {h.mu, h.E, h.nu, h.thick, h.rho_max, h.k_lin, h.time}) w.f64(v);
+        w.i32(h.ap);
+        for (double v : {h.k_bend, h.k_area, h.area_0, h.drag, h.ap_rate, h.ap_quantum, h.ap_density}) w.f64(v);
         w.vec(h.X, 2 * N);
         w.vec(h.x, 2 * N);
         w.vec(h.ea, 3 * N);
@@ -2082,7 +2336,8 @@ int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path) {
       char magic[8];
       r.raw(magic, 8);
       if (memcmp(magic, CKP_MAGIC, 8) != 0) fail(WCFE_ERR_ARG, "%s is not a wcfe checkpoint", path);
-      if (r.i32() != 1) fail(WCFE_ERR_ARG, "unsupported checkpoint version");
+      const int version = r.i32();
+      if (version != 1 && version != 2) fail(WCFE_ERR_ARG, "unsupported checkpoint version %d", version);
       const int ncell = r.i32(), ntab = r.i32();
       if (ncell < 1 || ncell >= (1 << 11) || ntab < 0 || ntab > 1024) fail(WCFE_ERR_ARG, "bad checkpoint header");
       for (int t = 0; t < ntab; ++t) {
@@ -2101,6 +2356,11 @@ int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path) {
         for (int *v : iv) *v = r.i32();
         double *dv[] = {&h.mu, &h.E, &h.nu, &h.thick, &h.rho_max, &h.k_lin, &h.time};
         for (double *v : dv) *v = r.f64();
+        if (version >= 2) {
+          h.ap = r.i32();
+          double *bv[] = {&h.k_bend, &h.k_area, &h.area_0, &h.drag, &h.ap_rate, &h.ap_quantum, &h.ap_density};
+          for (double *v : bv) *v = r.f64();
+        }
         if (h.n_nodes <= 0 || h.n_tris <= 0 || h.n_spr < 0) fail(WCFE_ERR_ARG, "bad cell sizes in checkpoint");
         const long long N = h.n_nodes, T = h.n_tris, S = h.n_spr;
         r.vec(h.X, 2 * N);

@@ -28,9 +28,31 @@ struct CellParams {
   int sym;                        // 1: symmetric tangent, only the lower triangle is updated
                                   //    and U_kj = D_k L_jk^T is written by transposition
   int csr;                        // 1: block-CSR system + PCG (no dense storage: npad = nblk = 0)
-  int pad2_;
+  int bterm;                      // BT_* flags: boundary and body forces of the reference cell model
+  int bnd_off, n_bnd;             // the cell's boundary vertices in the batch's boundary tables
+  int pad3_;
   long long k_off;                // offset of the dense matrix (elements)
   double mu, E, nu, thick, k_lin, rho_max;
+  double k_bend, k_area, area_0, drag, ap_rate, ap_quantum, ap_density;
+};
+
+enum : int { BT_BEND = 1, BT_AREA = 2, BT_AP = 4, BT_DRAG = 8 };
+
+// Boundary tables and per-evaluation scratch of the node terms (batch wide;
+// bidx == nullptr when no cell has boundary or body forces).
+struct BndView {
+  const int *bidx;    // per node: index into the boundary tables, -1 for interior nodes
+  const int *node;    // per boundary vertex: global node id
+  const int *cst;     // per boundary vertex: candidate range [cst[b], cst[b+1])
+  const int2 *cand;   // (nj, nint) candidates, global node ids, in the reference's visiting order
+  int2 *frame;        // per boundary vertex: (j1, j2) at the current positions, (-1, -1) if incomplete
+  double2 *g;         // per boundary vertex: dA/dx = R (x_j2 - x_j1) / 2
+  double *J;          // per boundary vertex: -dF/d(x_i, x_j1, x_j2), 2 x 6 row-major
+  double *area;       // per cell: triangle area sum at the current positions
+  double *gd;         // per cell: sum_b g_b . d_b (refinement matvec)
+  const int *replica; // per cell: Philox stream id
+  double dt;
+  unsigned key_lo, key_hi;
 };
 
 // Per-cell mutable solver state.
@@ -78,6 +100,31 @@ struct View {
   const double *tab_px, *tab_py, *tab_k;
   double kappa;
 };
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox4x32_10(unsigned c[4], unsigned k0, unsigned k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c[0];
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c[2];
+    const unsigned n0 = (unsigned)(p1 >> 32) ^ c[1] ^ k0;
+    const unsigned n2 = (unsigned)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (unsigned)p1;
+    c[3] = (unsigned)p0;
+    c[0] = n0;
+    c[2] = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Uniform in (0, 1): (w + 0.5) 2^-32 with w the first word of Philox(ctr).
+__device__ __forceinline__ double philox_uniform(unsigned a, unsigned b, unsigned c, unsigned d, unsigned k0,
+                                                 unsigned k1) {
+  unsigned ctr[4] = {a, b, c, d};
+  philox4x32_10(ctr, k0, k1);
+  return ((double)ctr[0] + 0.5) * (1.0 / 4294967296.0);
+}
 
 __device__ __forceinline__ double t_act_of(double time) {  // UL:743-757
   return (time < 2.0) ? time * 200.0 : 400.0;

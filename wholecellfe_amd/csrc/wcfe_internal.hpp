@@ -30,7 +30,7 @@ void launch_cell_norm(hipStream_t s, const CellParams *cp, CellState *cs, const 
 void launch_neg_copy(hipStream_t s, const double *src, double *dst, long long n);
 void launch_matvec_ir(hipStream_t s, const View &v, const int *node_cell, const int *adj_start, const int *adj,
                       const double *Ke64, const double *G, const double *delta, double *r, int node_begin,
-                      int node_end, const CellState *cs);
+                      int node_end, const CellState *cs, const BndView &bv);
 void launch_ir_control(hipStream_t s, CellState *cs, const int *cells, int ncell, double lin_rtol, int reset);
 void launch_update_x(hipStream_t s, double2 *x, const double *delta, int node_begin, int node_end);
 void launch_newton_control(hipStream_t s, CellState *cs, const int *cells, int ncell, double rtol, double atol);
@@ -44,6 +44,25 @@ void launch_commit_springs(hipStream_t s, const View &v, const int *sp_cell, con
 void launch_commit_cells(hipStream_t s, CellState *cs, const int *cells, int ncell, double dt);
 template <class T>
 void launch_diag_tau(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell, const T *K);
+
+// ---- boundary and body forces (k_boundary.hip) ----
+enum : int { REF_BEND = 0, REF_AREA = 1, REF_AP = 2 };
+struct RefBoundaryArgs {
+  double k_bend, tot_area, k_area, area_0, dt, rate, quantum, density;
+  unsigned key_lo, key_hi, draw;
+};
+// cst: N+1 candidate offsets (empty range = interior vertex), cand: (nj, nint)
+void launch_ref_boundary(hipStream_t s, int which, int n_nodes, const int *cst, const int2 *cand, const double *x,
+                         const RefBoundaryArgs &a, const double *u, double *F, int *n_open);
+void launch_ref_area_tot(hipStream_t s, int n_tris, const int *tris, const double *x, double *out);
+void launch_bnd_frames(hipStream_t s, const BndView &bv, const double2 *x, int b0, int b1);
+void launch_cell_area(hipStream_t s, const CellParams *cp, const int *cells, int ncell, const int *crange, int ncolor,
+                      const int *el_tri, const double2 *x, double *area);
+void launch_cell_gdot(hipStream_t s, const CellParams *cp, const int *cells, int ncell, const BndView &bv,
+                      const double *delta);
+template <class T>
+void launch_node_terms(hipStream_t s, bool full, const View &v, const BndView &bv, const int *node_cell, int n0, int n1,
+                       T *K, T *Kb, const int *row_ptr, const int *bcol, const int *diag_pos, double *G);
 
 // ---- dense LU and triangular solves (k_lu.hip) ----
 // Dinv holds, per cell and diagonal block, the inverses of the block's unit
