@@ -84,6 +84,9 @@ typedef struct wcfe_options {
   int32_t lu_groups;     /* batched dense LU: cells split in this many groups factored on concurrent
                             streams (1..16; 0 = 4) */
   int32_t lu_depth;      /* static pivoting: block steps per delayed trailing update (1..8; 0 = 4) */
+  int32_t band;          /* 1: dense cells under static pivoting are numbered to narrow their band
+                            (reverse Cuthill-McKee) and factored inside it (fill stays in the band);
+                            0: natural numbering, full matrix */
 } wcfe_options;
 
 typedef struct wcfe_cell_desc {

@@ -25,6 +25,52 @@ def _run(ds, **opt):
     return xs
 
 
+@pytest.mark.parametrize("sym,depth", [(1, 4), (0, 4), (1, 1), (0, 2), (1, 3)])
+def test_band_lu_matches_full_matrix(sym, depth):
+    """Band LU in the reverse Cuthill-McKee numbering (the default) against the
+    full-matrix LU in the natural numbering, for symmetric and full
+    factorisations and delayed-update depths below and above the block
+    bandwidth: one batched Newton step (every cell's first, so all are in the
+    same phase) gives the same update to the refinement tolerance, and a
+    whole load step of each cell alone the same converged positions.  (In a
+    batch, cells that converge early start their next load step while the
+    others iterate, so batch-final positions depend on the iteration counts.)"""
+    print("case sym", sym, "depth", depth, flush=True)
+    ds = [cells.cst_cell(K=k, n_spr=20 + 10 * k, law_table=(k % 2 == 1), seed=k, time=1.0) for k in (3, 9, 16, 22)]
+    steps = []
+    for band in (1, 0):
+        c = W.Context(precision=W.PREC_MIXED, t0=1.0, max_refine=12, symmetric=sym, lu_depth=depth, band=band)
+        for d in ds:
+            cells.add_to_context(c, d)
+        c.finalize()
+        st = c.newton_step()
+        assert st["max_lin_relres"] <= 1e-8
+        steps.append([c.positions(i) for i in range(len(ds))])
+        c.close()
+    for d, a, b in zip(ds, steps[0], steps[1]):
+        assert np.max(np.abs(a - b)) <= 1e-7 * np.max(np.abs(b - d["x"]))
+    for d in ds[2:]:
+        band = _run([d], symmetric=sym, lu_depth=depth)[0]
+        full = _run([d], symmetric=sym, lu_depth=depth, band=0)[0]
+        assert np.max(np.abs(band - full)) <= 1e-9 * np.max(np.abs(full - d["X"]))
+
+
+def test_band_assembled_system_is_natural_order():
+    """The assembled system read back is in the natural numbering whatever
+    numbering the band LU uses."""
+    d = cells.cst_cell(K=16, n_spr=60, law_table=True, seed=3, time=1.0)
+    d["x"] = d["X"] + 0.002 * np.random.default_rng(3).standard_normal(len(d["X"]))
+    out = []
+    for band in (1, 0):
+        c = W.Context(precision=W.PREC_FP64, t0=1.0, band=band)
+        cells.add_to_context(c, d)
+        c.finalize()
+        out.append(c.assembled_system(0))
+        c.close()
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.max(np.abs(out[0][0] - out[1][0])) <= 1e-12 * np.max(np.abs(out[1][0]))
+
+
 @pytest.mark.parametrize("depth,groups", [(1, 1), (2, 3), (3, 2), (6, 4), (8, 1)])
 def test_lu_depth_and_groups_same_solution(depth, groups):
     """Every delayed-update depth and group count factors the same systems:

@@ -23,10 +23,12 @@ __global__ __launch_bounds__(256) void k_fill_systems(const CellParams *cp, cons
   const CellParams p = cp[cells[ci]];
   typedef typename Vec16<T>::type V;
   constexpr int VN = Vec16<T>::n;
-  const int nv = p.npad / VN;
+  // blocks within the fill envelope |i - j| <= env only
   for (int r = blockIdx.x; r < p.npad; r += gridDim.x) {
+    const int rb = r / NB;
+    const int c0 = max(0, rb - p.env) * (NB / VN), c1 = min(p.nblk, rb + p.env + 1) * (NB / VN);
     V *row = reinterpret_cast<V *>(K + p.k_off + (long long)r * p.npad);
-    for (int c = threadIdx.x; c < nv; c += blockDim.x) {
+    for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
       T vals[VN];
 #pragma unroll
       for (int k = 0; k < VN; ++k) vals[k] = (r >= p.ndof && r == c * VN + k) ? T(1) : T(0);
@@ -112,14 +114,15 @@ __global__ __launch_bounds__(128) void k_assemble(View v, int e_begin, int e_end
   } else if (MODE == ASM_FULL) {
     const CellParams &p = v.cp[c];
     T *A = K + p.k_off;
+    const int kp[3] = {v.kpos[n[0]], v.kpos[n[1]], v.kpos[n[2]]};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int pp = 0; pp < 2; ++pp) {
-        T *Ar = A + (long long)(2 * (n[a] - p.node_off) + pp) * p.npad;
+        T *Ar = A + (long long)(2 * kp[a] + pp) * p.npad;
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-          const int col = 2 * (n[b] - p.node_off);
+          const int col = 2 * kp[b];
           Ar[col] += T(Ke[(2 * a + pp) * 6 + 2 * b]);
           Ar[col + 1] += T(Ke[(2 * a + pp) * 6 + 2 * b + 1]);
         }
@@ -272,6 +275,43 @@ void launch_matvec_ir(hipStream_t s, const View &v, const int *node_cell, const 
   if (n <= 0) return;
   hipLaunchKernelGGL(k_matvec_ir, dim3((n + 255) / 256), dim3(256), 0, s, v.el_tri, node_cell, adj_start, adj, Ke64,
                      G, delta, r, node_begin, node_end, cs, v.cp, bv);
+}
+
+// Dense cells solve in their band-reducing numbering: rp = P r before the
+// triangular solves, delta (+)= P^T dp after (cells still refining only).
+__global__ __launch_bounds__(256) void k_perm_gather(const int *node_cell, const CellParams *cp, const int *kpos,
+                                                     const double *r, double *rp, int n0, int n1) {
+  const int n = n0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n1) return;
+  const CellParams &p = cp[node_cell[n]];
+  if (p.csr) return;
+  const long long o = 2LL * (p.node_off + kpos[n]);
+  rp[o] = r[2LL * n];
+  rp[o + 1] = r[2LL * n + 1];
+}
+__global__ __launch_bounds__(256) void k_perm_scatter(const int *node_cell, const CellParams *cp, const CellState *cs,
+                                                      const int *kpos, const double *dp, double *delta, int n0, int n1,
+                                                      int accumulate) {
+  const int n = n0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n1) return;
+  const int c = node_cell[n];
+  const CellParams &p = cp[c];
+  if (p.csr || cs[c].ir_done) return;
+  const long long o = 2LL * (p.node_off + kpos[n]);
+  double *d = delta + 2LL * n;
+  d[0] = accumulate ? d[0] + dp[o] : dp[o];
+  d[1] = accumulate ? d[1] + dp[o + 1] : dp[o + 1];
+}
+void launch_perm_gather(hipStream_t s, const int *node_cell, const CellParams *cp, const int *kpos, const double *r,
+                        double *rp, int n0, int n1) {
+  if (n1 <= n0) return;
+  hipLaunchKernelGGL(k_perm_gather, dim3((n1 - n0 + 255) / 256), dim3(256), 0, s, node_cell, cp, kpos, r, rp, n0, n1);
+}
+void launch_perm_scatter(hipStream_t s, const int *node_cell, const CellParams *cp, const CellState *cs,
+                         const int *kpos, const double *dp, double *delta, int n0, int n1, int accumulate) {
+  if (n1 <= n0) return;
+  hipLaunchKernelGGL(k_perm_scatter, dim3((n1 - n0 + 255) / 256), dim3(256), 0, s, node_cell, cp, cs, kpos, dp, delta,
+                     n0, n1, accumulate);
 }
 
 __global__ void k_ir_control(CellState *cs, const int *cells, int ncell, double lin_rtol, int reset) {

@@ -389,11 +389,11 @@ __global__ __launch_bounds__(256) void k_node_terms(View v, BndView bv, const in
     return;
   }
   T *A = K + p.k_off;
-  const long long r0 = 2LL * (n - p.node_off);
+  const long long r0 = 2LL * v.kpos[n];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     if (m > 0 && !has_bnd) break;
-    const long long cc = 2LL * (cols[m] - p.node_off);
+    const long long cc = 2LL * v.kpos[cols[m]];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       A[(r0 + r) * p.npad + cc] += T(J[6 * r + 2 * m]);
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void k_node_terms(View v, BndView bv, const in
     for (int bb = p.bnd_off; bb < p.bnd_off + p.n_bnd; ++bb) {
       if (bv.frame[bb].x < 0) continue;
       const double2 gm = bv.g[bb];
-      const long long cc = 2LL * (bv.node[bb] - p.node_off);
+      const long long cc = 2LL * v.kpos[bv.node[bb]];
       A[r0 * p.npad + cc] += T(p.k_area * gn.x * gm.x);
       A[r0 * p.npad + cc + 1] += T(p.k_area * gn.x * gm.y);
       A[(r0 + 1) * p.npad + cc] += T(p.k_area * gn.y * gm.x);

@@ -371,7 +371,7 @@ __global__ __launch_bounds__(256) void k_lu_trsm_mm(const CellParams *cp, const 
   const int c = v >> 20;
   const CellParams p = cp[c];
   const int t = v & 0xFFFFF;
-  const int m = p.nblk - k - 1;
+  const int m = min(p.nblk - k - 1, p.bw);
   const bool upper = !p.sym && t < m;
   const int other = k + 1 + ((upper || p.sym) ? t : t - m);
   const long long ld = p.npad;
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
   const int s0 = k + np;
   int bi, bj;
   if (STRIP) {
-    const int m = p.nblk - s0;
+    const int m = min(p.nblk - s0, p.bw);
     if (t < m) { bi = s0 + t; bj = s0; }
     else { bi = s0; bj = s0 + 1 + (t - m); }
   } else if (p.sym) {
@@ -499,7 +499,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
     bi = s0 + i;
     bj = s0 + (t - i * (i + 1) / 2);
   } else {
-    const int mt = p.nblk - s0;
+    const int mt = min(p.nblk - s0, p.bw);
     bi = s0 + t / mt;
     bj = s0 + t % mt;
   }
@@ -868,8 +868,9 @@ __global__ __launch_bounds__(MW_THREADS) void k_lu_solve_mw(const CellParams *cp
   T pf_[S::NP], ps_[S::NP];
 #pragma unroll
   for (int u = 0; u < S::NP; ++u) pf_[u] = ps_[u] = T(0);
-  // the blocks before rf, in the order they are solved
-  for (int st = 0; st < pf; st += UNR) {
+  // the blocks before rf within the band, in the order they are solved (the
+  // first of them is one block beyond rs's band: zero in the fill envelope)
+  for (int st = max(0, pf - p.bw); st < pf; st += UNR) {
     T a0[UNR][S::NP][S::V], a1[UNR][S::NP][S::V];
     unsigned long long w[UNR][NW];
 #pragma unroll

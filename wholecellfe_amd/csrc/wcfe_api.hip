@@ -152,6 +152,93 @@ void boundary_candidates(int n_nodes, int n_tris, const int *tris, std::vector<i
   cst[n_nodes] = (int)cand.size();
 }
 
+// Reverse Cuthill-McKee order of a mesh's nodes (George & Liu, "Computer
+// Solution of Large Sparse Positive Definite Systems", 1981): per connected
+// component, a pseudo-peripheral start node (repeated breadth-first searches
+// from the last level's lowest-degree node while the eccentricity grows),
+// then breadth-first by increasing (degree, index); reversed.  Returns pos:
+// node -> position.
+std::vector<int> rcm_positions(int n, int n_tris, const int *tris) {
+  std::vector<std::vector<int>> adj(n);
+  for (int e = 0; e < n_tris; ++e)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        if (a != b) adj[tris[3 * e + a]].push_back(tris[3 * e + b]);
+  for (auto &l : adj) {
+    std::sort(l.begin(), l.end());
+    l.erase(std::unique(l.begin(), l.end()), l.end());
+  }
+  auto deg = [&](int v) { return (int)adj[v].size(); };
+  for (auto &l : adj)
+    std::stable_sort(l.begin(), l.end(), [&](int a, int b) { return deg(a) < deg(b); });
+  std::vector<int> order, level(n, -1);
+  order.reserve(n);
+  std::vector<char> done(n, 0);
+  auto bfs_levels = [&](int s, std::vector<int> &comp) {  // returns eccentricity
+    comp.clear();
+    comp.push_back(s);
+    level[s] = 0;
+    for (size_t h = 0; h < comp.size(); ++h)
+      for (int w : adj[comp[h]])
+        if (level[w] < 0) {
+          level[w] = level[comp[h]] + 1;
+          comp.push_back(w);
+        }
+    return level[comp.back()];
+  };
+  std::vector<int> comp;
+  for (int s0 = 0; s0 < n; ++s0) {
+    if (done[s0]) continue;
+    // start from the component's lowest-degree node
+    int s = s0;
+    bfs_levels(s, comp);
+    for (int v : comp)
+      if (deg(v) < deg(s)) s = v;
+    for (int v : comp) level[v] = -1;
+    int ecc = bfs_levels(s, comp);
+    for (int it = 0; it < 8; ++it) {
+      int best = -1;
+      for (int v : comp)
+        if (level[v] == ecc && (best < 0 || deg(v) < deg(best))) best = v;
+      for (int v : comp) level[v] = -1;
+      std::vector<int> c2;
+      const int e2 = bfs_levels(best, c2);
+      if (e2 <= ecc) {
+        for (int v : c2) level[v] = -1;
+        break;
+      }
+      s = best;
+      ecc = e2;
+      comp.swap(c2);
+    }
+    for (int v : comp) level[v] = -1;
+    // Cuthill-McKee from s
+    const size_t first = order.size();
+    order.push_back(s);
+    done[s] = 1;
+    for (size_t h = first; h < order.size(); ++h)
+      for (int w : adj[order[h]])
+        if (!done[w]) {
+          done[w] = 1;
+          order.push_back(w);
+        }
+  }
+  std::vector<int> pos(n);
+  for (int i = 0; i < n; ++i) pos[order[n - 1 - i]] = i;
+  return pos;
+}
+
+// Block bandwidth (64-row blocks, 2 rows per node) of a mesh's stiffness
+// pattern under the node positions pos.
+int block_bandwidth(int n_tris, const int *tris, const std::vector<int> &pos) {
+  int bw = 0;
+  for (int e = 0; e < n_tris; ++e)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        bw = std::max(bw, std::abs(pos[tris[3 * e + a]] / (NB / 2) - pos[tris[3 * e + b]] / (NB / 2)));
+  return bw;
+}
+
 // LU schedule: a list of stages (kernel kind, block step k, panel count),
 // each with the exclusive prefix sums of its work items over the cells of a
 // list (all cells batched, or one cell at a time).
@@ -182,18 +269,21 @@ struct LuPlan {
 
   // trailing tiles of an mt x mt block matrix: all, or the lower triangle
   static long long tiles(const CellParams &p, long long mt) { return p.sym ? mt * (mt + 1) / 2 : mt * mt; }
+  // Every trailing window is capped at the block bandwidth bw: with no
+  // pivoting the factors of a band matrix keep the band, and panel p only
+  // reaches blocks up to p + bw.
   static long long items(const CellParams &p, int kind, int k, int np, int pivoting) {
     const int nb = p.nblk;
-    const int m = nb - k - 1;
+    const int m = std::min(nb - k - 1, p.bw);
     switch (kind) {
       case ST_DIAG: return (k == 0 && nb > 0) ? 1 : 0;
       case ST_TRSM: return k < nb ? ((pivoting || p.sym) ? m : 2 * m) : 0;
       case ST_STRIP: {  // block column (and row) k + np, after panels k .. k+np-1
-        const int ms = nb - k - np;
+        const int ms = std::min(nb - k - np, p.bw);
         return ms >= 1 ? (p.sym ? ms : 2 * ms - 1) : 0;
       }
       case ST_GEMM1: return k < nb ? tiles(p, m) : 0;
-      case ST_GEMM: return nb - k - np >= 1 ? tiles(p, nb - k - np) : 0;
+      case ST_GEMM: return nb - k - np >= 1 ? tiles(p, std::min(nb - k - np, p.bw)) : 0;
       case ST_PANEL: return k < nb ? 1 : 0;
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
     }
@@ -359,6 +449,11 @@ struct Batch {
   void *Kb = nullptr, *cgvec = nullptr;         // block values; x, r, z, p, q (2N each) and dinv (4N)
   CgState *cg = nullptr;
   double *partials = nullptr;
+  // dense numbering: per node its position in the cell's dense system; the
+  // permuted right-hand side and correction of the triangular solves
+  int *kpos = nullptr;
+  std::vector<std::vector<int>> kpos_h;
+  double *rp = nullptr, *dp = nullptr;
   // boundary and body forces (cells with bterm != 0)
   int any_bterm = 0, any_area = 0, nbnd = 0;
   int *bt_bidx = nullptr, *bt_node = nullptr, *bt_cst = nullptr, *crange = nullptr;
@@ -370,7 +465,7 @@ struct Batch {
     void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
                     sp_local, cp, cs, node_cell, adj_start, adj, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
                     replica, transitions, plan.d_prefix, row_ptr, bcol, diag_pos, el_bpos, csr_cells, chunks, Kb,
-                    cgvec, cg, partials, bt_bidx, bt_node, bt_cst, crange, bt_cand, bt_frame, bt_g, bt_J, bt_area,
+                    cgvec, cg, partials, kpos, rp, dp, bt_bidx, bt_node, bt_cst, crange, bt_cand, bt_frame, bt_g, bt_J, bt_area,
                     bt_gd};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -471,7 +566,7 @@ struct PhaseScope {
 View make_view(const wcfe_ctx *ctx, const Batch &b) {
   View v;
   v.X = b.X; v.x = b.x; v.ea = b.ea; v.cC = b.cC; v.fau = b.fau;
-  v.el_tri = b.el_tri; v.el_cell = b.el_cell; v.el_spr = b.el_spr; v.el_bpos = b.el_bpos;
+  v.el_tri = b.el_tri; v.el_cell = b.el_cell; v.el_spr = b.el_spr; v.el_bpos = b.el_bpos; v.kpos = b.kpos;
   v.sp_bary = b.sp_bary; v.sp_anchor = b.sp_anchor; v.sp_state = b.sp_state;
   v.cp = b.cp; v.cs = b.cs;
   v.tab_px = ctx->tab_px; v.tab_py = ctx->tab_py; v.tab_k = ctx->tab_k;
@@ -556,6 +651,33 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   b.N = N;
   b.S = S;
   b.kpool = kp;
+  // dense cells under static pivoting: a band-reducing node numbering (RCM
+  // when it narrows the band) and the LU restricted to the band; partial
+  // pivoting (row swaps widen the band), the area term (rank-one coupling of
+  // the whole boundary) and ctx->opt.band == 0 keep the natural numbering and
+  // the full matrix
+  b.kpos_h.assign(b.ncell, {});
+  for (int c = 0; c < b.ncell; ++c) {
+    CellParams &p = b.cp_h[c];
+    const HostCell &h = hc[c];
+    std::vector<int> ident(h.n_nodes);
+    for (int i = 0; i < h.n_nodes; ++i) ident[i] = i;
+    const int full = std::max(p.nblk - 1, 0);
+    p.bw = full;
+    b.kpos_h[c] = ident;
+    if (!p.csr && ctx->opt.band && ctx->opt.pivoting == WCFE_PIVOT_STATIC && !(p.bterm & BT_AREA) && p.nblk > 1) {
+      const int bw_nat = block_bandwidth(h.n_tris, h.tris.data(), ident);
+      std::vector<int> pos = rcm_positions(h.n_nodes, h.n_tris, h.tris.data());
+      const int bw_rcm = block_bandwidth(h.n_tris, h.tris.data(), pos);
+      if (bw_rcm < bw_nat) {
+        b.kpos_h[c] = pos;
+        p.bw = std::min(full, bw_rcm);
+      } else {
+        p.bw = std::min(full, bw_nat);
+      }
+    }
+    p.env = std::min(full, p.bw + std::max(ctx->opt.lu_depth - 1, 1));
+  }
   // colour-major element order
   b.color_start.assign(b.ncolor + 1, 0);
   b.cell_color.assign(b.ncell, std::vector<int>(b.ncolor + 1, 0));
@@ -708,6 +830,16 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   b.adj = dupload(adj);
   b.cells = dupload(ident);
   b.replica = dupload(replica);
+  {
+    std::vector<int> kpos(N);
+    for (int c = 0; c < b.ncell; ++c)
+      for (int i = 0; i < hc[c].n_nodes; ++i) kpos[b.cp_h[c].node_off + i] = b.kpos_h[c][i];
+    b.kpos = dupload(kpos);
+  }
+  b.rp = dalloc<double>(2 * (size_t)N);
+  b.dp = dalloc<double>(2 * (size_t)N);
+  HIPCHECK(hipMemset(b.rp, 0, sizeof(double) * 2 * N));
+  HIPCHECK(hipMemset(b.dp, 0, sizeof(double) * 2 * N));
   if (b.any_bterm) {
     b.bt_bidx = dupload(bidx);
     b.bt_node = dupload(bnode);
@@ -1029,10 +1161,13 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
         launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.ipiv, piv, b.r,
                            b.delta, npad, it > 0);
       } else if (npad > 0) {
+        // in each dense cell's own numbering: rp = P r, solve, delta (+)= P^T dp
         const LuPlan::SolveLists &L = sc.cell < 0 ? b.plan.solve_batched : b.plan.solve_single[sc.cell];
+        launch_perm_gather(s, b.node_cell, b.cp, b.kpos, b.r, b.rp, sc.node_begin, sc.node_end);
         launch_lu_solve_mw<T>(s, b.cp, b.cs, sc.cells, sc.ncell, WorkList{b.plan.d_prefix + L.fwd_off, L.tot},
-                              WorkList{b.plan.d_prefix + L.bwd_off, L.tot}, K, static_cast<const T *>(b.Dinv), b.r,
-                              b.delta, b.ysol, b.sync, it > 0);
+                              WorkList{b.plan.d_prefix + L.bwd_off, L.tot}, K, static_cast<const T *>(b.Dinv), b.rp,
+                              b.dp, b.ysol, b.sync, 0);
+        launch_perm_scatter(s, b.node_cell, b.cp, b.cs, b.kpos, b.dp, b.delta, sc.node_begin, sc.node_end, it > 0);
       }
       enqueue_pcg<T>(ctx, b, sc, it > 0);
     }
@@ -1190,6 +1325,7 @@ void wcfe_default_options(wcfe_options *o) {
   o->cg_max_iter = 20000;
   o->lu_groups = 4;
   o->lu_depth = 4;
+  o->band = 1;
 }
 
 int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
@@ -1847,9 +1983,10 @@ int wcfe_assembled_system(wcfe_ctx *ctx, int32_t cell, double *K, double *G) {
       std::vector<unsigned char> raw((size_t)p.npad * p.npad * es);
       HIPCHECK(hipMemcpy(raw.data(), static_cast<unsigned char *>(b.K) + (size_t)p.k_off * es, raw.size(),
                          hipMemcpyDeviceToHost));
+      const std::vector<int> &kp = b.kpos_h[cell];  // back to the natural numbering
       for (int i = 0; i < p.ndof; ++i)
         for (int j = 0; j < p.ndof; ++j) {
-          const size_t o = (size_t)i * p.npad + j;
+          const size_t o = (size_t)(2 * kp[i / 2] + i % 2) * p.npad + 2 * kp[j / 2] + j % 2;
           K[(size_t)i * p.ndof + j] =
               es == 4 ? (double)reinterpret_cast<float *>(raw.data())[o] : reinterpret_cast<double *>(raw.data())[o];
         }
@@ -1905,6 +2042,7 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
     p.ndof = n[i];
     p.npad = ((n[i] + NB - 1) / NB) * NB;
     p.nblk = p.npad / NB;
+    p.bw = p.env = std::max(p.nblk - 1, 0);  // caller's matrices: no band assumed
     p.k_off = kp;
     p.sys_off = sys;
     p.vec_off = vo;

@@ -30,7 +30,11 @@ struct CellParams {
   int csr;                        // 1: block-CSR system + PCG (no dense storage: npad = nblk = 0)
   int bterm;                      // BT_* flags: boundary and body forces of the reference cell model
   int bnd_off, n_bnd;             // the cell's boundary vertices in the batch's boundary tables
-  int pad3_;
+  int bw;                         // block bandwidth of the dense system: A_ij = 0 for |i - j| > bw
+                                  // (nblk - 1 = full); the LU keeps its fill inside the band
+  int env;                        // blocks |i - j| <= env are zeroed before assembly (bw + delayed
+                                  // panels - 1: the trailing update reads that far off the band)
+  int pad4_;
   long long k_off;                // offset of the dense matrix (elements)
   double mu, E, nu, thick, k_lin, rho_max;
   double k_bend, k_area, area_0, drag, ap_rate, ap_quantum, ap_density;
@@ -89,6 +93,7 @@ struct View {
   const int *el_cell; // owning cell
   const int *el_spr;  // 2 per element: [begin, end) of its hosted barycentric springs
   const int *el_bpos; // 9 per element of a sparse cell: block-CSR index of block (a, b)
+  const int *kpos;    // per node: its position in the cell's dense numbering (row block 2 kpos)
   // barycentric springs (grouped by host element)
   const double *sp_bary;   // 3 per spring
   const double *sp_anchor; // 2 per spring

@@ -32,6 +32,10 @@ void launch_matvec_ir(hipStream_t s, const View &v, const int *node_cell, const 
                       const double *Ke64, const double *G, const double *delta, double *r, int node_begin,
                       int node_end, const CellState *cs, const BndView &bv);
 void launch_ir_control(hipStream_t s, CellState *cs, const int *cells, int ncell, double lin_rtol, int reset);
+void launch_perm_gather(hipStream_t s, const int *node_cell, const CellParams *cp, const int *kpos, const double *r,
+                        double *rp, int n0, int n1);
+void launch_perm_scatter(hipStream_t s, const int *node_cell, const CellParams *cp, const CellState *cs,
+                         const int *kpos, const double *dp, double *delta, int n0, int n1, int accumulate);
 void launch_update_x(hipStream_t s, double2 *x, const double *delta, int node_begin, int node_end);
 void launch_newton_control(hipStream_t s, CellState *cs, const int *cells, int ncell, double rtol, double atol);
 void launch_commit_nodes(hipStream_t s, const View &v, const int *node_cell, double2 *X, double2 *x, double *ea,
