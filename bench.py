@@ -213,7 +213,7 @@ def main():
     ctx.close()
     log("check step:", st)
     log("phases:", {k: round(v["ms"], 3) for k, v in prof.items()})
-    depth = tuned["lu_depth"] if tuned else ctx_depth
+    depth = tuned["lu_depth"] if tuned else (ctx_depth or 1)
     roofline = gemm_roofline(prof, a, depth) if prof["lu_gemm"]["launches"] > 0 else pcg_roofline(cells, prof, st)
     dense = [c for c in cells if c.get("solver", 0) != 2]
     lu_alg = configs.lu_flops(dense, symmetric=a.pivoting == "static")

@@ -92,7 +92,7 @@ def test_autotune_picks_valid_settings_and_keeps_results():
         cells.add_to_context(c, d)
     c.finalize()
     ch = c.autotune(steps=1)
-    assert ch["lu_depth"] in (2, 4, 6) and ch["lu_groups"] in (1, 2, 4) and ch["ms_per_step"] > 0
+    assert ch["lu_depth"] in (1, 2, 4, 6) and ch["lu_groups"] in (1, 2, 4) and ch["ms_per_step"] > 0
     # the batch restarts from the state the cells were added with
     st = c.load_step()
     assert st["max_lin_relres"] <= 1e-8

@@ -92,51 +92,6 @@ __device__ __forceinline__ void store_block(T *g, const T *s, long long ld) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// Diagonal block: factorisation without interchanges (static pivoting) and
-// explicit triangular inverses.  `a` is a 64 x 64 LDS tile (pitch P) holding
-// the block; `prow` a 2 x 64 LDS scratch.  Wave 0 owns the rows (lane r ->
-// row r in registers, loops fully unrolled); all threads of the workgroup
-// call these (the barrier count is uniform).  Returns the number of pivots
-// lane `threadIdx.x` replaced (0 or 1).
-// ---------------------------------------------------------------------------
-template <class T, int P>
-__device__ __forceinline__ int factor_diag(T *a, T *prow, T tau) {
-  const int lane = threadIdx.x;
-  const bool w0 = lane < 64;
-  T row[64];
-  int fixed = 0;
-  if (w0) {
-#pragma unroll
-    for (int c = 0; c < 64; ++c) row[c] = a[lane * P + c];
-  }
-#pragma unroll
-  for (int j = 0; j < 64; ++j) {
-    T *pr = prow + (j & 1) * 64;
-    if (lane == j) {
-      if (!(fabs((double)row[j]) >= (double)tau)) {
-        row[j] = row[j] >= T(0) ? tau : -tau;
-        fixed = 1;
-      }
-#pragma unroll
-      for (int c = j; c < 64; ++c) pr[c] = row[c];
-    }
-    __syncthreads();
-    if (w0 && lane > j) {
-      const T l = row[j] / pr[j];
-      row[j] = l;
-#pragma unroll
-      for (int c = j + 1; c < 64; ++c) row[c] -= l * pr[c];
-    }
-  }
-  if (w0) {
-#pragma unroll
-    for (int c = 0; c < 64; ++c) a[lane * P + c] = row[c];
-  }
-  __syncthreads();
-  return fixed;
-}
-
 // Inverses of the unit lower and the upper triangle of the factored block in
 // `a` (LDS), written row-major to Li and Ui (global).  Wave 0: column c of
 // L^-1 by forward substitution; wave 1: column c of U^-1 by back substitution.
@@ -169,12 +124,14 @@ __device__ __forceinline__ void invert_triangles(const T *a, T *Li, T *Ui) {
   }
 }
 
+template <class T, int PA> __device__ __forceinline__ int factor_invert_diag(T *a, T *w, T *Li, T *Ui, T tau);
+
 // k = 0 of static pivoting: factor block (0, 0) and invert its triangles.
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_diag_inv(const CellParams *cp, CellState *cs, const int *map, int k, T *K, T *Dinv) {
   constexpr int P = 65;
   __shared__ T a[64 * P];
-  __shared__ T prow[128];
+  __shared__ T wk[64 * P];
   const int v = map[blockIdx.x];
   const int c = v >> 20;
   const CellParams p = cp[c];
@@ -183,15 +140,11 @@ __global__ __launch_bounds__(256) void k_lu_diag_inv(const CellParams *cp, CellS
   for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
     a[(idx >> 6) * P + (idx & 63)] = blk[(idx >> 6) * ld + (idx & 63)];
   __syncthreads();
-  const int fixed = factor_diag<T, P>(a, prow, (T)cs[c].tau);
+  T *D = Dinv + dinv_off(p, k);
+  const int fixed = factor_invert_diag<T, P>(a, wk, D, D + 4096, (T)cs[c].tau);
   for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
     blk[(idx >> 6) * ld + (idx & 63)] = a[(idx >> 6) * P + (idx & 63)];
-  T *D = Dinv + dinv_off(p, k);
-  invert_triangles<T, P>(a, D, D + 4096);
-  if (threadIdx.x < 64) {
-    const int cnt = __popcll(__ballot(fixed != 0));
-    if (threadIdx.x == 0 && cnt) atomicAdd(&cs[c].n_static, cnt);
-  }
+  if (fixed) atomicAdd(&cs[c].n_static, fixed);
 }
 template <class T>
 void launch_lu_diag_inv(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K,
@@ -353,6 +306,197 @@ __device__ __forceinline__ void mma_64(const T *xs, const T *ys, typename Acc4<T
   }
 }
 
+// ---------------------------------------------------------------------------
+// Diagonal block, blocked: factorisation without interchanges (pivots below
+// tau replaced by +-tau) and the inverses of its triangles, by 16 x 16
+// sub-blocks.  Per sub-block step kb: wave 0 factors the diagonal sub-block
+// in registers (lane r owns row r, the pivot row read across lanes) and
+// forms the inverse of its unit lower triangle, wave 1 that of its upper
+// triangle; then all waves compute the panel solves as products with those
+// inverses and the trailing update, on MFMA.  Finally the inverses of the
+// whole triangles by block substitution on MFMA:
+//   (L^-1)_ik = -Linv_ii sum_{m=k}^{i-1} L_im (L^-1)_mk,
+//   (U^-1)_kj = -Uinv_kk sum_{m=k+1}^{j} U_km (U^-1)_mj.
+// a: the block in LDS (pitch PA), factored in place; w: 64 x PA LDS scratch
+// (diagonal sub-blocks: the sub-block inverses, Linv strictly below and
+// Uinv on and above the diagonal; off-diagonal: L^-1 below, U^-1 above).
+// Li, Ui: 64 x 64 row-major outputs.  All 256 threads call it (uniform
+// barriers).  Returns the pivots this thread replaced.
+// ---------------------------------------------------------------------------
+template <class T> __device__ __forceinline__ T rdlane(T v, int l);
+template <> __device__ __forceinline__ float rdlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+template <> __device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __hiloint2double(hi, lo);
+}
+
+// operand masks of the 16 x 16 products: as stored, unit lower, upper
+enum : int { M_FULL = 0, M_UNIT_LOWER = 1, M_UPPER = 2 };
+template <int M, class T> __device__ __forceinline__ T masked(T v, int r, int c) {
+  if (M == M_UNIT_LOWER) return c < r ? v : (c == r ? T(1) : T(0));
+  if (M == M_UPPER) return c >= r ? v : T(0);
+  return v;
+}
+
+// acc += s X Y for 16 x 16 sub-blocks X, Y in LDS (pitch PA), one wave
+template <class T, int PA, int MX, int MY>
+__device__ __forceinline__ void mm16(const T *X, const T *Y, typename Acc4<T>::type &acc, T s) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < 16; kk += 4) {
+    const T x = masked<MX>(X[lr * PA + kk + lk], lr, kk + lk);
+    const T y = masked<MY>(Y[(kk + lk) * PA + lr], kk + lk, lr);
+    acc = mfma16(s * x, y, acc);
+  }
+}
+template <class T, int PA>
+__device__ __forceinline__ void acc16_load(typename Acc4<T>::type &acc, const T *Z) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r] = Z[acc_row(T(0), lk, r) * PA + lr];
+}
+template <class T, int PA>
+__device__ __forceinline__ void acc16_store(const typename Acc4<T>::type &acc, T *Z) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Z[acc_row(T(0), lk, r) * PA + lr] = acc[r];
+}
+
+template <class T, int PA> __device__ __forceinline__ int factor_invert_diag(T *a, T *w, T *Li, T *Ui, T tau) {
+  typedef typename Acc4<T>::type acc_t;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15;
+  int fixed = 0;
+  auto sub = [](T *base, int i, int j) { return base + 16 * i * PA + 16 * j; };
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    T *D = sub(a, kb, kb), *WD = sub(w, kb, kb);
+    if (wave == 0) {
+      T row[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) row[c] = D[lr * PA + c];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        T piv = rdlane(row[j], j);
+        if (!(fabs((double)piv) >= (double)tau)) {
+          piv = piv >= T(0) ? tau : -tau;
+          if (lr == j) row[j] = piv;
+          if (lane == j) ++fixed;
+        }
+        if (lr > j) {
+          const T l = row[j] / piv;
+          row[j] = l;
+#pragma unroll
+          for (int c = j + 1; c < 16; ++c) row[c] -= l * rdlane(row[c], j);
+        }
+      }
+      if (lane < 16) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) D[lr * PA + c] = row[c];
+      }
+      // column lr of the inverse of the unit lower triangle (row i of L is in lane i)
+      T x[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = (i == lr) ? T(1) : T(0);
+#pragma unroll
+      for (int i = 1; i < 16; ++i)
+#pragma unroll
+        for (int m = 0; m < i; ++m) x[i] -= rdlane(row[m], i) * x[m];
+      if (lane < 16) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+          if (i > lr) WD[i * PA + lr] = x[i];
+      }
+    }
+    __syncthreads();
+    if (wave == 1) {
+      // column lr of the inverse of the upper triangle
+      T row[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) row[c] = D[lr * PA + c];
+      T x[16];
+#pragma unroll
+      for (int i = 15; i >= 0; --i) {
+        T s = (i == lr) ? T(1) : T(0);
+#pragma unroll
+        for (int m = i + 1; m < 16; ++m) s -= rdlane(row[m], i) * x[m];
+        x[i] = s / rdlane(row[i], i);
+      }
+      if (lane < 16) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+          if (i <= lr) WD[i * PA + lr] = x[i];
+      }
+    }
+    __syncthreads();
+    const int nb = 3 - kb;
+    // panel: L_ik = A_ik Uinv_kk (i > kb), U_kj = Linv_kk A_kj (j > kb)
+    for (int t = wave; t < 2 * nb; t += 4) {
+      acc_t acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = T(0);
+      if (t < nb) {
+        T *Z = sub(a, kb + 1 + t, kb);
+        mm16<T, PA, M_FULL, M_UPPER>(Z, WD, acc, T(1));
+        acc16_store<T, PA>(acc, Z);
+      } else {
+        T *Z = sub(a, kb, kb + 1 + (t - nb));
+        mm16<T, PA, M_UNIT_LOWER, M_FULL>(WD, Z, acc, T(1));
+        acc16_store<T, PA>(acc, Z);
+      }
+    }
+    __syncthreads();
+    // trailing update A_ij -= L_ik U_kj
+    for (int t = wave; t < nb * nb; t += 4) {
+      const int i = kb + 1 + t / nb, j = kb + 1 + t % nb;
+      acc_t acc;
+      acc16_load<T, PA>(acc, sub(a, i, j));
+      mm16<T, PA, M_FULL, M_FULL>(sub(a, i, kb), sub(a, kb, j), acc, T(-1));
+      acc16_store<T, PA>(acc, sub(a, i, j));
+    }
+    __syncthreads();
+  }
+  // inverses of the whole triangles, by distance d from the diagonal
+#pragma unroll
+  for (int d = 1; d < 4; ++d) {
+    const int nt = 4 - d;
+    for (int t = wave; t < 2 * nt; t += 4) {
+      acc_t acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = T(0);
+      if (t < nt) {  // (L^-1)_ik, i = k + d
+        const int k = t, i = k + d;
+        mm16<T, PA, M_FULL, M_UNIT_LOWER>(sub(a, i, k), sub(w, k, k), acc, T(1));
+        for (int m = k + 1; m < i; ++m) mm16<T, PA, M_FULL, M_FULL>(sub(a, i, m), sub(w, m, k), acc, T(1));
+        acc16_store<T, PA>(acc, sub(w, i, k));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = T(0);
+        mm16<T, PA, M_UNIT_LOWER, M_FULL>(sub(w, i, i), sub(w, i, k), acc, T(-1));
+        acc16_store<T, PA>(acc, sub(w, i, k));
+      } else {  // (U^-1)_kj, j = k + d
+        const int k = t - nt, j = k + d;
+        for (int m = k + 1; m < j; ++m) mm16<T, PA, M_FULL, M_FULL>(sub(a, k, m), sub(w, m, j), acc, T(1));
+        mm16<T, PA, M_FULL, M_UPPER>(sub(a, k, j), sub(w, j, j), acc, T(1));
+        acc16_store<T, PA>(acc, sub(w, k, j));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = T(0);
+        mm16<T, PA, M_UPPER, M_FULL>(sub(w, k, k), sub(w, k, j), acc, T(-1));
+        acc16_store<T, PA>(acc, sub(w, k, j));
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
+    const int r = idx >> 6, c = idx & 63;
+    const T v = w[r * PA + c];
+    Li[idx] = r > c ? v : (r == c ? T(1) : T(0));
+    Ui[idx] = r <= c ? v : T(0);
+  }
+  return fixed;
+}
+
 // block triangular solves as products with the stored inverses:
 //   items [0, m):  U_kj = L_kk^-1 A_kj
 //   items [m, 2m): L_ik = A_ik U_kk^-1   (static pivoting only)
@@ -479,7 +623,6 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
   typedef typename Acc4<T>::type acc_t;
   __shared__ __attribute__((aligned(16))) T ls[64 * P];
   __shared__ __attribute__((aligned(16))) T us[64 * P];
-  __shared__ T prow[128];
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int v = map[item];
   const int c = v >> 20;
@@ -553,15 +696,11 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
 #pragma unroll
       for (int r = 0; r < 4; ++r) a[(wr + ti * 16 + acc_row(T(0), lk, r)) * PA + wc + tj * 16 + lr] = acc[ti][tj][r];
   __syncthreads();
-  const int fixed = factor_diag<T, PA>(a, prow, (T)cs[c].tau);
+  T *D = Dinv + dinv_off(p, s0);
+  const int fixed = factor_invert_diag<T, PA>(a, us, D, D + 4096, (T)cs[c].tau);
   for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
     C[(idx >> 6) * ld + (idx & 63)] = a[(idx >> 6) * PA + (idx & 63)];
-  T *D = Dinv + dinv_off(p, s0);
-  invert_triangles<T, PA>(a, D, D + 4096);
-  if (threadIdx.x < 64) {
-    const int cnt = __popcll(__ballot(fixed != 0));
-    if (threadIdx.x == 0 && cnt) atomicAdd(&cs[c].n_static, cnt);
-  }
+  if (fixed) atomicAdd(&cs[c].n_static, fixed);
 }
 template <class T>
 void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
@@ -686,14 +825,14 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_lu_solve(const CellParams *cp
       __syncthreads();
     }
     diag_matvec<T>(y, k0, Dinv + dinv_off(p, kb), tmp);
-    block_column_update<T>(y, A, ld, k0, k0 + NB, p.npad);
+    block_column_update<T>(y, A, ld, k0, k0 + NB, min(p.npad, (kb + 1 + p.bw) * NB));  // rows in the band
     __syncthreads();
   }
   // backward: U with its diagonal
   for (int kb = p.nblk - 1; kb >= 0; --kb) {
     const int k0 = kb * NB;
     diag_matvec<T>(y, k0, Dinv + dinv_off(p, kb) + 4096, tmp);
-    block_column_update<T>(y, A, ld, k0, 0, k0);
+    block_column_update<T>(y, A, ld, k0, max(0, (kb - p.bw) * NB), k0);
     __syncthreads();
   }
   double *dd = delta + (long long)p.vec_off;

@@ -326,6 +326,7 @@ struct LuPlan {
   }
 
   void build(const std::vector<CellParams> &cp, int piv, int groups, int lu_depth) {
+    if (lu_depth < 1 || groups < 1) throw ApiError{WCFE_ERR_ARG, "LU schedule needs depth and groups >= 1"};
     pivoting = piv;
     depth = lu_depth;
     const int n = (int)cp.size();
@@ -454,6 +455,8 @@ struct Batch {
   int *kpos = nullptr;
   std::vector<std::vector<int>> kpos_h;
   double *rp = nullptr, *dp = nullptr;
+  bool solve_wg = false;  // static pivoting: one workgroup per cell walks its band (narrow bands)
+  int lu_depth = 4, lu_groups = 4;  // the LU schedule in effect (options, or their automatic choice)
   // boundary and body forces (cells with bterm != 0)
   int any_bterm = 0, any_area = 0, nbnd = 0;
   int *bt_bidx = nullptr, *bt_node = nullptr, *bt_cst = nullptr, *crange = nullptr;
@@ -676,7 +679,18 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
         p.bw = std::min(full, bw_nat);
       }
     }
-    p.env = std::min(full, p.bw + std::max(ctx->opt.lu_depth - 1, 1));
+  }
+  // automatic LU schedule: narrow bands are latency bound (one block step per
+  // round, one stream); wide or full matrices are throughput bound (delayed
+  // updates over 4 block steps, 4 concurrent cell groups)
+  bool narrow = true;
+  for (const CellParams &p : b.cp_h)
+    if (!p.csr && p.bw > SOLVE_WG_MAX_BW) narrow = false;
+  b.lu_depth = ctx->opt.lu_depth > 0 ? ctx->opt.lu_depth : (narrow ? 1 : 4);
+  b.lu_groups = ctx->opt.lu_groups > 0 ? ctx->opt.lu_groups : (narrow ? 1 : AUTO_GROUPS_MAX);
+  for (CellParams &p : b.cp_h) {
+    const int full = std::max(p.nblk - 1, 0);
+    p.env = std::min(full, p.bw + std::max(b.lu_depth - 1, 1));
   }
   // colour-major element order
   b.color_start.assign(b.ncolor + 1, 0);
@@ -942,7 +956,14 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.cg = dupload(cg);
     b.partials = dalloc<double>(2 * (size_t)std::max(b.nchunk, 1));
   }
-  b.plan.build(b.cp_h, ctx->opt.pivoting, ctx->opt.lu_groups, ctx->opt.lu_depth);
+  b.plan.build(b.cp_h, ctx->opt.pivoting, b.lu_groups, b.lu_depth);
+  // narrow bands: one workgroup per cell runs each triangular sweep (no
+  // cross-workgroup hops); otherwise the block rows of all cells are spread
+  // over workgroups (k_lu_solve_mw)
+  b.solve_wg = ctx->opt.pivoting == WCFE_PIVOT_STATIC && b.max_npad > 0 &&
+               lu_solve_lds_bytes(b.max_npad, elem_size) <= lu_solve_max_dynamic(ctx->max_lds);
+  for (const CellParams &p : b.cp_h)
+    if (!p.csr && p.bw > SOLVE_WG_MAX_BW) b.solve_wg = false;
   if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
       lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
@@ -1164,9 +1185,13 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
         // in each dense cell's own numbering: rp = P r, solve, delta (+)= P^T dp
         const LuPlan::SolveLists &L = sc.cell < 0 ? b.plan.solve_batched : b.plan.solve_single[sc.cell];
         launch_perm_gather(s, b.node_cell, b.cp, b.kpos, b.r, b.rp, sc.node_begin, sc.node_end);
-        launch_lu_solve_mw<T>(s, b.cp, b.cs, sc.cells, sc.ncell, WorkList{b.plan.d_prefix + L.fwd_off, L.tot},
-                              WorkList{b.plan.d_prefix + L.bwd_off, L.tot}, K, static_cast<const T *>(b.Dinv), b.rp,
-                              b.dp, b.ysol, b.sync, 0);
+        if (b.solve_wg)
+          launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.ipiv, 0, b.rp,
+                             b.dp, npad, 0);
+        else
+          launch_lu_solve_mw<T>(s, b.cp, b.cs, sc.cells, sc.ncell, WorkList{b.plan.d_prefix + L.fwd_off, L.tot},
+                                WorkList{b.plan.d_prefix + L.bwd_off, L.tot}, K, static_cast<const T *>(b.Dinv), b.rp,
+                                b.dp, b.ysol, b.sync, 0);
         launch_perm_scatter(s, b.node_cell, b.cp, b.cs, b.kpos, b.dp, b.delta, sc.node_begin, sc.node_end, it > 0);
       }
       enqueue_pcg<T>(ctx, b, sc, it > 0);
@@ -1323,8 +1348,8 @@ void wcfe_default_options(wcfe_options *o) {
   o->dense_max_dof = 8192;
   o->cg_rtol = 1e-5;
   o->cg_max_iter = 20000;
-  o->lu_groups = 4;
-  o->lu_depth = 4;
+  o->lu_groups = 0;  // auto
+  o->lu_depth = 0;   // auto
   o->band = 1;
 }
 
@@ -1354,11 +1379,10 @@ int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
     HIPCHECK(lu_prepare_attributes(ctx->max_lds));
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     if (ctx->opt.lu_groups < 0 || ctx->opt.lu_groups > 16) fail(WCFE_ERR_ARG, "lu_groups must be in [0, 16]");
-    if (ctx->opt.lu_groups == 0) ctx->opt.lu_groups = 4;
     if (ctx->opt.lu_depth < 0 || ctx->opt.lu_depth > 8) fail(WCFE_ERR_ARG, "lu_depth must be in [0, 8]");
-    if (ctx->opt.lu_depth == 0) ctx->opt.lu_depth = 4;
+    if (ctx->opt.band != 0 && ctx->opt.band != 1) fail(WCFE_ERR_ARG, "band must be 0 or 1");
     HIPCHECK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
-    for (int g = 0; g < ctx->opt.lu_groups; ++g) {
+    for (int g = 0; g < std::max(ctx->opt.lu_groups, AUTO_GROUPS_MAX); ++g) {
       hipStream_t st;
       hipEvent_t ev;
       HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -1664,7 +1688,7 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
     }
     // 2. delayed-update depth and concurrent cell groups of the dense LU
     if (flags & WCFE_TUNE_LU) {
-      const int depths[] = {2, 4, 6}, groups[] = {1, 2, 4};
+      const int depths[] = {1, 2, 4, 6}, groups[] = {1, 2, 4};
       const wcfe_options base = best;
       for (int d : depths)
         for (int g : groups) {
@@ -1683,8 +1707,8 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (chosen) {
-      chosen[0] = ctx->opt.lu_depth;
-      chosen[1] = ctx->opt.lu_groups;
+      chosen[0] = ctx->b.lu_depth;
+      chosen[1] = ctx->b.lu_groups;
       chosen[2] = ctx->opt.dense_max_dof;
     }
     if (ms_best) *ms_best = best_ms < 1e300 ? best_ms : 0.0;
@@ -1986,7 +2010,12 @@ int wcfe_assembled_system(wcfe_ctx *ctx, int32_t cell, double *K, double *G) {
       const std::vector<int> &kp = b.kpos_h[cell];  // back to the natural numbering
       for (int i = 0; i < p.ndof; ++i)
         for (int j = 0; j < p.ndof; ++j) {
-          const size_t o = (size_t)(2 * kp[i / 2] + i % 2) * p.npad + 2 * kp[j / 2] + j % 2;
+          const int pi = 2 * kp[i / 2] + i % 2, pj = 2 * kp[j / 2] + j % 2;
+          if (std::abs(pi / NB - pj / NB) > p.env) {  // outside the fill envelope: zero, never stored
+            K[(size_t)i * p.ndof + j] = 0.0;
+            continue;
+          }
+          const size_t o = (size_t)pi * p.npad + pj;
           K[(size_t)i * p.ndof + j] =
               es == 4 ? (double)reinterpret_cast<float *>(raw.data())[o] : reinterpret_cast<double *>(raw.data())[o];
         }
@@ -2090,7 +2119,7 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
   int *dipiv = sc.zeros<int>((size_t)std::max(sys, 1));
   T *dDinv = sc.zeros<T>((size_t)std::max(sys, 1) * 128);
   LuPlan plan;
-  plan.build(cp, ctx->opt.pivoting, 1, ctx->opt.lu_depth);
+  plan.build(cp, ctx->opt.pivoting, 1, ctx->opt.lu_depth > 0 ? ctx->opt.lu_depth : 4);  // full matrices
   sc.ptrs.push_back(plan.d_prefix);
   const bool piv = ctx->opt.pivoting == WCFE_PIVOT_PARTIAL;
   enqueue_lu<T>(ctx, ctx->stream, plan, plan.batched, dcells, n_sys, dcp, dcs, dK, dDinv, dipiv);

@@ -101,6 +101,10 @@ void launch_lu_solve_mw(hipStream_t s, const CellParams *cp, CellState *cs, cons
 size_t lu_solve_exchange_bytes(long long rows, int elem_size);
 size_t lu_solve_lds_bytes(int npad, int elem_size);
 size_t lu_solve_max_dynamic(size_t max_lds);
+// block bandwidth up to which one workgroup per cell runs the triangular sweeps
+constexpr int SOLVE_WG_MAX_BW = 8;
+// concurrent cell groups of the dense LU chosen automatically for wide bands
+constexpr int AUTO_GROUPS_MAX = 4;
 hipError_t lu_prepare_attributes(size_t max_lds);
 
 // ---- sparse path: block-CSR systems and batched PCG (k_csr.hip) ----
