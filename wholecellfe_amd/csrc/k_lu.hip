@@ -333,6 +333,11 @@ template <> __device__ __forceinline__ double rdlane(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
+// reciprocal: the hardware's in FP32 (1 ulp, the factor is refined in FP64),
+// IEEE division in FP64
+__device__ __forceinline__ float recip(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double recip(double v) { return 1.0 / v; }
+
 // operand masks of the 16 x 16 products: as stored, unit lower, upper
 enum : int { M_FULL = 0, M_UNIT_LOWER = 1, M_UPPER = 2 };
 template <int M, class T> __device__ __forceinline__ T masked(T v, int r, int c) {
@@ -373,8 +378,8 @@ template <class T, int PA> __device__ __forceinline__ int factor_invert_diag(T *
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
     T *D = sub(a, kb, kb), *WD = sub(w, kb, kb);
+    T row[16];
     if (wave == 0) {
-      T row[16];
 #pragma unroll
       for (int c = 0; c < 16; ++c) row[c] = D[lr * PA + c];
 #pragma unroll
@@ -385,8 +390,9 @@ template <class T, int PA> __device__ __forceinline__ int factor_invert_diag(T *
           if (lr == j) row[j] = piv;
           if (lane == j) ++fixed;
         }
+        const T inv = recip(piv);
         if (lr > j) {
-          const T l = row[j] / piv;
+          const T l = row[j] * inv;
           row[j] = l;
 #pragma unroll
           for (int c = j + 1; c < 16; ++c) row[c] -= l * rdlane(row[c], j);
@@ -396,33 +402,38 @@ template <class T, int PA> __device__ __forceinline__ int factor_invert_diag(T *
 #pragma unroll
         for (int c = 0; c < 16; ++c) D[lr * PA + c] = row[c];
       }
-      // column lr of the inverse of the unit lower triangle (row i of L is in lane i)
+    }
+    __syncthreads();
+    if (wave == 0) {
+      // column lr of the inverse of the unit lower triangle, column-oriented
+      // (row i of L is in lane i)
       T x[16];
 #pragma unroll
       for (int i = 0; i < 16; ++i) x[i] = (i == lr) ? T(1) : T(0);
 #pragma unroll
-      for (int i = 1; i < 16; ++i)
+      for (int m = 0; m < 15; ++m)
 #pragma unroll
-        for (int m = 0; m < i; ++m) x[i] -= rdlane(row[m], i) * x[m];
+        for (int i = m + 1; i < 16; ++i) x[i] -= rdlane(row[m], i) * x[m];
       if (lane < 16) {
 #pragma unroll
         for (int i = 0; i < 16; ++i)
           if (i > lr) WD[i * PA + lr] = x[i];
       }
-    }
-    __syncthreads();
-    if (wave == 1) {
-      // column lr of the inverse of the upper triangle
-      T row[16];
+    } else if (wave == 1) {
+      // column lr of the inverse of the upper triangle, column-oriented
 #pragma unroll
       for (int c = 0; c < 16; ++c) row[c] = D[lr * PA + c];
-      T x[16];
+      T rinv[16], x[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        rinv[i] = recip(rdlane(row[i], i));
+        x[i] = (i == lr) ? T(1) : T(0);
+      }
 #pragma unroll
       for (int i = 15; i >= 0; --i) {
-        T s = (i == lr) ? T(1) : T(0);
+        x[i] *= rinv[i];
 #pragma unroll
-        for (int m = i + 1; m < 16; ++m) s -= rdlane(row[m], i) * x[m];
-        x[i] = s / rdlane(row[i], i);
+        for (int m = 0; m < i; ++m) x[m] -= rdlane(row[i], m) * x[i];
       }
       if (lane < 16) {
 #pragma unroll
