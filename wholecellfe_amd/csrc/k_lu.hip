@@ -382,21 +382,19 @@ template <class T, int PA> __device__ __forceinline__ int factor_invert_diag(T *
     if (wave == 0) {
 #pragma unroll
       for (int c = 0; c < 16; ++c) row[c] = D[lr * PA + c];
+      // straight-line (no per-step branches, so steps overlap): rows at or
+      // above the pivot take a zero multiplier and stay unchanged
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        T piv = rdlane(row[j], j);
-        if (!(fabs((double)piv) >= (double)tau)) {
-          piv = piv >= T(0) ? tau : -tau;
-          if (lr == j) row[j] = piv;
-          if (lane == j) ++fixed;
-        }
+        const T p0 = rdlane(row[j], j);
+        const bool small = !(fabs((double)p0) >= (double)tau);
+        const T piv = small ? (p0 >= T(0) ? tau : -tau) : p0;
+        fixed += (small && lane == j) ? 1 : 0;
         const T inv = recip(piv);
-        if (lr > j) {
-          const T l = row[j] * inv;
-          row[j] = l;
+        const T l = lr > j ? row[j] * inv : T(0);
+        row[j] = lr > j ? l : (lr == j ? piv : row[j]);
 #pragma unroll
-          for (int c = j + 1; c < 16; ++c) row[c] -= l * rdlane(row[c], j);
-        }
+        for (int c = j + 1; c < 16; ++c) row[c] -= l * rdlane(row[c], j);
       }
       if (lane < 16) {
 #pragma unroll
@@ -413,7 +411,7 @@ template <class T, int PA> __device__ __forceinline__ int factor_invert_diag(T *
 #pragma unroll
       for (int m = 0; m < 15; ++m)
 #pragma unroll
-        for (int i = m + 1; i < 16; ++i) x[i] -= rdlane(row[m], i) * x[m];
+        for (int i = m + 1; i < 16; ++i) x[i] -= D[i * PA + m] * x[m];  // L_im, broadcast read
       if (lane < 16) {
 #pragma unroll
         for (int i = 0; i < 16; ++i)
@@ -421,19 +419,17 @@ template <class T, int PA> __device__ __forceinline__ int factor_invert_diag(T *
       }
     } else if (wave == 1) {
       // column lr of the inverse of the upper triangle, column-oriented
-#pragma unroll
-      for (int c = 0; c < 16; ++c) row[c] = D[lr * PA + c];
       T rinv[16], x[16];
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        rinv[i] = recip(rdlane(row[i], i));
+        rinv[i] = recip(D[i * PA + i]);
         x[i] = (i == lr) ? T(1) : T(0);
       }
 #pragma unroll
       for (int i = 15; i >= 0; --i) {
         x[i] *= rinv[i];
 #pragma unroll
-        for (int m = 0; m < i; ++m) x[m] -= rdlane(row[i], m) * x[i];
+        for (int m = 0; m < i; ++m) x[m] -= D[m * PA + i] * x[i];  // U_mi, broadcast read
       }
       if (lane < 16) {
 #pragma unroll
