@@ -846,6 +846,135 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_lu_solve(const CellParams *cp
   for (int i = threadIdx.x; i < p.ndof; i += SOLVE_THREADS) dd[i] = accumulate ? dd[i] + (double)y[i] : (double)y[i];
 }
 
+// Workgroup barrier for LDS only: outstanding global loads stay in flight
+// (a full __syncthreads also waits for them, which would serialise the
+// prefetch below).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// ---------------------------------------------------------------------------
+// Triangular sweeps of a band-factored cell by one workgroup (static
+// pivoting, block bandwidth bw <= BWM), left-looking:
+//   forward   y_b = Linv_b (r_b - sum_{j=b-bw}^{b-1} L_bj y_j)
+//   backward  x_b = Uinv_b (y_b - sum_{j=b+1}^{b+bw} U_bj x_j)
+// Thread (row r, part q) = (tid / 8, tid % 8) owns columns 8q .. 8q+7 of row
+// r of every 64-row block.  The band blocks and the diagonal inverse of the
+// next block are loaded into registers while the current block is solved
+// (they do not depend on the solution), so each block step costs its
+// arithmetic and two LDS barriers, not a memory round trip.
+// ---------------------------------------------------------------------------
+constexpr int BAND_THREADS = 512;
+
+template <class T, int BWM>
+struct BandRegs {
+  T a[BWM][8];
+  T d[8];
+};
+
+template <class T, int BWM, bool FWD>
+__device__ __forceinline__ void band_load(BandRegs<T, BWM> &R, const CellParams &p, const T *A, const T *Dinv, int b,
+                                          int bw) {
+  const int r = threadIdx.x >> 3, q = threadIdx.x & 7;
+  const long long ld = p.npad;
+  const bool ok = b >= 0 && b < p.nblk;
+#pragma unroll
+  for (int u = 0; u < BWM; ++u) {
+    const int j = FWD ? b - bw + u : b + 1 + u;
+    if (ok && u < bw && j >= 0 && j < p.nblk) {
+      const T *src = A + (long long)(b * NB + r) * ld + j * NB + q * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) R.a[u][e] = src[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) R.a[u][e] = T(0);
+    }
+  }
+  if (ok) {
+    const T *src = Dinv + dinv_off(p, b) + (FWD ? 0 : 4096) + r * 64 + q * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) R.d[e] = src[e];
+  }
+}
+
+template <class T, int BWM, bool FWD>
+__device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb, int b, int bw, int nb) {
+  const int r = threadIdx.x >> 3, q = threadIdx.x & 7;
+  T part = T(0);
+#pragma unroll
+  for (int u = 0; u < BWM; ++u) {
+    const int j = FWD ? b - bw + u : b + 1 + u;
+    if (u < bw && j >= 0 && j < nb) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) part += R.a[u][e] * y[j * NB + q * 8 + e];
+    }
+  }
+  part += __shfl_xor(part, 1, 64);
+  part += __shfl_xor(part, 2, 64);
+  part += __shfl_xor(part, 4, 64);
+  if (q == 0) sb[r] = y[b * NB + r] - part;
+  lds_barrier();
+  T t = T(0);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) t += R.d[e] * sb[q * 8 + e];
+  t += __shfl_xor(t, 1, 64);
+  t += __shfl_xor(t, 2, 64);
+  t += __shfl_xor(t, 4, 64);
+  if (q == 0) y[b * NB + r] = t;
+  lds_barrier();
+}
+
+template <class T, int BWM>
+__global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams *cp, const CellState *cs,
+                                                                const int *cells, const T *K, const T *Dinv,
+                                                                const double *rhs, double *delta) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ T sb[64];
+  T *y = reinterpret_cast<T *>(smem);
+  const int c = cells[blockIdx.x];
+  if (cs[c].ir_done) return;
+  const CellParams p = cp[c];
+  if (p.npad == 0) return;  // sparse cell: solved by PCG
+  const T *A = K + p.k_off;
+  const int bw = p.bw;      // <= BWM (checked by the launcher's caller)
+  const int nb = p.nblk;
+  const double *rr = rhs + (long long)p.vec_off;
+  for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = i < p.ndof ? (T)rr[i] : T(0);
+  BandRegs<T, BWM> R0, R1;
+  band_load<T, BWM, true>(R0, p, A, Dinv, 0, bw);
+  lds_barrier();
+  for (int b = 0; b < nb; b += 2) {
+    band_load<T, BWM, true>(R1, p, A, Dinv, b + 1, bw);
+    band_step<T, BWM, true>(R0, y, sb, b, bw, nb);
+    band_load<T, BWM, true>(R0, p, A, Dinv, b + 2, bw);
+    if (b + 1 < nb) band_step<T, BWM, true>(R1, y, sb, b + 1, bw, nb);
+  }
+  band_load<T, BWM, false>(R0, p, A, Dinv, nb - 1, bw);
+  for (int b = nb - 1; b >= 0; b -= 2) {
+    band_load<T, BWM, false>(R1, p, A, Dinv, b - 1, bw);
+    band_step<T, BWM, false>(R0, y, sb, b, bw, nb);
+    band_load<T, BWM, false>(R0, p, A, Dinv, b - 2, bw);
+    if (b - 1 >= 0) band_step<T, BWM, false>(R1, y, sb, b - 1, bw, nb);
+  }
+  double *dd = delta + (long long)p.vec_off;
+  for (int i = threadIdx.x; i < p.ndof; i += BAND_THREADS) dd[i] = (double)y[i];
+}
+
+template <class T>
+void launch_lu_solve_band(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
+                          const T *K, const T *Dinv, const double *r, double *delta, int max_npad, int max_bw) {
+  if (ncell <= 0) return;
+  const size_t lds = lu_solve_lds_bytes(max_npad, sizeof(T));
+  if (max_bw <= 4)
+    hipLaunchKernelGGL((k_lu_solve_band<T, 4>), dim3(ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv, r,
+                       delta);
+  else
+    hipLaunchKernelGGL((k_lu_solve_band<T, 8>), dim3(ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv, r,
+                       delta);
+}
+
 // ---------------------------------------------------------------------------
 // Triangular solves for static pivoting across many workgroups: one workgroup
 // per 64-row block row, every block row of every cell in one launch.
@@ -1151,10 +1280,14 @@ size_t lu_solve_max_dynamic(size_t max_lds) { return max_lds > 1024 ? max_lds - 
 
 hipError_t lu_prepare_attributes(size_t max_lds) {
   const int dyn = (int)lu_solve_max_dynamic(max_lds);
-  const void *fns[4] = {reinterpret_cast<const void *>(&k_lu_solve<float, true>),
+  const void *fns[8] = {reinterpret_cast<const void *>(&k_lu_solve<float, true>),
                         reinterpret_cast<const void *>(&k_lu_solve<float, false>),
                         reinterpret_cast<const void *>(&k_lu_solve<double, true>),
-                        reinterpret_cast<const void *>(&k_lu_solve<double, false>)};
+                        reinterpret_cast<const void *>(&k_lu_solve<double, false>),
+                        reinterpret_cast<const void *>(&k_lu_solve_band<float, 4>),
+                        reinterpret_cast<const void *>(&k_lu_solve_band<float, 8>),
+                        reinterpret_cast<const void *>(&k_lu_solve_band<double, 4>),
+                        reinterpret_cast<const void *>(&k_lu_solve_band<double, 8>)};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
     if (e != hipSuccess) return e;
@@ -1177,6 +1310,8 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
 }
 
 #define WCFE_INST(T)                                                                                                 \
+  template void launch_lu_solve_band<T>(hipStream_t, const CellParams *, const CellState *, const int *, int,        \
+                                        const T *, const T *, const double *, double *, int, int);                   \
   template void launch_lu_diag_inv<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *); \
   template void launch_lu_panel_pp<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *,       \
                                       int *, T *);                                                                    \

@@ -457,6 +457,7 @@ struct Batch {
   double *rp = nullptr, *dp = nullptr;
   bool solve_wg = false;  // static pivoting: one workgroup per cell walks its band (narrow bands)
   int lu_depth = 4, lu_groups = 4;  // the LU schedule in effect (options, or their automatic choice)
+  int max_bw = 0;                    // widest block band of the dense cells
   // boundary and body forces (cells with bterm != 0)
   int any_bterm = 0, any_area = 0, nbnd = 0;
   int *bt_bidx = nullptr, *bt_node = nullptr, *bt_cst = nullptr, *crange = nullptr;
@@ -962,8 +963,11 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   // over workgroups (k_lu_solve_mw)
   b.solve_wg = ctx->opt.pivoting == WCFE_PIVOT_STATIC && b.max_npad > 0 &&
                lu_solve_lds_bytes(b.max_npad, elem_size) <= lu_solve_max_dynamic(ctx->max_lds);
-  for (const CellParams &p : b.cp_h)
+  b.max_bw = 0;
+  for (const CellParams &p : b.cp_h) {
     if (!p.csr && p.bw > SOLVE_WG_MAX_BW) b.solve_wg = false;
+    if (!p.csr) b.max_bw = std::max(b.max_bw, p.bw);
+  }
   if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
       lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
@@ -1186,8 +1190,8 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
         const LuPlan::SolveLists &L = sc.cell < 0 ? b.plan.solve_batched : b.plan.solve_single[sc.cell];
         launch_perm_gather(s, b.node_cell, b.cp, b.kpos, b.r, b.rp, sc.node_begin, sc.node_end);
         if (b.solve_wg)
-          launch_lu_solve<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.ipiv, 0, b.rp,
-                             b.dp, npad, 0);
+          launch_lu_solve_band<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.rp, b.dp,
+                                  npad, b.max_bw);
         else
           launch_lu_solve_mw<T>(s, b.cp, b.cs, sc.cells, sc.ncell, WorkList{b.plan.d_prefix + L.fwd_off, L.tot},
                                 WorkList{b.plan.d_prefix + L.bwd_off, L.tot}, K, static_cast<const T *>(b.Dinv), b.rp,
