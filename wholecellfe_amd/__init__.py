@@ -35,7 +35,7 @@ EXPORTS = [
     "wcfe_calculate_res", "wcfe_force_fa", "wcfe_update_fa", "wcfe_profile_step",
     "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state", "wcfe_replace_cell", "wcfe_checkpoint_save",
     "wcfe_checkpoint_load", "wcfe_area_tot", "wcfe_bending_force_2d", "wcfe_area_constraint_force",
-    "wcfe_force_ap",
+    "wcfe_force_ap", "wcfe_get_schedule",
 ]
 TUNE_SOLVER, TUNE_LU = 1, 2
 # reference constants of the boundary and body forces (UL_growth_explicit.cpp)
@@ -254,6 +254,14 @@ class Context:
         d = st.as_dict()
         d["iterations"] = it.value
         return d
+
+    def schedule(self):
+        """The dense-LU schedule in effect: lu_depth, lu_groups, max_band
+        (widest block band of the dense cells), band_solve (1: one workgroup
+        per cell along the band)."""
+        v = [C.c_int32(0) for _ in range(4)]
+        self._check(self._L.wcfe_get_schedule(self._h, *[C.byref(x) for x in v]))
+        return dict(lu_depth=v[0].value, lu_groups=v[1].value, max_band=v[2].value, band_solve=v[3].value)
 
     def autotune(self, flags=TUNE_SOLVER | TUNE_LU, steps=2):
         """Time candidate settings on this batch and keep the fastest (call

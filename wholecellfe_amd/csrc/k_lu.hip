@@ -35,6 +35,8 @@
 // whole vector resident in LDS.
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "wcfe_internal.hpp"
 
 namespace wcfe {
@@ -711,14 +713,22 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
 }
 template <class T>
 void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
-                    int next_diag, int npanel, int strip) {
+                    int next_diag, int npanel, int strip, hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (wl.total <= 0) return;
-  if (strip)
+  if (ev_start) {  // profiling (never under graph capture): events at the kernel's own start and end
+    if (strip)
+      hipExtLaunchKernelGGL((k_lu_gemm<T, true>), dim3(wl.total), dim3(256), 0, s, ev_start, ev_stop, 0, cp, cs,
+                            wl.map, k, K, Dinv, next_diag, npanel);
+    else
+      hipExtLaunchKernelGGL((k_lu_gemm<T, false>), dim3(wl.total), dim3(256), 0, s, ev_start, ev_stop, 0, cp, cs,
+                            wl.map, k, K, Dinv, next_diag, npanel);
+  } else if (strip) {
     hipLaunchKernelGGL((k_lu_gemm<T, true>), dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag,
                        npanel);
-  else
+  } else {
     hipLaunchKernelGGL((k_lu_gemm<T, false>), dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag,
                        npanel);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1319,7 +1329,7 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
   template void launch_lu_trsm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *,           \
                                   const T *);                                                                         \
   template void launch_lu_gemm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *,      \
-                                  int, int, int);                                                                     \
+                                  int, int, int, hipEvent_t, hipEvent_t);                                             \
   template void launch_lu_solve<T>(hipStream_t, const CellParams *, const CellState *, const int *, int, const T *,   \
                                    const T *, const int *, int, const double *, double *, int, int);                  \
   template void launch_lu_solve_mw<T>(hipStream_t, const CellParams *, CellState *, const int *, int,               \

@@ -567,6 +567,24 @@ struct PhaseScope {
   }
 };
 
+// Timing of one kernel while wcfe_profile_step runs: events handed to the
+// launch mark the kernel's own start and end (not the queue around it).
+struct KernelTimer {
+  wcfe_ctx *ctx;
+  int phase;
+  long long work;
+  hipEvent_t a = nullptr, b = nullptr;
+  KernelTimer(wcfe_ctx *c, int ph, long long w) : ctx(c), phase(ph), work(w) {
+    if (ctx->prof) {
+      a = ctx->prof->get();
+      b = ctx->prof->get();
+    }
+  }
+  ~KernelTimer() {
+    if (ctx->prof && a) ctx->prof->recs.push_back({phase, a, b, work});
+  }
+};
+
 View make_view(const wcfe_ctx *ctx, const Batch &b) {
   View v;
   v.X = b.X; v.x = b.x; v.ea = b.ea; v.cC = b.cC; v.fau = b.fau;
@@ -689,9 +707,18 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     if (!p.csr && p.bw > SOLVE_WG_MAX_BW) narrow = false;
   b.lu_depth = ctx->opt.lu_depth > 0 ? ctx->opt.lu_depth : (narrow ? 1 : 4);
   b.lu_groups = ctx->opt.lu_groups > 0 ? ctx->opt.lu_groups : (narrow ? 1 : AUTO_GROUPS_MAX);
+  // narrow bands: one workgroup per cell runs each triangular sweep (no
+  // cross-workgroup hops); otherwise the block rows of all cells are spread
+  // over workgroups (k_lu_solve_mw)
+  b.solve_wg = ctx->opt.pivoting == WCFE_PIVOT_STATIC && narrow && b.max_npad > 0 &&
+               lu_solve_lds_bytes(b.max_npad, elem_size) <= lu_solve_max_dynamic(ctx->max_lds);
+  b.max_bw = 0;
   for (CellParams &p : b.cp_h) {
     const int full = std::max(p.nblk - 1, 0);
-    p.env = std::min(full, p.bw + std::max(b.lu_depth - 1, 1));
+    // the delayed update reads panel blocks up to depth - 1 past the band; the
+    // spread-out solve reads one block past it
+    p.env = std::min(full, p.bw + std::max(b.lu_depth - 1, b.solve_wg ? 0 : 1));
+    if (!p.csr) b.max_bw = std::max(b.max_bw, p.bw);
   }
   // colour-major element order
   b.color_start.assign(b.ncolor + 1, 0);
@@ -958,16 +985,6 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.partials = dalloc<double>(2 * (size_t)std::max(b.nchunk, 1));
   }
   b.plan.build(b.cp_h, ctx->opt.pivoting, b.lu_groups, b.lu_depth);
-  // narrow bands: one workgroup per cell runs each triangular sweep (no
-  // cross-workgroup hops); otherwise the block rows of all cells are spread
-  // over workgroups (k_lu_solve_mw)
-  b.solve_wg = ctx->opt.pivoting == WCFE_PIVOT_STATIC && b.max_npad > 0 &&
-               lu_solve_lds_bytes(b.max_npad, elem_size) <= lu_solve_max_dynamic(ctx->max_lds);
-  b.max_bw = 0;
-  for (const CellParams &p : b.cp_h) {
-    if (!p.csr && p.bw > SOLVE_WG_MAX_BW) b.solve_wg = false;
-    if (!p.csr) b.max_bw = std::max(b.max_bw, p.bw);
-  }
   if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
       lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
@@ -1086,18 +1103,18 @@ void enqueue_lu(wcfe_ctx *ctx, hipStream_t s, const LuPlan &plan, const std::vec
         break;
       }
       case ST_STRIP: {
-        PhaseScope ph(ctx, WCFE_PH_LU_STRIP, g.work);
-        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, g.np, 1);
+        KernelTimer kt(ctx, WCFE_PH_LU_STRIP, g.work);
+        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, g.np, 1, kt.a, kt.b);
         break;
       }
       case ST_GEMM1: {
-        PhaseScope ph(ctx, WCFE_PH_LU_GEMM, g.work);
-        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, piv ? 0 : 1, 1, 0);
+        KernelTimer kt(ctx, WCFE_PH_LU_GEMM, g.work);
+        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, piv ? 0 : 1, 1, 0, kt.a, kt.b);
         break;
       }
       case ST_GEMM: {
-        PhaseScope ph(ctx, WCFE_PH_LU_GEMM, g.work);
-        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, g.np, 0);
+        KernelTimer kt(ctx, WCFE_PH_LU_GEMM, g.work);
+        launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, g.np, 0, kt.a, kt.b);
         break;
       }
     }
@@ -2306,6 +2323,17 @@ int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double
     int un = 0;
     HIPCHECK(hipMemcpy(&un, dun, sizeof(int), hipMemcpyDeviceToHost));
     if (n_unlocated) *n_unlocated = un;
+  });
+}
+
+int wcfe_get_schedule(wcfe_ctx *ctx, int32_t *lu_depth, int32_t *lu_groups, int32_t *max_band, int32_t *band_solve) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
+    if (lu_depth) *lu_depth = ctx->b.lu_depth;
+    if (lu_groups) *lu_groups = ctx->b.lu_groups;
+    if (max_band) *max_band = ctx->b.max_bw;
+    if (band_solve) *band_solve = ctx->b.solve_wg ? 1 : 0;
   });
 }
 
