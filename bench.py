@@ -41,7 +41,7 @@ MFMA_FP32_PEAK = 157.3e12
 HBM_PEAK = 8.0e12
 
 
-def cpu_baseline(cells, budget_s=20.0):
+def cpu_baseline(cells, budget_s=15.0):
     """The FP64 oracle (oracle/liboracle.so, one thread) timed on a sample of
     the batch's cells: one Newton step each (assemble K and G, partial-pivot
     LU, solve, update).  The batch rate is extrapolated with the measured
@@ -53,7 +53,8 @@ def cpu_baseline(cells, budget_s=20.0):
     order = sorted(range(len(cells)), key=lambda i: len(cells[i]["X"]))
     sample, spent, rec = [], 0.0, []
     px, py, kk = mesh.stiffening_table()
-    for idx in order[:: max(1, len(order) // 24)]:
+    # every cell with n <= 3200, smallest first, until the budget is spent
+    for idx in order:
         c = cells[idx]
         n = 2 * len(c["X"])
         if n > 3200 or spent > budget_s:
