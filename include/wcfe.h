@@ -13,7 +13,8 @@
  *
  *  (1) Reference-shaped, stateless calls with the reference's argument meaning
  *      (3 components per node, host arrays in and out): wcfe_calculate_res,
- *      wcfe_force_fa, wcfe_update_fa.  They are what include/wcfe_compat.hpp
+ *      wcfe_force_fa, wcfe_force_fa_subs, wcfe_update_fa and the boundary
+ *      forces.  They are what include/wcfe_compat.hpp
  *      forwards the reference call sites to.
  *  (2) The implicit (Newton) step the reference lacks: a context holding a
  *      batch of cells resident in HBM, assembled with an analytic tangent,
@@ -131,6 +132,11 @@ typedef struct wcfe_cell_desc {
   double ap_rate;         /* [1/s], reference 10 (UL:1552) */
   double ap_quantum;      /* [pN] per firing trial, reference 5 (UL:1561) */
   double ap_density;      /* trials per um of boundary (x L/2), reference 100 (UL:1555) */
+  /* the reference's substrate (updateForces_Explicit substrate overload,
+   * UL:429-504) as a cell of the batch */
+  int32_t passive;        /* 1: no active stress (mat = 2: t_act = 0, UL:763); with mu = 1e6 (UL:727) */
+  const double *f_ext;    /* 2N constant nodal load, G -= f_ext (calculateForce_FA_subs, UL:467-470);
+                             NULL = none; update between load steps with wcfe_set_external_forces */
 } wcfe_cell_desc;
 
 typedef struct wcfe_step_stats {
@@ -243,6 +249,11 @@ int wcfe_get_ul_state(wcfe_ctx *ctx, int32_t cell, double *X /* 2N */, double *e
  * starts a new load step. */
 int wcfe_set_ul_state(wcfe_ctx *ctx, int32_t cell, const double *X /* 2N */, const double *ea /* 3N */,
                       const double *cC /* N */, const double *fau /* 2N */, double time);
+/* Replace the constant nodal load of a cell added with f_ext != NULL (2N,
+ * NULL = zero), e.g. the cell's integrin forces on the substrate from
+ * wcfe_force_fa_subs in a staggered cell/substrate step.  It enters the next
+ * residual evaluation; call it between load steps. */
+int wcfe_set_external_forces(wcfe_ctx *ctx, int32_t cell, const double *f_ext);
 int wcfe_get_springs(wcfe_ctx *ctx, int32_t cell, double *state /* n_spr */, double *anchor /* 2 n_spr */);
 int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step, int32_t *newton_iter,
                        int32_t *n_colors, int32_t *ndof);
@@ -267,11 +278,13 @@ int wcfe_dense_solve(wcfe_ctx *ctx, int32_t n_sys, const int32_t *n, const doubl
                      double *relres, int32_t *iters, int32_t *ipiv_out /* optional, sum n */);
 
 /* ---- reference-shaped stateless calls (3 components per node) ----
- * wcfe_calculate_res: calculateRes (UL:632-707) for the cell law (mat = 1,
- * UL:721-768 constants) on planar meshes (z = 0, planar ea).  X, x: 3N; ea:
+ * wcfe_calculate_res: calculateRes (UL:632-707 cell, mat = 1: mu 1000,
+ * active stress; UL:931-987 substrate, mat = 2: mu 1e6, no active stress;
+ * constants UL:721-768) on planar meshes (z = 0, planar ea).  X, x: 3N; ea:
  * 6N Voigt [xx,yy,zz,xy,xz,yz]; outputs Res 3N, Res_ea 6N, MM N. */
 int wcfe_calculate_res(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *X,
-                       const double *x, const double *ea, double time, double *Res, double *Res_ea, double *MM);
+                       const double *x, const double *ea, int32_t mat, double time, double *Res, double *Res_ea,
+                       double *MM);
 /* calculateForce_FA (UL:1198 kMD table, law 1; UL1:1196 linear kint, law 0).
  * x: 3N, cC: N, fau: 3N, F: 3N. */
 int wcfe_force_fa(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
@@ -283,6 +296,16 @@ int wcfe_force_fa(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t 
 int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double *x, double *cA, double *cC,
                    double *fau, double dt, int32_t n_snodes, int32_t n_stris, const int32_t *stris, const double *SX,
                    const double *Sx, int32_t law, double kint, int32_t *n_unlocated);
+/* calculateForce_FA_subs (UL:1305-1475 kMD table, law 1; UL1:1297 linear
+ * kint, law 0): the cell's integrin forces applied to the substrate.  Each
+ * cell node's force acts at x_cell - fau on every substrate triangle that
+ * contains that point in the substrate's current positions Sx, split by
+ * barycentric weight and scaled by c_C * (one-ring area / 3) * rho_max (100).
+ * Cell: x_cell 3N, cC N, fau 3N; substrate: Sx 3M; F: 3M (overwritten).
+ * A point may lie in at most 16 substrate triangles (WCFE_ERR_ARG beyond). */
+int wcfe_force_fa_subs(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x_cell,
+                       const double *cC, const double *fau, int32_t n_snodes, int32_t n_stris, const int32_t *stris,
+                       const double *Sx, int32_t law, double kint, double *F);
 /* calculateAreaTot (UL:1662-1688): out[4] = {area, centroid x, y, z}; x: 3N. */
 int wcfe_area_tot(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
                   double *out);

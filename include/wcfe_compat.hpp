@@ -80,7 +80,8 @@ template <class Mesh> std::vector<int32_t> flat_tris(const Mesh &mesh) {
 }
 
 // calculateRes, cell overload (UL_growth_explicit.cpp:632): FP64 residual,
-// strain increment and lumped mass of the cell law (mat = 1) on the GPU.
+// strain increment and lumped mass of the cell law (mat = 1) or the
+// substrate law (mat = 2) on the GPU.
 // Res_Adif / Res_Cdif (diffusion terms the reference computes but never uses,
 // UL:617-618) are returned as zero vectors.
 template <class Vec3, class VecX, class Mesh>
@@ -89,14 +90,13 @@ void calculateRes(const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t, st
                   VecX &Res, VecX &Res_ea, VecX &MM, VecX &Res_Adif, VecX &Res_Cdif, int mat, const void *centroid,
                   double time) {
   (void)c_A; (void)c_C; (void)dt_real; (void)centroid;
-  if (mat != 1) throw error(WCFE_ERR_ARG, "calculateRes adapter: only the cell law (mat = 1) runs on the GPU");
   const int n = (int)X_t.size();
   std::vector<double> X = flat3(X_t), x = flat3(x_t), ea(6 * (size_t)n);
   for (int i = 0; i < n; ++i)
     for (int q = 0; q < 6; ++q) ea[6 * i + q] = ea_t[i](q);
   std::vector<int32_t> tris = flat_tris(mesh);
   std::vector<double> R(3 * (size_t)n), Rea(6 * (size_t)n), M(n);
-  check(wcfe_calculate_res(context(), n, (int32_t)(tris.size() / 3), tris.data(), X.data(), x.data(), ea.data(),
+  check(wcfe_calculate_res(context(), n, (int32_t)(tris.size() / 3), tris.data(), X.data(), x.data(), ea.data(), mat,
                            time, R.data(), Rea.data(), M.data()));
   Res.resize(3 * n);
   Res_ea.resize(6 * n);
@@ -112,6 +112,16 @@ void calculateRes(const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t, st
   }
 }
 
+// calculateRes, 10-argument overload (UL_growth_explicit.cpp:931), as the
+// substrate's updateForces_Explicit calls it with mat = 2 (UL:451).
+template <class Vec3, class VecX, class Mesh>
+void calculateRes(const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t, std::vector<VecX> &ea_t,
+                  const Mesh &mesh, double dt_real, VecX &Res, VecX &Res_ea, VecX &MM, int mat, double time) {
+  VecX RA, RC;
+  const std::vector<double> none(X_t.size(), 0.0);
+  calculateRes(X_t, x_t, ea_t, none, none, mesh, dt_real, Res, Res_ea, MM, RA, RC, mat, nullptr, time);
+}
+
 // calculateForce_FA (UL_growth_explicit.cpp:1198 / UL1:1196).
 template <class VecX, class Vec3, class Mesh>
 void calculateForce_FA(VecX &F_constraint, const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t,
@@ -124,6 +134,23 @@ void calculateForce_FA(VecX &F_constraint, const std::vector<Vec3> &X_t, const s
                       force_law(), kint, F.data()));
   F_constraint.resize(3 * n);
   for (int i = 0; i < 3 * n; ++i) F_constraint(i) = F[i];
+}
+
+// calculateForce_FA_subs (UL_growth_explicit.cpp:1305 / UL1:1297), as called
+// by the substrate's updateForces_Explicit (UL:467).  F_constraint receives 3
+// components per substrate node.
+template <class VecX, class Vec3, class Mesh>
+void calculateForce_FA_subs(VecX &F_constraint, const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t,
+                            const Mesh &mesh, const std::vector<double> &c_C, const std::vector<Vec3> &x_cell,
+                            std::vector<Vec3> &fa_u, const Mesh &mesh_cell, double kint) {
+  (void)X_t;
+  const int m = (int)x_t.size(), n = (int)x_cell.size();
+  std::vector<double> Sx = flat3(x_t), xc = flat3(x_cell), fau = flat3(fa_u), F(3 * (size_t)m);
+  std::vector<int32_t> st = flat_tris(mesh), ct = flat_tris(mesh_cell);
+  check(wcfe_force_fa_subs(context(), n, (int32_t)(ct.size() / 3), ct.data(), xc.data(), c_C.data(), fau.data(), m,
+                           (int32_t)(st.size() / 3), st.data(), Sx.data(), force_law(), kint, F.data()));
+  F_constraint.resize(3 * m);
+  for (int i = 0; i < 3 * m; ++i) F_constraint(i) = F[i];
 }
 
 // updateFA, 11-argument form (UL_growth_explicit.cpp:2026 / UL1:2005), as

@@ -7,7 +7,7 @@
 // in and out of the reference functions; it contains no mechanics of its own.
 //
 // Usage:  refharness_<variant> <mode> <case_dir>
-//   modes: elem | calcres | forcefa | updatefa | areatot | bend | areacon | ap
+//   modes: elem | calcres | forcefa | updatefa | areatot | bend | areacon | ap | forcefasubs | calcressubs
 // Inputs are read from <case_dir>/<name>.bin (float64 or int64, row-major),
 // outputs written to <case_dir>/out_<name>.bin.  Scalars come from
 // <case_dir>/meta.txt ("key value" lines).
@@ -21,6 +21,8 @@
 //   calculateBendingForce2D            UL:1571
 //   calculateAreaConstraintForce       UL:1691 (tot_area from calculateAreaTot at x)
 //   calculateForce_AP                  UL:1477 (its own srand(time(NULL)) / rand() draws)
+//   calculateForce_FA_subs             UL:1305 (kMD) / UL1:1297 (k1, k31)
+//   calculateRes (substrate overload)  UL:931 with mat = 2
 #include "vec.h"
 #include "nondestructivetrimesh.h"
 #include <Eigen/Dense>
@@ -50,6 +52,13 @@ void updateFA(std::vector<double> &c_A, std::vector<double> &c_C, std::vector<Ve
               const NonDestructiveTriMesh &mesh_cell, const std::vector<Vec3d> &X_t_subs,
               const std::vector<Vec3d> &x_t_subs, const NonDestructiveTriMesh &mesh_subs, double kint);
 Eigen::Vector4d calculateAreaTot(const std::vector<Vec3d> &x_t, const NonDestructiveTriMesh &mesh);
+void calculateForce_FA_subs(Eigen::VectorXd &F_constraint, const std::vector<Vec3d> &X_t, const std::vector<Vec3d> &x_t,
+                            const NonDestructiveTriMesh &mesh, const std::vector<double> &c_C,
+                            const std::vector<Vec3d> &x_cell, std::vector<Vec3d> &fa_u,
+                            const NonDestructiveTriMesh &mesh_cell, double kint);
+void calculateRes(const std::vector<Vec3d> &X_t, const std::vector<Vec3d> &x_t, std::vector<Eigen::VectorXd> &ea_t,
+                  const NonDestructiveTriMesh &mesh, double dt_real, Eigen::VectorXd &Res, Eigen::VectorXd &Res_ea,
+                  Eigen::VectorXd &MM, int mat, double time);
 void calculateForce_AP(Eigen::VectorXd &F_constraint, const std::vector<Vec3d> &X_t, const std::vector<Vec3d> &x_t,
                        const std::vector<double> &c_A, const NonDestructiveTriMesh &mesh, double dt_real);
 void calculateBendingForce2D(Eigen::VectorXd &F_bending, const std::vector<Vec3d> &X_t, const std::vector<Vec3d> &x_t,
@@ -198,6 +207,31 @@ int main(int argc, char **argv) {
     Eigen::VectorXd F(3 * n);
     calculateForce_FA(F, X, x, cC, fau, mesh, meta["kint"]);
     store("F_FA", F.data(), 3 * n);
+    return 0;
+  }
+  if (mode == "forcefasubs") {
+    std::vector<double> cC = load<double>("cC");
+    std::vector<Vec3d> fau = vec3(load<double>("fau"));
+    std::vector<Vec3d> SX = vec3(load<double>("SX")), Sx = vec3(load<double>("Sx"));
+    NonDestructiveTriMesh smesh;
+    build_mesh(smesh, load<long long>("Stris"));
+    Eigen::VectorXd F(3 * SX.size());
+    calculateForce_FA_subs(F, SX, Sx, smesh, cC, x, fau, mesh, meta["kint"]);
+    store("F_subs", F.data(), 3 * SX.size());
+    return 0;
+  }
+  if (mode == "calcressubs") {
+    std::vector<double> ea_raw = load<double>("ea");
+    std::vector<Eigen::VectorXd> ea(n);
+    for (size_t i = 0; i < n; ++i) {
+      ea[i].resize(6);
+      for (int q = 0; q < 6; ++q) ea[i](q) = ea_raw[6 * i + q];
+    }
+    Eigen::VectorXd Res(3 * n), Res_ea(6 * n), MM(n);
+    calculateRes(X, x, ea, mesh, meta["dt"], Res, Res_ea, MM, 2, meta["time"]);
+    store("Res", Res.data(), 3 * n);
+    store("Res_ea", Res_ea.data(), 6 * n);
+    store("MM", MM.data(), n);
     return 0;
   }
   if (mode == "updatefa") {

@@ -473,6 +473,68 @@ extern "C" void orc_update_fa(int n_nodes, int n_tris, const int32_t *tris, cons
   if (n_unlocated) *n_unlocated = unloc;
 }
 
+extern "C" void orc_force_fa_subs(int n_nodes, int n_tris, const int32_t *tris, const double *x_cell,
+                                  const double *cC, const double *fau, int n_snodes, int n_stris,
+                                  const int32_t *stris, const double *Sx, int law, double kint, double *F) {
+  // UL:1305-1475 (law 1, 6-knot MD table) / UL1:1297 (law 0, linear kint):
+  // the integrin force of every cell node, applied to every substrate
+  // triangle containing x_cell - fa_u in the substrate's current positions
+  // (no early exit: a point on a shared edge loads both triangles), split by
+  // barycentric weight.
+  const double rho_i_max = 100;
+  for (int i = 0; i < 3 * n_snodes; ++i) F[i] = 0.0;
+  std::vector<int> st, li;
+  one_rings(n_nodes, n_tris, tris, st, li);
+  for (int ni = 0; ni < n_nodes; ++ni) {
+    double p[3];
+    for (int c = 0; c < 3; ++c) p[c] = x_cell[3 * ni + c] - fau[3 * ni + c];
+    for (int e = 0; e < n_stris; ++e) {
+      const int *t = stris + 3 * e;
+      V3<double> e1, e2, ep;
+      for (int c = 0; c < 3; ++c) {
+        e1.a[c] = Sx[3 * t[1] + c] - Sx[3 * t[0] + c];
+        e2.a[c] = Sx[3 * t[2] + c] - Sx[3 * t[0] + c];
+        ep.a[c] = p[c] - Sx[3 * t[0] + c];
+      }
+      const double A_tri = 0.5 * norm(cross(e1, e2));
+      V3<double> A3n = cross(e1, ep);
+      double A3 = -1;
+      if (A3n.a[2] >= 0) A3 = 0.5 * norm(A3n);
+      double A2 = -1;
+      V3<double> A2n = cross(ep, e2);
+      if (A2n.a[2] >= 0) A2 = 0.5 * norm(A2n);
+      const double A1 = A_tri - A2 - A3;
+      if (!(A1 >= 0 && A2 >= 0 && A3 >= 0)) continue;
+      double Asum = 0.0;  // one-ring area / 3 of the cell node (UL:1358-1371)
+      for (int k = st[ni]; k < st[ni + 1]; ++k) {
+        const int *ct = tris + 3 * li[k];
+        V3<double> E1, E2;
+        for (int c = 0; c < 3; ++c) {
+          E1.a[c] = x_cell[3 * ct[1] + c] - x_cell[3 * ct[0] + c];
+          E2.a[c] = x_cell[3 * ct[2] + c] - x_cell[3 * ct[0] + c];
+        }
+        Asum += 0.5 * norm(cross(E1, E2)) / 3;
+      }
+      const double *u = fau + 3 * ni;
+      double fs[3];
+      if (law == 0) {
+        for (int c = 0; c < 3; ++c) fs[c] = kint * u[c];
+      } else {
+        const double D = ::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        double uD[3] = {0, 0, 0};
+        if (D != 0)
+          for (int c = 0; c < 3; ++c) uD[c] = u[c] / D;
+        bool found;
+        const double f = table_force<double>(MD_N, MD_PX, MD_PY, MD_K, D, &found);
+        for (int c = 0; c < 3; ++c) fs[c] = f * uD[c];
+      }
+      const double w[3] = {A1 / A_tri, A2 / A_tri, A3 / A_tri};
+      for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) F[3 * t[a] + c] += cC[ni] * fs[c] * w[a] * Asum * rho_i_max;
+    }
+  }
+}
+
 // --------------------------------------------------------------------------
 // Boundary forces (UL:1477-1568 calculateForce_AP, UL:1571-1660
 // calculateBendingForce2D, UL:1691-1784 calculateAreaConstraintForce)
@@ -651,6 +713,7 @@ extern "C" int orc_force_ap(int n_nodes, int n_tris, const int32_t *tris, const 
 namespace {
 
 double t_act_of(double time) { return (time < 2) ? time * 200.0 : 400.0; }  // UL:743-757
+double cell_t_act(const orc_cell *c) { return c->passive ? 0.0 : t_act_of(c->time); }  // mat 2: UL:763
 
 // Spring force vector for extension u (2D): (D == 0) ? k0 * u : f(D) * u / D.
 template <class S> void spring_vec(const orc_cell *c, const S *u, S *out) {
@@ -696,7 +759,7 @@ void element_G(const orc_cell *c, int e, const S *xe /*6*/, const std::vector<in
     // 0/0 of an element centroid coinciding with the cell centroid.
     const double far[3] = {1e30, 1e30, 0};
     S Re[9], Rea[18], MMe[3], RA[3], RC[3];
-    ul_element<S>(Xe, x9, ea6, zero3, zero3, 0.0, c->mu, t_act_of(c->time), c->thick, far, Re, Rea, MMe, RA, RC);
+    ul_element<S>(Xe, x9, ea6, zero3, zero3, 0.0, c->mu, cell_t_act(c), c->thick, far, Re, Rea, MMe, RA, RC);
     for (int a = 0; a < 3; ++a) {
       Ge[2 * a] = Re[3 * a];
       Ge[2 * a + 1] = Re[3 * a + 1];
@@ -720,7 +783,7 @@ void element_G(const orc_cell *c, int e, const S *xe /*6*/, const std::vector<in
     S eyy = d[0] * uy[0] + d[1] * uy[1] + d[2] * uy[2];
     S gxy = (d[0] * ux[0] + d[1] * ux[1] + d[2] * ux[2]) + (b[0] * uy[0] + b[1] * uy[1] + b[2] * uy[2]);
     double Dc = c->E / (1.0 - c->nu * c->nu);
-    double sa = 0.5 * t_act_of(c->time);
+    double sa = 0.5 * cell_t_act(c);
     S sxx = Dc * (exx + c->nu * eyy) + sa;
     S syy = Dc * (c->nu * exx + eyy) + sa;
     S sxy = Dc * 0.5 * (1.0 - c->nu) * gxy;
@@ -853,6 +916,8 @@ std::vector<double> planar3(const double *x2, int n) {
 void add_node_terms(const orc_cell *c, double *G, double *K) {
   const int N = c->n_nodes, n = 2 * N;
   const bool bnd = c->k_bend != 0 || c->k_area != 0 || c->ap_u;
+  if (c->f_ext)
+    for (int i = 0; i < n; ++i) G[i] -= c->f_ext[i];
   if (c->drag != 0) {
     const double cd = c->drag / c->dt;
     for (int i = 0; i < n; ++i) {
@@ -1159,7 +1224,7 @@ extern "C" void orc_commit_ul_strain(const orc_cell *c, double *ea_out) {
       E6[0] = E3[0]; E6[1] = E3[1]; E6[2] = 0; E6[3] = E3[2]; E6[4] = 0; E6[5] = 0;
     }
     double Re[9], Re_ea[18], MMe[3], RA[3], RC[3];
-    ul_element<double>(Xe, xe, ea6, zero3, zero3, 0.0, c->mu, t_act_of(c->time), c->thick, far, Re, Re_ea, MMe, RA,
+    ul_element<double>(Xe, xe, ea6, zero3, zero3, 0.0, c->mu, cell_t_act(c), c->thick, far, Re, Re_ea, MMe, RA,
                        RC);
     for (int a = 0; a < 3; ++a) {
       for (int q = 0; q < 6; ++q) Rea[6 * t[a] + q] += Re_ea[6 * a + q];

@@ -49,6 +49,13 @@ void orc_update_fa(int n_nodes, int n_tris, const int32_t *tris, const double *X
                    double *cC, double *fau, double dt, int n_snodes, int n_stris, const int32_t *stris,
                    const double *SX, const double *Sx, int law, double kint, int *n_unlocated);
 
+/* UL:1305-1475 calculateForce_FA_subs (law 1: MD table) / UL1:1297 (law 0:
+ * linear kint): reaction of the integrin springs on the substrate, F (3 Ns).
+ * x_cell, fau: 3N; Sx: current substrate positions, 3 Ns. */
+void orc_force_fa_subs(int n_nodes, int n_tris, const int32_t *tris, const double *x_cell, const double *cC,
+                       const double *fau, int n_snodes, int n_stris, const int32_t *stris, const double *Sx, int law,
+                       double kint, double *F);
+
 /* Boundary vertices and the neighbour candidates the reference's boundary
  * loops visit (UL:1497-1528 and its copies at :1594-1607, :1714-1746), by El
  * Topo's rule (odd edge valence, nondestructivetrimesh.cpp:527-543).
@@ -106,6 +113,11 @@ typedef struct orc_cell {
   double k_bend, k_area, area_0, drag, dt;
   const double *ap_u;    /* N draws in (0,1), NULL = no actin polymerisation force */
   double ap_rate, ap_quantum, ap_density;
+  /* substrate form (UL:429-504): passive = 1 drops the active stress (mat = 2,
+   * t_act = 0, UL:763); f_ext (2N, NULL = none) is a constant nodal load,
+   * G -= f_ext (the cell's integrin forces on the substrate, UL:467-470) */
+  int32_t passive;
+  const double *f_ext;
 } orc_cell;
 
 /* Spring force magnitude f(D) and slope f'(D) (UL:1258-1264 table form). */

@@ -6,7 +6,8 @@
 // NonDestructiveTriMesh num_triangles/get_triangle) keep this file free of
 // the reference's headers.
 //
-// Usage: compat_check <case_dir> <mode>   (mode: calcres | forcefa | updatefa | boundary)
+// Usage: compat_check <case_dir> <mode>   (mode: calcres | calcressubs | forcefa |
+//                                          forcesubs | updatefa | boundary)
 // Reads <name>.bin arrays written by tests/test_gpu_compat.py and writes
 // out_<name>.bin results.
 #include <array>
@@ -84,6 +85,19 @@ int main(int argc, char **argv) {
       store("Res", Res.d);
       store("Res_ea", Rea.d);
       store("MM", MM.d);
+    } else if (mode == "calcressubs") {
+      // the substrate's call (UL:451): 10-argument overload, mat = 2
+      std::vector<double> e = load<double>("ea");
+      std::vector<VecX> ea(X.size());
+      for (size_t i = 0; i < X.size(); ++i) {
+        ea[i].resize(6);
+        for (int q = 0; q < 6; ++q) ea[i](q) = e[6 * i + q];
+      }
+      VecX Res, Rea, MM;
+      wcfe_compat::calculateRes(X, x, ea, m, meta[1], Res, Rea, MM, 2, meta[0]);
+      store("Res", Res.d);
+      store("Res_ea", Rea.d);
+      store("MM", MM.d);
     } else if (mode == "boundary") {
       // updateForces_Explicit's boundary calls (UL:527, :565, :582, :587)
       Vec4 area = wcfe_compat::calculateAreaTot<Vec4>(x, m);
@@ -96,6 +110,15 @@ int main(int argc, char **argv) {
       store("F_bend", Fb.d);
       store("F_area", Fa.d);
       store("F_ap", Fp.d);
+    } else if (mode == "forcesubs") {
+      // the substrate's updateForces_Explicit (UL:467): X, x are the
+      // substrate's, the cell enters through x_cell, c_C, fa_u, mesh_cell
+      std::vector<double> cC = load<double>("cC");
+      std::vector<Vec3> fau = vec3(load<double>("fau")), xc = vec3(load<double>("x_cell"));
+      Mesh cm = mesh(load<int>("cell_tris"));
+      VecX F;
+      wcfe_compat::calculateForce_FA_subs(F, X, x, m, cC, xc, fau, cm, meta[2]);
+      store("F_subs", F.d);
     } else if (mode == "forcefa") {
       std::vector<double> cC = load<double>("cC");
       std::vector<Vec3> fau = vec3(load<double>("fau"));

@@ -8,7 +8,7 @@ small compressed .npz fixtures next to this script.  The fixtures are data
 files, seeded inputs, and the reference's outputs); nothing of the reference's
 source travels.
 
-Usage:  python tests/golden/make_golden.py [--only boundary]
+Usage:  python tests/golden/make_golden.py [--only boundary|substrate]
 """
 import os
 import shutil
@@ -106,6 +106,43 @@ def boundary_cases(disk296, t296, disk483, t483):
     print("wrote bound_disk296_notch")
 
 
+def substrate_cases(disk296, t296, subs, tsubs):
+    """calculateForce_FA_subs (k31 linear, kMD table) and the substrate
+    overload of calculateRes (mat = 2)."""
+    rng = np.random.default_rng(20251004)
+    N = len(disk296)
+    for name, variant, kint in [("forcesubs_k31", "k31", 31000.0), ("forcesubs_kMD", "kMD", -1.0)]:
+        x = deform(disk296, rng, 0.01)
+        D = rng.uniform(0.0, 0.02, N)
+        th = rng.uniform(0, 2 * np.pi, N)
+        fau = np.zeros((N, 3))
+        fau[:, 0], fau[:, 1] = D * np.cos(th), D * np.sin(th)
+        # node 0's substrate point on a substrate vertex: every triangle around
+        # that vertex is loaded (the reference does not stop at the first hit)
+        v = int(np.argmin(np.sum((subs[:, :2] - x[0, :2]) ** 2, axis=1)))
+        fau[0, :2] = x[0, :2] - subs[v, :2]
+        cC = rng.uniform(0, 1, N)
+        SX = subs.copy()
+        Sx = SX.copy()
+        Sx[:, 0] += 0.01 * np.sin(SX[:, 1] / 7.0)
+        Sx[:, 1] += 0.01 * np.cos(SX[:, 0] / 5.0)
+        fau[0, :2] = x[0, :2] - Sx[v, :2]
+        inp = dict(X=disk296, x=x, cC=cC, fau=fau, SX=SX, Sx=Sx, tris=t296, Stris=tsubs)
+        o = run(variant, "forcefasubs", inp, dict(kint=kint))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), kint=kint, **inp, F_subs=o["F_subs"])
+        print("wrote", name)
+    Ns = len(subs)
+    x = subs.copy()
+    x[:, :2] += 0.002 * rng.standard_normal((Ns, 2))
+    ea = np.zeros((Ns, 6))
+    ea[:, [0, 1, 3]] = 1e-4 * rng.standard_normal((Ns, 3))
+    inp = dict(X=subs, x=x, ea=ea, tris=tsubs)
+    o = run("kMD", "calcressubs", inp, dict(dt=0.005, time=1.0))
+    np.savez_compressed(os.path.join(HERE, "calcres_subs.npz"), dt=0.005, time=1.0, **inp, res_Res=o["Res"],
+                        res_Res_ea=o["Res_ea"], res_MM=o["MM"])
+    print("wrote calcres_subs")
+
+
 def main():
     if not os.path.exists(HARNESS.format("kMD")):
         sys.exit("build the reference harness first: make -C oracle ref")
@@ -115,6 +152,9 @@ def main():
     subs, tsubs = read_ecomsol(os.path.join(SCRIPTS, "sq_subs_100um.txt"))
     if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "boundary":
         boundary_cases(disk296, t296, disk483, t483)
+        return
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "substrate":
+        substrate_cases(disk296, t296, subs, tsubs)
         return
 
     # --- element residual / calculateRes (law identical in all variants) ---
@@ -191,6 +231,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "mesh_disk483.npz"), X=disk483[:, :2], tris=t483.astype(np.int32))
     print("wrote mesh_disk483")
     boundary_cases(disk296, t296, disk483, t483)
+    substrate_cases(disk296, t296, subs, tsubs)
 
 
 if __name__ == "__main__":

@@ -56,7 +56,7 @@ class Cell(C.Structure):
         ("tab_n", C.c_int32), ("tab_px", D), ("tab_py", D), ("tab_k", D),
         ("k_bend", C.c_double), ("k_area", C.c_double), ("area_0", C.c_double), ("drag", C.c_double),
         ("dt", C.c_double), ("ap_u", D), ("ap_rate", C.c_double), ("ap_quantum", C.c_double),
-        ("ap_density", C.c_double),
+        ("ap_density", C.c_double), ("passive", C.c_int32), ("f_ext", D),
     ]
 
 
@@ -115,6 +115,8 @@ def make_cell(model):
     c.ap_rate = model.get("ap_rate", 10.0)
     c.ap_quantum = model.get("ap_quantum", 5.0)
     c.ap_density = model.get("ap_density", 100.0)
+    c.passive = model.get("passive", 0)
+    c.f_ext = dp(arr("f_ext", f64))
     c._keep = keep
     return c
 
@@ -159,6 +161,16 @@ def update_fa(tris, X, x, cA, cC, fau, dt, stris, SX, Sx, law, kint):
                         dp(fau), C.c_double(dt), C.c_int(len(SX)), C.c_int(len(stris)), ip(i32(stris)),
                         dp(f64(SX)), dp(f64(Sx)), C.c_int(law), C.c_double(kint), C.byref(unloc))
     return cA, cC, fau, unloc.value
+
+
+def force_fa_subs(tris, x_cell, cC, fau, stris, Sx, law, kint):
+    n = len(x_cell)
+    ns = len(Sx)
+    F = np.zeros(3 * ns)
+    lib().orc_force_fa_subs(C.c_int(n), C.c_int(len(tris)), ip(i32(tris)), dp(f64(x_cell)), dp(f64(cC)),
+                            dp(f64(fau)), C.c_int(ns), C.c_int(len(stris)), ip(i32(stris)), dp(f64(Sx)),
+                            C.c_int(law), C.c_double(kint), dp(F))
+    return F
 
 
 def boundary_candidates(n_nodes, tris):

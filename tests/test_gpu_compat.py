@@ -47,6 +47,14 @@ def test_calculate_res_adapter():
     assert relerr(o["MM"], g["res_MM"]) < 1e-12
 
 
+def test_calculate_res_substrate_adapter():
+    g = golden("calcres_subs")
+    o = run("calcressubs", dict(X=g["X"], x=g["x"], ea=g["ea"], tris=g["tris"]), [float(g["time"]), 0.005, 0, 0])
+    assert relerr(o["Res"], g["res_Res"]) < 1e-10
+    assert relerr(o["Res_ea"], g["res_Res_ea"]) < 1e-10
+    assert relerr(o["MM"], g["res_MM"]) < 1e-12
+
+
 @pytest.mark.parametrize("case,law", [("forcefa_k31", 0), ("forcefa_kMD", 1)])
 def test_force_fa_adapter(case, law):
     g = golden(case)
@@ -83,3 +91,11 @@ def test_boundary_force_adapters(case):
     on = np.abs(a).max(axis=1) > 0
     assert on.sum() > 0
     assert relerr(a[on], f[on]) < 1e-13
+
+
+@pytest.mark.parametrize("case,law", [("forcesubs_k31", 0), ("forcesubs_kMD", 1)])
+def test_force_fa_subs_adapter(case, law):
+    g = golden(case)
+    o = run("forcesubs", dict(X=g["SX"], x=g["Sx"], tris=g["Stris"], x_cell=g["x"], cC=g["cC"], fau=g["fau"],
+                              cell_tris=g["tris"]), [0.0, 0.0, float(g["kint"]), law])
+    assert relerr(o["F_subs"], g["F_subs"]) < 1e-12

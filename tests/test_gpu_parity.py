@@ -71,6 +71,71 @@ def test_update_fa_vs_reference(ctx, case, law):
     assert relerr(fau, g["out_fau"]) < 1e-11
 
 
+@pytest.mark.parametrize("case,law", [("forcesubs_k31", 0), ("forcesubs_kMD", 1)])
+def test_force_fa_subs_vs_reference(ctx, case, law):
+    g = golden(case)
+    print("case", case, flush=True)
+    F = ctx.force_fa_subs(g["tris"], g["x"], g["cC"], g["fau"], g["Stris"], g["Sx"], law, float(g["kint"]))
+    assert relerr(F, g["F_subs"]) < 1e-12
+
+
+def _substrate_variant(g, kind):
+    """The fixture's substrate as given, with every triangle clockwise (the
+    containment test then accepts the triangle's reflection through its first
+    vertex), or lifted off z = 0 (point location falls back to every
+    triangle)."""
+    Stris, SX, Sx = g["Stris"].copy(), g["SX"].copy(), g["Sx"].copy()
+    if kind == "clockwise":
+        Stris = Stris[:, [0, 2, 1]]
+    elif kind == "lifted":
+        SX[:, 2] = 1e-3 * np.sin(SX[:, 0])
+        Sx[:, 2] = SX[:, 2]
+    return Stris, SX, Sx
+
+
+@pytest.mark.parametrize("kind", ["as_given", "clockwise", "lifted"])
+@pytest.mark.parametrize("law", [0, 1])
+def test_force_fa_subs_and_update_fa_vs_oracle(ctx, kind, law):
+    g = golden("forcesubs_kMD")
+    print("substrate", kind, "law", law, flush=True)
+    Stris, SX, Sx = _substrate_variant(g, kind)
+    kint = 31000.0 if law == 0 else -1.0
+    F = ctx.force_fa_subs(g["tris"], g["x"], g["cC"], g["fau"], Stris, Sx, law, kint)
+    Fo = O.force_fa_subs(g["tris"], g["x"], g["cC"], g["fau"], Stris, Sx, law, kint)
+    np.testing.assert_array_equal(np.abs(F) > 0, np.abs(Fo) > 0)
+    assert relerr(F, Fo) < 1e-13
+    cA = np.full(len(g["X"]), 50.0)
+    out = ctx.update_fa(g["X"], g["x"], cA, g["cC"], g["fau"], 0.005, Stris, SX, Sx, law, kint)
+    ref = O.update_fa(g["tris"], g["X"], g["x"], cA, g["cC"], g["fau"], 0.005, Stris, SX, Sx, law, kint)
+    assert out[3] == ref[3]
+    for a, b in zip(out[:3], ref[:3]):
+        assert relerr(a, b) < 1e-12
+
+
+def test_force_fa_subs_large_substrate_vs_oracle(ctx):
+    # 20k-triangle substrate grid, 2k anchors, some on vertices and edges
+    n = 101
+    gx, gy = np.meshgrid(np.linspace(-50, 50, n), np.linspace(-50, 50, n))
+    Sx = np.zeros((n * n, 3))
+    Sx[:, 0], Sx[:, 1] = gx.ravel(), gy.ravel()
+    q = np.arange(n * n).reshape(n, n)
+    a, b, c, d = q[:-1, :-1].ravel(), q[:-1, 1:].ravel(), q[1:, 1:].ravel(), q[1:, :-1].ravel()
+    Stris = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)])
+    g = golden("forcesubs_kMD")
+    rng = np.random.default_rng(5)
+    reps = 7
+    x = np.concatenate([g["x"] + [rng.uniform(-40, 40), rng.uniform(-40, 40), 0] for _ in range(reps)])
+    tris = np.concatenate([g["tris"] + k * len(g["x"]) for k in range(reps)])
+    fau = rng.uniform(-0.01, 0.01, x.shape)
+    fau[:, 2] = 0
+    fau[:40] = x[:40] - Sx[rng.integers(0, n * n, 40)]                      # on vertices
+    fau[40:80] = x[40:80] - (Sx[Stris[:40, 0]] + Sx[Stris[:40, 2]]) / 2  # on diagonals
+    cC = rng.uniform(0, 1, len(x))
+    F = ctx.force_fa_subs(tris, x, cC, fau, Stris, Sx, 0, 1000.0)
+    Fo = O.force_fa_subs(tris, x, cC, fau, Stris, Sx, 0, 1000.0)
+    assert relerr(F, Fo) < 1e-13
+
+
 def test_calculate_res_rejects_out_of_plane(ctx):
     g = golden("ul_disk296_t1p3")
     x = g["x"].copy()

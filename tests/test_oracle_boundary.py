@@ -190,3 +190,26 @@ def test_node_terms_newton_converges_quadratically():
     # quadratic phase: some step squares the relative residual
     rel = h / h[0]
     assert any(rel[k + 1] < 10 * rel[k] ** 2 for k in range(1, it) if rel[k] < 1e-3)
+
+
+# ---------------- substrate form (passive law, constant nodal load) ----------------
+
+def test_passive_cell_is_the_cell_law_without_active_stress():
+    # t_act(0) = 0: a passive cell at t = 1.3 equals an active one at t = 0
+    m = ul_cell(k_bend=0.0, k_area=0.0, drag=0.0, ap_u=None, mu=1.0e6)
+    Gp, Kp = O.assemble(O.make_cell(dict(m, passive=1, time=1.3)))
+    G0, K0 = O.assemble(O.make_cell(dict(m, passive=0, time=0.0)))
+    Ga, _ = O.assemble(O.make_cell(dict(m, passive=0, time=1.3)))
+    np.testing.assert_array_equal(Gp, G0)
+    np.testing.assert_array_equal(Kp, K0)
+    assert np.abs(Ga - Gp).max() > 1e-6 * np.abs(Gp).max()
+
+
+def test_external_load_enters_the_residual_only():
+    m = ul_cell(mu=1.0e6)
+    N = len(m["X"]) // 2
+    f = np.random.default_rng(11).standard_normal(2 * N) * 1e3
+    G1, K1 = O.assemble(O.make_cell(dict(m, passive=1, f_ext=f)))
+    G0, K0 = O.assemble(O.make_cell(dict(m, passive=1)))
+    np.testing.assert_allclose(G1 - G0, -f, rtol=0, atol=1e-12 * np.abs(G0).max())
+    np.testing.assert_array_equal(K1, K0)

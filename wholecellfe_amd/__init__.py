@@ -32,7 +32,7 @@ EXPORTS = [
     "wcfe_add_cell", "wcfe_finalize", "wcfe_newton_step", "wcfe_load_step", "wcfe_synchronize",
     "wcfe_get_positions", "wcfe_set_positions", "wcfe_get_ul_state", "wcfe_get_springs", "wcfe_get_cell_info",
     "wcfe_residual_fp64", "wcfe_element_systems", "wcfe_assembled_system", "wcfe_get_colors", "wcfe_dense_solve",
-    "wcfe_calculate_res", "wcfe_force_fa", "wcfe_update_fa", "wcfe_profile_step",
+    "wcfe_calculate_res", "wcfe_force_fa", "wcfe_force_fa_subs", "wcfe_update_fa", "wcfe_set_external_forces", "wcfe_profile_step",
     "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state", "wcfe_replace_cell", "wcfe_checkpoint_save",
     "wcfe_checkpoint_load", "wcfe_area_tot", "wcfe_bending_force_2d", "wcfe_area_constraint_force",
     "wcfe_force_ap", "wcfe_get_schedule",
@@ -67,6 +67,7 @@ class CellDesc(C.Structure):
         ("k_lin", C.c_double), ("table_id", C.c_int32), ("kinetics", C.c_int32), ("solver", C.c_int32),
         ("k_bend", C.c_double), ("k_area", C.c_double), ("area_0", C.c_double), ("drag", C.c_double),
         ("ap", C.c_int32), ("ap_rate", C.c_double), ("ap_quantum", C.c_double), ("ap_density", C.c_double),
+        ("passive", C.c_int32), ("f_ext", D),
     ]
 
 
@@ -208,12 +209,13 @@ class Context:
               node_springs=0, cC=None, fau=None, rho_max=100.0, spr_tri=None, spr_bary=None, spr_anchor=None,
               spr_state=None, spring_law=SPRING_LINEAR, k_lin=1000.0, table_id=0, kinetics=KIN_NONE,
               solver=SOLVER_AUTO, k_bend=0.0, k_area=0.0, area_0=AREA_0_REF, drag=0.0, ap=0, ap_rate=10.0,
-              ap_quantum=5.0, ap_density=100.0):
+              ap_quantum=5.0, ap_density=100.0, passive=0, f_ext=None):
         X = _f64(X).reshape(-1)
         tris = _i32(tris).reshape(-1)
         arrs = dict(X=X, x=_f64(x), tris=tris, ea=_f64(ea), cC=_f64(cC), fau=_f64(fau), spr_tri=_i32(spr_tri),
-                    spr_bary=_f64(spr_bary), spr_anchor=_f64(spr_anchor), spr_state=_f64(spr_state))
-        for k in ("x", "ea", "cC", "fau", "spr_bary", "spr_anchor", "spr_state"):
+                    spr_bary=_f64(spr_bary), spr_anchor=_f64(spr_anchor), spr_state=_f64(spr_state),
+                    f_ext=_f64(f_ext))
+        for k in ("x", "ea", "cC", "fau", "spr_bary", "spr_anchor", "spr_state", "f_ext"):
             if arrs[k] is not None:
                 arrs[k] = arrs[k].reshape(-1)
         if arrs["spr_tri"] is not None:
@@ -233,6 +235,7 @@ class Context:
         d.solver = solver
         d.k_bend, d.k_area, d.area_0, d.drag = k_bend, k_area, area_0, drag
         d.ap, d.ap_rate, d.ap_quantum, d.ap_density = int(ap), ap_rate, ap_quantum, ap_density
+        d.passive, d.f_ext = int(passive), _dp(arrs["f_ext"])
         return d
 
     def finalize(self):
@@ -300,6 +303,11 @@ class Context:
         self._check(self._L.wcfe_get_positions(self._h, cell, _dp(x)))
         return x
 
+    def set_external_forces(self, cell, f_ext):
+        """New constant nodal load (2N, None = zero) of a cell added with f_ext."""
+        f = _f64(f_ext)
+        self._check(self._L.wcfe_set_external_forces(self._h, cell, _dp(None if f is None else f.reshape(-1))))
+
     def set_positions(self, cell, x):
         x = _f64(x).reshape(-1)
         self._check(self._L.wcfe_set_positions(self._h, cell, _dp(x)))
@@ -363,14 +371,14 @@ class Context:
         return out, rel, it, piv
 
     # ---- reference-shaped stateless calls (3 components per node) ----
-    def calculate_res(self, tris, X, x, ea, time):
-        """GPU counterpart of calculateRes (UL_growth_explicit.cpp:632) for the
-        cell law; returns Res (3N), Res_ea (6N), MM (N)."""
+    def calculate_res(self, tris, X, x, ea, time, mat=1):
+        """GPU counterpart of calculateRes (UL_growth_explicit.cpp:632 cell,
+        mat 1; :931 substrate, mat 2); returns Res (3N), Res_ea (6N), MM (N)."""
         tris, X, x, ea = _i32(tris).reshape(-1), _f64(X).reshape(-1), _f64(x).reshape(-1), _f64(ea).reshape(-1)
         n = len(X) // 3
         Res, Rea, MM = np.zeros(3 * n), np.zeros(6 * n), np.zeros(n)
         self._check(self._L.wcfe_calculate_res(self._h, n, len(tris) // 3, _ip(tris), _dp(X), _dp(x), _dp(ea),
-                                               C.c_double(time), _dp(Res), _dp(Rea), _dp(MM)))
+                                               mat, C.c_double(time), _dp(Res), _dp(Rea), _dp(MM)))
         return Res, Rea, MM
 
     def force_fa(self, tris, x, cC, fau, law, kint):
@@ -380,6 +388,17 @@ class Context:
         F = np.zeros(3 * n)
         self._check(self._L.wcfe_force_fa(self._h, n, len(tris) // 3, _ip(tris), _dp(x), _dp(cC), _dp(fau), law,
                                           C.c_double(kint), _dp(F)))
+        return F
+
+    def force_fa_subs(self, tris, x_cell, cC, fau, stris, Sx, law, kint):
+        """GPU counterpart of calculateForce_FA_subs (UL:1305 / UL1:1297):
+        the cell's integrin forces on the substrate nodes (3 per node)."""
+        tris, x_cell, cC, fau = _i32(tris).reshape(-1), _f64(x_cell).reshape(-1), _f64(cC), _f64(fau).reshape(-1)
+        stris, Sx = _i32(stris).reshape(-1), _f64(Sx).reshape(-1)
+        F = np.zeros(len(Sx))
+        self._check(self._L.wcfe_force_fa_subs(self._h, len(x_cell) // 3, len(tris) // 3, _ip(tris), _dp(x_cell),
+                                               _dp(cC), _dp(fau), len(Sx) // 3, len(stris) // 3, _ip(stris),
+                                               _dp(Sx), law, C.c_double(kint), _dp(F)))
         return F
 
     def area_tot(self, tris, x):

@@ -291,6 +291,10 @@ __global__ __launch_bounds__(256) void k_node_terms(View v, BndView bv, const in
   if (!p.bterm) return;
   const double2 xn = v.x[n];
   double gx = 0.0, gy = 0.0, cd = 0.0;
+  if (p.bterm & BT_EXT) {  // constant load: the cell's integrin forces on a substrate (UL:467-470)
+    gx -= bv.fext[2LL * n];
+    gy -= bv.fext[2LL * n + 1];
+  }
   if (p.bterm & BT_DRAG) {  // F_vd = -drag v, v = (x - X) / dt (UL:568-574)
     cd = p.drag / bv.dt;
     const double2 Xn = v.X[n];

@@ -61,12 +61,43 @@ void launch_ref_force_fa(hipStream_t s, int n_nodes, const int *ring_start, cons
                      cC, fau, law, kint, F);
 }
 
+// Point location on the substrate: the reference's containment test (signed
+// z of two sub-triangle normals, A1 = A_tri - A2 - A3 >= 0; UL:2070-2095,
+// UL:1335-1347) on the triangles of the point's cell of a uniform grid.  A
+// cell lists every triangle whose (slightly widened) bounding box meets it,
+// in increasing index, so scanning it visits every containing triangle in
+// the reference's order.
+__device__ __forceinline__ int grid_cell(const PointGrid &g, double px, double py) {
+#pragma clang fp contract(off)
+  const double fx = floor((px - g.ox) * g.inv_h), fy = floor((py - g.oy) * g.inv_h);
+  const int ix = fx >= 0.0 ? (fx < g.gx ? (int)fx : g.gx - 1) : 0;  // NaN falls to cell 0 (no triangle contains it)
+  const int iy = fy >= 0.0 ? (fy < g.gy ? (int)fy : g.gy - 1) : 0;
+  return iy * g.gx + ix;
+}
+__device__ __forceinline__ bool tri_contains(const double *q0, const double *q1, const double *q2, const double *p,
+                                             double &A_tri, double &A1, double &A2, double &A3) {
+#pragma clang fp contract(off)
+  const double e1[3] = {q1[0] - q0[0], q1[1] - q0[1], q1[2] - q0[2]};
+  const double e2[3] = {q2[0] - q0[0], q2[1] - q0[1], q2[2] - q0[2]};
+  const double ep[3] = {p[0] - q0[0], p[1] - q0[1], p[2] - q0[2]};
+  const double nn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  A_tri = 0.5 * sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+  const double a3[3] = {e1[1] * ep[2] - e1[2] * ep[1], e1[2] * ep[0] - e1[0] * ep[2], e1[0] * ep[1] - e1[1] * ep[0]};
+  A3 = -1.0;
+  if (a3[2] >= 0.0) A3 = 0.5 * sqrt(a3[0] * a3[0] + a3[1] * a3[1] + a3[2] * a3[2]);
+  const double a2[3] = {ep[1] * e2[2] - ep[2] * e2[1], ep[2] * e2[0] - ep[0] * e2[2], ep[0] * e2[1] - ep[1] * e2[0]};
+  A2 = -1.0;
+  if (a2[2] >= 0.0) A2 = 0.5 * sqrt(a2[0] * a2[0] + a2[1] * a2[1] + a2[2] * a2[2]);
+  A1 = A_tri - A2 - A3;
+  return A1 >= 0.0 && A2 >= 0.0 && A3 >= 0.0;
+}
+
 // updateFA 11-argument form (UL:2026-2174, law 1; UL1:2005-2115, law 0), one
-// thread per cell node.  The substrate point X - fa_u is located by testing
-// every substrate triangle in its reference configuration with the
+// thread per cell node.  The substrate point X - fa_u is located among the
+// substrate triangles (reference configuration) of its grid cell with the
 // reference's z-sign tests; the last containing triangle wins (UL:2098).
 __global__ __launch_bounds__(256) void k_ref_update_fa(int n_nodes, const double *X, const double *x, double *cA,
-                                                       double *cC, double *fau, double dt, int n_stris,
+                                                       double *cC, double *fau, double dt, PointGrid grid,
                                                        const int *stris, const double *SX, const double *Sx, int law,
                                                        double kint, int *n_unlocated) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,22 +108,11 @@ __global__ __launch_bounds__(256) void k_ref_update_fa(int n_nodes, const double
     sX[c] = X[3 * i + c] - fau[3 * i + c];
   }
   bool located = false;
-  for (int e = 0; e < n_stris; ++e) {
-    const int *t = stris + 3 * e;
-    const double *q0 = SX + 3 * t[0], *q1 = SX + 3 * t[1], *q2 = SX + 3 * t[2];
-    const double e1[3] = {q1[0] - q0[0], q1[1] - q0[1], q1[2] - q0[2]};
-    const double e2[3] = {q2[0] - q0[0], q2[1] - q0[1], q2[2] - q0[2]};
-    const double ep[3] = {sX[0] - q0[0], sX[1] - q0[1], sX[2] - q0[2]};
-    const double nn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-    const double A_tri = 0.5 * sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
-    const double a3[3] = {e1[1] * ep[2] - e1[2] * ep[1], e1[2] * ep[0] - e1[0] * ep[2], e1[0] * ep[1] - e1[1] * ep[0]};
-    double A3 = -1.0;
-    if (a3[2] >= 0.0) A3 = 0.5 * sqrt(a3[0] * a3[0] + a3[1] * a3[1] + a3[2] * a3[2]);
-    const double a2[3] = {ep[1] * e2[2] - ep[2] * e2[1], ep[2] * e2[0] - ep[0] * e2[2], ep[0] * e2[1] - ep[1] * e2[0]};
-    double A2 = -1.0;
-    if (a2[2] >= 0.0) A2 = 0.5 * sqrt(a2[0] * a2[0] + a2[1] * a2[1] + a2[2] * a2[2]);
-    const double A1 = A_tri - A2 - A3;
-    if (A1 >= 0.0 && A2 >= 0.0 && A3 >= 0.0) {
+  const int gc = grid_cell(grid, sX[0], sX[1]);
+  for (int k = grid.start[gc]; k < grid.start[gc + 1]; ++k) {  // last containing triangle wins (UL:2098)
+    const int *t = stris + 3 * grid.list[k];
+    double A_tri, A1, A2, A3;
+    if (tri_contains(SX + 3 * t[0], SX + 3 * t[1], SX + 3 * t[2], sX, A_tri, A1, A2, A3)) {
       for (int c = 0; c < 3; ++c)
         subs_u[c] = (A1 / A_tri) * (Sx[3 * t[0] + c] - SX[3 * t[0] + c]) +
                     (A2 / A_tri) * (Sx[3 * t[1] + c] - SX[3 * t[1] + c]) +
@@ -125,11 +145,79 @@ __global__ __launch_bounds__(256) void k_ref_update_fa(int n_nodes, const double
 }
 
 void launch_ref_update_fa(hipStream_t s, int n_nodes, const double *X, const double *x, double *cA, double *cC,
-                          double *fau, double dt, int n_stris, const int *stris, const double *SX, const double *Sx,
-                          int law, double kint, int *n_unlocated) {
+                          double *fau, double dt, const PointGrid &grid, const int *stris, const double *SX,
+                          const double *Sx, int law, double kint, int *n_unlocated) {
   if (n_nodes <= 0) return;
   hipLaunchKernelGGL(k_ref_update_fa, dim3((n_nodes + 255) / 256), dim3(256), 0, s, n_nodes, X, x, cA, cC, fau, dt,
-                     n_stris, stris, SX, Sx, law, kint, n_unlocated);
+                     grid, stris, SX, Sx, law, kint, n_unlocated);
+}
+
+// calculateForce_FA_subs (UL:1305-1475, law 1 MD table; UL1:1297, law 0
+// linear kint), one thread per cell node: the node's integrin force at the
+// substrate point x_cell - fa_u, for every substrate triangle containing it
+// in the substrate's current positions, split by barycentric weight.  Each
+// hit's 9 contributions are written out (hits in increasing triangle order,
+// at most SUBS_MAX_HITS per node) and summed on the host in the reference's
+// order.
+__global__ __launch_bounds__(256) void k_ref_force_fa_subs(int n_nodes, const int *ring_start, const int *ring,
+                                                           const int *tris, const double *x_cell, const double *cC,
+                                                           const double *fau, PointGrid grid, const int *stris,
+                                                           const double *Sx, int law, double kint, int *hit_tri,
+                                                           double *hit_val, int *n_hits, int *overflow) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  const double p[3] = {x_cell[3 * i] - fau[3 * i], x_cell[3 * i + 1] - fau[3 * i + 1],
+                       x_cell[3 * i + 2] - fau[3 * i + 2]};
+  const int gc = grid_cell(grid, p[0], p[1]);
+  int h = 0;
+  double Asum = -1.0, fs[3] = {0.0, 0.0, 0.0};
+  for (int k = grid.start[gc]; k < grid.start[gc + 1]; ++k) {
+    const int e = grid.list[k];
+    const int *t = stris + 3 * e;
+    double A_tri, A1, A2, A3;
+    if (!tri_contains(Sx + 3 * t[0], Sx + 3 * t[1], Sx + 3 * t[2], p, A_tri, A1, A2, A3)) continue;
+    if (Asum < 0.0) {  // first hit: the node's one-ring area / 3 and its spring force
+      Asum = 0.0;
+      for (int r = ring_start[i]; r < ring_start[i + 1]; ++r) {
+        const int *ct = tris + 3 * ring[r];
+        const double *p0 = x_cell + 3 * ct[0], *p1 = x_cell + 3 * ct[1], *p2 = x_cell + 3 * ct[2];
+        const double E1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        const double E2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+        const double nx = E1[1] * E2[2] - E1[2] * E2[1], ny = E1[2] * E2[0] - E1[0] * E2[2],
+                     nz = E1[0] * E2[1] - E1[1] * E2[0];
+        Asum += 0.5 * sqrt(nx * nx + ny * ny + nz * nz) / 3;
+      }
+      const double *u = fau + 3 * i;
+      if (law == 0) {
+        for (int c = 0; c < 3; ++c) fs[c] = kint * u[c];
+      } else {
+        const double D = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        const double f = md_force(D);
+        for (int c = 0; c < 3; ++c) fs[c] = D == 0.0 ? 0.0 : f * (u[c] / D);
+      }
+    }
+    if (h == SUBS_MAX_HITS) {
+      atomicAdd(overflow, 1);
+      break;
+    }
+    const double w[3] = {A1 / A_tri, A2 / A_tri, A3 / A_tri};
+    hit_tri[i * SUBS_MAX_HITS + h] = e;
+    double *v = hit_val + 9LL * (i * SUBS_MAX_HITS + h);
+    const double rho_i_max = 100.0;  // UL:1391
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) v[3 * a + c] = cC[i] * fs[c] * w[a] * Asum * rho_i_max;
+    ++h;
+  }
+  n_hits[i] = h;
+}
+void launch_ref_force_fa_subs(hipStream_t s, int n_nodes, const int *ring_start, const int *ring, const int *tris,
+                              const double *x_cell, const double *cC, const double *fau, const PointGrid &grid,
+                              const int *stris, const double *Sx, int law, double kint, int *hit_tri,
+                              double *hit_val, int *n_hits, int *overflow) {
+  if (n_nodes <= 0) return;
+  hipLaunchKernelGGL(k_ref_force_fa_subs, dim3((n_nodes + 255) / 256), dim3(256), 0, s, n_nodes, ring_start, ring,
+                     tris, x_cell, cC, fau, grid, stris, Sx, law, kint, hit_tri, hit_val, n_hits, overflow);
 }
 
 // r = b - A x in FP64 for dense systems (one workgroup per system, one row per

@@ -67,6 +67,9 @@ struct HostCell {
   // boundary and body forces of the reference cell model
   int ap = 0;
   double k_bend = 0, k_area = 0, area_0 = 0, drag = 0, ap_rate = 0, ap_quantum = 0, ap_density = 0;
+  // substrate form: no active stress; constant nodal load (empty = none)
+  int passive = 0;
+  std::vector<double> f_ext;
   std::vector<double> X, x, ea, cC, fau, spr_bary, spr_anchor, spr_state;
   std::vector<int> tris, spr_tri;
 };
@@ -463,14 +466,14 @@ struct Batch {
   int *bt_bidx = nullptr, *bt_node = nullptr, *bt_cst = nullptr, *crange = nullptr;
   int2 *bt_cand = nullptr, *bt_frame = nullptr;
   double2 *bt_g = nullptr;
-  double *bt_J = nullptr, *bt_area = nullptr, *bt_gd = nullptr;
+  double *bt_J = nullptr, *bt_area = nullptr, *bt_gd = nullptr, *bt_fext = nullptr;
 
   void release() {
     void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
                     sp_local, cp, cs, node_cell, adj_start, adj, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
                     replica, transitions, plan.d_prefix, row_ptr, bcol, diag_pos, el_bpos, csr_cells, chunks, Kb,
                     cgvec, cg, partials, kpos, rp, dp, bt_bidx, bt_node, bt_cst, crange, bt_cand, bt_frame, bt_g, bt_J, bt_area,
-                    bt_gd};
+                    bt_gd, bt_fext};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     *this = Batch();
@@ -650,7 +653,8 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     p.sys_off = sys;
     p.vec_off = 2 * N;
     p.bterm = (h.k_bend != 0 ? BT_BEND : 0) | (h.k_area != 0 ? BT_AREA : 0) | (h.ap ? BT_AP : 0) |
-              (h.drag != 0 ? BT_DRAG : 0);
+              (h.drag != 0 ? BT_DRAG : 0) | (!h.f_ext.empty() ? BT_EXT : 0);
+    p.passive = h.passive;
     p.k_bend = h.k_bend; p.k_area = h.k_area; p.area_0 = h.area_0; p.drag = h.drag;
     p.ap_rate = h.ap_rate; p.ap_quantum = h.ap_quantum; p.ap_density = h.ap_density;
     // the tangent is symmetric unless node-attached springs (area derivative)
@@ -898,6 +902,15 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.bt_gd = dalloc<double>(b.ncell);
     HIPCHECK(hipMemset(b.bt_area, 0, sizeof(double) * b.ncell));
     HIPCHECK(hipMemset(b.bt_gd, 0, sizeof(double) * b.ncell));
+    bool any_ext = false;
+    for (int c = 0; c < b.ncell; ++c) any_ext |= !hc[c].f_ext.empty();
+    if (any_ext) {
+      std::vector<double> fext(2 * (size_t)N, 0.0);
+      for (int c = 0; c < b.ncell; ++c)
+        if (!hc[c].f_ext.empty())
+          std::copy(hc[c].f_ext.begin(), hc[c].f_ext.end(), fext.begin() + 2LL * b.cp_h[c].node_off);
+      b.bt_fext = dupload(fext);
+    }
   }
   b.transitions = dalloc<int>(1);
   HIPCHECK(hipMemset(b.transitions, 0, sizeof(int)));
@@ -1043,6 +1056,7 @@ BndView bnd_view(const wcfe_ctx *ctx, const Batch &b) {
   bv.bidx = b.bt_bidx; bv.node = b.bt_node; bv.cst = b.bt_cst; bv.cand = b.bt_cand;
   bv.frame = b.bt_frame; bv.g = b.bt_g; bv.J = b.bt_J; bv.area = b.bt_area; bv.gd = b.bt_gd;
   bv.replica = b.replica;
+  bv.fext = b.bt_fext;
   bv.dt = ctx->opt.dt;
   bv.key_lo = ctx->opt.seed_lo;
   bv.key_hi = ctx->opt.seed_hi;
@@ -1503,6 +1517,12 @@ HostCell host_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d) {
     h.k_bend = d->k_bend; h.k_area = d->k_area; h.area_0 = d->area_0; h.drag = d->drag;
     h.ap = d->ap ? 1 : 0;
     h.ap_rate = d->ap_rate; h.ap_quantum = d->ap_quantum; h.ap_density = d->ap_density;
+    h.passive = d->passive ? 1 : 0;
+    if (d->f_ext) {
+      h.f_ext.assign(d->f_ext, d->f_ext + 2 * d->n_nodes);
+      for (double v : h.f_ext)
+        if (!std::isfinite(v)) fail(WCFE_ERR_ARG, "f_ext holds a non-finite value");
+    }
     h.n_spr = d->n_spr;
     h.X.assign(d->X, d->X + 2 * d->n_nodes);
     if (d->x) h.x.assign(d->x, d->x + 2 * d->n_nodes);
@@ -1861,6 +1881,28 @@ int wcfe_set_positions(wcfe_ctx *ctx, int32_t cell, const double *x) {
   });
 }
 
+int wcfe_set_external_forces(wcfe_ctx *ctx, int32_t cell, const double *f_ext) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (cell < 0 || cell >= (int)ctx->cells.size()) fail(WCFE_ERR_ARG, "cell %d out of range", cell);
+    HostCell &h = ctx->cells[cell];
+    if (h.f_ext.empty()) fail(WCFE_ERR_ARG, "cell %d was added without f_ext", cell);
+    const size_t n = 2 * (size_t)h.n_nodes;
+    if (f_ext) {
+      for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(f_ext[i])) fail(WCFE_ERR_ARG, "f_ext holds a non-finite value");
+      h.f_ext.assign(f_ext, f_ext + n);
+    } else {
+      std::fill(h.f_ext.begin(), h.f_ext.end(), 0.0);
+    }
+    if (ctx->finalized) {
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+      HIPCHECK(hipMemcpy(ctx->b.bt_fext + 2LL * ctx->b.cp_h[cell].node_off, h.f_ext.data(), sizeof(double) * n,
+                         hipMemcpyHostToDevice));
+    }
+  });
+}
+
 int wcfe_get_ul_state(wcfe_ctx *ctx, int32_t cell, double *X, double *ea, double *cC, double *fau) {
   if (!ctx) return WCFE_ERR_ARG;
   return guarded(ctx, [&] {
@@ -2195,6 +2237,81 @@ void one_rings(int n_nodes, int n_tris, const int32_t *tris, std::vector<int> &s
     for (int a = 0; a < 3; ++a) list[fill[tris[3 * e + a]]++] = e;
 }
 
+// Uniform grid over a substrate's triangles for point location.  Cell size
+// ~ sqrt(area / triangles), at most GRID_MAX cells a side.  A triangle is
+// listed in every cell its bounding box meets, widened by 1e-9 of the mesh's
+// coordinate scale so that points the containment test accepts through
+// rounding are still found.  The z-sign test also accepts the reflection of
+// a clockwise (or nearly degenerate) triangle through its first vertex, so
+// such triangles list that region too.  A substrate off the plane z = 0
+// gets one cell holding every triangle.  Lists are in increasing triangle
+// index, the reference's visiting order.
+struct HostGrid {
+  PointGrid g;
+  std::vector<int> start, list;
+};
+constexpr int GRID_MAX = 2048;
+HostGrid build_point_grid(int n_snodes, int n_stris, const int32_t *stris, const double *P) {
+  HostGrid h;
+  h.g = PointGrid{0.0, 0.0, 1.0, 1, 1, nullptr, nullptr};
+  bool planar = true;
+  double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  for (int i = 0; i < n_snodes; ++i) {
+    const double *q = P + 3 * (size_t)i;
+    if (q[2] != 0.0 || !std::isfinite(q[0]) || !std::isfinite(q[1])) planar = false;
+    x0 = std::min(x0, q[0]); x1 = std::max(x1, q[0]);
+    y0 = std::min(y0, q[1]); y1 = std::max(y1, q[1]);
+  }
+  const double W = x1 - x0, H = y1 - y0, ext = std::max(W, H);
+  if (!planar || n_stris == 0 || !(ext > 0.0)) {
+    h.start = {0, n_stris};
+    h.list.resize(n_stris);
+    for (int e = 0; e < n_stris; ++e) h.list[e] = e;
+    return h;
+  }
+  const double scale = std::max({ext, fabs(x0), fabs(x1), fabs(y0), fabs(y1)});
+  const double pad = 1e-9 * scale;
+  double cell = std::sqrt(std::max(W, ext * 1e-6) * std::max(H, ext * 1e-6) / n_stris);
+  cell = std::max(cell, ext / GRID_MAX);
+  const double ox = x0 - 2 * pad, oy = y0 - 2 * pad, inv_h = 1.0 / cell;
+  const int gx = std::min(GRID_MAX, (int)((W + 4 * pad) * inv_h) + 1);
+  const int gy = std::min(GRID_MAX, (int)((H + 4 * pad) * inv_h) + 1);
+  auto idx = [&](double v, double o, int g) {
+    const double f = std::floor((v - o) * inv_h);
+    return f < 0.0 ? 0 : (f >= g ? g - 1 : (int)f);
+  };
+  std::vector<int4> box(n_stris);
+  h.start.assign((size_t)gx * gy + 1, 0);
+  for (int e = 0; e < n_stris; ++e) {
+    const double *q0 = P + 3 * stris[3 * e], *q1 = P + 3 * stris[3 * e + 1], *q2 = P + 3 * stris[3 * e + 2];
+    double bx0 = std::min({q0[0], q1[0], q2[0]}), bx1 = std::max({q0[0], q1[0], q2[0]});
+    double by0 = std::min({q0[1], q1[1], q2[1]}), by1 = std::max({q0[1], q1[1], q2[1]});
+    const double e1x = q1[0] - q0[0], e1y = q1[1] - q0[1], e2x = q2[0] - q0[0], e2y = q2[1] - q0[1];
+    const double orient = e1x * e2y - e1y * e2x;
+    if (!(orient > 1e-8 * std::hypot(e1x, e1y) * std::hypot(e2x, e2y))) {  // reflected region q0 - (q - q0)
+      bx0 = std::min({bx0, q0[0] - e1x, q0[0] - e2x}); bx1 = std::max({bx1, q0[0] - e1x, q0[0] - e2x});
+      by0 = std::min({by0, q0[1] - e1y, q0[1] - e2y}); by1 = std::max({by1, q0[1] - e1y, q0[1] - e2y});
+    }
+    box[e] = int4{idx(bx0 - pad, ox, gx), idx(bx1 + pad, ox, gx), idx(by0 - pad, oy, gy), idx(by1 + pad, oy, gy)};
+    for (int iy = box[e].z; iy <= box[e].w; ++iy)
+      for (int ix = box[e].x; ix <= box[e].y; ++ix) h.start[(size_t)iy * gx + ix + 1]++;
+  }
+  for (size_t c = 0; c < (size_t)gx * gy; ++c) h.start[c + 1] += h.start[c];
+  h.list.resize(h.start.back());
+  std::vector<int> fill(h.start.begin(), h.start.end() - 1);
+  for (int e = 0; e < n_stris; ++e)
+    for (int iy = box[e].z; iy <= box[e].w; ++iy)
+      for (int ix = box[e].x; ix <= box[e].y; ++ix) h.list[fill[(size_t)iy * gx + ix]++] = e;
+  h.g = PointGrid{ox, oy, inv_h, gx, gy, nullptr, nullptr};
+  return h;
+}
+PointGrid upload_grid(Scratch &sc, const HostGrid &h) {
+  PointGrid g = h.g;
+  g.start = sc.up(h.start.data(), h.start.size());
+  g.list = sc.up(h.list.data(), std::max<size_t>(h.list.size(), 1));
+  return g;
+}
+
 void check_mesh(int n_nodes, int n_tris, const int32_t *tris) {
   if (n_nodes <= 0 || n_tris < 0 || (n_tris > 0 && !tris)) fail(WCFE_ERR_ARG, "invalid mesh sizes");
   for (long long i = 0; i < 3LL * n_tris; ++i)
@@ -2216,17 +2333,20 @@ int wcfe_dense_solve(wcfe_ctx *ctx, int32_t n_sys, const int32_t *n, const doubl
 }
 
 int wcfe_calculate_res(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *X,
-                       const double *x, const double *ea, double time, double *Res, double *Res_ea, double *MM) {
+                       const double *x, const double *ea, int32_t mat, double time, double *Res, double *Res_ea,
+                       double *MM) {
   if (!ctx || !X || !x || !ea || !Res || !Res_ea || !MM) return WCFE_ERR_ARG;
   return guarded(ctx, [&] {
     check_mesh(n_nodes, n_tris, tris);
+    if (mat != 1 && mat != 2) fail(WCFE_ERR_ARG, "mat must be 1 (cell) or 2 (substrate)");
     HIPCHECK(hipSetDevice(ctx->opt.device));
     HostCell h;
     h.n_nodes = n_nodes;
     h.n_tris = n_tris;
     h.law = WCFE_LAW_UL;
-    h.mu = 1000.0;  // UL:724
-    h.thick = 1.0;  // UL:768
+    h.mu = mat == 2 ? 1.0e6 : 1000.0;  // UL:724-727
+    h.passive = mat == 2;              // t_act = 0 (UL:763)
+    h.thick = 1.0;                     // UL:768
     h.X.resize(2 * (size_t)n_nodes);
     h.x.resize(2 * (size_t)n_nodes);
     h.ea.resize(3 * (size_t)n_nodes);
@@ -2314,7 +2434,8 @@ int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double
     int *dst = sc.up(stris, 3 * (size_t)n_stris);
     double *dSX = sc.up(SX, 3 * (size_t)n_snodes), *dSx = sc.up(Sx, 3 * (size_t)n_snodes);
     int *dun = sc.zeros<int>(1);
-    launch_ref_update_fa(ctx->stream, n_nodes, dX, dx, dA, dC, dF, dt, n_stris, dst, dSX, dSx, law, kint, dun);
+    const PointGrid grid = upload_grid(sc, build_point_grid(n_snodes, n_stris, stris, SX));
+    launch_ref_update_fa(ctx->stream, n_nodes, dX, dx, dA, dC, dF, dt, grid, dst, dSX, dSx, law, kint, dun);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     HIPCHECK(hipMemcpy(cA, dA, sizeof(double) * n_nodes, hipMemcpyDeviceToHost));
@@ -2323,6 +2444,50 @@ int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double
     int un = 0;
     HIPCHECK(hipMemcpy(&un, dun, sizeof(int), hipMemcpyDeviceToHost));
     if (n_unlocated) *n_unlocated = un;
+  });
+}
+
+int wcfe_force_fa_subs(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x_cell,
+                       const double *cC, const double *fau, int32_t n_snodes, int32_t n_stris, const int32_t *stris,
+                       const double *Sx, int32_t law, double kint, double *F) {
+  if (!ctx || !x_cell || !cC || !fau || !Sx || !F) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_mesh(n_nodes, n_tris, tris);
+    check_mesh(n_snodes, n_stris, stris);
+    if (law != 0 && law != 1) fail(WCFE_ERR_ARG, "law must be 0 (linear kint) or 1 (MD table)");
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    std::vector<int> st, li;
+    one_rings(n_nodes, n_tris, tris, st, li);
+    Scratch sc;
+    int *drs = sc.up(st.data(), st.size()), *dri = sc.up(li.data(), std::max<size_t>(li.size(), 1));
+    int *dtr = sc.up(tris, 3 * (size_t)n_tris), *dst = sc.up(stris, 3 * (size_t)n_stris);
+    double *dx = sc.up(x_cell, 3 * (size_t)n_nodes), *dc = sc.up(cC, (size_t)n_nodes);
+    double *df = sc.up(fau, 3 * (size_t)n_nodes), *dSx = sc.up(Sx, 3 * (size_t)n_snodes);
+    const PointGrid grid = upload_grid(sc, build_point_grid(n_snodes, n_stris, stris, Sx));
+    int *dht = sc.zeros<int>((size_t)n_nodes * SUBS_MAX_HITS), *dnh = sc.zeros<int>((size_t)n_nodes);
+    double *dhv = sc.zeros<double>((size_t)n_nodes * SUBS_MAX_HITS * 9);
+    int *dov = sc.zeros<int>(1);
+    launch_ref_force_fa_subs(ctx->stream, n_nodes, drs, dri, dtr, dx, dc, df, grid, dst, dSx, law, kint, dht, dhv,
+                             dnh, dov);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    int ov = 0;
+    HIPCHECK(hipMemcpy(&ov, dov, sizeof(int), hipMemcpyDeviceToHost));
+    if (ov) fail(WCFE_ERR_ARG, "%d cell nodes lie in more than %d substrate triangles", ov, SUBS_MAX_HITS);
+    std::vector<int> ht((size_t)n_nodes * SUBS_MAX_HITS), nh(n_nodes);
+    std::vector<double> hv(ht.size() * 9);
+    HIPCHECK(hipMemcpy(ht.data(), dht, sizeof(int) * ht.size(), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(nh.data(), dnh, sizeof(int) * nh.size(), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(hv.data(), dhv, sizeof(double) * hv.size(), hipMemcpyDeviceToHost));
+    // the reference's accumulation order: cell node, then triangle, ascending
+    std::fill(F, F + 3 * (size_t)n_snodes, 0.0);
+    for (int i = 0; i < n_nodes; ++i)
+      for (int k = 0; k < nh[i]; ++k) {
+        const size_t hk = (size_t)i * SUBS_MAX_HITS + k;
+        const int32_t *t = stris + 3 * (size_t)ht[hk];
+        for (int a = 0; a < 3; ++a)
+          for (int c = 0; c < 3; ++c) F[3 * (size_t)t[a] + c] += hv[9 * hk + 3 * a + c];
+      }
   });
 }
 
@@ -2423,8 +2588,9 @@ int wcfe_force_ap(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t 
 
 namespace {
 const char CKP_MAGIC[8] = {'W', 'C', 'F', 'E', 'C', 'K', 'P', '1'};
-// version 2 adds the boundary and body force parameters after the scalars
-const int CKP_VERSION = 2;
+// version 2 adds the boundary and body force parameters after the scalars,
+// version 3 the substrate form (passive flag, constant nodal load)
+const int CKP_VERSION = 3;
 struct CkpWriter {
   FILE *f;
   void raw(const void *p, size_t n) {
@@ -2502,6 +2668,9 @@ int wcfe_checkpoint_save(wcfe_ctx *ctx, const char *path) {
         for (double v : {h.mu, h.E, h.nu, h.thick, h.rho_max, h.k_lin, h.time}) w.f64(v);
         w.i32(h.ap);
         for (double v : {h.k_bend, h.k_area, h.area_0, h.drag, h.ap_rate, h.ap_quantum, h.ap_density}) w.f64(v);
+        w.i32(h.passive);
+        w.i32(h.f_ext.empty() ? 0 : 1);
+        if (!h.f_ext.empty()) w.vec(h.f_ext, 2 * N);
         w.vec(h.X, 2 * N);
         w.vec(h.x, 2 * N);
         w.vec(h.ea, 3 * N);
@@ -2536,7 +2705,7 @@ int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path) {
       r.raw(magic, 8);
       if (memcmp(magic, CKP_MAGIC, 8) != 0) fail(WCFE_ERR_ARG, "%s is not a wcfe checkpoint", path);
       const int version = r.i32();
-      if (version != 1 && version != 2) fail(WCFE_ERR_ARG, "unsupported checkpoint version %d", version);
+      if (version < 1 || version > 3) fail(WCFE_ERR_ARG, "unsupported checkpoint version %d", version);
       const int ncell = r.i32(), ntab = r.i32();
       if (ncell < 1 || ncell >= (1 << 11) || ntab < 0 || ntab > 1024) fail(WCFE_ERR_ARG, "bad checkpoint header");
       for (int t = 0; t < ntab; ++t) {
@@ -2560,8 +2729,14 @@ int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path) {
           double *bv[] = {&h.k_bend, &h.k_area, &h.area_0, &h.drag, &h.ap_rate, &h.ap_quantum, &h.ap_density};
           for (double *v : bv) *v = r.f64();
         }
+        int has_ext = 0;
+        if (version >= 3) {
+          h.passive = r.i32() ? 1 : 0;
+          has_ext = r.i32();
+        }
         if (h.n_nodes <= 0 || h.n_tris <= 0 || h.n_spr < 0) fail(WCFE_ERR_ARG, "bad cell sizes in checkpoint");
         const long long N = h.n_nodes, T = h.n_tris, S = h.n_spr;
+        if (has_ext) r.vec(h.f_ext, 2 * N);
         r.vec(h.X, 2 * N);
         r.vec(h.x, 2 * N);
         r.vec(h.ea, 3 * N);

@@ -34,13 +34,13 @@ struct CellParams {
                                   // (nblk - 1 = full); the LU keeps its fill inside the band
   int env;                        // blocks |i - j| <= env are zeroed before assembly (bw + delayed
                                   // panels - 1: the trailing update reads that far off the band)
-  int pad4_;
+  int passive;                    // 1: no active stress (the reference's substrate, mat = 2)
   long long k_off;                // offset of the dense matrix (elements)
   double mu, E, nu, thick, k_lin, rho_max;
   double k_bend, k_area, area_0, drag, ap_rate, ap_quantum, ap_density;
 };
 
-enum : int { BT_BEND = 1, BT_AREA = 2, BT_AP = 4, BT_DRAG = 8 };
+enum : int { BT_BEND = 1, BT_AREA = 2, BT_AP = 4, BT_DRAG = 8, BT_EXT = 16 };
 
 // Boundary tables and per-evaluation scratch of the node terms (batch wide;
 // bidx == nullptr when no cell has boundary or body forces).
@@ -55,6 +55,7 @@ struct BndView {
   double *area;       // per cell: triangle area sum at the current positions
   double *gd;         // per cell: sum_b g_b . d_b (refinement matvec)
   const int *replica; // per cell: Philox stream id
+  const double *fext; // per node: constant external load (2 per node; cells with BT_EXT)
   double dt;
   unsigned key_lo, key_hi;
 };
@@ -221,7 +222,7 @@ __device__ __forceinline__ bool element_system(const View &v, int e, double Ge[6
     const double b00 = m11 / detM, b11 = m00 / detM, b01 = -m01 / detM;
     // S = mu be_t + sigma_a I + p C^-1 with p = -mu / (theta^2 theta_e_t^2)
     // (UL:848-880); first PK-type stress P = F S = F S0 - (beta/theta^2) F^-T
-    const double sa = v.kappa * t_act_of(time);
+    const double sa = p.passive ? 0.0 : v.kappa * t_act_of(time);
     const double S0[2][2] = {{p.mu * b00 + sa, p.mu * b01}, {p.mu * b01, p.mu * b11 + sa}};
     const double beta = p.mu * detM;  // mu / theta_e_t^2
     const double ith = 1.0 / th;
@@ -263,7 +264,7 @@ __device__ __forceinline__ bool element_system(const View &v, int e, double Ge[6
       eyy += dN[a][1] * uy;
       gxy += dN[a][1] * ux + dN[a][0] * uy;
     }
-    const double sa = v.kappa * t_act_of(time);
+    const double sa = p.passive ? 0.0 : v.kappa * t_act_of(time);
     const double sxx = Dc * (exx + p.nu * eyy) + sa;
     const double syy = Dc * (p.nu * exx + eyy) + sa;
     const double sxy = Dc * 0.5 * (1.0 - p.nu) * gxy;

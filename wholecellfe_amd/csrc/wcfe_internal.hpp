@@ -145,9 +145,21 @@ void launch_pcg_finish(hipStream_t s, const int4 *chunks, int nchunk, const Cell
 // ---- reference-shaped stateless kernels (k_reference.hip) ----
 void launch_ref_force_fa(hipStream_t s, int n_nodes, const int *ring_start, const int *ring, const int *tris,
                          const double *x, const double *cC, const double *fau, int law, double kint, double *F);
+// uniform grid over substrate triangles (point location); list[start[c] ..
+// start[c+1]) = triangles whose widened bounding box meets cell c, ascending
+struct PointGrid {
+  double ox, oy, inv_h;
+  int gx, gy;
+  const int *start, *list;
+};
+constexpr int SUBS_MAX_HITS = 16;  // containing substrate triangles per cell node (shared vertices)
 void launch_ref_update_fa(hipStream_t s, int n_nodes, const double *X, const double *x, double *cA, double *cC,
-                          double *fau, double dt, int n_stris, const int *stris, const double *SX, const double *Sx,
-                          int law, double kint, int *n_unlocated);
+                          double *fau, double dt, const PointGrid &grid, const int *stris, const double *SX,
+                          const double *Sx, int law, double kint, int *n_unlocated);
+void launch_ref_force_fa_subs(hipStream_t s, int n_nodes, const int *ring_start, const int *ring, const int *tris,
+                              const double *x_cell, const double *cC, const double *fau, const PointGrid &grid,
+                              const int *stris, const double *Sx, int law, double kint, int *hit_tri,
+                              double *hit_val, int *n_hits, int *overflow);
 void launch_dense_residual(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                            const double *A64, const long long *a_off, const double *b, const double *x, double *r);
 
