@@ -56,6 +56,8 @@ enum wcfe_pivoting { WCFE_PIVOT_STATIC = 0, WCFE_PIVOT_PARTIAL = 1 };
  * (both inside the FP64 iterative refinement); AUTO picks CSR above
  * wcfe_options.dense_max_dof unknowns */
 enum wcfe_solver { WCFE_SOLVER_AUTO = 0, WCFE_SOLVER_DENSE = 1, WCFE_SOLVER_CSR = 2 };
+/* frame file layouts: the reference's cell and substrate write_vtk */
+enum wcfe_vtk_style { WCFE_VTK_CELL = 0, WCFE_VTK_SUBSTRATE = 1 };
 
 typedef struct wcfe_ctx wcfe_ctx;
 
@@ -254,6 +256,22 @@ int wcfe_set_ul_state(wcfe_ctx *ctx, int32_t cell, const double *X /* 2N */, con
  * wcfe_force_fa_subs in a staggered cell/substrate step.  It enters the next
  * residual evaluation; call it between load steps. */
 int wcfe_set_external_forces(wcfe_ctx *ctx, int32_t cell, const double *f_ext);
+/* ---- frame output (the reference's per-frame write_vtk, main.cpp:447-454) ----
+ * wcfe_frame_capture snapshots every cell's committed state (start-of-step
+ * positions, strain, bound fraction, spring extension, load time) on the
+ * device in stream order after the work already enqueued, and copies it to
+ * pinned host memory on a separate stream while the following Newton steps
+ * run.  Two frames can be in flight: a capture reuses the slot of the frame
+ * captured two captures earlier.  wcfe_frame_get and wcfe_frame_write_vtk
+ * wait for the frame's copy; a batch laid out again (wcfe_replace_cell)
+ * invalidates earlier frames. */
+int wcfe_frame_capture(wcfe_ctx *ctx, int32_t *frame);
+int wcfe_frame_get(wcfe_ctx *ctx, int32_t frame, int32_t cell, double *X /* 2N */, double *ea /* 3N */,
+                   double *cC /* N */, double *fau /* 2N */, double *time);
+/* One cell of a captured frame as a VTK file (wcfe_write_vtk's layout; z = 0;
+ * cA NULL = the reference's initial 100, main.cpp:674-677). */
+int wcfe_frame_write_vtk(wcfe_ctx *ctx, int32_t frame, int32_t cell, const char *path, int32_t style,
+                         const double *cA /* N */, int32_t law, double kint);
 int wcfe_get_springs(wcfe_ctx *ctx, int32_t cell, double *state /* n_spr */, double *anchor /* 2 n_spr */);
 int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step, int32_t *newton_iter,
                        int32_t *n_colors, int32_t *ndof);
@@ -306,6 +324,15 @@ int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double
 int wcfe_force_fa_subs(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x_cell,
                        const double *cC, const double *fau, int32_t n_snodes, int32_t n_stris, const int32_t *stris,
                        const double *Sx, int32_t law, double kint, double *F);
+/* write_vtk (iomesh.cpp:844-1047, style WCFE_VTK_CELL: c_A, theta_e_t, c_C and
+ * the spring force per node -- law 1: the MD table at |fa_u| (kMD), law 0:
+ * |kint fa_u_xy| (k1, k31) -- then fa_u; iomesh.cpp:730-842, style
+ * WCFE_VTK_SUBSTRATE: theta_e_t, then fa_u, fixed notation).  Host only, no
+ * context.  x, fau: 3N; ea: 6N Voigt (NULL = 0); cA, cC: N (cell style; NULL
+ * = 100 and 0). */
+int wcfe_write_vtk(const char *path, int32_t style, int32_t n_nodes, int32_t n_tris, const int32_t *tris,
+                   const double *x, const double *ea, const double *fau, const double *cA, const double *cC,
+                   int32_t law, double kint, double time);
 /* calculateAreaTot (UL:1662-1688): out[4] = {area, centroid x, y, z}; x: 3N. */
 int wcfe_area_tot(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
                   double *out);

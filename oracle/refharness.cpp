@@ -7,7 +7,8 @@
 // in and out of the reference functions; it contains no mechanics of its own.
 //
 // Usage:  refharness_<variant> <mode> <case_dir>
-//   modes: elem | calcres | forcefa | updatefa | areatot | bend | areacon | ap | forcefasubs | calcressubs
+//   modes: elem | calcres | forcefa | updatefa | areatot | bend | areacon | ap | forcefasubs | calcressubs |
+//          vtkcell | vtksubs
 // Inputs are read from <case_dir>/<name>.bin (float64 or int64, row-major),
 // outputs written to <case_dir>/out_<name>.bin.  Scalars come from
 // <case_dir>/meta.txt ("key value" lines).
@@ -23,6 +24,8 @@
 //   calculateForce_AP                  UL:1477 (its own srand(time(NULL)) / rand() draws)
 //   calculateForce_FA_subs             UL:1305 (kMD) / UL1:1297 (k1, k31)
 //   calculateRes (substrate overload)  UL:931 with mat = 2
+//   write_vtk (cell frame)             iomesh.cpp:844 (mode vtkcell, writes <case_dir>/out.vtk)
+//   write_vtk (substrate frame)        iomesh.cpp:730 (mode vtksubs)
 #include "vec.h"
 #include "nondestructivetrimesh.h"
 #include <Eigen/Dense>
@@ -65,6 +68,19 @@ void calculateBendingForce2D(Eigen::VectorXd &F_bending, const std::vector<Vec3d
                              const NonDestructiveTriMesh &mesh);
 void calculateAreaConstraintForce(Eigen::VectorXd &F_areaConstraint, const std::vector<Vec3d> &X_t,
                                   const std::vector<Vec3d> &x_t, const NonDestructiveTriMesh &mesh, double tot_area);
+
+void write_vtk(const NonDestructiveTriMesh &mesh, const std::vector<Vec3d> &x, const std::vector<Eigen::VectorXd> &ea_t,
+               const std::vector<Vec3d> &fa_u, double current_time, const char *filename);
+// the cell writer takes the force law's kint in the kMD variant only
+#ifdef REF_VARIANT_kMD
+void write_vtk(const NonDestructiveTriMesh &mesh, const std::vector<Vec3d> &x, const std::vector<double> &c_A,
+               const std::vector<double> &c_B, const std::vector<double> &c_C, const std::vector<Vec3d> &fa_u,
+               const std::vector<Eigen::VectorXd> &ea_t, double current_time, const char *filename, double kint);
+#else
+void write_vtk(const NonDestructiveTriMesh &mesh, const std::vector<Vec3d> &x, const std::vector<double> &c_A,
+               const std::vector<double> &c_B, const std::vector<double> &c_C, const std::vector<Vec3d> &fa_u,
+               const std::vector<Eigen::VectorXd> &ea_t, double current_time, const char *filename);
+#endif
 
 namespace {
 
@@ -218,6 +234,27 @@ int main(int argc, char **argv) {
     Eigen::VectorXd F(3 * SX.size());
     calculateForce_FA_subs(F, SX, Sx, smesh, cC, x, fau, mesh, meta["kint"]);
     store("F_subs", F.data(), 3 * SX.size());
+    return 0;
+  }
+  if (mode == "vtkcell" || mode == "vtksubs") {
+    std::vector<double> ea_raw = load<double>("ea");
+    std::vector<Eigen::VectorXd> ea(n);
+    for (size_t i = 0; i < n; ++i) {
+      ea[i].resize(6);
+      for (int q = 0; q < 6; ++q) ea[i](q) = ea_raw[6 * i + q];
+    }
+    std::vector<Vec3d> fau = vec3(load<double>("fau"));
+    const std::string out = g_dir + "/out.vtk";
+    if (mode == "vtksubs") {
+      write_vtk(mesh, x, ea, fau, meta["time"], out.c_str());
+    } else {
+      std::vector<double> cA = load<double>("cA"), cC = load<double>("cC"), cB(n, 0.0);
+#ifdef REF_VARIANT_kMD
+      write_vtk(mesh, x, cA, cB, cC, fau, ea, meta["time"], out.c_str(), meta["kint"]);
+#else
+      write_vtk(mesh, x, cA, cB, cC, fau, ea, meta["time"], out.c_str());
+#endif
+    }
     return 0;
   }
   if (mode == "calcressubs") {

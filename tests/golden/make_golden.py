@@ -8,7 +8,7 @@ small compressed .npz fixtures next to this script.  The fixtures are data
 files, seeded inputs, and the reference's outputs); nothing of the reference's
 source travels.
 
-Usage:  python tests/golden/make_golden.py [--only boundary|substrate]
+Usage:  python tests/golden/make_golden.py [--only boundary|substrate|vtk]
 """
 import os
 import shutil
@@ -53,6 +53,8 @@ def run(variant, mode, inputs, meta):
         for fn in os.listdir(d):
             if fn.startswith("out_"):
                 out[fn[4:-4]] = np.fromfile(os.path.join(d, fn), dtype=np.float64)
+            elif fn == "out.vtk":
+                out["vtk"] = np.frombuffer(open(os.path.join(d, fn), "rb").read(), dtype=np.uint8)
         return out
     finally:
         shutil.rmtree(d)
@@ -143,6 +145,40 @@ def substrate_cases(disk296, t296, subs, tsubs):
     print("wrote calcres_subs")
 
 
+def vtk_cases(disk296, t296, subs, tsubs):
+    """The per-frame VTK writers (iomesh.cpp:844 cell frame, per variant;
+    iomesh.cpp:730 substrate frame).  The fixture holds the inputs and the
+    written text."""
+    rng = np.random.default_rng(20251102)
+    N = len(disk296)
+    x = deform(disk296, rng, 0.02)
+    ea = np.zeros((N, 6))
+    ea[:, [0, 1, 3]] = 0.05 * rng.standard_normal((N, 3))
+    cA = rng.uniform(0, 100, N)
+    cC = rng.uniform(0, 1, N)
+    D = rng.uniform(0.0, 0.022, N)  # every MD-table segment and beyond
+    th = rng.uniform(0, 2 * np.pi, N)
+    fau = np.zeros((N, 3))
+    fau[:, 0], fau[:, 1] = D * np.cos(th), D * np.sin(th)
+    fau[0] = 0.0
+    inp = dict(X=disk296, x=x, tris=t296, ea=ea, cA=cA, cC=cC, fau=fau)
+    for variant in ("k1", "k31", "kMD"):
+        o = run(variant, "vtkcell", inp, dict(time=1.2345678901, kint=-1.0))
+        np.savez_compressed(os.path.join(HERE, "vtk_cell_%s.npz" % variant), time=1.2345678901, **inp, vtk=o["vtk"])
+        print("wrote vtk_cell_%s" % variant)
+    Ns = len(subs)
+    xs = subs.copy()
+    xs[:, :2] += 0.002 * rng.standard_normal((Ns, 2))
+    eas = np.zeros((Ns, 6))
+    eas[:, [0, 1, 3]] = 1e-4 * rng.standard_normal((Ns, 3))
+    faus = np.zeros((Ns, 3))
+    faus[:, :2] = 1e-3 * rng.standard_normal((Ns, 2))
+    inp = dict(X=subs, x=xs, tris=tsubs, ea=eas, fau=faus)
+    o = run("kMD", "vtksubs", inp, dict(time=0.04))
+    np.savez_compressed(os.path.join(HERE, "vtk_subs.npz"), time=0.04, **inp, vtk=o["vtk"])
+    print("wrote vtk_subs")
+
+
 def main():
     if not os.path.exists(HARNESS.format("kMD")):
         sys.exit("build the reference harness first: make -C oracle ref")
@@ -155,6 +191,9 @@ def main():
         return
     if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "substrate":
         substrate_cases(disk296, t296, subs, tsubs)
+        return
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "vtk":
+        vtk_cases(disk296, t296, subs, tsubs)
         return
 
     # --- element residual / calculateRes (law identical in all variants) ---
@@ -232,6 +271,7 @@ def main():
     print("wrote mesh_disk483")
     boundary_cases(disk296, t296, disk483, t483)
     substrate_cases(disk296, t296, subs, tsubs)
+    vtk_cases(disk296, t296, subs, tsubs)
 
 
 if __name__ == "__main__":

@@ -32,7 +32,8 @@ EXPORTS = [
     "wcfe_add_cell", "wcfe_finalize", "wcfe_newton_step", "wcfe_load_step", "wcfe_synchronize",
     "wcfe_get_positions", "wcfe_set_positions", "wcfe_get_ul_state", "wcfe_get_springs", "wcfe_get_cell_info",
     "wcfe_residual_fp64", "wcfe_element_systems", "wcfe_assembled_system", "wcfe_get_colors", "wcfe_dense_solve",
-    "wcfe_calculate_res", "wcfe_force_fa", "wcfe_force_fa_subs", "wcfe_update_fa", "wcfe_set_external_forces", "wcfe_profile_step",
+    "wcfe_calculate_res", "wcfe_force_fa", "wcfe_force_fa_subs", "wcfe_update_fa", "wcfe_set_external_forces",
+    "wcfe_write_vtk", "wcfe_frame_capture", "wcfe_frame_get", "wcfe_frame_write_vtk", "wcfe_profile_step",
     "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state", "wcfe_replace_cell", "wcfe_checkpoint_save",
     "wcfe_checkpoint_load", "wcfe_area_tot", "wcfe_bending_force_2d", "wcfe_area_constraint_force",
     "wcfe_force_ap", "wcfe_get_schedule",
@@ -131,6 +132,20 @@ def default_options(**kw):
             raise KeyError(k)
         setattr(o, k, v)
     return o
+
+
+VTK_CELL, VTK_SUBSTRATE = 0, 1
+
+
+def write_vtk(path, tris, x, ea=None, fau=None, cA=None, cC=None, style=VTK_CELL, law=1, kint=-1.0, time=0.0):
+    """The reference's write_vtk (iomesh.cpp:844 cell frame / :730 substrate
+    frame) from host arrays: x, fau 3 per node, ea 6 per node (Voigt)."""
+    tris, x = _i32(tris).reshape(-1), _f64(x).reshape(-1)
+    arr = [None if a is None else _f64(a).reshape(-1) for a in (ea, fau, cA, cC)]
+    rc = lib().wcfe_write_vtk(str(path).encode(), style, len(x) // 3, len(tris) // 3, _ip(tris), _dp(x),
+                              *[_dp(a) for a in arr], law, C.c_double(kint), C.c_double(time))
+    if rc != OK:
+        raise WcfeError(rc, "wcfe_write_vtk failed")
 
 
 class Context:
@@ -302,6 +317,26 @@ class Context:
         x = np.zeros(2 * n)
         self._check(self._L.wcfe_get_positions(self._h, cell, _dp(x)))
         return x
+
+    def frame_capture(self):
+        """Snapshot every cell's committed state; the copy to host memory
+        overlaps the following Newton steps.  Returns the frame id."""
+        fr = C.c_int32(-1)
+        self._check(self._L.wcfe_frame_capture(self._h, C.byref(fr)))
+        return fr.value
+
+    def frame_get(self, frame, cell):
+        """(X 2N, ea 3N, cC N, fau 2N, time) of one cell in a captured frame."""
+        n = self.cells[cell]["n_nodes"]
+        X, ea, cC, fau, t = np.zeros(2 * n), np.zeros(3 * n), np.zeros(n), np.zeros(2 * n), C.c_double(0)
+        self._check(self._L.wcfe_frame_get(self._h, frame, cell, _dp(X), _dp(ea), _dp(cC), _dp(fau), C.byref(t)))
+        return X, ea, cC, fau, t.value
+
+    def frame_write_vtk(self, frame, cell, path, style=0, cA=None, law=1, kint=-1.0):
+        """One cell of a captured frame as the reference's VTK file."""
+        cA = _f64(cA)
+        self._check(self._L.wcfe_frame_write_vtk(self._h, frame, cell, str(path).encode(), style, _dp(cA), law,
+                                                 C.c_double(kint)))
 
     def set_external_forces(self, cell, f_ext):
         """New constant nodal load (2N, None = zero) of a cell added with f_ext."""

@@ -506,6 +506,16 @@ struct Profiler {
 
 }  // namespace
 
+// One captured frame: the batch's committed state copied to pinned host
+// memory on the copy stream.  Layout (doubles): X[2N] ea[3N] cC[N] fau[2N],
+// then the CellState array.
+struct FrameSlot {
+  int id = -1, layout = -1;
+  size_t bytes = 0;
+  double *dev = nullptr, *host = nullptr;
+  hipEvent_t done = nullptr;
+};
+
 struct wcfe_ctx {
   wcfe_options opt;
   std::string err;
@@ -527,6 +537,12 @@ struct wcfe_ctx {
   std::vector<hipStream_t> side;            // concurrent LU groups
   hipEvent_t fork_ev = nullptr;
   std::vector<hipEvent_t> join_ev;
+  // frame output: two slots in flight, copied on their own stream
+  FrameSlot frames[2];
+  int next_frame = 0;
+  int layout = 0;  // bumped whenever the batch is laid out (finalize, rebuild)
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t frame_ready = nullptr;
 };
 
 namespace {
@@ -1448,6 +1464,14 @@ int wcfe_destroy(wcfe_ctx *ctx) {
   for (hipStream_t st : ctx->side) (void)hipStreamDestroy(st);
   for (hipEvent_t ev : ctx->join_ev) (void)hipEventDestroy(ev);
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+  for (FrameSlot &f : ctx->frames) {
+    if (f.dev) (void)hipFree(f.dev);
+    if (f.host) (void)hipHostFree(f.host);
+    if (f.done) (void)hipEventDestroy(f.done);
+  }
+  if (ctx->frame_ready) (void)hipEventDestroy(ctx->frame_ready);
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   delete ctx;
   return WCFE_OK;
 }
@@ -1614,6 +1638,7 @@ int wcfe_finalize(wcfe_ctx *ctx) {
     ctx->tab_k = dupload(kk);
     build_batch(ctx, ctx->b, ctx->cells, mixed(ctx) ? 4 : 8);
     ctx->finalized = true;
+    ++ctx->layout;
   });
 }
 
@@ -1653,6 +1678,7 @@ void rebuild_batch(wcfe_ctx *ctx) {
   ctx->b.release();
   build_batch(ctx, ctx->b, ctx->cells, mixed(ctx) ? 4 : 8);
   ctx->stepped = false;
+  ++ctx->layout;
 }
 
 // Device bytes of the dense systems the cells would need at a threshold.
@@ -1878,6 +1904,113 @@ int wcfe_set_positions(wcfe_ctx *ctx, int32_t cell, const double *x) {
     const CellParams &p = ctx->b.cp_h[cell];
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     HIPCHECK(hipMemcpy(ctx->b.x + p.node_off, x, sizeof(double2) * p.n_nodes, hipMemcpyHostToDevice));
+  });
+}
+
+int wcfe_frame_capture(wcfe_ctx *ctx, int32_t *frame) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    Batch &b = ctx->b;
+    if (!ctx->copy_stream) {
+      HIPCHECK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+      HIPCHECK(hipEventCreateWithFlags(&ctx->frame_ready, hipEventDisableTiming));
+    }
+    const int id = ctx->next_frame;
+    FrameSlot &f = ctx->frames[id & 1];
+    if (f.done) HIPCHECK(hipEventSynchronize(f.done));  // the slot's previous copy has landed
+    const size_t N = (size_t)b.N;
+    const size_t bytes = 8 * N * sizeof(double) + sizeof(CellState) * (size_t)b.ncell;
+    if (bytes > f.bytes) {
+      if (f.dev) HIPCHECK(hipFree(f.dev));
+      if (f.host) HIPCHECK(hipHostFree(f.host));
+      f.dev = nullptr;
+      f.host = nullptr;
+      f.bytes = 0;
+      HIPCHECK(hipMalloc(&f.dev, bytes));
+      HIPCHECK(hipHostMalloc(&f.host, bytes, hipHostMallocDefault));
+      f.bytes = bytes;
+    }
+    if (!f.done) HIPCHECK(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
+    // snapshot in stream order on the device, then D2H on the copy stream
+    // while the following Newton steps run
+    hipStream_t s = ctx->stream;
+    double *d = f.dev;
+    HIPCHECK(hipMemcpyAsync(d, b.X, sizeof(double) * 2 * N, hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(d + 2 * N, b.ea, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(d + 5 * N, b.cC, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(d + 6 * N, b.fau, sizeof(double) * 2 * N, hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(d + 8 * N, b.cs, sizeof(CellState) * b.ncell, hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipEventRecord(ctx->frame_ready, s));
+    HIPCHECK(hipStreamWaitEvent(ctx->copy_stream, ctx->frame_ready, 0));
+    HIPCHECK(hipMemcpyAsync(f.host, f.dev, bytes, hipMemcpyDeviceToHost, ctx->copy_stream));
+    HIPCHECK(hipEventRecord(f.done, ctx->copy_stream));
+    f.id = id;
+    f.layout = ctx->layout;
+    ++ctx->next_frame;
+    if (frame) *frame = id;
+  });
+}
+
+namespace {
+// The landed host copy of a captured frame.
+const FrameSlot &frame_slot(wcfe_ctx *ctx, int frame) {
+  if (frame < 0 || frame >= ctx->next_frame) fail(WCFE_ERR_ARG, "frame %d has not been captured", frame);
+  const FrameSlot &f = ctx->frames[frame & 1];
+  if (f.id != frame) fail(WCFE_ERR_ARG, "frame %d was replaced by a later capture (two frames in flight)", frame);
+  if (f.layout != ctx->layout) fail(WCFE_ERR_STATE, "the batch was laid out again after frame %d", frame);
+  HIPCHECK(hipEventSynchronize(f.done));
+  return f;
+}
+}  // namespace
+
+int wcfe_frame_get(wcfe_ctx *ctx, int32_t frame, int32_t cell, double *X, double *ea, double *cC, double *fau,
+                   double *time) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    const FrameSlot &f = frame_slot(ctx, frame);
+    const size_t N = (size_t)ctx->b.N, o = (size_t)ctx->b.cp_h[cell].node_off, n = (size_t)ctx->b.cp_h[cell].n_nodes;
+    const double *h = f.host;
+    if (X) std::copy(h + 2 * o, h + 2 * (o + n), X);
+    if (ea) std::copy(h + 2 * N + 3 * o, h + 2 * N + 3 * (o + n), ea);
+    if (cC) std::copy(h + 5 * N + o, h + 5 * N + o + n, cC);
+    if (fau) std::copy(h + 6 * N + 2 * o, h + 6 * N + 2 * (o + n), fau);
+    if (time) *time = reinterpret_cast<const CellState *>(h + 8 * N)[cell].time;
+  });
+}
+
+int wcfe_frame_write_vtk(wcfe_ctx *ctx, int32_t frame, int32_t cell, const char *path, int32_t style,
+                         const double *cA, int32_t law, double kint) {
+  if (!ctx || !path) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    if (style != WCFE_VTK_CELL && style != WCFE_VTK_SUBSTRATE) fail(WCFE_ERR_ARG, "unknown VTK style %d", style);
+    if (law != 0 && law != 1) fail(WCFE_ERR_ARG, "law must be 0 (linear kint) or 1 (MD table)");
+    const FrameSlot &f = frame_slot(ctx, frame);
+    const size_t N = (size_t)ctx->b.N, o = (size_t)ctx->b.cp_h[cell].node_off;
+    const HostCell &hc = ctx->cells[cell];
+    VtkInput in;
+    in.path = path;
+    in.style = style;
+    in.n_nodes = hc.n_nodes;
+    in.n_tris = hc.n_tris;
+    in.tris = hc.tris.data();
+    in.x = f.host + 2 * o;
+    in.x_stride = 2;
+    in.ea = f.host + 2 * N + 3 * o;
+    in.ea_kind = 3;
+    in.cC = f.host + 5 * N + o;
+    in.fau = f.host + 6 * N + 2 * o;
+    in.fau_stride = 2;
+    in.cA = cA;
+    in.law = law;
+    in.kint = kint;
+    in.time = reinterpret_cast<const CellState *>(f.host + 8 * N)[cell].time;
+    std::string err;
+    const int rc = write_vtk_file(in, err);
+    if (rc != WCFE_OK) fail(rc, "%s", err.c_str());
   });
 }
 

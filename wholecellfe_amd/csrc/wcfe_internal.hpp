@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "wcfe_device.hpp"
 
 namespace wcfe {
@@ -162,5 +164,20 @@ void launch_ref_force_fa_subs(hipStream_t s, int n_nodes, const int *ring_start,
                               double *hit_val, int *n_hits, int *overflow);
 void launch_dense_residual(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                            const double *A64, const long long *a_off, const double *b, const double *x, double *r);
+
+// Host-side VTK frame writer (wcfe_vtk.hip): one cell or substrate, the
+// reference's file layout.  Positions and fa_u with 2 (z = 0) or 3 components
+// per node; strain 3 (planar [xx,yy,xy]) or 6 (Voigt) per node.
+struct VtkInput {
+  const char *path = nullptr;
+  int style = 0, n_nodes = 0, n_tris = 0;
+  const int32_t *tris = nullptr;
+  const double *x = nullptr, *fau = nullptr, *ea = nullptr, *cA = nullptr, *cC = nullptr;
+  int x_stride = 3, fau_stride = 3, ea_kind = 6;
+  double cA_default = 100.0;  // the reference's initial c_A (main.cpp:674-677)
+  int law = 1;
+  double kint = 0.0, time = 0.0;
+};
+int write_vtk_file(const VtkInput &in, std::string &err);
 
 }  // namespace wcfe
