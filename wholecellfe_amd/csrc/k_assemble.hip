@@ -157,6 +157,105 @@ template void launch_assemble<float>(hipStream_t, const View &, int, int, int, f
 template void launch_assemble<double>(hipStream_t, const View &, int, int, int, double *, double *, double *,
                                       double *, double *, double *);
 
+// ---------------------------------------------------------------------------
+// whole-batch assembly in two launches: every element of every colour at once,
+// then one thread per node gathering its rows.  A node's rows are written by
+// that node's thread only, and it adds its elements' contributions in element
+// order -- colour-major, the order the colour-by-colour scatter adds them -- so
+// G and K are bit for bit the scatter's.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(128) void k_element_pass(View v, int e_begin, int e_end, double *Ge_out, double *Ke64,
+                                                      CellState *cs) {
+  const int e = e_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e_end) return;
+  double Ge[6], Ke[36];
+  const bool ok = element_system<MODE == ASM_FULL>(v, e, Ge, Ke);
+  if (!ok) cs[v.el_cell[e]].degenerate = 1;
+  double2 *g = reinterpret_cast<double2 *>(Ge_out + 6LL * e);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) g[k] = make_double2(Ge[2 * k], Ge[2 * k + 1]);
+  if (MODE == ASM_FULL) {
+    double2 *dst = reinterpret_cast<double2 *>(Ke64 + 36LL * e);
+#pragma unroll
+    for (int k = 0; k < 18; ++k) dst[k] = make_double2(Ke[2 * k], Ke[2 * k + 1]);
+  }
+}
+
+template <class T, int MODE>
+__global__ __launch_bounds__(256) void k_node_gather(View v, const int *node_cell, const int *adj_start,
+                                                     const int *adj, GatherPlan gp, int n0, int n1, const double *Ge,
+                                                     const double *Ke64, double *G, T *K, T *Kb) {
+  const int n = n0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n1) return;
+  double g0 = G[2 * n], g1 = G[2 * n + 1];
+  for (int k = adj_start[n]; k < adj_start[n + 1]; ++k) {
+    const int pk = adj[k];
+    const int e = pk >> 2, a = pk & 3;
+    g0 += Ge[6LL * e + 2 * a];
+    g1 += Ge[6LL * e + 2 * a + 1];
+  }
+  G[2 * n] = g0;
+  G[2 * n + 1] = g1;
+  if (MODE != ASM_FULL) return;
+  // each 2 x 2 block of the node's rows: read once, add the element blocks
+  // in element order, write once
+  const CellParams &p = v.cp[node_cell[n]];
+  T *A = K + p.k_off;
+  const long long r0 = 2LL * v.kpos[n];
+  for (int g = gp.start[n]; g < gp.start[n + 1]; ++g) {
+    T *b0, *b1;
+    if (p.csr) {
+      b0 = Kb + 4LL * gp.bpos[g];
+      b1 = b0 + 2;
+    } else {
+      b0 = A + r0 * p.npad + 2 * v.kpos[gp.col[g]];
+      b1 = b0 + p.npad;
+    }
+    T a00 = b0[0], a01 = b0[1], a10 = b1[0], a11 = b1[1];
+    for (int q = gp.cstart[g]; q < gp.cstart[g + 1]; ++q) {
+      const double *Ke = Ke64 + gp.ke[q];
+      a00 += T(Ke[0]);
+      a01 += T(Ke[1]);
+      a10 += T(Ke[6]);
+      a11 += T(Ke[7]);
+    }
+    b0[0] = a00;
+    b0[1] = a01;
+    b1[0] = a10;
+    b1[1] = a11;
+  }
+}
+
+template <class T>
+void launch_assemble_gather(hipStream_t s, const View &v, int e_begin, int e_end, int mode, const int *node_cell,
+                            const int *adj_start, const int *adj, const GatherPlan &gp, int n0, int n1, double *Ge,
+                            double *Ke64, double *G, T *K, T *Kb) {
+  CellState *cs = const_cast<CellState *>(v.cs);
+  const int ne = e_end - e_begin, nn = n1 - n0;
+  if (ne > 0) {
+    if (mode == ASM_FULL)
+      hipLaunchKernelGGL(k_element_pass<ASM_FULL>, dim3((ne + 127) / 128), dim3(128), 0, s, v, e_begin, e_end, Ge,
+                         Ke64, cs);
+    else
+      hipLaunchKernelGGL(k_element_pass<ASM_RESIDUAL>, dim3((ne + 127) / 128), dim3(128), 0, s, v, e_begin, e_end,
+                         Ge, Ke64, cs);
+  }
+  if (nn <= 0) return;
+  if (mode == ASM_FULL)
+    hipLaunchKernelGGL((k_node_gather<T, ASM_FULL>), dim3((nn + 255) / 256), dim3(256), 0, s, v, node_cell, adj_start,
+                       adj, gp, n0, n1, Ge, Ke64, G, K, Kb);
+  else
+    hipLaunchKernelGGL((k_node_gather<T, ASM_RESIDUAL>), dim3((nn + 255) / 256), dim3(256), 0, s, v, node_cell,
+                       adj_start, adj, gp, n0, n1, Ge, Ke64, G, K, Kb);
+}
+template void launch_assemble_gather<float>(hipStream_t, const View &, int, int, int, const int *, const int *,
+                                            const int *, const GatherPlan &, int, int, double *, double *, double *,
+                                            float *, float *);
+template void launch_assemble_gather<double>(hipStream_t, const View &, int, int, int, const int *, const int *,
+                                             const int *, const GatherPlan &, int, int, double *, double *, double *,
+                                             double *, double *);
+
 __global__ __launch_bounds__(128) void k_element_dump(View v, int e_begin, int e_end, double *Ge, double *Ke) {
   const int e = e_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= e_end) return;

@@ -431,6 +431,10 @@ struct Batch {
   CellParams *cp = nullptr;
   CellState *cs = nullptr;
   int *node_cell = nullptr, *adj_start = nullptr, *adj = nullptr;
+  double *Ge = nullptr;  // per element: residual (6) for the node gather
+  // node gather plan: per node, its column nodes (groups) and, per group, the
+  // element contributions in element order (offsets of the 2 x 2 block in Ke64)
+  int *g_start = nullptr, *g_col = nullptr, *g_bpos = nullptr, *g_cstart = nullptr, *g_ke = nullptr;
   double *Ke64 = nullptr, *G = nullptr, *r = nullptr, *delta = nullptr, *Rea = nullptr, *MM = nullptr;
   void *K = nullptr;
   void *Dinv = nullptr;
@@ -470,7 +474,7 @@ struct Batch {
 
   void release() {
     void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
-                    sp_local, cp, cs, node_cell, adj_start, adj, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
+                    sp_local, cp, cs, node_cell, adj_start, adj, Ge, g_start, g_col, g_bpos, g_cstart, g_ke, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
                     replica, transitions, plan.d_prefix, row_ptr, bcol, diag_pos, el_bpos, csr_cells, chunks, Kb,
                     cgvec, cg, partials, kpos, rp, dp, bt_bidx, bt_node, bt_cst, crange, bt_cand, bt_frame, bt_g, bt_J, bt_area,
                     bt_gd, bt_fext};
@@ -931,6 +935,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   b.transitions = dalloc<int>(1);
   HIPCHECK(hipMemset(b.transitions, 0, sizeof(int)));
   b.Ke64 = dalloc<double>(36 * (size_t)T);
+  b.Ge = dalloc<double>(6 * (size_t)T);
   b.G = dalloc<double>(2 * (size_t)N);
   b.r = dalloc<double>(2 * (size_t)N);
   b.delta = dalloc<double>(2 * (size_t)N);
@@ -953,6 +958,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   b.nchunk_h.assign(b.ncell, 0);
   b.blk_begin_h.assign(b.ncell, 0);
   b.blk_end_h.assign(b.ncell, 0);
+  std::vector<int> bpos_all;  // block-CSR position of element block (a, b), sparse cells
   if (b.any_csr) {
     std::vector<std::vector<int>> nbr(N);
     for (int e = 0; e < T; ++e) {
@@ -999,6 +1005,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.bcol = dupload(bcol);
     b.diag_pos = dupload(dpos);
     b.el_bpos = dupload(bpos);
+    bpos_all = std::move(bpos);
     b.chunks = dupload(ch);
     b.csr_cells = dupload(b.csr_cells_h);
     b.Kb = (void *)dalloc<unsigned char>((size_t)std::max<long long>(b.nnzb, 1) * 4 * elem_size);
@@ -1012,6 +1019,50 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     }
     b.cg = dupload(cg);
     b.partials = dalloc<double>(2 * (size_t)std::max(b.nchunk, 1));
+  }
+  {
+    // node gather plan: for node n, its distinct column nodes in order of
+    // first appearance along its elements (element order) and, for each, the
+    // contributing element blocks in element order -- the order in which the
+    // colour-by-colour scatter adds them to the same entries
+    std::vector<int> gs(N + 1, 0), gcol, gbp, gcs(1, 0), gke;
+    gcol.reserve(7 * (size_t)N);
+    gbp.reserve(7 * (size_t)N);
+    gke.reserve(9 * (size_t)T);
+    std::vector<int> cols;
+    std::vector<std::vector<int>> lists;
+    for (int n = 0; n < N; ++n) {
+      cols.clear();
+      for (auto &l : lists) l.clear();
+      std::vector<int> bps;
+      for (int k = adj_start[n]; k < adj_start[n + 1]; ++k) {
+        const int e = adj[k] >> 2, a = adj[k] & 3;
+        for (int q = 0; q < 3; ++q) {
+          const int col = el_tri[3 * (size_t)e + q];
+          size_t g = 0;
+          while (g < cols.size() && cols[g] != col) ++g;
+          if (g == cols.size()) {
+            cols.push_back(col);
+            if (lists.size() < cols.size()) lists.emplace_back();
+            bps.push_back(bpos_all.empty() ? -1 : bpos_all[9 * (size_t)e + 3 * a + q]);
+          }
+          lists[g].push_back(36 * e + 12 * a + 2 * q);
+        }
+      }
+      for (size_t g = 0; g < cols.size(); ++g) {
+        gcol.push_back(cols[g]);
+        gbp.push_back(bps[g]);
+        gke.insert(gke.end(), lists[g].begin(), lists[g].end());
+        gcs.push_back((int)gke.size());
+      }
+      gs[n + 1] = (int)gcol.size();
+    }
+    if (36LL * T >= (1LL << 31)) fail(WCFE_ERR_ARG, "batch of %d elements exceeds the gather plan's 32-bit offsets", T);
+    b.g_start = dupload(gs);
+    b.g_col = dupload(gcol);
+    b.g_bpos = dupload(gbp);
+    b.g_cstart = dupload(gcs);
+    b.g_ke = dupload(gke);
   }
   b.plan.build(b.cp_h, ctx->opt.pivoting, b.lu_groups, b.lu_depth);
   if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
@@ -1085,10 +1136,17 @@ template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &s
   hipStream_t s = ctx->stream;
   View v = make_view(ctx, b);
   T *K = static_cast<T *>(b.K);
-  for (int k = 0; k < b.ncolor; ++k) {
-    int e0, e1;
-    color_range(b, sc, k, e0, e1);
-    launch_assemble<T>(s, v, e0, e1, mode, K, static_cast<T *>(b.Kb), b.G, b.Ke64, b.Rea, b.MM);
+  if (sc.cell < 0 && mode != ASM_STRAIN) {
+    // whole batch: every element at once, then each node gathers its rows
+    const GatherPlan gp{b.g_start, b.g_col, b.g_bpos, b.g_cstart, b.g_ke};
+    launch_assemble_gather<T>(s, v, 0, b.T, mode, b.node_cell, b.adj_start, b.adj, gp, sc.node_begin, sc.node_end,
+                              b.Ge, b.Ke64, b.G, K, static_cast<T *>(b.Kb));
+  } else {
+    for (int k = 0; k < b.ncolor; ++k) {
+      int e0, e1;
+      color_range(b, sc, k, e0, e1);
+      launch_assemble<T>(s, v, e0, e1, mode, K, static_cast<T *>(b.Kb), b.G, b.Ke64, b.Rea, b.MM);
+    }
   }
   if (!b.any_bterm || mode == ASM_STRAIN) return;
   const BndView bv = bnd_view(ctx, b);

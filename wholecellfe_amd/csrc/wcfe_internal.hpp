@@ -26,6 +26,19 @@ void launch_copy(hipStream_t s, const void *src, void *dst, long long bytes);
 template <class T>
 void launch_assemble(hipStream_t s, const View &v, int e_begin, int e_end, int mode, T *K, T *Kb, double *G,
                      double *Ke64, double *Rea, double *MM);
+// whole-batch assembly: element pass over [e_begin, e_end) (all colours),
+// then a node gather over [n0, n1) (ASM_FULL or ASM_RESIDUAL)
+struct GatherPlan {
+  const int *start;   // per node: groups [start[n], start[n+1])
+  const int *col;     // per group: column node
+  const int *bpos;    // per group: block-CSR position (sparse cells), else -1
+  const int *cstart;  // per group: contributions [cstart[g], cstart[g+1])
+  const int *ke;      // per contribution: offset of the 2 x 2 block in Ke64
+};
+template <class T>
+void launch_assemble_gather(hipStream_t s, const View &v, int e_begin, int e_end, int mode, const int *node_cell,
+                            const int *adj_start, const int *adj, const GatherPlan &gp, int n0, int n1, double *Ge,
+                            double *Ke64, double *G, T *K, T *Kb);
 void launch_element_dump(hipStream_t s, const View &v, int e_begin, int e_end, double *Ge, double *Ke);
 void launch_cell_norm(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell,
                       const double *vec, int which, int only_active);
