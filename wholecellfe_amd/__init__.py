@@ -55,7 +55,7 @@ class Options(C.Structure):
         ("t0", C.c_double), ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32), ("k_on", C.c_double),
         ("k_off0", C.c_double), ("f_bell", C.c_double), ("kappa", C.c_double), ("symmetric", C.c_int32),
         ("dense_max_dof", C.c_int32), ("cg_rtol", C.c_double), ("cg_max_iter", C.c_int32),
-        ("lu_groups", C.c_int32), ("lu_depth", C.c_int32), ("band", C.c_int32),
+        ("lu_groups", C.c_int32), ("lu_depth", C.c_int32), ("band", C.c_int32), ("fused_step", C.c_int32),
     ]
 
 

@@ -126,14 +126,14 @@ __device__ __forceinline__ void invert_triangles(const T *a, T *Li, T *Ui) {
   }
 }
 
-template <class T, int PA> __device__ __forceinline__ int factor_invert_diag(T *a, T *w, T *Li, T *Ui, T tau);
+template <class T, int PA, bool STORE = true>
+__device__ __forceinline__ int invert_diag(T *a, T *Li, T *Ui, T tau);
 
-// k = 0 of static pivoting: factor block (0, 0) and invert its triangles.
+// k = 0 of static pivoting: invert block (0, 0).
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_diag_inv(const CellParams *cp, CellState *cs, const int *map, int k, T *K, T *Dinv) {
   constexpr int P = 65;
   __shared__ T a[64 * P];
-  __shared__ T wk[64 * P];
   const int v = map[blockIdx.x];
   const int c = v >> 20;
   const CellParams p = cp[c];
@@ -143,9 +143,7 @@ __global__ __launch_bounds__(256) void k_lu_diag_inv(const CellParams *cp, CellS
     a[(idx >> 6) * P + (idx & 63)] = blk[(idx >> 6) * ld + (idx & 63)];
   __syncthreads();
   T *D = Dinv + dinv_off(p, k);
-  const int fixed = factor_invert_diag<T, P>(a, wk, D, D + 4096, (T)cs[c].tau);
-  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
-    blk[(idx >> 6) * ld + (idx & 63)] = a[(idx >> 6) * P + (idx & 63)];
+  const int fixed = invert_diag<T, P>(a, D, D + 4096, (T)cs[c].tau);
   if (fixed) atomicAdd(&cs[c].n_static, fixed);
 }
 template <class T>
@@ -309,21 +307,20 @@ __device__ __forceinline__ void mma_64(const T *xs, const T *ys, typename Acc4<T
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block, blocked: factorisation without interchanges (pivots below
-// tau replaced by +-tau) and the inverses of its triangles, by 16 x 16
-// sub-blocks.  Per sub-block step kb: wave 0 factors the diagonal sub-block
-// in registers (lane r owns row r, the pivot row read across lanes) and
-// forms the inverse of its unit lower triangle, wave 1 that of its upper
-// triangle; then all waves compute the panel solves as products with those
-// inverses and the trailing update, on MFMA.  Finally the inverses of the
-// whole triangles by block substitution on MFMA:
-//   (L^-1)_ik = -Linv_ii sum_{m=k}^{i-1} L_im (L^-1)_mk,
-//   (U^-1)_kj = -Uinv_kk sum_{m=k+1}^{j} U_km (U^-1)_mj.
-// a: the block in LDS (pitch PA), factored in place; w: 64 x PA LDS scratch
-// (diagonal sub-blocks: the sub-block inverses, Linv strictly below and
-// Uinv on and above the diagonal; off-diagonal: L^-1 below, U^-1 above).
-// Li, Ui: 64 x 64 row-major outputs.  All 256 threads call it (uniform
-// barriers).  Returns the pivots this thread replaced.
+// Diagonal block: its inverse by Gauss-Jordan elimination without
+// interchanges (pivots below tau replaced by +-tau), in 16 x 16 sub-blocks.
+// Per sub-block step kb: wave 0 inverts the diagonal sub-block P in place
+// (cross-lane reads), then on MFMA
+//   row kb:       A_kb,j <- P^-1 A_kb,j                 (j != kb)
+//   other rows i: A_i,j  <- A_i,j - A_i,kb A_kb,j  (j != kb),  A_i,kb <- -A_i,kb P^-1
+// one wave per block row, so each row's old A_i,kb is read before it is
+// replaced.  The block is taken as factored A_kk = I A_kk: the stored pair is
+// Li = I and Ui = A_kk^-1, so the band LU's off-diagonal factors are
+// L_ik = A_ik A_kk^-1 and U_kj = A_kj (block LDU form), and every solve that
+// applies Li then Ui to a diagonal block applies A_kk^-1.
+// a: the block in LDS (pitch PA), inverted in place; Li, Ui: 64 x 64
+// row-major outputs (STORE).  All 256 threads call it (uniform barriers).
+// Returns the pivots this thread replaced.
 // ---------------------------------------------------------------------------
 template <class T> __device__ __forceinline__ T rdlane(T v, int l);
 template <> __device__ __forceinline__ float rdlane(float v, int l) {
@@ -372,146 +369,93 @@ __device__ __forceinline__ void acc16_store(const typename Acc4<T>::type &acc, T
   for (int r = 0; r < 4; ++r) Z[acc_row(T(0), lk, r) * PA + lr] = acc[r];
 }
 
-template <class T, int PA> __device__ __forceinline__ int factor_invert_diag(T *a, T *w, T *Li, T *Ui, T tau) {
+// 16 x 16 sub-block at D (pitch PA) inverted in place, one wave: lane l
+// holds row l & 15, columns 4 (l >> 4) .. +3; the pivot and the pivot row and
+// column reach the other lanes by cross-lane reads.
+template <class T, int PA> __device__ __forceinline__ int gj16(T *D, T tau) {
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  T v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = D[r * PA + 4 * g + q];
+  int fixed = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int gj = j >> 2, qj = j & 3;
+    const T p0 = rdlane(v[qj], gj * 16 + j);
+    const bool small = !(fabs((double)p0) >= (double)tau);
+    const T piv = small ? (p0 >= T(0) ? tau : -tau) : p0;
+    fixed += (small && lane == 0) ? 1 : 0;
+    const T ip = recip(piv);
+    const T f = __shfl(v[qj], gj * 16 + r, 64);  // A[r][j]
+    T rq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rq[q] = __shfl(v[q], g * 16 + j, 64) * ip;  // pivot row / pivot
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const bool cj = 4 * g + q == j;
+      v[q] = (r == j) ? (cj ? ip : rq[q]) : (cj ? -f * ip : v[q] - f * rq[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) D[r * PA + 4 * g + q] = v[q];
+  return fixed;
+}
+
+template <class T, int PA, bool STORE>
+__device__ __forceinline__ int invert_diag(T *a, T *Li, T *Ui, T tau) {
   typedef typename Acc4<T>::type acc_t;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15;
+  const int wave = threadIdx.x >> 6;
   int fixed = 0;
   auto sub = [](T *base, int i, int j) { return base + 16 * i * PA + 16 * j; };
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
-    T *D = sub(a, kb, kb), *WD = sub(w, kb, kb);
-    T row[16];
-    if (wave == 0) {
-#pragma unroll
-      for (int c = 0; c < 16; ++c) row[c] = D[lr * PA + c];
-      // straight-line (no per-step branches, so steps overlap): rows at or
-      // above the pivot take a zero multiplier and stay unchanged
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const T p0 = rdlane(row[j], j);
-        const bool small = !(fabs((double)p0) >= (double)tau);
-        const T piv = small ? (p0 >= T(0) ? tau : -tau) : p0;
-        fixed += (small && lane == j) ? 1 : 0;
-        const T inv = recip(piv);
-        const T l = lr > j ? row[j] * inv : T(0);
-        row[j] = lr > j ? l : (lr == j ? piv : row[j]);
-#pragma unroll
-        for (int c = j + 1; c < 16; ++c) row[c] -= l * rdlane(row[c], j);
-      }
-      if (lane < 16) {
-#pragma unroll
-        for (int c = 0; c < 16; ++c) D[lr * PA + c] = row[c];
-      }
-    }
+    if (wave == 0) fixed += gj16<T, PA>(sub(a, kb, kb), tau);
     __syncthreads();
-    if (wave == 0) {
-      // column lr of the inverse of the unit lower triangle, column-oriented
-      // (row i of L is in lane i)
-      T x[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) x[i] = (i == lr) ? T(1) : T(0);
-#pragma unroll
-      for (int m = 0; m < 15; ++m)
-#pragma unroll
-        for (int i = m + 1; i < 16; ++i) x[i] -= D[i * PA + m] * x[m];  // L_im, broadcast read
-      if (lane < 16) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-          if (i > lr) WD[i * PA + lr] = x[i];
-      }
-    } else if (wave == 1) {
-      // column lr of the inverse of the upper triangle, column-oriented
-      T rinv[16], x[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        rinv[i] = recip(D[i * PA + i]);
-        x[i] = (i == lr) ? T(1) : T(0);
-      }
-#pragma unroll
-      for (int i = 15; i >= 0; --i) {
-        x[i] *= rinv[i];
-#pragma unroll
-        for (int m = 0; m < i; ++m) x[m] -= D[m * PA + i] * x[i];  // U_mi, broadcast read
-      }
-      if (lane < 16) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-          if (i <= lr) WD[i * PA + lr] = x[i];
-      }
-    }
-    __syncthreads();
-    const int nb = 3 - kb;
-    // panel: L_ik = A_ik Uinv_kk (i > kb), U_kj = Linv_kk A_kj (j > kb)
-    for (int t = wave; t < 2 * nb; t += 4) {
+    if (wave < 3) {  // row kb
+      const int j = wave + (wave >= kb ? 1 : 0);
       acc_t acc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = T(0);
-      if (t < nb) {
-        T *Z = sub(a, kb + 1 + t, kb);
-        mm16<T, PA, M_FULL, M_UPPER>(Z, WD, acc, T(1));
-        acc16_store<T, PA>(acc, Z);
-      } else {
-        T *Z = sub(a, kb, kb + 1 + (t - nb));
-        mm16<T, PA, M_UNIT_LOWER, M_FULL>(WD, Z, acc, T(1));
-        acc16_store<T, PA>(acc, Z);
-      }
+      mm16<T, PA, M_FULL, M_FULL>(sub(a, kb, kb), sub(a, kb, j), acc, T(1));
+      acc16_store<T, PA>(acc, sub(a, kb, j));
     }
     __syncthreads();
-    // trailing update A_ij -= L_ik U_kj
-    for (int t = wave; t < nb * nb; t += 4) {
-      const int i = kb + 1 + t / nb, j = kb + 1 + t % nb;
-      acc_t acc;
-      acc16_load<T, PA>(acc, sub(a, i, j));
-      mm16<T, PA, M_FULL, M_FULL>(sub(a, i, kb), sub(a, kb, j), acc, T(-1));
-      acc16_store<T, PA>(acc, sub(a, i, j));
-    }
-    __syncthreads();
-  }
-  // inverses of the whole triangles, by distance d from the diagonal
+    if (wave < 3) {  // the other rows, one per wave
+      const int i = wave + (wave >= kb ? 1 : 0);
 #pragma unroll
-  for (int d = 1; d < 4; ++d) {
-    const int nt = 4 - d;
-    for (int t = wave; t < 2 * nt; t += 4) {
+      for (int j = 0; j < 4; ++j) {
+        if (j == kb) continue;
+        acc_t acc;
+        acc16_load<T, PA>(acc, sub(a, i, j));
+        mm16<T, PA, M_FULL, M_FULL>(sub(a, i, kb), sub(a, kb, j), acc, T(-1));
+        acc16_store<T, PA>(acc, sub(a, i, j));
+      }
       acc_t acc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = T(0);
-      if (t < nt) {  // (L^-1)_ik, i = k + d
-        const int k = t, i = k + d;
-        mm16<T, PA, M_FULL, M_UNIT_LOWER>(sub(a, i, k), sub(w, k, k), acc, T(1));
-        for (int m = k + 1; m < i; ++m) mm16<T, PA, M_FULL, M_FULL>(sub(a, i, m), sub(w, m, k), acc, T(1));
-        acc16_store<T, PA>(acc, sub(w, i, k));
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = T(0);
-        mm16<T, PA, M_UNIT_LOWER, M_FULL>(sub(w, i, i), sub(w, i, k), acc, T(-1));
-        acc16_store<T, PA>(acc, sub(w, i, k));
-      } else {  // (U^-1)_kj, j = k + d
-        const int k = t - nt, j = k + d;
-        for (int m = k + 1; m < j; ++m) mm16<T, PA, M_FULL, M_FULL>(sub(a, k, m), sub(w, m, j), acc, T(1));
-        mm16<T, PA, M_FULL, M_UPPER>(sub(a, k, j), sub(w, j, j), acc, T(1));
-        acc16_store<T, PA>(acc, sub(w, k, j));
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = T(0);
-        mm16<T, PA, M_UPPER, M_FULL>(sub(w, k, k), sub(w, k, j), acc, T(-1));
-        acc16_store<T, PA>(acc, sub(w, k, j));
-      }
+      mm16<T, PA, M_FULL, M_FULL>(sub(a, i, kb), sub(a, kb, kb), acc, T(-1));
+      acc16_store<T, PA>(acc, sub(a, i, kb));
     }
     __syncthreads();
   }
-  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
-    const int r = idx >> 6, c = idx & 63;
-    const T v = w[r * PA + c];
-    Li[idx] = r > c ? v : (r == c ? T(1) : T(0));
-    Ui[idx] = r <= c ? v : T(0);
+  if (STORE) {
+    for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
+      const int r = idx >> 6, c = idx & 63;
+      Li[idx] = r == c ? T(1) : T(0);
+      Ui[idx] = a[r * PA + c];
+    }
   }
   return fixed;
 }
 
-// block triangular solves as products with the stored inverses:
-//   items [0, m):  U_kj = L_kk^-1 A_kj
-//   items [m, 2m): L_ik = A_ik U_kk^-1   (static pivoting only)
-// Symmetric cells (p.sym) have only the L items, [0, m): with A = L U and
-// A symmetric, U = D L^T (D = diag U), so each L_ik tile also yields
-// U_ki = D_k L_ik^T, stored by transposition through LDS.
+// block triangular solves as products with the stored pair (Li, Ui):
+//   items [0, m):  U_kj = Li_k A_kj
+//   items [m, 2m): L_ik = A_ik Ui_k   (static pivoting only)
+// Under static pivoting the pair is (I, A_kk^-1) (block LDU form, see
+// invert_diag), so U_kj = A_kj and L_ik = A_ik A_kk^-1; under partial
+// pivoting (U items only) Li_k is the inverse of the unit lower factor.
+// Symmetric cells (p.sym) have only the L items, [0, m): U_ki = A_ik^T is
+// stored by transposition of the block before the product.
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_trsm_mm(const CellParams *cp, const int *map, int k, T *K,
                                                     const T *Dinv) {
@@ -557,19 +501,12 @@ __global__ __launch_bounds__(256) void k_lu_trsm_mm(const CellParams *cp, const 
       for (int r = 0; r < 4; ++r)
         blk[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
   if (!p.sym) return;
-  // U_k,other[col][row] = L_other,k[row][col] * U_kk[col][col]
-  __syncthreads();  // every wave is done reading xs
-#pragma unroll
-  for (int tj = 0; tj < 2; ++tj) {
-    const int col = wc + tj * 16 + lr;
-    const T d = A[(long long)(k * NB + col) * ld + k * NB + col];
-#pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) xs[col * P + wr + ti * 16 + acc_row(T(0), lk, r)] = acc[ti][tj][r] * d;
+  // U_{k,other} = A_{other,k}^T, the block before the product (still in xs)
+  T *U = A + (long long)(k * NB) * ld + other * NB;
+  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
+    const int r = idx >> 6, cc = idx & 63;
+    U[(long long)r * ld + cc] = xs[cc * P + r];
   }
-  __syncthreads();
-  store_block<T, P>(A + (long long)(k * NB) * ld + other * NB, xs, ld);
 }
 template <class T>
 void launch_lu_trsm(hipStream_t s, const CellParams *cp, CellState *, const WorkList &wl, int k, T *K,
@@ -706,9 +643,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
       for (int r = 0; r < 4; ++r) a[(wr + ti * 16 + acc_row(T(0), lk, r)) * PA + wc + tj * 16 + lr] = acc[ti][tj][r];
   __syncthreads();
   T *D = Dinv + dinv_off(p, s0);
-  const int fixed = factor_invert_diag<T, PA>(a, us, D, D + 4096, (T)cs[c].tau);
-  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x)
-    C[(idx >> 6) * ld + (idx & 63)] = a[(idx >> 6) * PA + (idx & 63)];
+  const int fixed = invert_diag<T, PA>(a, D, D + 4096, (T)cs[c].tau);
   if (fixed) atomicAdd(&cs[c].n_static, fixed);
 }
 template <class T>
@@ -729,6 +664,164 @@ void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const Wo
     hipLaunchKernelGGL((k_lu_gemm<T, false>), dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, next_diag,
                        npanel);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Fused block step for narrow bands under static pivoting, one launch per
+// block step k = -1 .. nblk-2 (s0 = k + 1): the trailing update of panel k,
+// the inverse of diagonal block s0, and the block solves of panel s0.  Items
+// of a cell (lu_step_items):
+//   t = 0          diagonal block s0: invert, store the inverse
+//   t = 1 .. w     L_{s0+t, s0} = A_{s0+t, s0} A_{s0,s0}^-1
+//   t = w+1 .. 2w  U_{s0, s0+t-w} = A_{s0, s0+t-w} (non-symmetric; the block
+//                  LDU form leaves the row as it is)
+//   then           the other blocks of panel k's window
+// with w = min(nblk - s0 - 1, bw).  Blocks inside panel k's reach (row and
+// column <= k + bw) take the update A -= L_{i,k} U_{k,j} first.  Every L item
+// computes the inverse of block s0 itself from the same inputs (so bit for
+// bit the diagonal item's), so no workgroup waits for another; only the
+// diagonal item stores it.  Diagonal blocks of K keep their unfactored
+// values: nothing reads them afterwards.  Symmetric cells: items are the
+// lower blocks; an L item also stores U_{s0,i} = A_{i,s0}^T.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void step_tile(typename Acc4<T>::type (&acc)[2][2], const T *A, long long ld, int bi,
+                                          int bj, int k, bool upd, T *ls, T *us) {
+  constexpr int P = Pitch<T>::v;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int lk = lane >> 4, lr = lane & 15;
+  const T *C = A + (long long)(bi * NB) * ld + bj * NB;
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        acc[ti][tj][r] = C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr];
+  if (!upd) return;
+  T lt[TileShape<T>::E], ut[TileShape<T>::E];
+  tile_load<T>(lt, A + (long long)(bi * NB) * ld + k * NB, ld);
+  tile_load<T>(ut, A + (long long)(k * NB) * ld + bj * NB, ld);
+  __syncthreads();  // earlier users of ls / us are done
+  tile_to_lds<T, P>(lt, ls);
+  tile_to_lds<T, P>(ut, us);
+  __syncthreads();
+  mma_64<T, P, P>(ls, us, acc, T(-1));
+}
+
+__device__ __forceinline__ int lu_step_w(const CellParams &p, int k) { return min(p.nblk - k - 2, p.bw); }
+__device__ __forceinline__ int lu_step_np(const CellParams &p, int k) {
+  return k >= 0 ? max(0, min(p.nblk - 1, k + p.bw) - (k + 1)) : 0;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState *cs, const int *map, int k, T *K,
+                                                 T *Dinv) {
+  constexpr int P = Pitch<T>::v, PA = 65;
+  typedef typename Acc4<T>::type acc_t;
+  __shared__ __attribute__((aligned(16))) T ls[64 * P];
+  __shared__ __attribute__((aligned(16))) T us[64 * P];
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int v = map[item];
+  const int c = v >> 20;
+  const CellParams p = cp[c];
+  const int t = v & 0xFFFFF;
+  const int s0 = k + 1, wl = lu_step_w(p, k), npl = lu_step_np(p, k);
+  const int nsolve = p.sym ? wl : 2 * wl;
+  // 0 diagonal, 1 L block, 2 U block (stored as updated), 3 other
+  int role, bi, bj;
+  if (t == 0) { role = 0; bi = s0; bj = s0; }
+  else if (t <= wl) { role = 1; bi = s0 + t; bj = s0; }
+  else if (t <= nsolve) { role = 2; bi = s0; bj = s0 + (t - wl); }
+  else {
+    const int q = t - 1 - nsolve;
+    role = 3;
+    if (p.sym) {
+      int i = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+      while (i * (i + 1) / 2 > q) --i;
+      while ((i + 1) * (i + 2) / 2 <= q) ++i;
+      bi = s0 + 1 + i;
+      bj = s0 + 1 + (q - i * (i + 1) / 2);
+    } else {
+      bi = s0 + 1 + q / npl;
+      bj = s0 + 1 + q % npl;
+    }
+  }
+  T *A = K + p.k_off;
+  const long long ld = p.npad;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int lk = lane >> 4, lr = lane & 15;
+  const int reach = k + p.bw;  // panel k's last block row / column
+  acc_t acc[2][2];
+  T *C = A + (long long)(bi * NB) * ld + bj * NB;
+  if (role != 0) {
+    step_tile<T>(acc, A, ld, bi, bj, k, k >= 0 && bi <= reach && bj <= reach, ls, us);
+    if (role != 1) {
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
+      return;
+    }
+  }
+  // diagonal block s0 after the update, inverted in us (pitch PA)
+  acc_t dacc[2][2];
+  step_tile<T>(dacc, A, ld, s0, s0, k, k >= 0, ls, us);
+  __syncthreads();
+  T *a = us;  // 64 x 65 fits in 64 x P
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[(wr + ti * 16 + acc_row(T(0), lk, r)) * PA + wc + tj * 16 + lr] = dacc[ti][tj][r];
+  __syncthreads();
+  T *D = Dinv + dinv_off(p, s0);
+  if (role == 0) {
+    const int fixed = invert_diag<T, PA, true>(a, D, D + 4096, (T)cs[c].tau);
+    if (fixed) atomicAdd(&cs[c].n_static, fixed);
+    return;
+  }
+  (void)invert_diag<T, PA, false>(a, D, D + 4096, (T)cs[c].tau);
+  // L = A_{bi,s0} A_{s0,s0}^-1 (own block staged in ls, pitch P)
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        ls[(wr + ti * 16 + acc_row(T(0), lk, r)) * P + wc + tj * 16 + lr] = acc[ti][tj][r];
+        acc[ti][tj][r] = T(0);
+      }
+  __syncthreads();
+  mma_64<T, P, PA>(ls, a, acc, T(1));
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
+  if (!p.sym) return;
+  // U_{s0,bi} = A_{bi,s0}^T, the block before the product (still in ls)
+  T *U = A + (long long)(s0 * NB) * ld + bi * NB;
+  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
+    const int r = idx >> 6, cc = idx & 63;
+    U[(long long)r * ld + cc] = ls[cc * P + r];
+  }
+}
+template <class T>
+void launch_lu_step(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv) {
+  if (wl.total <= 0) return;
+  // FP32 only: three 64 x 68 LDS tiles (the FP64 variant would exceed the
+  // static LDS limit); the schedule never selects it for FP64 factors
+  if constexpr (sizeof(T) == 4)
+    hipLaunchKernelGGL(k_lu_step<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv);
 }
 
 // ---------------------------------------------------------------------------
@@ -1330,6 +1423,7 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
                                   const T *);                                                                         \
   template void launch_lu_gemm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *,      \
                                   int, int, int, hipEvent_t, hipEvent_t);                                             \
+  template void launch_lu_step<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *);     \
   template void launch_lu_solve<T>(hipStream_t, const CellParams *, const CellState *, const int *, int, const T *,   \
                                    const T *, const int *, int, const double *, double *, int, int);                  \
   template void launch_lu_solve_mw<T>(hipStream_t, const CellParams *, CellState *, const int *, int,               \

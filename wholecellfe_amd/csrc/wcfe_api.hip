@@ -245,7 +245,7 @@ int block_bandwidth(int n_tris, const int *tris, const std::vector<int> &pos) {
 // LU schedule: a list of stages (kernel kind, block step k, panel count),
 // each with the exclusive prefix sums of its work items over the cells of a
 // list (all cells batched, or one cell at a time).
-enum StageKind { ST_DIAG, ST_TRSM, ST_STRIP, ST_GEMM1, ST_GEMM, ST_PANEL, ST_SWP };
+enum StageKind { ST_DIAG, ST_TRSM, ST_STRIP, ST_GEMM1, ST_GEMM, ST_PANEL, ST_SWP, ST_STEP };
 struct Stage {
   int kind, k, np, off, tot;
   long long work;  // 64 x 64 x 64 block products (GEMM and strip stages)
@@ -288,11 +288,19 @@ struct LuPlan {
       case ST_GEMM1: return k < nb ? tiles(p, m) : 0;
       case ST_GEMM: return nb - k - np >= 1 ? tiles(p, std::min(nb - k - np, p.bw)) : 0;
       case ST_PANEL: return k < nb ? 1 : 0;
+      case ST_STEP: {  // fused block step k (k = -1 .. nb - 2), see k_lu_step
+        const int s0 = k + 1;
+        if (s0 >= nb) return 0;
+        const int w = std::min(nb - s0 - 1, p.bw);
+        const long long npl = k >= 0 ? std::max(0, std::min(nb - 1, k + p.bw) - s0) : 0;
+        return 1 + (p.sym ? w : 2 * w) + (p.sym ? npl * (npl + 1) / 2 : npl * npl);
+      }
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
     }
   }
   static long long block_products(const CellParams &p, int kind, int k, int np, int pivoting) {
     if (kind == ST_STRIP || kind == ST_GEMM || kind == ST_GEMM1) return (long long)np * items(p, kind, k, np, pivoting);
+    if (kind == ST_STEP) return items(p, kind, k, np, pivoting);
     return 0;
   }
   // static pivoting: LU_DEPTH block steps per round -- panel k, then the
@@ -304,9 +312,12 @@ struct LuPlan {
   // update depth): the trailing matrix is read and written once per `depth`
   // block steps
   int depth = 4;
+  bool fused = false;  // static pivoting, depth 1: one fused launch per block step
   std::vector<std::array<int, 3>> schedule(int nblk, int pivoting) const {
     std::vector<std::array<int, 3>> s;
-    if (!pivoting) {
+    if (!pivoting && fused) {
+      for (int k = -1; k < nblk - 1; ++k) s.push_back({ST_STEP, k, 1});
+    } else if (!pivoting) {
       s.push_back({ST_DIAG, 0, 0});
       for (int k = 0; k < nblk; k += depth) {
         const int R = std::min(depth, nblk - k);
@@ -328,10 +339,11 @@ struct LuPlan {
     return s;
   }
 
-  void build(const std::vector<CellParams> &cp, int piv, int groups, int lu_depth) {
+  void build(const std::vector<CellParams> &cp, int piv, int groups, int lu_depth, bool fuse = false) {
     if (lu_depth < 1 || groups < 1) throw ApiError{WCFE_ERR_ARG, "LU schedule needs depth and groups >= 1"};
     pivoting = piv;
     depth = lu_depth;
+    fused = fuse && !piv && lu_depth == 1;
     const int n = (int)cp.size();
     std::vector<int> buf;
     max_nblk = 0;
@@ -1064,7 +1076,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.g_cstart = dupload(gcs);
     b.g_ke = dupload(gke);
   }
-  b.plan.build(b.cp_h, ctx->opt.pivoting, b.lu_groups, b.lu_depth);
+  b.plan.build(b.cp_h, ctx->opt.pivoting, b.lu_groups, b.lu_depth, ctx->opt.fused_step && elem_size == 4);
   if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
       lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
@@ -1203,6 +1215,12 @@ void enqueue_lu(wcfe_ctx *ctx, hipStream_t s, const LuPlan &plan, const std::vec
       case ST_GEMM: {
         KernelTimer kt(ctx, WCFE_PH_LU_GEMM, g.work);
         launch_lu_gemm<T>(s, cp, cs, wl, g.k, K, Dinv, 1, g.np, 0, kt.a, kt.b);
+        break;
+      }
+      case ST_STEP: {
+        if (sizeof(T) != 4) fail(WCFE_ERR_STATE, "the fused block step is built for FP32 factors only");
+        PhaseScope ph(ctx, WCFE_PH_LU_GEMM);
+        launch_lu_step<T>(s, cp, cs, wl, g.k, K, Dinv);
         break;
       }
     }
@@ -1460,6 +1478,7 @@ void wcfe_default_options(wcfe_options *o) {
   o->lu_groups = 0;  // auto
   o->lu_depth = 0;   // auto
   o->band = 1;
+  o->fused_step = 1;
 }
 
 int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
