@@ -715,6 +715,31 @@ __device__ __forceinline__ int lu_step_np(const CellParams &p, int k) {
   return k >= 0 ? max(0, min(p.nblk - 1, k + p.bw) - (k + 1)) : 0;
 }
 
+// acc <- the 64 x 64 block at C (leading dimension ld), MFMA accumulator layout
+template <class T>
+__device__ __forceinline__ void acc_load64(typename Acc4<T>::type (&acc)[2][2], const T *C, long long ld) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32, lk = lane >> 4, lr = lane & 15;
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        acc[ti][tj][r] = C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr];
+}
+template <class T, int PS>
+__device__ __forceinline__ void acc_store64(const typename Acc4<T>::type (&acc)[2][2], T *C, long long ld) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32, lk = lane >> 4, lr = lane & 15;
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState *cs, const int *map, int k, T *K,
                                                  T *Dinv) {
@@ -722,6 +747,7 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
   typedef typename Acc4<T>::type acc_t;
   __shared__ __attribute__((aligned(16))) T ls[64 * P];
   __shared__ __attribute__((aligned(16))) T us[64 * P];
+  __shared__ __attribute__((aligned(16))) T xs[64 * P];
   const int item = xcd_item(blockIdx.x, gridDim.x);
   const int v = map[item];
   const int c = v >> 20;
@@ -754,25 +780,49 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int lk = lane >> 4, lr = lane & 15;
   const int reach = k + p.bw;  // panel k's last block row / column
-  acc_t acc[2][2];
   T *C = A + (long long)(bi * NB) * ld + bj * NB;
-  if (role != 0) {
+  acc_t acc[2][2];
+  if (role >= 2) {
     step_tile<T>(acc, A, ld, bi, bj, k, k >= 0 && bi <= reach && bj <= reach, ls, us);
-    if (role != 1) {
-#pragma unroll
-      for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
-      return;
-    }
+    acc_store64<T, 0>(acc, C, ld);
+    return;
   }
-  // diagonal block s0 after the update, inverted in us (pitch PA)
+  // diagonal item and L items: every global load first (own block, block
+  // s0, and the panel tiles L_{bi,k}, L_{s0,k}, U_{k,s0})
+  const bool own = role == 1, upd = k >= 0, upd_own = own && upd && bi <= reach;
   acc_t dacc[2][2];
-  step_tile<T>(dacc, A, ld, s0, s0, k, k >= 0, ls, us);
-  __syncthreads();
+  T lt[TileShape<T>::E], ld2[TileShape<T>::E], ut[TileShape<T>::E];
+  acc_load64<T>(dacc, A + (long long)(s0 * NB) * ld + s0 * NB, ld);
+  if (own) acc_load64<T>(acc, C, ld);
+  if (upd) {
+    tile_load<T>(ld2, A + (long long)(s0 * NB) * ld + k * NB, ld);
+    tile_load<T>(ut, A + (long long)(k * NB) * ld + s0 * NB, ld);
+  }
+  if (upd_own) tile_load<T>(lt, A + (long long)(bi * NB) * ld + k * NB, ld);
+  if (upd) {
+    tile_to_lds<T, P>(ld2, ls);
+    tile_to_lds<T, P>(ut, us);
+    __syncthreads();
+    mma_64<T, P, P>(ls, us, dacc, T(-1));  // block s0 after panel k
+    if (upd_own) {
+      __syncthreads();
+      tile_to_lds<T, P>(lt, ls);  // us still holds U_{k,s0}
+      __syncthreads();
+      mma_64<T, P, P>(ls, us, acc, T(-1));
+    }
+    __syncthreads();
+  }
+  // the own block staged in xs for the product after the inverse; for a
+  // symmetric cell U_{s0,bi} = A_{bi,s0}^T is stored now (its stores overlap
+  // the inverse)
+  if (own) {
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xs[(wr + ti * 16 + acc_row(T(0), lk, r)) * P + wc + tj * 16 + lr] = acc[ti][tj][r];
+  }
   T *a = us;  // 64 x 65 fits in 64 x P
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
@@ -781,39 +831,29 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
 #pragma unroll
       for (int r = 0; r < 4; ++r) a[(wr + ti * 16 + acc_row(T(0), lk, r)) * PA + wc + tj * 16 + lr] = dacc[ti][tj][r];
   __syncthreads();
+  if (own && p.sym) {
+    T *U = A + (long long)(s0 * NB) * ld + bi * NB;
+    for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
+      const int r = idx >> 6, cc = idx & 63;
+      U[(long long)r * ld + cc] = xs[cc * P + r];
+    }
+  }
   T *D = Dinv + dinv_off(p, s0);
-  if (role == 0) {
+  if (!own) {
     const int fixed = invert_diag<T, PA, true>(a, D, D + 4096, (T)cs[c].tau);
     if (fixed) atomicAdd(&cs[c].n_static, fixed);
     return;
   }
   (void)invert_diag<T, PA, false>(a, D, D + 4096, (T)cs[c].tau);
-  // L = A_{bi,s0} A_{s0,s0}^-1 (own block staged in ls, pitch P)
+  // L = A_{bi,s0} A_{s0,s0}^-1
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        ls[(wr + ti * 16 + acc_row(T(0), lk, r)) * P + wc + tj * 16 + lr] = acc[ti][tj][r];
-        acc[ti][tj][r] = T(0);
-      }
-  __syncthreads();
-  mma_64<T, P, PA>(ls, a, acc, T(1));
-#pragma unroll
-  for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
-  if (!p.sym) return;
-  // U_{s0,bi} = A_{bi,s0}^T, the block before the product (still in ls)
-  T *U = A + (long long)(s0 * NB) * ld + bi * NB;
-  for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
-    const int r = idx >> 6, cc = idx & 63;
-    U[(long long)r * ld + cc] = ls[cc * P + r];
-  }
+      for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
+  mma_64<T, P, PA>(xs, a, acc, T(1));
+  acc_store64<T, 0>(acc, C, ld);
 }
 template <class T>
 void launch_lu_step(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv) {
@@ -960,9 +1000,10 @@ __device__ __forceinline__ void lds_barrier() {
 
 // ---------------------------------------------------------------------------
 // Triangular sweeps of a band-factored cell by one workgroup (static
-// pivoting, block bandwidth bw <= BWM), left-looking:
-//   forward   y_b = Linv_b (r_b - sum_{j=b-bw}^{b-1} L_bj y_j)
-//   backward  x_b = Uinv_b (y_b - sum_{j=b+1}^{b+bw} U_bj x_j)
+// pivoting, block bandwidth bw <= BWM), left-looking, in the block LDU form
+// of the static-pivoting factorisation (Li = I, Ui = A_bb^-1):
+//   forward   y_b = r_b - sum_{j=b-bw}^{b-1} L_bj y_j
+//   backward  x_b = Ui_b (y_b - sum_{j=b+1}^{b+bw} U_bj x_j)
 // Thread (row r, part q) = (tid / 8, tid % 8) owns columns 8q .. 8q+7 of row
 // r of every 64-row block.  The band blocks and the diagonal inverse of the
 // next block are loaded into registers while the current block is solved
@@ -995,8 +1036,8 @@ __device__ __forceinline__ void band_load(BandRegs<T, BWM> &R, const CellParams 
       for (int e = 0; e < 8; ++e) R.a[u][e] = T(0);
     }
   }
-  if (ok) {
-    const T *src = Dinv + dinv_off(p, b) + (FWD ? 0 : 4096) + r * 64 + q * 8;
+  if (ok && !FWD) {  // forward: the block-LDU lower factor has identity diagonal blocks
+    const T *src = Dinv + dinv_off(p, b) + 4096 + r * 64 + q * 8;
 #pragma unroll
     for (int e = 0; e < 8; ++e) R.d[e] = src[e];
   }
@@ -1017,6 +1058,11 @@ __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb
   part += __shfl_xor(part, 1, 64);
   part += __shfl_xor(part, 2, 64);
   part += __shfl_xor(part, 4, 64);
+  if (FWD) {  // identity diagonal block (static pivoting stores Li = I)
+    if (q == 0) y[b * NB + r] -= part;
+    lds_barrier();
+    return;
+  }
   if (q == 0) sb[r] = y[b * NB + r] - part;
   lds_barrier();
   T t = T(0);
