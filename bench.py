@@ -83,12 +83,12 @@ def cpu_baseline(cells, budget_s=15.0):
                       (len(rec), len(cells), max(n for n, _ in rec), spent, coef)}
 
 
-def measured_traffic(n_cells, pivoting):
-    """HBM bytes per launch of the trailing-update kernel, from the rocprofv3
-    FETCH_SIZE and WRITE_SIZE passes summarised in profiles/ (scaled by the
-    same counters' reading for a plain copy of known size); None when no
-    summary exists for this workload."""
-    path = os.path.join(REPO, "profiles", "k_lu_gemm_batched%d_%s_traffic.txt" % (n_cells, pivoting))
+def measured_traffic(kernel, n_cells, pivoting):
+    """HBM bytes per launch of the LU kernel, from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes summarised in profiles/ (scaled by the same counters'
+    reading for a plain copy of known size); None when no summary exists for
+    this workload."""
+    path = os.path.join(REPO, "profiles", "%s_batched%d_%s_traffic.txt" % (kernel, n_cells, pivoting))
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -98,28 +98,39 @@ def measured_traffic(n_cells, pivoting):
     return None
 
 
-def gemm_roofline(prof, a, depth, diag_per_step=0):
-    """The LU's trailing-update kernel (with the fused 64 x 64 diagonal-block
-    factorisation and triangle inverses): achieved = (block products x 2 64^3
-    + diagonal blocks x (2/3 + 2/3) 64^3) flop / its summed launch time (HIP
-    events).  With the band LU it is latency bound: each launch is one block
-    step of the chain of dependent diagonal factorisations."""
+def gemm_roofline(prof, a, depth, diag_per_step=0, fused=False):
+    """The LU's dominant kernel, latency bound under the band LU (each launch
+    is one block step of the chain of dependent diagonal blocks).
+    fused (one launch per block step, k_lu_step): achieved = useful 64^3 block
+    products (trailing updates, L block solves, and the diagonal inverse, whose
+    Gauss-Jordan elimination is 64^3 multiply-adds) x 2 64^3 flop / summed
+    launch time (HIP events); the inverses that L items recompute are not
+    counted.  Otherwise (k_lu_gemm): trailing-update products x 2 64^3 +
+    diagonal blocks x (4/3) 64^3 flop."""
     g = prof["lu_gemm"]
     tile_flops = 2.0 * 64 ** 3
-    diag_flops = (4.0 / 3.0) * 64 ** 3
+    diag_flops = 0.0 if fused else (4.0 / 3.0) * 64 ** 3
     flops = g["work"] * tile_flops + diag_per_step * diag_flops
     achieved = flops / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
     launches = max(g["launches"], 1)
-    traffic = measured_traffic(a.cells, a.pivoting) if a.workload == "batched256" else None
-    return {"kernel": "k_lu_gemm<float,false> (LU trailing update + fused diagonal factorisation, "
-                      "v_mfma_f32_16x16x4f32)", "bound": "mfma",
-            "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / MFMA_FP32_PEAK, "traffic": traffic,
-            "launches_per_step": g["launches"], "block_products_per_step": g["work"],
-            "flops_per_block_product": tile_flops, "avg_launch_us": g["ms"] * 1e3 / launches,
-            "c_tile_bytes_per_launch": g["work"] / launches / depth * 2 * 64 * 64 * 4,
-            "panels_per_update": depth, "diag_blocks_per_step": diag_per_step,
-            "flops_per_diag_block": diag_flops}
+    kernel = "k_lu_step" if fused else "k_lu_gemm"
+    traffic = measured_traffic(kernel, a.cells, a.pivoting) if a.workload == "batched256" else None
+    if fused:
+        name = ("k_lu_step<float> (fused band-LU block step: trailing update, Gauss-Jordan inverse of the "
+                "diagonal block, L block solves; v_mfma_f32_16x16x4f32)")
+    else:
+        name = ("k_lu_gemm<float,false> (LU trailing update + fused diagonal factorisation, "
+                "v_mfma_f32_16x16x4f32)")
+    out = {"kernel": name, "bound": "mfma",
+           "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
+           "frac": achieved / MFMA_FP32_PEAK, "traffic": traffic,
+           "launches_per_step": g["launches"], "block_products_per_step": g["work"],
+           "flops_per_block_product": tile_flops, "avg_launch_us": g["ms"] * 1e3 / launches}
+    if not fused:
+        out.update({"c_tile_bytes_per_launch": g["work"] / launches / depth * 2 * 64 * 64 * 4,
+                    "panels_per_update": depth, "diag_blocks_per_step": diag_per_step,
+                    "flops_per_diag_block": diag_flops})
+    return out
 
 
 def pcg_roofline(cells, prof, st):
@@ -226,8 +237,9 @@ def main():
     # diagonal blocks factored per step inside the trailing-update launches
     # (block 0 of each cell has its own kernel)
     diag_per_step = sum(max((2 * len(c["X"]) + 63) // 64 - 1, 0) for c in dense)
-    roofline = (gemm_roofline(prof, a, sched["lu_depth"], diag_per_step) if prof["lu_gemm"]["launches"] > 0
-                else pcg_roofline(cells, prof, st))
+    roofline = (gemm_roofline(prof, a, sched["lu_depth"], 0 if sched["fused_step"] else diag_per_step,
+                              fused=bool(sched["fused_step"]))
+                if prof["lu_gemm"]["launches"] > 0 else pcg_roofline(cells, prof, st))
     lu_alg = configs.lu_flops(dense, symmetric=a.pivoting == "static")
     lu_ms = prof["lu_gemm"]["ms"] + prof["lu_strip"]["ms"] + prof["lu_panel"]["ms"]
 

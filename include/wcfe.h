@@ -216,8 +216,9 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
  * automatic choice): delayed-update depth, concurrent cell groups, the widest
  * block band of the dense cells (64-row blocks) and whether the triangular
  * sweeps run one workgroup per cell along the band (1) or spread over
- * workgroups (0). */
-int wcfe_get_schedule(wcfe_ctx *ctx, int32_t *lu_depth, int32_t *lu_groups, int32_t *max_band, int32_t *band_solve);
+ * workgroups (0), and whether each block step is one fused launch (1). */
+int wcfe_get_schedule(wcfe_ctx *ctx, int32_t *lu_depth, int32_t *lu_groups, int32_t *max_band, int32_t *band_solve,
+                      int32_t *fused_step);
 
 /* Diagnostics: time `reps` plain device copies of `bytes` bytes (one read and
  * one write of every byte, 16 bytes per thread) on the context's stream; the

@@ -276,10 +276,11 @@ class Context:
     def schedule(self):
         """The dense-LU schedule in effect: lu_depth, lu_groups, max_band
         (widest block band of the dense cells), band_solve (1: one workgroup
-        per cell along the band)."""
-        v = [C.c_int32(0) for _ in range(4)]
+        per cell along the band), fused_step (1: one launch per block step)."""
+        v = [C.c_int32(0) for _ in range(5)]
         self._check(self._L.wcfe_get_schedule(self._h, *[C.byref(x) for x in v]))
-        return dict(lu_depth=v[0].value, lu_groups=v[1].value, max_band=v[2].value, band_solve=v[3].value)
+        return dict(lu_depth=v[0].value, lu_groups=v[1].value, max_band=v[2].value, band_solve=v[3].value,
+                    fused_step=v[4].value)
 
     def autotune(self, flags=TUNE_SOLVER | TUNE_LU, steps=2):
         """Time candidate settings on this batch and keep the fastest (call

@@ -132,7 +132,7 @@ __device__ __forceinline__ int invert_diag(T *a, T *Li, T *Ui, T tau);
 // k = 0 of static pivoting: invert block (0, 0).
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_diag_inv(const CellParams *cp, CellState *cs, const int *map, int k, T *K, T *Dinv) {
-  constexpr int P = 65;
+  constexpr int P = Pitch<T>::v;
   __shared__ T a[64 * P];
   const int v = map[blockIdx.x];
   const int c = v >> 20;
@@ -633,8 +633,8 @@ __global__ __launch_bounds__(256) void k_lu_gemm(const CellParams *cp, CellState
   }
   // diagonal block s0 is final: factor it here
   __syncthreads();
-  constexpr int PA = 65;
-  T *a = ls;  // reuse: 64 x 65 fits in 64 x P
+  constexpr int PA = P;
+  T *a = ls;
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -743,7 +743,7 @@ __device__ __forceinline__ void acc_store64(const typename Acc4<T>::type (&acc)[
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState *cs, const int *map, int k, T *K,
                                                  T *Dinv) {
-  constexpr int P = Pitch<T>::v, PA = 65;
+  constexpr int P = Pitch<T>::v, PA = P;
   typedef typename Acc4<T>::type acc_t;
   __shared__ __attribute__((aligned(16))) T ls[64 * P];
   __shared__ __attribute__((aligned(16))) T us[64 * P];
@@ -823,7 +823,7 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
 #pragma unroll
         for (int r = 0; r < 4; ++r) xs[(wr + ti * 16 + acc_row(T(0), lk, r)) * P + wc + tj * 16 + lr] = acc[ti][tj][r];
   }
-  T *a = us;  // 64 x 65 fits in 64 x P
+  T *a = us;
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -856,12 +856,18 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
   acc_store64<T, 0>(acc, C, ld);
 }
 template <class T>
-void launch_lu_step(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv) {
+void launch_lu_step(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
+                    hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (wl.total <= 0) return;
   // FP32 only: three 64 x 68 LDS tiles (the FP64 variant would exceed the
   // static LDS limit); the schedule never selects it for FP64 factors
-  if constexpr (sizeof(T) == 4)
-    hipLaunchKernelGGL(k_lu_step<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv);
+  if constexpr (sizeof(T) == 4) {
+    if (ev_start)  // profiling (never under graph capture): events at the kernel's own start and end
+      hipExtLaunchKernelGGL(k_lu_step<T>, dim3(wl.total), dim3(256), 0, s, ev_start, ev_stop, 0, cp, cs, wl.map, k,
+                            K, Dinv);
+    else
+      hipLaunchKernelGGL(k_lu_step<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1469,7 +1475,8 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
                                   const T *);                                                                         \
   template void launch_lu_gemm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *,      \
                                   int, int, int, hipEvent_t, hipEvent_t);                                             \
-  template void launch_lu_step<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *);     \
+  template void launch_lu_step<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *,      \
+                                  hipEvent_t, hipEvent_t);                                                            \
   template void launch_lu_solve<T>(hipStream_t, const CellParams *, const CellState *, const int *, int, const T *,   \
                                    const T *, const int *, int, const double *, double *, int, int);                  \
   template void launch_lu_solve_mw<T>(hipStream_t, const CellParams *, CellState *, const int *, int,               \

@@ -300,7 +300,15 @@ struct LuPlan {
   }
   static long long block_products(const CellParams &p, int kind, int k, int np, int pivoting) {
     if (kind == ST_STRIP || kind == ST_GEMM || kind == ST_GEMM1) return (long long)np * items(p, kind, k, np, pivoting);
-    if (kind == ST_STEP) return items(p, kind, k, np, pivoting);
+    if (kind == ST_STEP) {  // useful 64^3 products: updated blocks, L block solves, the inverse
+      const int nb = p.nblk, s0 = k + 1;
+      if (s0 >= nb) return 0;
+      const int w = std::min(nb - s0 - 1, p.bw);
+      const long long npl = k >= 0 ? std::max(0, std::min(nb - 1, k + p.bw) - s0) : 0;
+      const long long upd =
+          k >= 0 ? 1 + (p.sym ? 1 : 2) * std::min(w, p.bw - 1) + (p.sym ? npl * (npl + 1) / 2 : npl * npl) : 0;
+      return upd + w + 1;
+    }
     return 0;
   }
   // static pivoting: LU_DEPTH block steps per round -- panel k, then the
@@ -1219,8 +1227,8 @@ void enqueue_lu(wcfe_ctx *ctx, hipStream_t s, const LuPlan &plan, const std::vec
       }
       case ST_STEP: {
         if (sizeof(T) != 4) fail(WCFE_ERR_STATE, "the fused block step is built for FP32 factors only");
-        PhaseScope ph(ctx, WCFE_PH_LU_GEMM);
-        launch_lu_step<T>(s, cp, cs, wl, g.k, K, Dinv);
+        KernelTimer kt(ctx, WCFE_PH_LU_GEMM, g.work);
+        launch_lu_step<T>(s, cp, cs, wl, g.k, K, Dinv, kt.a, kt.b);
         break;
       }
     }
@@ -2701,7 +2709,8 @@ int wcfe_force_fa_subs(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int
   });
 }
 
-int wcfe_get_schedule(wcfe_ctx *ctx, int32_t *lu_depth, int32_t *lu_groups, int32_t *max_band, int32_t *band_solve) {
+int wcfe_get_schedule(wcfe_ctx *ctx, int32_t *lu_depth, int32_t *lu_groups, int32_t *max_band, int32_t *band_solve,
+                      int32_t *fused_step) {
   if (!ctx) return WCFE_ERR_ARG;
   return guarded(ctx, [&] {
     if (!ctx->finalized) fail(WCFE_ERR_STATE, "wcfe_finalize has not been called");
@@ -2709,6 +2718,7 @@ int wcfe_get_schedule(wcfe_ctx *ctx, int32_t *lu_depth, int32_t *lu_groups, int3
     if (lu_groups) *lu_groups = ctx->b.lu_groups;
     if (max_band) *max_band = ctx->b.max_bw;
     if (band_solve) *band_solve = ctx->b.solve_wg ? 1 : 0;
+    if (fused_step) *fused_step = ctx->b.plan.fused ? 1 : 0;
   });
 }
 
