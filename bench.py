@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--autotune", choices=["none", "solver", "lu", "all"], default=None,
                     help="run wcfe_autotune on the batch before timing (default: solver for replicas1024, "
                          "none otherwise)")
+    ap.add_argument("--fused-step", type=int, choices=[0, 1], default=None,
+                    help="band-LU block-step mode (library default when omitted)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (HBM probe / counter reference) and exit")
     a = ap.parse_args()
@@ -203,6 +205,8 @@ def main():
                                                                time.perf_counter() - t_setup))
     piv = W.PIVOT_STATIC if a.pivoting == "static" else W.PIVOT_PARTIAL
     common = dict(lin_rtol=1e-8, newton_rtol=1e-7, newton_atol=1e-12, t0=0.5, dt=0.005, pivoting=piv)
+    if a.fused_step is not None:
+        common["fused_step"] = a.fused_step
 
     # ---- batched mixed precision, one hipGraph replay per Newton step ----
     ctx = W.Context(precision=W.PREC_MIXED, batched=1, use_graph=1, max_refine=8, **common)

@@ -205,7 +205,8 @@ int wcfe_synchronize(wcfe_ctx *ctx);
  * candidate setting and keeps the fastest.  WCFE_TUNE_SOLVER: dense / CSR
  * threshold (dense_max_dof, cells with solver AUTO; candidates that would not
  * fit in free device memory are skipped); WCFE_TUNE_LU: delayed-update depth
- * (2, 4, 6) x concurrent cell groups (1, 2, 4).  Every trial starts from the
+ * (1, 2, 4, 6) x concurrent cell groups (1, 2, 4), at depth 1 with and without
+ * the fused block step.  Every trial starts from the
  * state the cells were added with, and so does the batch afterwards: call it
  * after wcfe_finalize and before any Newton step.  chosen (optional, 3):
  * lu_depth, lu_groups, dense_max_dof; ms_best (optional): ms per step. */

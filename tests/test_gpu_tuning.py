@@ -25,8 +25,9 @@ def _run(ds, **opt):
     return xs
 
 
-@pytest.mark.parametrize("sym,depth", [(1, 4), (0, 4), (1, 1), (0, 2), (1, 3)])
-def test_band_lu_matches_full_matrix(sym, depth):
+@pytest.mark.parametrize("sym,depth,fused", [(1, 4, 1), (0, 4, 1), (1, 1, 1), (0, 1, 1), (1, 1, 0), (0, 1, 0),
+                                             (0, 2, 1), (1, 3, 1)])
+def test_band_lu_matches_full_matrix(sym, depth, fused):
     """Band LU in the reverse Cuthill-McKee numbering (the default) against the
     full-matrix LU in the natural numbering, for symmetric and full
     factorisations and delayed-update depths below and above the block
@@ -35,11 +36,12 @@ def test_band_lu_matches_full_matrix(sym, depth):
     whole load step of each cell alone the same converged positions.  (In a
     batch, cells that converge early start their next load step while the
     others iterate, so batch-final positions depend on the iteration counts.)"""
-    print("case sym", sym, "depth", depth, flush=True)
+    print("case sym", sym, "depth", depth, "fused", fused, flush=True)
     ds = [cells.cst_cell(K=k, n_spr=20 + 10 * k, law_table=(k % 2 == 1), seed=k, time=1.0) for k in (3, 9, 16, 22)]
     steps = []
     for band in (1, 0):
-        c = W.Context(precision=W.PREC_MIXED, t0=1.0, max_refine=12, symmetric=sym, lu_depth=depth, band=band)
+        c = W.Context(precision=W.PREC_MIXED, t0=1.0, max_refine=12, symmetric=sym, lu_depth=depth, band=band,
+                      fused_step=fused)
         for d in ds:
             cells.add_to_context(c, d)
         c.finalize()
@@ -50,7 +52,7 @@ def test_band_lu_matches_full_matrix(sym, depth):
     for d, a, b in zip(ds, steps[0], steps[1]):
         assert np.max(np.abs(a - b)) <= 1e-7 * np.max(np.abs(b - d["x"]))
     for d in ds[2:]:
-        band = _run([d], symmetric=sym, lu_depth=depth)[0]
+        band = _run([d], symmetric=sym, lu_depth=depth, fused_step=fused)[0]
         full = _run([d], symmetric=sym, lu_depth=depth, band=0)[0]
         assert np.max(np.abs(band - full)) <= 1e-9 * np.max(np.abs(full - d["X"]))
 

@@ -1806,13 +1806,20 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
     auto trial = [&]() {
       try {
         rebuild_batch(ctx);
+        step_impl(ctx);  // warm-up (captures the graph)
       } catch (const ApiError &e) {
         if (e.code != WCFE_ERR_OOM) throw;
-        ctx->b.release();  // this candidate does not fit: skip it
+        // this candidate does not fit (its batch or a workspace of its
+        // first step): skip it
+        (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+        if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+        ctx->gexec = nullptr;
+        ctx->graph = nullptr;
+        ctx->b.release();
         (void)hipGetLastError();
         return 1e300;
       }
-      step_impl(ctx);  // warm-up (captures the graph)
       HIPCHECK(hipEventRecord(e0, ctx->stream));
       for (int i = 0; i < steps; ++i) step_impl(ctx);
       HIPCHECK(hipEventRecord(e1, ctx->stream));
@@ -1821,41 +1828,67 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
       HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
       return (double)ms / steps;
     };
+    const wcfe_options orig = ctx->opt;
     wcfe_options best = ctx->opt;
     double best_ms = 1e300;
+    std::vector<std::pair<double, wcfe_options>> tried;  // every feasible candidate, for the final rebuild
     // 1. dense / CSR threshold (cells with solver AUTO)
     if (flags & WCFE_TUNE_SOLVER) {
       const int cands[] = {0, 2048, 4096, 8192, 16384, 1 << 30};
       for (int dm : cands) {
         if (dense_bytes(ctx, dm) > std::min(current_dense + 0.8 * (double)free_b, 0.6 * (double)total_b))
-          continue;  // would not fit
+          continue;  // would not fit (with room for the workspaces)
         ctx->opt = best;
         ctx->opt.dense_max_dof = dm;
         const double ms = trial();
+        if (ms < 1e300) tried.push_back({ms, ctx->opt});
         if (ms < best_ms) {
           best_ms = ms;
           best = ctx->opt;
         }
       }
     }
-    // 2. delayed-update depth and concurrent cell groups of the dense LU
+    // 2. delayed-update depth and concurrent cell groups of the dense LU;
+    //    at depth 1 also separate launches instead of the fused block step
+    //    (fewer inverses per step: better when many cells share a launch)
     if (flags & WCFE_TUNE_LU) {
       const int depths[] = {1, 2, 4, 6}, groups[] = {1, 2, 4};
       const wcfe_options base = best;
       for (int d : depths)
-        for (int g : groups) {
-          ctx->opt = base;
-          ctx->opt.lu_depth = d;
-          ctx->opt.lu_groups = g;
-          const double ms = trial();
-          if (ms < best_ms) {
-            best_ms = ms;
-            best = ctx->opt;
+        for (int g : groups)
+          for (int f = 1; f >= (d == 1 ? 0 : 1); --f) {
+            ctx->opt = base;
+            ctx->opt.lu_depth = d;
+            ctx->opt.lu_groups = g;
+            ctx->opt.fused_step = f;
+            const double ms = trial();
+            if (ms < 1e300) tried.push_back({ms, ctx->opt});
+            if (ms < best_ms) {
+              best_ms = ms;
+              best = ctx->opt;
+            }
           }
-        }
     }
-    ctx->opt = best;
-    rebuild_batch(ctx);
+    // the fastest candidate that can be laid out again (a setting near the
+    // memory limit may not fit a second time); else the batch's own
+    std::stable_sort(tried.begin(), tried.end(),
+                     [](const std::pair<double, wcfe_options> &a, const std::pair<double, wcfe_options> &b) {
+                       return a.first < b.first;
+                     });
+    tried.push_back({0.0, orig});
+    best_ms = 0.0;
+    for (size_t q = 0; q < tried.size(); ++q) {
+      ctx->opt = tried[q].second;
+      try {
+        rebuild_batch(ctx);
+        best_ms = q + 1 < tried.size() ? tried[q].first : 0.0;
+        break;
+      } catch (const ApiError &e) {
+        if (e.code != WCFE_ERR_OOM || q + 1 == tried.size()) throw;
+        ctx->b.release();
+        (void)hipGetLastError();
+      }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (chosen) {
