@@ -34,6 +34,7 @@
 // k_lu_solve applies the factors to one right-hand side per cell with the
 // whole vector resident in LDS.
 #include <type_traits>
+#include <utility>
 
 #include <hip/hip_ext.h>
 
@@ -372,8 +373,55 @@ __device__ __forceinline__ void acc16_store(const typename Acc4<T>::type &acc, T
 // 16 x 16 sub-block at D (pitch PA) inverted in place, one wave: lane l
 // holds row l & 15, columns 4 (l >> 4) .. +3; the pivot and the pivot row and
 // column reach the other lanes by cross-lane reads.
+// FP32: the pivot row by DPP row broadcast (row_newbcast: lane j of each
+// 16-lane row) and the pivot column by permlane32 / permlane16 swaps (row gj
+// of the wave copied to all four rows): no LDS round trip on the chain.
+template <int J> __device__ __forceinline__ float row_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + J, 0xF, 0xF, false));
+}
+template <int G> __device__ __forceinline__ float wave_row_bcast(float v) {
+  const int x = __float_as_int(v);
+  const auto h = __builtin_amdgcn_permlane32_swap(x, x, false, false);  // rows (0,1,0,1) | (2,3,2,3)
+  const int y = G < 2 ? h[0] : h[1];
+  const auto q = __builtin_amdgcn_permlane16_swap(y, y, false, false);  // even row | odd row, everywhere
+  return __int_as_float(G % 2 == 0 ? q[0] : q[1]);
+}
+template <int J>
+__device__ __forceinline__ void gj16_step(float (&v)[4], int lane, int r, int g, float tau, int &fixed) {
+  constexpr int GJ = J >> 2, QJ = J & 3;
+  const float p0 = rdlane(v[QJ], GJ * 16 + J);
+  const bool small = !(fabsf(p0) >= tau);
+  const float piv = small ? (p0 >= 0.0f ? tau : -tau) : p0;
+  fixed += (small && lane == 0) ? 1 : 0;
+  const float ip = recip(piv);
+  const float f = wave_row_bcast<GJ>(v[QJ]);  // A[r][J]
+  float rq[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) rq[q] = row_bcast<J>(v[q]) * ip;  // pivot row / pivot
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const bool cj = 4 * g + q == J;
+    v[q] = (r == J) ? (cj ? ip : rq[q]) : (cj ? -f * ip : v[q] - f * rq[q]);
+  }
+}
+template <int... J>
+__device__ __forceinline__ void gj16_steps(float (&v)[4], int lane, int r, int g, float tau, int &fixed,
+                                           std::integer_sequence<int, J...>) {
+  (gj16_step<J>(v, lane, r, g, tau, fixed), ...);
+}
+
 template <class T, int PA> __device__ __forceinline__ int gj16(T *D, T tau) {
   const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  if constexpr (sizeof(T) == 4) {
+    float v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = D[r * PA + 4 * g + q];
+    int fixed = 0;
+    gj16_steps(v, lane, r, g, (float)tau, fixed, std::make_integer_sequence<int, 16>{});
+#pragma unroll
+    for (int q = 0; q < 4; ++q) D[r * PA + 4 * g + q] = v[q];
+    return fixed;
+  }
   T v[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) v[q] = D[r * PA + 4 * g + q];
