@@ -496,6 +496,78 @@ __device__ __forceinline__ int invert_diag(T *a, T *Li, T *Ui, T tau) {
   return fixed;
 }
 
+// invert_diag with work for the idle waves: wave 0 runs pre() and then the
+// Gauss-Jordan inverse of sub-block kb while waves 1-3 run side(kb); every
+// thread runs post(kb) right after that window's barrier.  The inverse itself
+// is invert_diag's, step for step.
+template <class T, int PA, bool STORE, class Pre, class Side, class Post>
+__device__ __forceinline__ int invert_diag_ov(T *a, T *Li, T *Ui, T tau, Pre pre, Side side, Post post) {
+  typedef typename Acc4<T>::type acc_t;
+  const int wave = threadIdx.x >> 6;
+  int fixed = 0;
+  auto sub = [](T *base, int i, int j) { return base + 16 * i * PA + 16 * j; };
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    if (wave == 0) {
+      if (kb == 0) pre();
+      fixed += gj16<T, PA>(sub(a, kb, kb), tau);
+    } else {
+      side(kb);
+    }
+    __syncthreads();
+    post(kb);
+    if (wave < 3) {  // row kb
+      const int j = wave + (wave >= kb ? 1 : 0);
+      acc_t acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = T(0);
+      mm16<T, PA, M_FULL, M_FULL>(sub(a, kb, kb), sub(a, kb, j), acc, T(1));
+      acc16_store<T, PA>(acc, sub(a, kb, j));
+    }
+    __syncthreads();
+    if (wave < 3) {  // the other rows, one per wave
+      const int i = wave + (wave >= kb ? 1 : 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j == kb) continue;
+        acc_t acc;
+        acc16_load<T, PA>(acc, sub(a, i, j));
+        mm16<T, PA, M_FULL, M_FULL>(sub(a, i, kb), sub(a, kb, j), acc, T(-1));
+        acc16_store<T, PA>(acc, sub(a, i, j));
+      }
+      acc_t acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = T(0);
+      mm16<T, PA, M_FULL, M_FULL>(sub(a, i, kb), sub(a, kb, kb), acc, T(-1));
+      acc16_store<T, PA>(acc, sub(a, i, kb));
+    }
+    __syncthreads();
+  }
+  if (STORE) {
+    for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
+      const int r = idx >> 6, c = idx & 63;
+      Li[idx] = r == c ? T(1) : T(0);
+      Ui[idx] = a[r * PA + c];
+    }
+  }
+  return fixed;
+}
+
+// acc += -X_I Y_J over k = 0..63 for the 16 x 16 output tile (I, J) of a
+// 64 x 64 product, one wave; X, Y in LDS (pitch P).  The MFMA sequence per
+// tile is mma_64's (same operands, same k order), so the tile comes out bit
+// for bit as mma_64 computes it.
+template <class T, int P>
+__device__ __forceinline__ void mmneg_tile(const T *X, const T *Y, int I, int J, typename Acc4<T>::type &acc) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int kk = 0; kk < 64; kk += 4) {
+    const T x = T(-1) * X[(I * 16 + lr) * P + kk + lk];
+    const T y = Y[(kk + lk) * P + J * 16 + lr];
+    acc = mfma16(x, y, acc);
+  }
+}
+
 // block triangular solves as products with the stored pair (Li, Ui):
 //   items [0, m):  U_kj = Li_k A_kj
 //   items [m, 2m): L_ik = A_ik Ui_k   (static pivoting only)
@@ -835,31 +907,101 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
     acc_store64<T, 0>(acc, C, ld);
     return;
   }
-  // diagonal item and L items: every global load first (own block, block
-  // s0, and the panel tiles L_{bi,k}, L_{s0,k}, U_{k,s0})
   const bool own = role == 1, upd = k >= 0, upd_own = own && upd && bi <= reach;
-  acc_t dacc[2][2];
-  T lt[TileShape<T>::E], ld2[TileShape<T>::E], ut[TileShape<T>::E];
-  acc_load64<T>(dacc, A + (long long)(s0 * NB) * ld + s0 * NB, ld);
-  if (own) acc_load64<T>(acc, C, ld);
   if (upd) {
+    // Panel k updates block s0 (and the own block): the updates run inside
+    // the inverse.  Block s0 -= L_{s0,k} U_{k,s0}: wave 0 takes sub-block
+    // (0,0) just before its Gauss-Jordan step, waves 1-3 the other 15
+    // meanwhile.  The own block's 16 sub-blocks live in waves 1-3's
+    // registers; they take the update A -= L_{bi,k} U_{k,s0} during steps
+    // 1-3 and are staged in ls for the final product.  LDS: xs block s0
+    // (then its inverse), ls L_{s0,k} (then L_{bi,k}, then the own block),
+    // us U_{k,s0}.  Every tile's MFMA sequence is mma_64's.
+    T lt[TileShape<T>::E], ld2[TileShape<T>::E], ut[TileShape<T>::E];
+    tile_load<T>(lt, A + (long long)(s0 * NB) * ld + s0 * NB, ld);
     tile_load<T>(ld2, A + (long long)(s0 * NB) * ld + k * NB, ld);
     tile_load<T>(ut, A + (long long)(k * NB) * ld + s0 * NB, ld);
-  }
-  if (upd_own) tile_load<T>(lt, A + (long long)(bi * NB) * ld + k * NB, ld);
-  if (upd) {
+    acc_t ow[6];
+    if (own && w > 0) {
+#pragma unroll
+      for (int sl = 0; sl < 6; ++sl) {
+        const int tt = (w - 1) + 3 * sl;
+        if (tt < 16) {
+          const T *Ct = C + (long long)((tt >> 2) * 16) * ld + (tt & 3) * 16;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) ow[sl][r] = Ct[(long long)acc_row(T(0), lk, r) * ld + lr];
+        }
+      }
+    }
+    tile_to_lds<T, P>(lt, xs);
     tile_to_lds<T, P>(ld2, ls);
     tile_to_lds<T, P>(ut, us);
+    if (upd_own) tile_load<T>(lt, A + (long long)(bi * NB) * ld + k * NB, ld);
     __syncthreads();
-    mma_64<T, P, P>(ls, us, dacc, T(-1));  // block s0 after panel k
-    if (upd_own) {
-      __syncthreads();
-      tile_to_lds<T, P>(lt, ls);  // us still holds U_{k,s0}
-      __syncthreads();
-      mma_64<T, P, P>(ls, us, acc, T(-1));
+    auto pre = [&]() {
+      acc_t d;
+      acc16_load<T, PA>(d, xs);
+      mmneg_tile<T, P>(ls, us, 0, 0, d);
+      acc16_store<T, PA>(d, xs);
+    };
+    T *Ut = A + (long long)(s0 * NB) * ld + bi * NB;  // U_{s0,bi} (symmetric cells)
+    auto side = [&](int kb) {
+      if (kb == 0) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          const int u = w + 3 * q, I = u >> 2, J = u & 3;
+          acc_t d;
+          acc16_load<T, PA>(d, xs + I * 16 * PA + J * 16);
+          mmneg_tile<T, P>(ls, us, I, J, d);
+          acc16_store<T, PA>(d, xs + I * 16 * PA + J * 16);
+        }
+        return;
+      }
+      if (!own) return;
+#pragma unroll
+      for (int sl = 2 * (kb - 1); sl < 2 * kb; ++sl) {
+        const int tt = (w - 1) + 3 * sl;
+        if (tt >= 16) continue;
+        const int I = tt >> 2, J = tt & 3;
+        if (upd_own) mmneg_tile<T, P>(ls, us, I, J, ow[sl]);
+        if (p.sym) {  // U_{s0,bi} = A_{bi,s0}^T: four consecutive entries of one row per lane
+          const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+          *reinterpret_cast<float4 *>(Ut + (long long)(J * 16 + lr) * ld + I * 16 + 4 * lk) =
+              make_float4(ow[sl][0], ow[sl][1], ow[sl][2], ow[sl][3]);
+        }
+      }
+    };
+    auto post = [&](int kb) {
+      if (kb == 0 && upd_own) tile_to_lds<T, P>(lt, ls);  // L_{s0,k} is done with
+      if (kb == 3 && own && w > 0) {
+#pragma unroll
+        for (int sl = 0; sl < 6; ++sl) {
+          const int tt = (w - 1) + 3 * sl;
+          if (tt < 16) acc16_store<T, P>(ow[sl], ls + (tt >> 2) * 16 * P + (tt & 3) * 16);
+        }
+      }
+    };
+    T *D = Dinv + dinv_off(p, s0);
+    if (!own) {
+      const int fixed = invert_diag_ov<T, PA, true>(xs, D, D + 4096, (T)cs[c].tau, pre, side, post);
+      if (fixed) atomicAdd(&cs[c].n_static, fixed);
+      return;
     }
-    __syncthreads();
+    (void)invert_diag_ov<T, PA, false>(xs, D, D + 4096, (T)cs[c].tau, pre, side, post);
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
+    mma_64<T, P, PA>(ls, xs, acc, T(1));
+    acc_store64<T, 0>(acc, C, ld);
+    return;
   }
+  // first step (no panel before it): block s0 and the own block as they are
+  acc_t dacc[2][2];
+  acc_load64<T>(dacc, A + (long long)(s0 * NB) * ld + s0 * NB, ld);
+  if (own) acc_load64<T>(acc, C, ld);
   // the own block staged in xs for the product after the inverse; for a
   // symmetric cell U_{s0,bi} = A_{bi,s0}^T is stored now (its stores overlap
   // the inverse)
