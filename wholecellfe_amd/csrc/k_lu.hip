@@ -1287,21 +1287,28 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams
   const int nb = p.nblk;
   const double *rr = rhs + (long long)p.vec_off;
   for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = i < p.ndof ? (T)rr[i] : T(0);
-  BandRegs<T, BWM> R0, R1;
-  band_load<T, BWM, true>(R0, p, A, Dinv, 0, bw);
+  // a ring of PF block rows in registers: the loads of block b + PF are in
+  // flight while blocks b + 1 .. b + PF - 1 are applied
+  constexpr int PF = (sizeof(T) == 4 && BWM <= 4) ? 4 : 2;
+  BandRegs<T, BWM> R[PF];
+#pragma unroll
+  for (int i = 0; i < PF; ++i) band_load<T, BWM, true>(R[i], p, A, Dinv, i, bw);
   lds_barrier();
-  for (int b = 0; b < nb; b += 2) {
-    band_load<T, BWM, true>(R1, p, A, Dinv, b + 1, bw);
-    band_step<T, BWM, true>(R0, y, sb, b, bw, nb);
-    band_load<T, BWM, true>(R0, p, A, Dinv, b + 2, bw);
-    if (b + 1 < nb) band_step<T, BWM, true>(R1, y, sb, b + 1, bw, nb);
+  for (int b = 0; b < nb; b += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      if (b + i < nb) band_step<T, BWM, true>(R[i], y, sb, b + i, bw, nb);
+      band_load<T, BWM, true>(R[i], p, A, Dinv, b + i + PF, bw);
+    }
   }
-  band_load<T, BWM, false>(R0, p, A, Dinv, nb - 1, bw);
-  for (int b = nb - 1; b >= 0; b -= 2) {
-    band_load<T, BWM, false>(R1, p, A, Dinv, b - 1, bw);
-    band_step<T, BWM, false>(R0, y, sb, b, bw, nb);
-    band_load<T, BWM, false>(R0, p, A, Dinv, b - 2, bw);
-    if (b - 1 >= 0) band_step<T, BWM, false>(R1, y, sb, b - 1, bw, nb);
+#pragma unroll
+  for (int i = 0; i < PF; ++i) band_load<T, BWM, false>(R[i], p, A, Dinv, nb - 1 - i, bw);
+  for (int b = nb - 1; b >= 0; b -= PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      if (b - i >= 0) band_step<T, BWM, false>(R[i], y, sb, b - i, bw, nb);
+      band_load<T, BWM, false>(R[i], p, A, Dinv, b - i - PF, bw);
+    }
   }
   double *dd = delta + (long long)p.vec_off;
   for (int i = threadIdx.x; i < p.ndof; i += BAND_THREADS) dd[i] = (double)y[i];
