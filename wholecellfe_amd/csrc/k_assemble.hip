@@ -182,10 +182,9 @@ __global__ __launch_bounds__(128) void k_element_pass(View v, int e_begin, int e
   }
 }
 
-template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_node_gather(View v, const int *node_cell, const int *adj_start,
-                                                     const int *adj, GatherPlan gp, int n0, int n1, const double *Ge,
-                                                     const double *Ke64, double *G, T *K, T *Kb) {
+// RESIDUAL mode: each node gathers its two residual rows, element order
+__global__ __launch_bounds__(256) void k_node_gather(const int *adj_start, const int *adj, int n0, int n1,
+                                                     const double *Ge, double *G) {
   const int n = n0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= n1) return;
   double g0 = G[2 * n], g1 = G[2 * n + 1];
@@ -197,34 +196,51 @@ __global__ __launch_bounds__(256) void k_node_gather(View v, const int *node_cel
   }
   G[2 * n] = g0;
   G[2 * n + 1] = g1;
-  if (MODE != ASM_FULL) return;
-  // each 2 x 2 block of the node's rows: read once, add the element blocks
-  // in element order, write once
-  const CellParams &p = v.cp[node_cell[n]];
-  T *A = K + p.k_off;
-  const long long r0 = 2LL * v.kpos[n];
-  for (int g = gp.start[n]; g < gp.start[n + 1]; ++g) {
-    T *b0, *b1;
-    if (p.csr) {
-      b0 = Kb + 4LL * gp.bpos[g];
-      b1 = b0 + 2;
-    } else {
-      b0 = A + r0 * p.npad + 2 * v.kpos[gp.col[g]];
-      b1 = b0 + p.npad;
+}
+
+// FULL mode, one thread per group (row node n, column node): the 2 x 2
+// block's contributions in element order, the order in which the
+// colour-by-colour scatter adds them; the thread of the node's first group
+// also gathers the node's residual rows.
+template <class T>
+__global__ __launch_bounds__(256) void k_group_gather(View v, const int *node_cell, const int *adj_start,
+                                                      const int *adj, GatherPlan gp, const double *Ge,
+                                                      const double *Ke64, double *G, T *K, T *Kb) {
+  const int g = gp.g0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= gp.g1) return;
+  const int n = gp.node[g];
+  if (g == gp.start[n]) {
+    double g0 = G[2 * n], g1 = G[2 * n + 1];
+    for (int k = adj_start[n]; k < adj_start[n + 1]; ++k) {
+      const int pk = adj[k];
+      const int e = pk >> 2, a = pk & 3;
+      g0 += Ge[6LL * e + 2 * a];
+      g1 += Ge[6LL * e + 2 * a + 1];
     }
-    T a00 = b0[0], a01 = b0[1], a10 = b1[0], a11 = b1[1];
-    for (int q = gp.cstart[g]; q < gp.cstart[g + 1]; ++q) {
-      const double *Ke = Ke64 + gp.ke[q];
-      a00 += T(Ke[0]);
-      a01 += T(Ke[1]);
-      a10 += T(Ke[6]);
-      a11 += T(Ke[7]);
-    }
-    b0[0] = a00;
-    b0[1] = a01;
-    b1[0] = a10;
-    b1[1] = a11;
+    G[2 * n] = g0;
+    G[2 * n + 1] = g1;
   }
+  const CellParams &p = v.cp[node_cell[n]];
+  T *b0, *b1;
+  if (p.csr) {
+    b0 = Kb + 4LL * gp.bpos[g];
+    b1 = b0 + 2;
+  } else {
+    b0 = K + p.k_off + 2LL * v.kpos[n] * p.npad + 2 * v.kpos[gp.col[g]];
+    b1 = b0 + p.npad;
+  }
+  T a00 = b0[0], a01 = b0[1], a10 = b1[0], a11 = b1[1];
+  for (int q = gp.cstart[g]; q < gp.cstart[g + 1]; ++q) {
+    const double *Ke = Ke64 + gp.ke[q];
+    a00 += T(Ke[0]);
+    a01 += T(Ke[1]);
+    a10 += T(Ke[6]);
+    a11 += T(Ke[7]);
+  }
+  b0[0] = a00;
+  b0[1] = a01;
+  b1[0] = a10;
+  b1[1] = a11;
 }
 
 template <class T>
@@ -242,12 +258,13 @@ void launch_assemble_gather(hipStream_t s, const View &v, int e_begin, int e_end
                          Ge, Ke64, cs);
   }
   if (nn <= 0) return;
-  if (mode == ASM_FULL)
-    hipLaunchKernelGGL((k_node_gather<T, ASM_FULL>), dim3((nn + 255) / 256), dim3(256), 0, s, v, node_cell, adj_start,
-                       adj, gp, n0, n1, Ge, Ke64, G, K, Kb);
-  else
-    hipLaunchKernelGGL((k_node_gather<T, ASM_RESIDUAL>), dim3((nn + 255) / 256), dim3(256), 0, s, v, node_cell,
-                       adj_start, adj, gp, n0, n1, Ge, Ke64, G, K, Kb);
+  const int ng = gp.g1 - gp.g0;
+  if (mode == ASM_FULL) {
+    if (ng > 0)
+      hipLaunchKernelGGL((k_group_gather<T>), dim3((ng + 255) / 256), dim3(256), 0, s, v, node_cell, adj_start, adj,
+                         gp, Ge, Ke64, G, K, Kb);
+  } else
+    hipLaunchKernelGGL(k_node_gather, dim3((nn + 255) / 256), dim3(256), 0, s, adj_start, adj, n0, n1, Ge, G);
 }
 template void launch_assemble_gather<float>(hipStream_t, const View &, int, int, int, const int *, const int *,
                                             const int *, const GatherPlan &, int, int, double *, double *, double *,

@@ -454,7 +454,8 @@ struct Batch {
   double *Ge = nullptr;  // per element: residual (6) for the node gather
   // node gather plan: per node, its column nodes (groups) and, per group, the
   // element contributions in element order (offsets of the 2 x 2 block in Ke64)
-  int *g_start = nullptr, *g_col = nullptr, *g_bpos = nullptr, *g_cstart = nullptr, *g_ke = nullptr;
+  int *g_start = nullptr, *g_col = nullptr, *g_bpos = nullptr, *g_cstart = nullptr, *g_ke = nullptr, *g_node = nullptr;
+  std::vector<int> g_start_h;  // host copy of g_start
   double *Ke64 = nullptr, *G = nullptr, *r = nullptr, *delta = nullptr, *Rea = nullptr, *MM = nullptr;
   void *K = nullptr;
   void *Dinv = nullptr;
@@ -494,7 +495,7 @@ struct Batch {
 
   void release() {
     void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
-                    sp_local, cp, cs, node_cell, adj_start, adj, Ge, g_start, g_col, g_bpos, g_cstart, g_ke, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
+                    sp_local, cp, cs, node_cell, adj_start, adj, Ge, g_start, g_col, g_bpos, g_cstart, g_ke, g_node, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
                     replica, transitions, plan.d_prefix, row_ptr, bcol, diag_pos, el_bpos, csr_cells, chunks, Kb,
                     cgvec, cg, partials, kpos, rp, dp, bt_bidx, bt_node, bt_cst, crange, bt_cand, bt_frame, bt_g, bt_J, bt_area,
                     bt_gd, bt_fext};
@@ -1045,7 +1046,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     // first appearance along its elements (element order) and, for each, the
     // contributing element blocks in element order -- the order in which the
     // colour-by-colour scatter adds them to the same entries
-    std::vector<int> gs(N + 1, 0), gcol, gbp, gcs(1, 0), gke;
+    std::vector<int> gs(N + 1, 0), gcol, gbp, gcs(1, 0), gke, gnode;
     gcol.reserve(7 * (size_t)N);
     gbp.reserve(7 * (size_t)N);
     gke.reserve(9 * (size_t)T);
@@ -1071,6 +1072,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
       }
       for (size_t g = 0; g < cols.size(); ++g) {
         gcol.push_back(cols[g]);
+        gnode.push_back(n);
         gbp.push_back(bps[g]);
         gke.insert(gke.end(), lists[g].begin(), lists[g].end());
         gcs.push_back((int)gke.size());
@@ -1083,6 +1085,8 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.g_bpos = dupload(gbp);
     b.g_cstart = dupload(gcs);
     b.g_ke = dupload(gke);
+    b.g_node = dupload(gnode);
+    b.g_start_h = gs;
   }
   b.plan.build(b.cp_h, ctx->opt.pivoting, b.lu_groups, b.lu_depth, ctx->opt.fused_step && elem_size == 4);
   if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
@@ -1158,7 +1162,8 @@ template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &s
   T *K = static_cast<T *>(b.K);
   if (sc.cell < 0 && mode != ASM_STRAIN) {
     // whole batch: every element at once, then each node gathers its rows
-    const GatherPlan gp{b.g_start, b.g_col, b.g_bpos, b.g_cstart, b.g_ke};
+    const GatherPlan gp{b.g_start, b.g_col, b.g_bpos, b.g_cstart, b.g_ke, b.g_node,
+                        b.g_start_h[sc.node_begin], b.g_start_h[sc.node_end]};
     launch_assemble_gather<T>(s, v, 0, b.T, mode, b.node_cell, b.adj_start, b.adj, gp, sc.node_begin, sc.node_end,
                               b.Ge, b.Ke64, b.G, K, static_cast<T *>(b.Kb));
   } else {

@@ -34,6 +34,8 @@ struct GatherPlan {
   const int *bpos;    // per group: block-CSR position (sparse cells), else -1
   const int *cstart;  // per group: contributions [cstart[g], cstart[g+1])
   const int *ke;      // per contribution: offset of the 2 x 2 block in Ke64
+  const int *node;    // per group: its row node
+  int g0, g1;         // groups of the node range [n0, n1)
 };
 template <class T>
 void launch_assemble_gather(hipStream_t s, const View &v, int e_begin, int e_end, int mode, const int *node_cell,
