@@ -347,6 +347,22 @@ struct LuPlan {
     return s;
   }
 
+  // Fused block step lists: k_lu_step hands item i of a launch to XCD
+  // chunk xcd_item^-1(i) (eight contiguous chunks, dispatched in order within
+  // each); inside every chunk the diagonal and L items (the inverse on their
+  // critical path) go first and the trailing tiles fill in after them.
+  static void step_heavy_first(std::vector<int> &buf, const std::vector<CellParams> &cp, const Stage &g) {
+    if (g.kind != ST_STEP) return;
+    const int q = g.tot >> 3, r = g.tot & 7;
+    for (int x = 0; x < 8; ++x) {
+      const int b0 = g.off + x * q + std::min(x, r), nx = q + (x < r ? 1 : 0);
+      std::stable_partition(buf.begin() + b0, buf.begin() + b0 + nx, [&](int v) {
+        const CellParams &p = cp[v >> 20];
+        return (v & 0xFFFFF) <= std::min(p.nblk - g.k - 2, p.bw);
+      });
+    }
+  }
+
   void build(const std::vector<CellParams> &cp, int piv, int groups, int lu_depth, bool fuse = false) {
     if (lu_depth < 1 || groups < 1) throw ApiError{WCFE_ERR_ARG, "LU schedule needs depth and groups >= 1"};
     pivoting = piv;
@@ -367,6 +383,7 @@ struct LuPlan {
         g.tot += (int)t;
         g.work += block_products(cp[c], st[0], st[1], st[2], pivoting);
       }
+      step_heavy_first(buf, cp, g);
       if (g.tot > 0) batched.push_back(g);
     }
     single.assign(n, {});
@@ -417,6 +434,7 @@ struct LuPlan {
             gs.tot += (int)t;
             gs.work += block_products(cp[c], sk[0], sk[1], sk[2], pivoting);
           }
+          step_heavy_first(buf, cp, gs);
           if (gs.tot > 0) st.push_back(gs);
         }
         grp.push_back(st);
