@@ -172,7 +172,7 @@ def main():
     ap.add_argument("--autotune", choices=["none", "solver", "lu", "all"], default=None,
                     help="run wcfe_autotune on the batch before timing (default: solver for replicas1024, "
                          "none otherwise)")
-    ap.add_argument("--fused-step", type=int, choices=[0, 1], default=None,
+    ap.add_argument("--fused-step", type=int, choices=[0, 1, 2, 3], default=None,
                     help="band-LU block-step mode (library default when omitted)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (HBM probe / counter reference) and exit")

@@ -91,9 +91,12 @@ typedef struct wcfe_options {
   int32_t band;          /* 1: dense cells under static pivoting are numbered to narrow their band
                             (reverse Cuthill-McKee) and factored inside it (fill stays in the band);
                             0: natural numbering, full matrix */
-  int32_t fused_step;    /* 1: with one panel per round (lu_depth 1) in FP32, one launch per block step:
-                            trailing update, next diagonal factor and its block solves together;
-                            0: separate launches */
+  int32_t fused_step;    /* with one panel per round (lu_depth 1) in FP32, one launch per block step:
+                            trailing update, next diagonal inverse and its block solves together.
+                            1: per launch, L items (each inverts the diagonal block itself: shortest
+                            chain) or, above 2 items per compute unit, one panel item per cell (one
+                            inverse, then every L block: least work); 2: L items only; 3: panel items
+                            after the first step; 0: separate launches */
 } wcfe_options;
 
 typedef struct wcfe_cell_desc {

@@ -862,7 +862,7 @@ __device__ __forceinline__ void acc_store64(const typename Acc4<T>::type (&acc)[
 
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState *cs, const int *map, int k, T *K,
-                                                 T *Dinv) {
+                                                 T *Dinv, int panel) {
   constexpr int P = Pitch<T>::v, PA = P;
   typedef typename Acc4<T>::type acc_t;
   __shared__ __attribute__((aligned(16))) T ls[64 * P];
@@ -874,12 +874,16 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
   const CellParams p = cp[c];
   const int t = v & 0xFFFFF;
   const int s0 = k + 1, wl = lu_step_w(p, k), npl = lu_step_np(p, k);
-  const int nsolve = p.sym ? wl : 2 * wl;
+  if (k < 0) panel = 0;  // the first step always runs L items
+  // panel mode: item 0 inverts block s0 once and solves every L block of
+  // the panel (no L items)
+  const int nl = panel ? 0 : wl;
+  const int nsolve = nl + (p.sym ? 0 : wl);
   // 0 diagonal, 1 L block, 2 U block (stored as updated), 3 other
   int role, bi, bj;
-  if (t == 0) { role = 0; bi = s0; bj = s0; }
-  else if (t <= wl) { role = 1; bi = s0 + t; bj = s0; }
-  else if (t <= nsolve) { role = 2; bi = s0; bj = s0 + (t - wl); }
+  if (t == 0) { role = 0; bi = panel && wl > 0 ? s0 + 1 : s0; bj = s0; }
+  else if (t <= nl) { role = 1; bi = s0 + t; bj = s0; }
+  else if (t <= nsolve) { role = 2; bi = s0; bj = s0 + (t - nl); }
   else {
     const int q = t - 1 - nsolve;
     role = 3;
@@ -907,7 +911,8 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
     acc_store64<T, 0>(acc, C, ld);
     return;
   }
-  const bool own = role == 1, upd = k >= 0, upd_own = own && upd && bi <= reach;
+  const bool store = role == 0, own = role == 1 || (store && panel && wl > 0), upd = k >= 0,
+             upd_own = own && upd && bi <= reach;
   if (upd) {
     // Panel k updates block s0 (and the own block): the updates run inside
     // the inverse.  Block s0 -= L_{s0,k} U_{k,s0}: wave 0 takes sub-block
@@ -982,12 +987,13 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
       }
     };
     T *D = Dinv + dinv_off(p, s0);
-    if (!own) {
+    if (store) {
       const int fixed = invert_diag_ov<T, PA, true>(xs, D, D + 4096, (T)cs[c].tau, pre, side, post);
       if (fixed) atomicAdd(&cs[c].n_static, fixed);
-      return;
+    } else {
+      (void)invert_diag_ov<T, PA, false>(xs, D, D + 4096, (T)cs[c].tau, pre, side, post);
     }
-    (void)invert_diag_ov<T, PA, false>(xs, D, D + 4096, (T)cs[c].tau, pre, side, post);
+    if (!own) return;
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -996,6 +1002,44 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
         for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
     mma_64<T, P, PA>(ls, xs, acc, T(1));
     acc_store64<T, 0>(acc, C, ld);
+    if (!(store && panel)) return;
+    // panel item: L blocks s0+2 .. s0+w after the inverse, each as an L item
+    // computes it (same products in the same order)
+    for (int t2 = 2; t2 <= wl; ++t2) {
+      const int b2 = s0 + t2;
+      T *C2 = A + (long long)(b2 * NB) * ld + s0 * NB;
+      acc_load64<T>(acc, C2, ld);
+      __syncthreads();  // the previous block's product is done with ls
+      if (b2 <= reach) {
+        tile_load<T>(lt, A + (long long)(b2 * NB) * ld + k * NB, ld);
+        tile_to_lds<T, P>(lt, ls);
+        __syncthreads();
+        mma_64<T, P, P>(ls, us, acc, T(-1));  // us: U_{k,s0}
+        __syncthreads();
+      }
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) ls[(wr + ti * 16 + acc_row(T(0), lk, r)) * P + wc + tj * 16 + lr] = acc[ti][tj][r];
+      __syncthreads();
+      if (p.sym) {  // U_{s0,b2} = A_{b2,s0}^T
+        T *U2 = A + (long long)(s0 * NB) * ld + b2 * NB;
+        for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
+          const int r = idx >> 6, cc = idx & 63;
+          U2[(long long)r * ld + cc] = ls[cc * P + r];
+        }
+      }
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
+      mma_64<T, P, PA>(ls, xs, acc, T(1));
+      acc_store64<T, 0>(acc, C2, ld);
+    }
     return;
   }
   // first step (no panel before it): block s0 and the own block as they are
@@ -1047,16 +1091,16 @@ __global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState
 }
 template <class T>
 void launch_lu_step(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
-                    hipEvent_t ev_start, hipEvent_t ev_stop) {
+                    int panel, hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (wl.total <= 0) return;
   // FP32 only: three 64 x 68 LDS tiles (the FP64 variant would exceed the
   // static LDS limit); the schedule never selects it for FP64 factors
   if constexpr (sizeof(T) == 4) {
     if (ev_start)  // profiling (never under graph capture): events at the kernel's own start and end
       hipExtLaunchKernelGGL(k_lu_step<T>, dim3(wl.total), dim3(256), 0, s, ev_start, ev_stop, 0, cp, cs, wl.map, k,
-                            K, Dinv);
+                            K, Dinv, panel);
     else
-      hipLaunchKernelGGL(k_lu_step<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv);
+      hipLaunchKernelGGL(k_lu_step<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, panel);
   }
 }
 
@@ -1672,7 +1716,7 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
                                   const T *);                                                                         \
   template void launch_lu_gemm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *,      \
                                   int, int, int, hipEvent_t, hipEvent_t);                                             \
-  template void launch_lu_step<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *,      \
+  template void launch_lu_step<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *, int, \
                                   hipEvent_t, hipEvent_t);                                                            \
   template void launch_lu_solve<T>(hipStream_t, const CellParams *, const CellState *, const int *, int, const T *,   \
                                    const T *, const int *, int, const double *, double *, int, int);                  \

@@ -288,12 +288,13 @@ struct LuPlan {
       case ST_GEMM1: return k < nb ? tiles(p, m) : 0;
       case ST_GEMM: return nb - k - np >= 1 ? tiles(p, std::min(nb - k - np, p.bw)) : 0;
       case ST_PANEL: return k < nb ? 1 : 0;
-      case ST_STEP: {  // fused block step k (k = -1 .. nb - 2), see k_lu_step
+      case ST_STEP: {  // fused block step k (k = -1 .. nb - 2), see k_lu_step; np 2: panel mode
         const int s0 = k + 1;
         if (s0 >= nb) return 0;
         const int w = std::min(nb - s0 - 1, p.bw);
         const long long npl = k >= 0 ? std::max(0, std::min(nb - 1, k + p.bw) - s0) : 0;
-        return 1 + (p.sym ? w : 2 * w) + (p.sym ? npl * (npl + 1) / 2 : npl * npl);
+        const int nl = np == 2 ? 0 : w;
+        return 1 + nl + (p.sym ? 0 : w) + (p.sym ? npl * (npl + 1) / 2 : npl * npl);
       }
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
     }
@@ -358,16 +359,32 @@ struct LuPlan {
       const int b0 = g.off + x * q + std::min(x, r), nx = q + (x < r ? 1 : 0);
       std::stable_partition(buf.begin() + b0, buf.begin() + b0 + nx, [&](int v) {
         const CellParams &p = cp[v >> 20];
-        return (v & 0xFFFFF) <= std::min(p.nblk - g.k - 2, p.bw);
+        return (v & 0xFFFFF) <= (g.np == 2 ? 0 : std::min(p.nblk - g.k - 2, p.bw));
       });
     }
   }
 
-  void build(const std::vector<CellParams> &cp, int piv, int groups, int lu_depth, bool fuse = false) {
+  // fused block steps (fuse = the fused_step option): 1 panel items in the
+  // launches with more than panel_min items, L items in the others; 2 L items
+  // only; 3 panel items after the first step
+  void build(const std::vector<CellParams> &cp, int piv, int groups, int lu_depth, int fuse = 0, int panel_min = 512) {
     if (lu_depth < 1 || groups < 1) throw ApiError{WCFE_ERR_ARG, "LU schedule needs depth and groups >= 1"};
     pivoting = piv;
     depth = lu_depth;
     fused = fuse && !piv && lu_depth == 1;
+    // the step's form: L items (np 1) or one panel item per cell (np 2)
+    auto step_form = [&](const std::vector<int> &cells, std::array<int, 3> &st) {
+      if (st[0] != ST_STEP || st[1] < 0 || fuse == 2) return;
+      if (fuse == 3) {
+        st[2] = 2;
+        return;
+      }
+      long long tot = 0;
+      for (int c : cells) tot += items(cp[c], ST_STEP, st[1], 1, pivoting);
+      if (tot > panel_min) st[2] = 2;
+    };
+    std::vector<int> all((size_t)cp.size());
+    for (size_t c = 0; c < cp.size(); ++c) all[c] = (int)c;
     const int n = (int)cp.size();
     std::vector<int> buf;
     max_nblk = 0;
@@ -375,6 +392,7 @@ struct LuPlan {
     if (n >= (1 << 11)) throw ApiError{WCFE_ERR_ARG, "at most 2047 systems per batch"};
     batched.clear();
     for (auto st : schedule(max_nblk, pivoting)) {
+      step_form(all, st);
       Stage g{st[0], st[1], st[2], (int)buf.size(), 0, 0};
       for (int c = 0; c < n; ++c) {
         const long long t = items(cp[c], st[0], st[1], st[2], pivoting);
@@ -389,6 +407,7 @@ struct LuPlan {
     single.assign(n, {});
     for (int c = 0; c < n; ++c)
       for (auto st : schedule(cp[c].nblk, pivoting)) {
+        step_form({c}, st);
         const long long t = items(cp[c], st[0], st[1], st[2], pivoting);
         if (t == 0) continue;
         if (t >= (1 << 20)) throw ApiError{WCFE_ERR_ARG, "too many LU work items per cell"};
@@ -427,6 +446,7 @@ struct LuPlan {
         for (int c : mine) gmax = std::max(gmax, cp[c].nblk);
         std::vector<Stage> st;
         for (auto sk : schedule(gmax, pivoting)) {
+          step_form(mine, sk);
           Stage gs{sk[0], sk[1], sk[2], (int)buf.size(), 0, 0};
           for (int c : mine) {
             const long long t = items(cp[c], sk[0], sk[1], sk[2], pivoting);
@@ -566,6 +586,7 @@ struct wcfe_ctx {
   hipStream_t stream = nullptr;
   std::string arch;
   size_t max_lds = 65536;
+  int n_cu = 256;  // compute units of the device
   std::map<int, Table> tables;
   std::vector<HostCell> cells;
   bool finalized = false;
@@ -1106,7 +1127,8 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.g_node = dupload(gnode);
     b.g_start_h = gs;
   }
-  b.plan.build(b.cp_h, ctx->opt.pivoting, b.lu_groups, b.lu_depth, ctx->opt.fused_step && elem_size == 4);
+  b.plan.build(b.cp_h, ctx->opt.pivoting, b.lu_groups, b.lu_depth, elem_size == 4 ? ctx->opt.fused_step : 0,
+               2 * ctx->n_cu);
   if (ctx->opt.pivoting == WCFE_PIVOT_PARTIAL &&
       lu_solve_lds_bytes(b.max_npad, elem_size) > lu_solve_max_dynamic(ctx->max_lds))
     fail(WCFE_ERR_ARG, "system of %d unknowns exceeds the LDS-resident triangular solve (%zu bytes max)", b.max_npad,
@@ -1251,7 +1273,7 @@ void enqueue_lu(wcfe_ctx *ctx, hipStream_t s, const LuPlan &plan, const std::vec
       case ST_STEP: {
         if (sizeof(T) != 4) fail(WCFE_ERR_STATE, "the fused block step is built for FP32 factors only");
         KernelTimer kt(ctx, WCFE_PH_LU_GEMM, g.work);
-        launch_lu_step<T>(s, cp, cs, wl, g.k, K, Dinv, kt.a, kt.b);
+        launch_lu_step<T>(s, cp, cs, wl, g.k, K, Dinv, g.np == 2 ? 1 : 0, kt.a, kt.b);
         break;
       }
     }
@@ -1535,10 +1557,14 @@ int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
     int lds = 0;
     HIPCHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->opt.device));
     ctx->max_lds = (size_t)lds;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->opt.device) == hipSuccess && ncu > 0)
+      ctx->n_cu = ncu;
     HIPCHECK(lu_prepare_attributes(ctx->max_lds));
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     if (ctx->opt.lu_groups < 0 || ctx->opt.lu_groups > 16) fail(WCFE_ERR_ARG, "lu_groups must be in [0, 16]");
     if (ctx->opt.lu_depth < 0 || ctx->opt.lu_depth > 8) fail(WCFE_ERR_ARG, "lu_depth must be in [0, 8]");
+    if (ctx->opt.fused_step < 0 || ctx->opt.fused_step > 3) fail(WCFE_ERR_ARG, "fused_step must be in [0, 3]");
     if (ctx->opt.band != 0 && ctx->opt.band != 1) fail(WCFE_ERR_ARG, "band must be 0 or 1");
     HIPCHECK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     for (int g = 0; g < std::max(ctx->opt.lu_groups, AUTO_GROUPS_MAX); ++g) {

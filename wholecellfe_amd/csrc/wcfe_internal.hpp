@@ -105,7 +105,7 @@ void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const Wo
 // panel k, factor of diagonal block k + 1 and the block solves of panel k + 1
 template <class T>
 void launch_lu_step(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
-                    hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
+                    int panel, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 template <class T>
 void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                      const T *K, const T *Dinv, const int *ipiv, int pivoting, const double *r, double *delta,
