@@ -1283,6 +1283,27 @@ __device__ __forceinline__ void band_load(BandRegs<T, BWM> &R, const CellParams 
   }
 }
 
+// sum over the 8 lanes of each aligned group, in every lane.  FP32 by DPP
+// (quad_perm 1,0,3,2 and 2,3,0,1, then row_half_mirror: lane i adds the other
+// quad's sum); the partial sums and their order are the xor butterfly's, so
+// the result is the same bit for bit.
+template <class T> __device__ __forceinline__ T sum8(T v) {
+  if constexpr (sizeof(T) == 4) {
+    auto dpp = [](float x, auto ctrl) {
+      return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), decltype(ctrl)::value, 0xF, 0xF, false));
+    };
+    v += dpp(v, std::integral_constant<int, 0xB1>{});
+    v += dpp(v, std::integral_constant<int, 0x4E>{});
+    v += dpp(v, std::integral_constant<int, 0x141>{});
+    return v;
+  } else {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    return v;
+  }
+}
+
 template <class T, int BWM, bool FWD>
 __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb, int b, int bw, int nb) {
   const int r = threadIdx.x >> 3, q = threadIdx.x & 7;
@@ -1295,9 +1316,7 @@ __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb
       for (int e = 0; e < 8; ++e) part += R.a[u][e] * y[j * NB + q * 8 + e];
     }
   }
-  part += __shfl_xor(part, 1, 64);
-  part += __shfl_xor(part, 2, 64);
-  part += __shfl_xor(part, 4, 64);
+  part = sum8(part);
   if (FWD) {  // identity diagonal block (static pivoting stores Li = I)
     if (q == 0) y[b * NB + r] -= part;
     lds_barrier();
@@ -1308,9 +1327,7 @@ __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb
   T t = T(0);
 #pragma unroll
   for (int e = 0; e < 8; ++e) t += R.d[e] * sb[q * 8 + e];
-  t += __shfl_xor(t, 1, 64);
-  t += __shfl_xor(t, 2, 64);
-  t += __shfl_xor(t, 4, 64);
+  t = sum8(t);
   if (q == 0) y[b * NB + r] = t;
   lds_barrier();
 }
