@@ -498,51 +498,67 @@ __device__ __forceinline__ int invert_diag(T *a, T *Li, T *Ui, T tau) {
 
 // invert_diag with work for the idle waves: wave 0 runs pre() and then the
 // Gauss-Jordan inverse of sub-block kb while waves 1-3 run side(kb); every
-// thread runs post(kb) right after that window's barrier.  The inverse itself
-// is invert_diag's, step for step.
+// thread runs post(kb) right after that window's barrier.  The step's
+// products take one phase (two barriers per sub-block step instead of
+// three): with P the inverted pivot sub-block, wave 3 forms the new pivot
+// row P A_kj in registers and stores it in the next window, while each other
+// row's wave forms A'_ik = -A_ik P in place and then A_ij + A'_ik A_kj with
+// the old pivot row -- the same elimination as invert_diag
+// (A_ij - A_ik (P A_kj)), products associated the other way.
 template <class T, int PA, bool STORE, class Pre, class Side, class Post>
 __device__ __forceinline__ int invert_diag_ov(T *a, T *Li, T *Ui, T tau, Pre pre, Side side, Post post) {
   typedef typename Acc4<T>::type acc_t;
   const int wave = threadIdx.x >> 6;
   int fixed = 0;
   auto sub = [](T *base, int i, int j) { return base + 16 * i * PA + 16 * j; };
+  acc_t rowk[3];  // wave 3: the new pivot row of the previous step
+  auto store_row = [&](int kb) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc16_store<T, PA>(rowk[q], sub(a, kb, q + (q >= kb ? 1 : 0)));
+  };
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
     if (wave == 0) {
       if (kb == 0) pre();
       fixed += gj16<T, PA>(sub(a, kb, kb), tau);
     } else {
+      if (wave == 3 && kb > 0) store_row(kb - 1);
       side(kb);
     }
     __syncthreads();
     post(kb);
-    if (wave < 3) {  // row kb
-      const int j = wave + (wave >= kb ? 1 : 0);
-      acc_t acc;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] = T(0);
-      mm16<T, PA, M_FULL, M_FULL>(sub(a, kb, kb), sub(a, kb, j), acc, T(1));
-      acc16_store<T, PA>(acc, sub(a, kb, j));
-    }
-    __syncthreads();
-    if (wave < 3) {  // the other rows, one per wave
+    if (wave < 3) {  // row i: A'_ik = -A_ik P in place, then A_ij += A'_ik A_kj (old row kb)
       const int i = wave + (wave >= kb ? 1 : 0);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (j == kb) continue;
-        acc_t acc;
-        acc16_load<T, PA>(acc, sub(a, i, j));
-        mm16<T, PA, M_FULL, M_FULL>(sub(a, i, kb), sub(a, kb, j), acc, T(-1));
-        acc16_store<T, PA>(acc, sub(a, i, j));
-      }
       acc_t acc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = T(0);
       mm16<T, PA, M_FULL, M_FULL>(sub(a, i, kb), sub(a, kb, kb), acc, T(-1));
       acc16_store<T, PA>(acc, sub(a, i, kb));
+      // the wave reads back what its lanes wrote: LDS operations of one
+      // wave are performed in order; keep the compiler from reordering them
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j == kb) continue;
+        acc16_load<T, PA>(acc, sub(a, i, j));
+        mm16<T, PA, M_FULL, M_FULL>(sub(a, i, kb), sub(a, kb, j), acc, T(1));
+        acc16_store<T, PA>(acc, sub(a, i, j));
+      }
+    } else {  // wave 3: the new row kb, P A_kj, kept until the next window
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int j = q + (q >= kb ? 1 : 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) rowk[q][r] = T(0);
+        mm16<T, PA, M_FULL, M_FULL>(sub(a, kb, kb), sub(a, kb, j), rowk[q], T(1));
+      }
     }
     __syncthreads();
   }
+  if (wave == 3) store_row(3);
+  __syncthreads();
   if (STORE) {
     for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
       const int r = idx >> 6, c = idx & 63;
