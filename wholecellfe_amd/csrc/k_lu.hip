@@ -877,7 +877,7 @@ __device__ __forceinline__ void acc_store64(const typename Acc4<T>::type (&acc)[
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_step(const CellParams *cp, CellState *cs, const int *map, int k, T *K,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_lu_step(const CellParams *cp, CellState *cs, const int *map, int k, T *K,
                                                  T *Dinv, int panel) {
   constexpr int P = Pitch<T>::v, PA = P;
   typedef typename Acc4<T>::type acc_t;
