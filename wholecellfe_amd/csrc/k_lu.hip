@@ -1351,7 +1351,8 @@ __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb
 template <class T, int BWM>
 __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams *cp, const CellState *cs,
                                                                 const int *cells, const T *K, const T *Dinv,
-                                                                const double *rhs, double *delta) {
+                                                                const int *kpos, const double *rhs, double *delta,
+                                                                int accumulate) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   __shared__ T sb[64];
   T *y = reinterpret_cast<T *>(smem);
@@ -1362,8 +1363,15 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams
   const T *A = K + p.k_off;
   const int bw = p.bw;      // <= BWM (checked by the launcher's caller)
   const int nb = p.nblk;
-  const double *rr = rhs + (long long)p.vec_off;
-  for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = i < p.ndof ? (T)rr[i] : T(0);
+  // the right-hand side in the cell's band numbering: node n's rows go to
+  // 2 kpos[n] (the padding rows stay zero)
+  for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = T(0);
+  lds_barrier();
+  for (int m = threadIdx.x; m < p.n_nodes; m += BAND_THREADS) {
+    const int n = p.node_off + m, o = 2 * kpos[n];
+    y[o] = (T)rhs[2LL * n];
+    y[o + 1] = (T)rhs[2LL * n + 1];
+  }
   // a ring of PF block rows in registers: the loads of block b + PF are in
   // flight while blocks b + 1 .. b + PF - 1 are applied
   constexpr int PF = (sizeof(T) == 4 && BWM <= 4) ? 4 : 2;
@@ -1387,21 +1395,28 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams
       band_load<T, BWM, false>(R[i], p, A, Dinv, b - i - PF, bw);
     }
   }
-  double *dd = delta + (long long)p.vec_off;
-  for (int i = threadIdx.x; i < p.ndof; i += BAND_THREADS) dd[i] = (double)y[i];
+  // back to the batch numbering; accumulate: delta += the solution
+  for (int m = threadIdx.x; m < p.n_nodes; m += BAND_THREADS) {
+    const int n = p.node_off + m, o = 2 * kpos[n];
+    double *d = delta + 2LL * n;
+    const double d0 = (double)y[o], d1 = (double)y[o + 1];
+    d[0] = accumulate ? d[0] + d0 : d0;
+    d[1] = accumulate ? d[1] + d1 : d1;
+  }
 }
 
 template <class T>
 void launch_lu_solve_band(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
-                          const T *K, const T *Dinv, const double *r, double *delta, int max_npad, int max_bw) {
+                          const T *K, const T *Dinv, const int *kpos, const double *r, double *delta, int accumulate,
+                          int max_npad, int max_bw) {
   if (ncell <= 0) return;
   const size_t lds = lu_solve_lds_bytes(max_npad, sizeof(T));
   if (max_bw <= 4)
-    hipLaunchKernelGGL((k_lu_solve_band<T, 4>), dim3(ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv, r,
-                       delta);
+    hipLaunchKernelGGL((k_lu_solve_band<T, 4>), dim3(ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv, kpos, r,
+                       delta, accumulate);
   else
-    hipLaunchKernelGGL((k_lu_solve_band<T, 8>), dim3(ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv, r,
-                       delta);
+    hipLaunchKernelGGL((k_lu_solve_band<T, 8>), dim3(ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv, kpos, r,
+                       delta, accumulate);
 }
 
 // ---------------------------------------------------------------------------
@@ -1740,7 +1755,7 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
 
 #define WCFE_INST(T)                                                                                                 \
   template void launch_lu_solve_band<T>(hipStream_t, const CellParams *, const CellState *, const int *, int,        \
-                                        const T *, const T *, const double *, double *, int, int);                   \
+                                        const T *, const T *, const int *, const double *, double *, int, int, int); \
   template void launch_lu_diag_inv<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *); \
   template void launch_lu_panel_pp<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *,       \
                                       int *, T *);                                                                    \

@@ -1364,15 +1364,16 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
       } else if (npad > 0) {
         // in each dense cell's own numbering: rp = P r, solve, delta (+)= P^T dp
         const LuPlan::SolveLists &L = sc.cell < 0 ? b.plan.solve_batched : b.plan.solve_single[sc.cell];
-        launch_perm_gather(s, b.node_cell, b.cp, b.kpos, b.r, b.rp, sc.node_begin, sc.node_end);
-        if (b.solve_wg)
-          launch_lu_solve_band<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.rp, b.dp,
-                                  npad, b.max_bw);
-        else
+        if (b.solve_wg) {  // the sweep kernel reads r and writes delta through the numbering itself
+          launch_lu_solve_band<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.kpos, b.r,
+                                  b.delta, it > 0, npad, b.max_bw);
+        } else {
+          launch_perm_gather(s, b.node_cell, b.cp, b.kpos, b.r, b.rp, sc.node_begin, sc.node_end);
           launch_lu_solve_mw<T>(s, b.cp, b.cs, sc.cells, sc.ncell, WorkList{b.plan.d_prefix + L.fwd_off, L.tot},
                                 WorkList{b.plan.d_prefix + L.bwd_off, L.tot}, K, static_cast<const T *>(b.Dinv), b.rp,
                                 b.dp, b.ysol, b.sync, 0);
-        launch_perm_scatter(s, b.node_cell, b.cp, b.cs, b.kpos, b.dp, b.delta, sc.node_begin, sc.node_end, it > 0);
+          launch_perm_scatter(s, b.node_cell, b.cp, b.cs, b.kpos, b.dp, b.delta, sc.node_begin, sc.node_end, it > 0);
+        }
       }
       enqueue_pcg<T>(ctx, b, sc, it > 0);
     }

@@ -127,7 +127,8 @@ size_t lu_solve_max_dynamic(size_t max_lds);
 // block's data prefetched (static pivoting, max_bw <= SOLVE_WG_MAX_BW)
 template <class T>
 void launch_lu_solve_band(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
-                          const T *K, const T *Dinv, const double *r, double *delta, int max_npad, int max_bw);
+                          const T *K, const T *Dinv, const int *kpos, const double *r, double *delta, int accumulate,
+                          int max_npad, int max_bw);
 // block bandwidth up to which one workgroup per cell runs the triangular sweeps
 constexpr int SOLVE_WG_MAX_BW = 8;
 // concurrent cell groups of the dense LU chosen automatically for wide bands
