@@ -295,7 +295,7 @@ void launch_element_dump(hipStream_t s, const View &v, int e_begin, int e_end, d
 //        update (gnorm)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_cell_norm(const CellParams *cp, CellState *cs, const int *cells,
-                                                   const double *vec, int which, int only_active) {
+                                                   const double *vec, int which, int only_active, double lin_rtol) {
   const int c = cells[blockIdx.x];
   if (only_active && cs[c].ir_done) return;
   const CellParams &p = cp[c];
@@ -318,15 +318,20 @@ __global__ __launch_bounds__(256) void k_cell_norm(const CellParams *cp, CellSta
       if (s.newton_it == 0) s.g0 = nrm;
     } else if (which == 1) {
       s.rnorm = nrm;
+      if (lin_rtol >= 0.0 && !s.ir_done) {  // refinement control (k_ir_control) for this cell
+        s.ir_iters += 1;
+        s.lin_relres = s.bnorm > 0.0 ? s.rnorm / s.bnorm : 0.0;
+        if (s.rnorm <= lin_rtol * s.bnorm) s.ir_done = 1;
+      }
     } else {
       s.gnorm = nrm;
     }
   }
 }
 void launch_cell_norm(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell,
-                      const double *vec, int which, int only_active) {
+                      const double *vec, int which, int only_active, double lin_rtol) {
   if (ncell <= 0) return;
-  hipLaunchKernelGGL(k_cell_norm, dim3(ncell), dim3(256), 0, s, cp, cs, cells, vec, which, only_active);
+  hipLaunchKernelGGL(k_cell_norm, dim3(ncell), dim3(256), 0, s, cp, cs, cells, vec, which, only_active, lin_rtol);
 }
 
 // ---------------------------------------------------------------------------

@@ -1382,8 +1382,7 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
     if (b.any_area) launch_cell_gdot(s, b.cp, sc.cells, sc.ncell, bv, b.delta);
     launch_matvec_ir(s, v, b.node_cell, b.adj_start, b.adj, b.Ke64, b.G, b.delta, b.r, sc.node_begin, sc.node_end,
                      b.cs, bv);
-    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.r, 1, 1);
-    launch_ir_control(s, b.cs, sc.cells, sc.ncell, ctx->opt.lin_rtol, 0);
+    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.r, 1, 1, ctx->opt.lin_rtol);
   }
 }
 
@@ -2502,8 +2501,7 @@ void dense_solve_impl(wcfe_ctx *ctx, int n_sys, const int32_t *n, const double *
                             WorkList{plan.d_prefix + plan.solve_batched.bwd_off, plan.solve_batched.tot}, dK, dDinv,
                             dr, dx, dy, dsync, it > 0);
     launch_dense_residual(s, dcp, dcs, dcells, n_sys, dA, dAoff, db, dx, dr);
-    launch_cell_norm(s, dcp, dcs, dcells, n_sys, dr, 1, 1);
-    launch_ir_control(s, dcs, dcells, n_sys, ctx->opt.lin_rtol, 0);
+    launch_cell_norm(s, dcp, dcs, dcells, n_sys, dr, 1, 1, ctx->opt.lin_rtol);
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(s));

@@ -42,8 +42,10 @@ void launch_assemble_gather(hipStream_t s, const View &v, int e_begin, int e_end
                             const int *adj_start, const int *adj, const GatherPlan &gp, int n0, int n1, double *Ge,
                             double *Ke64, double *G, T *K, T *Kb);
 void launch_element_dump(hipStream_t s, const View &v, int e_begin, int e_end, double *Ge, double *Ke);
+// which 1 with lin_rtol >= 0 also applies the refinement control to each
+// refining cell (as launch_ir_control with reset 0)
 void launch_cell_norm(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell,
-                      const double *vec, int which, int only_active);
+                      const double *vec, int which, int only_active, double lin_rtol = -1.0);
 void launch_neg_copy(hipStream_t s, const double *src, double *dst, long long n);
 void launch_matvec_ir(hipStream_t s, const View &v, const int *node_cell, const int *adj_start, const int *adj,
                       const double *Ke64, const double *G, const double *delta, double *r, int node_begin,
