@@ -17,7 +17,11 @@ template <> struct Vec16<float> { typedef float4 type; static constexpr int n = 
 template <> struct Vec16<double> { typedef double2 type; static constexpr int n = 2; };
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fill_systems(const CellParams *cp, const int *cells, int ncell, T *K) {
+// lower_only: symmetric cells get the blocks up to the diagonal only (the
+// fused band LU writes every upper band block, U = L^T, before anything
+// reads it)
+__global__ __launch_bounds__(256) void k_fill_systems(const CellParams *cp, const int *cells, int ncell, T *K,
+                                                      int lower_only) {
   const int ci = blockIdx.y;
   if (ci >= ncell) return;
   const CellParams p = cp[cells[ci]];
@@ -26,7 +30,8 @@ __global__ __launch_bounds__(256) void k_fill_systems(const CellParams *cp, cons
   // blocks within the fill envelope |i - j| <= env only
   for (int r = blockIdx.x; r < p.npad; r += gridDim.x) {
     const int rb = r / NB;
-    const int c0 = max(0, rb - p.env) * (NB / VN), c1 = min(p.nblk, rb + p.env + 1) * (NB / VN);
+    const int c0 = max(0, rb - p.env) * (NB / VN),
+              c1 = min(p.nblk, rb + 1 + (lower_only && p.sym ? 0 : p.env)) * (NB / VN);
     V *row = reinterpret_cast<V *>(K + p.k_off + (long long)r * p.npad);
     for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
       T vals[VN];
@@ -40,14 +45,15 @@ __global__ __launch_bounds__(256) void k_fill_systems(const CellParams *cp, cons
 }
 
 template <class T>
-void launch_fill_systems(hipStream_t s, const CellParams *cp, const int *cells, int ncell, int max_npad, T *K) {
+void launch_fill_systems(hipStream_t s, const CellParams *cp, const int *cells, int ncell, int max_npad, T *K,
+                         int lower_only) {
   if (ncell <= 0) return;
   int gx = max_npad < 1024 ? max_npad : 1024;
   if (gx < 1) gx = 1;
-  hipLaunchKernelGGL(k_fill_systems<T>, dim3(gx, ncell), dim3(256), 0, s, cp, cells, ncell, K);
+  hipLaunchKernelGGL(k_fill_systems<T>, dim3(gx, ncell), dim3(256), 0, s, cp, cells, ncell, K, lower_only);
 }
-template void launch_fill_systems<float>(hipStream_t, const CellParams *, const int *, int, int, float *);
-template void launch_fill_systems<double>(hipStream_t, const CellParams *, const int *, int, int, double *);
+template void launch_fill_systems<float>(hipStream_t, const CellParams *, const int *, int, int, float *, int);
+template void launch_fill_systems<double>(hipStream_t, const CellParams *, const int *, int, int, double *, int);
 
 __global__ __launch_bounds__(256) void k_fill(double *p, long long n, double value) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)

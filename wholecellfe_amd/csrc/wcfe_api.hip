@@ -1394,7 +1394,9 @@ template <class T> void enqueue_newton(wcfe_ctx *ctx, Batch &b, const Scope &sc)
   // 1. assemble K(x), G(x)
   {
     PhaseScope ph(ctx, WCFE_PH_FILL);
-    launch_fill_systems<T>(s, b.cp, sc.cells, sc.ncell, npad, K);
+    // fused band LU with the one-workgroup sweeps: the upper band blocks of
+    // symmetric cells are written by the factorization before any read
+    launch_fill_systems<T>(s, b.cp, sc.cells, sc.ncell, npad, K, b.plan.fused && b.solve_wg ? 1 : 0);
     if (b.any_csr) {
       const long long bb = sc.cell < 0 ? 0 : b.blk_begin_h[sc.cell], be = sc.cell < 0 ? b.nnzb : b.blk_end_h[sc.cell];
       launch_zero<T>(s, static_cast<T *>(b.Kb) + 4 * bb, 4 * (be - bb));

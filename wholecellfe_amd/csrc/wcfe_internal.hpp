@@ -19,7 +19,8 @@ struct WorkList {
 enum AsmMode : int { ASM_FULL = 0, ASM_RESIDUAL = 1, ASM_STRAIN = 2 };
 
 template <class T>
-void launch_fill_systems(hipStream_t s, const CellParams *cp, const int *cells, int ncell, int max_npad, T *K);
+void launch_fill_systems(hipStream_t s, const CellParams *cp, const int *cells, int ncell, int max_npad, T *K,
+                         int lower_only = 0);
 void launch_fill(hipStream_t s, double *p, long long n, double value);
 void launch_copy(hipStream_t s, const void *src, void *dst, long long bytes);
 // K: dense systems; Kb: block-CSR values of sparse cells (View::el_bpos)
