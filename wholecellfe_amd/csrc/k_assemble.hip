@@ -27,13 +27,15 @@ __global__ __launch_bounds__(256) void k_fill_systems(const CellParams *cp, cons
   const CellParams p = cp[cells[ci]];
   typedef typename Vec16<T>::type V;
   constexpr int VN = Vec16<T>::n;
-  // blocks within the fill envelope |i - j| <= env only
-  for (int r = blockIdx.x; r < p.npad; r += gridDim.x) {
+  // blocks within the fill envelope |i - j| <= env only; one wave per row
+  // (a row holds at most (2 env + 1) * 64 / VN vectors)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = 4 * blockIdx.x + wave; r < p.npad; r += 4 * gridDim.x) {
     const int rb = r / NB;
     const int c0 = max(0, rb - p.env) * (NB / VN),
               c1 = min(p.nblk, rb + 1 + (lower_only && p.sym ? 0 : p.env)) * (NB / VN);
     V *row = reinterpret_cast<V *>(K + p.k_off + (long long)r * p.npad);
-    for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
+    for (int c = c0 + lane; c < c1; c += 64) {
       T vals[VN];
 #pragma unroll
       for (int k = 0; k < VN; ++k) vals[k] = (r >= p.ndof && r == c * VN + k) ? T(1) : T(0);
@@ -48,7 +50,9 @@ template <class T>
 void launch_fill_systems(hipStream_t s, const CellParams *cp, const int *cells, int ncell, int max_npad, T *K,
                          int lower_only) {
   if (ncell <= 0) return;
-  int gx = max_npad < 1024 ? max_npad : 1024;
+  // 4 rows per block and pass; at most 128 blocks per cell (the largest
+  // cells loop) so that small cells launch few empty blocks
+  int gx = (max_npad + 3) / 4 < 128 ? (max_npad + 3) / 4 : 128;
   if (gx < 1) gx = 1;
   hipLaunchKernelGGL(k_fill_systems<T>, dim3(gx, ncell), dim3(256), 0, s, cp, cells, ncell, K, lower_only);
 }
