@@ -310,10 +310,18 @@ __global__ __launch_bounds__(256) void k_cell_norm(const CellParams *cp, CellSta
   if (only_active && cs[c].ir_done) return;
   const CellParams &p = cp[c];
   const double *v = vec + (long long)p.vec_off;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < p.ndof; i += 256) acc += v[i] * v[i];
+  // four partial sums per thread: four loads in flight
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i0 = threadIdx.x; i0 < p.ndof; i0 += 4 * 256) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 256 * u;
+      const double x = i < p.ndof ? v[i] : 0.0;
+      acc[u] += x * x;
+    }
+  }
   __shared__ double red[256];
-  red[threadIdx.x] = acc;
+  red[threadIdx.x] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
