@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void k_node_gather(const int *adj_start, const
 template <class T>
 __global__ __launch_bounds__(256) void k_group_gather(View v, const int *node_cell, const int *adj_start,
                                                       const int *adj, GatherPlan gp, const double *Ge,
-                                                      const double *Ke64, double *G, T *K, T *Kb) {
+                                                      const double *Ke64, double *G, T *K, T *Kb, int lower_only) {
   const int g = gp.g0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= gp.g1) return;
   const int n = gp.node[g];
@@ -231,6 +231,9 @@ __global__ __launch_bounds__(256) void k_group_gather(View v, const int *node_ce
     G[2 * n + 1] = g1;
   }
   const CellParams &p = v.cp[node_cell[n]];
+  // lower_only: a symmetric cell's blocks above the diagonal block are
+  // written by the fused band LU (U = L^T) before any read
+  if (lower_only && p.sym && !p.csr && (v.kpos[n] >> 5) < (v.kpos[gp.col[g]] >> 5)) return;
   T *b0, *b1;
   if (p.csr) {
     b0 = Kb + 4LL * gp.bpos[g];
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(256) void k_group_gather(View v, const int *node_ce
 template <class T>
 void launch_assemble_gather(hipStream_t s, const View &v, int e_begin, int e_end, int mode, const int *node_cell,
                             const int *adj_start, const int *adj, const GatherPlan &gp, int n0, int n1, double *Ge,
-                            double *Ke64, double *G, T *K, T *Kb) {
+                            double *Ke64, double *G, T *K, T *Kb, int lower_only) {
   CellState *cs = const_cast<CellState *>(v.cs);
   const int ne = e_end - e_begin, nn = n1 - n0;
   if (ne > 0) {
@@ -272,16 +275,16 @@ void launch_assemble_gather(hipStream_t s, const View &v, int e_begin, int e_end
   if (mode == ASM_FULL) {
     if (ng > 0)
       hipLaunchKernelGGL((k_group_gather<T>), dim3((ng + 255) / 256), dim3(256), 0, s, v, node_cell, adj_start, adj,
-                         gp, Ge, Ke64, G, K, Kb);
+                         gp, Ge, Ke64, G, K, Kb, lower_only);
   } else
     hipLaunchKernelGGL(k_node_gather, dim3((nn + 255) / 256), dim3(256), 0, s, adj_start, adj, n0, n1, Ge, G);
 }
 template void launch_assemble_gather<float>(hipStream_t, const View &, int, int, int, const int *, const int *,
                                             const int *, const GatherPlan &, int, int, double *, double *, double *,
-                                            float *, float *);
+                                            float *, float *, int);
 template void launch_assemble_gather<double>(hipStream_t, const View &, int, int, int, const int *, const int *,
                                              const int *, const GatherPlan &, int, int, double *, double *, double *,
-                                             double *, double *);
+                                             double *, double *, int);
 
 __global__ __launch_bounds__(128) void k_element_dump(View v, int e_begin, int e_end, double *Ge, double *Ke) {
   const int e = e_begin + blockIdx.x * blockDim.x + threadIdx.x;

@@ -1205,7 +1205,7 @@ template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &s
     const GatherPlan gp{b.g_start, b.g_col, b.g_bpos, b.g_cstart, b.g_ke, b.g_node,
                         b.g_start_h[sc.node_begin], b.g_start_h[sc.node_end]};
     launch_assemble_gather<T>(s, v, 0, b.T, mode, b.node_cell, b.adj_start, b.adj, gp, sc.node_begin, sc.node_end,
-                              b.Ge, b.Ke64, b.G, K, static_cast<T *>(b.Kb));
+                              b.Ge, b.Ke64, b.G, K, static_cast<T *>(b.Kb), b.plan.fused && b.solve_wg ? 1 : 0);
   } else {
     for (int k = 0; k < b.ncolor; ++k) {
       int e0, e1;

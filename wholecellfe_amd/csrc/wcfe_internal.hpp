@@ -41,7 +41,7 @@ struct GatherPlan {
 template <class T>
 void launch_assemble_gather(hipStream_t s, const View &v, int e_begin, int e_end, int mode, const int *node_cell,
                             const int *adj_start, const int *adj, const GatherPlan &gp, int n0, int n1, double *Ge,
-                            double *Ke64, double *G, T *K, T *Kb);
+                            double *Ke64, double *G, T *K, T *Kb, int lower_only = 0);
 void launch_element_dump(hipStream_t s, const View &v, int e_begin, int e_end, double *Ge, double *Ke);
 // which 1 with lin_rtol >= 0 also applies the refinement control to each
 // refining cell (as launch_ir_control with reset 0)
