@@ -95,7 +95,8 @@ typedef struct wcfe_options {
                             trailing update, next diagonal inverse and its block solves together.
                             1: per launch, L items (each inverts the diagonal block itself: shortest
                             chain) or, above 2 items per compute unit, one panel item per cell (one
-                            inverse, then every L block: least work); 2: L items only; 3: panel items
+                            inverse, then every L block: least work) and one item for the
+                            cell's trailing blocks; 2: L items only; 3: panel items
                             after the first step; 0: separate launches */
 } wcfe_options;
 

@@ -892,27 +892,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const int s0 = k + 1, wl = lu_step_w(p, k), npl = lu_step_np(p, k);
   if (k < 0) panel = 0;  // the first step always runs L items
   // panel mode: item 0 inverts block s0 once and solves every L block of
-  // the panel (no L items)
+  // the panel (no L items); one item takes all trailing blocks of the window
   const int nl = panel ? 0 : wl;
   const int nsolve = nl + (p.sym ? 0 : wl);
-  // 0 diagonal, 1 L block, 2 U block (stored as updated), 3 other
-  int role, bi, bj;
-  if (t == 0) { role = 0; bi = panel && wl > 0 ? s0 + 1 : s0; bj = s0; }
-  else if (t <= nl) { role = 1; bi = s0 + t; bj = s0; }
-  else if (t <= nsolve) { role = 2; bi = s0; bj = s0 + (t - nl); }
-  else {
-    const int q = t - 1 - nsolve;
-    role = 3;
+  // trailing tile q of the window (lower triangle for symmetric cells)
+  auto plain_tile = [&](int q, int &ti, int &tj) {
     if (p.sym) {
       int i = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
       while (i * (i + 1) / 2 > q) --i;
       while ((i + 1) * (i + 2) / 2 <= q) ++i;
-      bi = s0 + 1 + i;
-      bj = s0 + 1 + (q - i * (i + 1) / 2);
+      ti = s0 + 1 + i;
+      tj = s0 + 1 + (q - i * (i + 1) / 2);
     } else {
-      bi = s0 + 1 + q / npl;
-      bj = s0 + 1 + q % npl;
+      ti = s0 + 1 + q / npl;
+      tj = s0 + 1 + q % npl;
     }
+  };
+  // 0 diagonal, 1 L block, 2 U block (stored as updated), 3 other, 4 every
+  // other block of the window (panel mode: one item per cell)
+  int role, bi, bj;
+  if (t == 0) { role = 0; bi = panel && wl > 0 ? s0 + 1 : s0; bj = s0; }
+  else if (t <= nl) { role = 1; bi = s0 + t; bj = s0; }
+  else if (t <= nsolve) { role = 2; bi = s0; bj = s0 + (t - nl); }
+  else if (panel) { role = 4; bi = s0; bj = s0; }
+  else {
+    role = 3;
+    plain_tile(t - 1 - nsolve, bi, bj);
   }
   T *A = K + p.k_off;
   const long long ld = p.npad;
@@ -922,6 +927,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const int reach = k + p.bw;  // panel k's last block row / column
   T *C = A + (long long)(bi * NB) * ld + bj * NB;
   acc_t acc[2][2];
+  if (role == 4) {  // the window's tiles one after another, as items of role 3 take them
+    const int ntile = p.sym ? npl * (npl + 1) / 2 : npl * npl;
+    for (int q = 0; q < ntile; ++q) {
+      int ti, tj;
+      plain_tile(q, ti, tj);
+      step_tile<T>(acc, A, ld, ti, tj, k, k >= 0 && ti <= reach && tj <= reach, ls, us);
+      acc_store64<T, 0>(acc, A + (long long)(ti * NB) * ld + tj * NB, ld);
+    }
+    return;
+  }
   if (role >= 2) {
     step_tile<T>(acc, A, ld, bi, bj, k, k >= 0 && bi <= reach && bj <= reach, ls, us);
     acc_store64<T, 0>(acc, C, ld);

@@ -293,8 +293,9 @@ struct LuPlan {
         if (s0 >= nb) return 0;
         const int w = std::min(nb - s0 - 1, p.bw);
         const long long npl = k >= 0 ? std::max(0, std::min(nb - 1, k + p.bw) - s0) : 0;
-        const int nl = np == 2 ? 0 : w;
-        return 1 + nl + (p.sym ? 0 : w) + (p.sym ? npl * (npl + 1) / 2 : npl * npl);
+        const long long tiles = p.sym ? npl * (npl + 1) / 2 : npl * npl;
+        if (np == 2) return 1 + (p.sym ? 0 : w) + (tiles > 0 ? 1 : 0);  // panel item, U items, one for the tiles
+        return 1 + w + (p.sym ? 0 : w) + tiles;
       }
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
     }
