@@ -877,8 +877,8 @@ __device__ __forceinline__ void acc_store64(const typename Acc4<T>::type (&acc)[
 }
 
 template <class T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_lu_step(const CellParams *cp, CellState *cs, const int *map, int k, T *K,
-                                                 T *Dinv, int panel) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_lu_step(
+    const CellParams *cp, CellState *cs, const int *map, int k, T *K, T *Dinv, int panel, StepFwd fwd) {
   constexpr int P = Pitch<T>::v, PA = P;
   typedef typename Acc4<T>::type acc_t;
   __shared__ __attribute__((aligned(16))) T ls[64 * P];
@@ -910,6 +910,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   };
   // 0 diagonal, 1 L block, 2 U block (stored as updated), 3 other, 4 every
   // other block of the window (panel mode: one item per cell)
+  const int ntile = p.sym ? npl * (npl + 1) / 2 : npl * npl;
+  const int t_fwd = 1 + nsolve + (panel ? (ntile > 0 ? 1 : 0) : ntile);  // the last item
+  if (t == t_fwd) {
+    // forward substitution of block s0 for the first solve (right-hand side
+    // -G in the band numbering): y_s0 = (-G)_s0 - sum_j L_s0,j y_j over the
+    // band, with L_s0,j and y_j final since earlier steps; 4 lanes per row
+    if (!fwd.y) return;
+    const T *A = K + p.k_off;
+    const long long ld = p.npad;
+    float *y = fwd.y + p.sys_off;
+    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+    float part = 0.0f;
+    for (int j = max(0, s0 - p.bw); j < s0; ++j) {
+      const T *Lr = A + (long long)(s0 * NB + r) * ld + j * NB + q * 16;
+      const float *yj = y + j * NB + q * 16;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) part += (float)Lr[e] * yj[e];
+    }
+    part += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(part), 0xB1, 0xF, 0xF, false));
+    part += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(part), 0x4E, 0xF, 0xF, false));
+    if (q == 0) {
+      const int i = s0 * NB + r, dof = fwd.kinv[p.sys_off + i];
+      y[i] = (dof >= 0 ? (float)(-fwd.G[dof]) : 0.0f) - part;
+    }
+    return;
+  }
   int role, bi, bj;
   if (t == 0) { role = 0; bi = panel && wl > 0 ? s0 + 1 : s0; bj = s0; }
   else if (t <= nl) { role = 1; bi = s0 + t; bj = s0; }
@@ -928,7 +954,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   T *C = A + (long long)(bi * NB) * ld + bj * NB;
   acc_t acc[2][2];
   if (role == 4) {  // the window's tiles one after another, as items of role 3 take them
-    const int ntile = p.sym ? npl * (npl + 1) / 2 : npl * npl;
     for (int q = 0; q < ntile; ++q) {
       int ti, tj;
       plain_tile(q, ti, tj);
@@ -1122,16 +1147,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 }
 template <class T>
 void launch_lu_step(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
-                    int panel, hipEvent_t ev_start, hipEvent_t ev_stop) {
+                    int panel, StepFwd fwd, hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (wl.total <= 0) return;
   // FP32 only: three 64 x 68 LDS tiles (the FP64 variant would exceed the
   // static LDS limit); the schedule never selects it for FP64 factors
   if constexpr (sizeof(T) == 4) {
     if (ev_start)  // profiling (never under graph capture): events at the kernel's own start and end
       hipExtLaunchKernelGGL(k_lu_step<T>, dim3(wl.total), dim3(256), 0, s, ev_start, ev_stop, 0, cp, cs, wl.map, k,
-                            K, Dinv, panel);
+                            K, Dinv, panel, fwd);
     else
-      hipLaunchKernelGGL(k_lu_step<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, panel);
+      hipLaunchKernelGGL(k_lu_step<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, k, K, Dinv, panel, fwd);
   }
 }
 
@@ -1367,7 +1392,7 @@ template <class T, int BWM>
 __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams *cp, const CellState *cs,
                                                                 const int *cells, const T *K, const T *Dinv,
                                                                 const int *kpos, const double *rhs, double *delta,
-                                                                int accumulate) {
+                                                                int accumulate, const float *yfwd) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   __shared__ T sb[64];
   T *y = reinterpret_cast<T *>(smem);
@@ -1379,28 +1404,36 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams
   const int bw = p.bw;      // <= BWM (checked by the launcher's caller)
   const int nb = p.nblk;
   // the right-hand side in the cell's band numbering: node n's rows go to
-  // 2 kpos[n] (the padding rows stay zero)
-  for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = T(0);
-  lds_barrier();
-  for (int m = threadIdx.x; m < p.n_nodes; m += BAND_THREADS) {
-    const int n = p.node_off + m, o = 2 * kpos[n];
-    y[o] = (T)rhs[2LL * n];
-    y[o + 1] = (T)rhs[2LL * n + 1];
+  // 2 kpos[n] (the padding rows stay zero); yfwd: the forward sweep already
+  // ran inside the fused block steps, y is its result
+  if (yfwd) {
+    for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = (T)yfwd[p.sys_off + i];
+  } else {
+    for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = T(0);
+    lds_barrier();
+    for (int m = threadIdx.x; m < p.n_nodes; m += BAND_THREADS) {
+      const int n = p.node_off + m, o = 2 * kpos[n];
+      y[o] = (T)rhs[2LL * n];
+      y[o + 1] = (T)rhs[2LL * n + 1];
+    }
   }
   // a ring of PF block rows in registers: the loads of block b + PF are in
   // flight while blocks b + 1 .. b + PF - 1 are applied
   constexpr int PF = (sizeof(T) == 4 && BWM <= 4) ? 4 : 2;
   BandRegs<T, BWM> R[PF];
+  if (!yfwd) {
 #pragma unroll
-  for (int i = 0; i < PF; ++i) band_load<T, BWM, true>(R[i], p, A, Dinv, i, bw);
-  lds_barrier();
-  for (int b = 0; b < nb; b += PF) {
+    for (int i = 0; i < PF; ++i) band_load<T, BWM, true>(R[i], p, A, Dinv, i, bw);
+    lds_barrier();
+    for (int b = 0; b < nb; b += PF) {
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      if (b + i < nb) band_step<T, BWM, true>(R[i], y, sb, b + i, bw, nb);
-      band_load<T, BWM, true>(R[i], p, A, Dinv, b + i + PF, bw);
+      for (int i = 0; i < PF; ++i) {
+        if (b + i < nb) band_step<T, BWM, true>(R[i], y, sb, b + i, bw, nb);
+        band_load<T, BWM, true>(R[i], p, A, Dinv, b + i + PF, bw);
+      }
     }
   }
+  lds_barrier();
 #pragma unroll
   for (int i = 0; i < PF; ++i) band_load<T, BWM, false>(R[i], p, A, Dinv, nb - 1 - i, bw);
   for (int b = nb - 1; b >= 0; b -= PF) {
@@ -1423,15 +1456,15 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams
 template <class T>
 void launch_lu_solve_band(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                           const T *K, const T *Dinv, const int *kpos, const double *r, double *delta, int accumulate,
-                          int max_npad, int max_bw) {
+                          int max_npad, int max_bw, const float *yfwd) {
   if (ncell <= 0) return;
   const size_t lds = lu_solve_lds_bytes(max_npad, sizeof(T));
   if (max_bw <= 4)
     hipLaunchKernelGGL((k_lu_solve_band<T, 4>), dim3(ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv, kpos, r,
-                       delta, accumulate);
+                       delta, accumulate, yfwd);
   else
     hipLaunchKernelGGL((k_lu_solve_band<T, 8>), dim3(ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv, kpos, r,
-                       delta, accumulate);
+                       delta, accumulate, yfwd);
 }
 
 // ---------------------------------------------------------------------------
@@ -1770,7 +1803,8 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
 
 #define WCFE_INST(T)                                                                                                 \
   template void launch_lu_solve_band<T>(hipStream_t, const CellParams *, const CellState *, const int *, int,        \
-                                        const T *, const T *, const int *, const double *, double *, int, int, int); \
+                                        const T *, const T *, const int *, const double *, double *, int, int, int,  \
+                                        const float *);                                                              \
   template void launch_lu_diag_inv<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *); \
   template void launch_lu_panel_pp<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *,       \
                                       int *, T *);                                                                    \
@@ -1780,6 +1814,7 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
   template void launch_lu_gemm<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *,      \
                                   int, int, int, hipEvent_t, hipEvent_t);                                             \
   template void launch_lu_step<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *, int, \
+                                  StepFwd,                                                                         \
                                   hipEvent_t, hipEvent_t);                                                            \
   template void launch_lu_solve<T>(hipStream_t, const CellParams *, const CellState *, const int *, int, const T *,   \
                                    const T *, const int *, int, const double *, double *, int, int);                  \

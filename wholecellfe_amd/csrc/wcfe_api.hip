@@ -294,8 +294,9 @@ struct LuPlan {
         const int w = std::min(nb - s0 - 1, p.bw);
         const long long npl = k >= 0 ? std::max(0, std::min(nb - 1, k + p.bw) - s0) : 0;
         const long long tiles = p.sym ? npl * (npl + 1) / 2 : npl * npl;
-        if (np == 2) return 1 + (p.sym ? 0 : w) + (tiles > 0 ? 1 : 0);  // panel item, U items, one for the tiles
-        return 1 + w + (p.sym ? 0 : w) + tiles;
+        // + 1: forward substitution of block s0 for the first solve
+        if (np == 2) return 2 + (p.sym ? 0 : w) + (tiles > 0 ? 1 : 0);  // panel item, U items, one for the tiles
+        return 2 + w + (p.sym ? 0 : w) + tiles;
       }
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
     }
@@ -520,6 +521,10 @@ struct Batch {
   // dense numbering: per node its position in the cell's dense system; the
   // permuted right-hand side and correction of the triangular solves
   int *kpos = nullptr;
+  // the first solve's forward sweep, run inside the fused block steps: per
+  // band row its batch dof (-1: padding) and the forward result
+  int *kinv = nullptr;
+  float *yfwd = nullptr;
   std::vector<std::vector<int>> kpos_h;
   double *rp = nullptr, *dp = nullptr;
   bool solve_wg = false;  // static pivoting: one workgroup per cell walks its band (narrow bands)
@@ -536,7 +541,7 @@ struct Batch {
     void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
                     sp_local, cp, cs, node_cell, adj_start, adj, Ge, g_start, g_col, g_bpos, g_cstart, g_ke, g_node, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
                     replica, transitions, plan.d_prefix, row_ptr, bcol, diag_pos, el_bpos, csr_cells, chunks, Kb,
-                    cgvec, cg, partials, kpos, rp, dp, bt_bidx, bt_node, bt_cst, crange, bt_cand, bt_frame, bt_g, bt_J, bt_area,
+                    cgvec, cg, partials, kpos, kinv, yfwd, rp, dp, bt_bidx, bt_node, bt_cst, crange, bt_cand, bt_frame, bt_g, bt_J, bt_area,
                     bt_gd, bt_fext};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -1006,6 +1011,17 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   HIPCHECK(hipMemset(b.ipiv, 0, sizeof(int) * std::max(sys, 1)));
   b.K = (void *)dalloc<unsigned char>((size_t)kp * elem_size);
   b.Dinv = (void *)dalloc<unsigned char>((size_t)std::max(sys, 1) * 128 * elem_size);
+  {
+    std::vector<int> kinv(std::max(sys, 1), -1);
+    for (int c = 0; c < b.ncell; ++c) {
+      const CellParams &p = b.cp_h[c];
+      if (p.csr) continue;
+      for (int i = 0; i < p.n_nodes; ++i)
+        for (int d = 0; d < 2; ++d) kinv[p.sys_off + 2 * b.kpos_h[c][i] + d] = 2 * (p.node_off + i) + d;
+    }
+    b.kinv = dupload(kinv);
+    b.yfwd = dalloc<float>(std::max(sys, 1));
+  }
   const size_t yb = lu_solve_exchange_bytes(std::max(sys, 1), elem_size);
   b.ysol = (void *)dalloc<unsigned char>(yb);
   HIPCHECK(hipMemset(b.ysol, 0, yb));
@@ -1227,7 +1243,7 @@ template <class T> void enqueue_assemble(wcfe_ctx *ctx, Batch &b, const Scope &s
 // Launch the LU stages of a plan for a list of cells.
 template <class T>
 void enqueue_lu(wcfe_ctx *ctx, hipStream_t s, const LuPlan &plan, const std::vector<Stage> &stages, const int *cells,
-                int ncell, const CellParams *cp, CellState *cs, T *K, T *Dinv, int *ipiv) {
+                int ncell, const CellParams *cp, CellState *cs, T *K, T *Dinv, int *ipiv, StepFwd fwd = StepFwd{}) {
   const bool piv = plan.pivoting != 0;
   {
     PhaseScope ph(ctx, WCFE_PH_LU_PANEL);
@@ -1274,7 +1290,7 @@ void enqueue_lu(wcfe_ctx *ctx, hipStream_t s, const LuPlan &plan, const std::vec
       case ST_STEP: {
         if (sizeof(T) != 4) fail(WCFE_ERR_STATE, "the fused block step is built for FP32 factors only");
         KernelTimer kt(ctx, WCFE_PH_LU_GEMM, g.work);
-        launch_lu_step<T>(s, cp, cs, wl, g.k, K, Dinv, g.np == 2 ? 1 : 0, kt.a, kt.b);
+        launch_lu_step<T>(s, cp, cs, wl, g.k, K, Dinv, g.np == 2 ? 1 : 0, fwd, kt.a, kt.b);
         break;
       }
     }
@@ -1292,13 +1308,14 @@ template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc)
       hipStream_t gs = ctx->side[g];
       HIPCHECK(hipStreamWaitEvent(gs, ctx->fork_ev, 0));
       enqueue_lu<T>(ctx, gs, b.plan, b.plan.grp[g], b.plan.d_prefix + b.plan.grp_cells_off[g], b.plan.grp_ncell[g],
-                    b.cp, b.cs, static_cast<T *>(b.K), static_cast<T *>(b.Dinv), b.ipiv);
+                    b.cp, b.cs, static_cast<T *>(b.K), static_cast<T *>(b.Dinv), b.ipiv,
+                    StepFwd{b.yfwd, b.kinv, b.G});
       HIPCHECK(hipEventRecord(ctx->join_ev[g], gs));
       HIPCHECK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[g], 0));
     }
   } else {
     enqueue_lu<T>(ctx, ctx->stream, b.plan, stages, sc.cells, sc.ncell, b.cp, b.cs, static_cast<T *>(b.K),
-                  static_cast<T *>(b.Dinv), b.ipiv);
+                  static_cast<T *>(b.Dinv), b.ipiv, StepFwd{b.yfwd, b.kinv, b.G});
   }
   const int *cl;
   int cb, nch, ncl;
@@ -1366,8 +1383,10 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
         // in each dense cell's own numbering: rp = P r, solve, delta (+)= P^T dp
         const LuPlan::SolveLists &L = sc.cell < 0 ? b.plan.solve_batched : b.plan.solve_single[sc.cell];
         if (b.solve_wg) {  // the sweep kernel reads r and writes delta through the numbering itself
+          // first solve after a fused factorization: its forward sweep ran in the block steps (r = -G)
+          const float *yf = (it == 0 && b.plan.fused) ? b.yfwd : nullptr;
           launch_lu_solve_band<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.kpos, b.r,
-                                  b.delta, it > 0, npad, b.max_bw);
+                                  b.delta, it > 0, npad, b.max_bw, yf);
         } else {
           launch_perm_gather(s, b.node_cell, b.cp, b.kpos, b.r, b.rp, sc.node_begin, sc.node_end);
           launch_lu_solve_mw<T>(s, b.cp, b.cs, sc.cells, sc.ncell, WorkList{b.plan.d_prefix + L.fwd_off, L.tot},

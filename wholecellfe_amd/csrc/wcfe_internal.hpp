@@ -106,9 +106,17 @@ void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const Wo
                     int next_diag, int npanel, int strip, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 // fused block step k (narrow bands, static pivoting): trailing update of
 // panel k, factor of diagonal block k + 1 and the block solves of panel k + 1
+// the forward substitution of the first solve inside the fused block steps:
+// y (band numbering, per cell at sys_off) = L^-1 P (-G); kinv: per band row
+// its batch dof (-1: padding); y null: no forward items work
+struct StepFwd {
+  float *y = nullptr;
+  const int *kinv = nullptr;
+  const double *G = nullptr;
+};
 template <class T>
 void launch_lu_step(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
-                    int panel, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
+                    int panel, StepFwd fwd, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 template <class T>
 void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                      const T *K, const T *Dinv, const int *ipiv, int pivoting, const double *r, double *delta,
@@ -131,7 +139,7 @@ size_t lu_solve_max_dynamic(size_t max_lds);
 template <class T>
 void launch_lu_solve_band(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                           const T *K, const T *Dinv, const int *kpos, const double *r, double *delta, int accumulate,
-                          int max_npad, int max_bw);
+                          int max_npad, int max_bw, const float *yfwd = nullptr);
 // block bandwidth up to which one workgroup per cell runs the triangular sweeps
 constexpr int SOLVE_WG_MAX_BW = 8;
 // concurrent cell groups of the dense LU chosen automatically for wide bands
