@@ -911,11 +911,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // 0 diagonal, 1 L block, 2 U block (stored as updated), 3 other, 4 every
   // other block of the window (panel mode: one item per cell)
   const int ntile = p.sym ? npl * (npl + 1) / 2 : npl * npl;
-  const int t_fwd = 1 + nsolve + (panel ? (ntile > 0 ? 1 : 0) : ntile);  // the last item
-  if (t == t_fwd) {
-    // forward substitution of block s0 for the first solve (right-hand side
-    // -G in the band numbering): y_s0 = (-G)_s0 - sum_j L_s0,j y_j over the
-    // band, with L_s0,j and y_j final since earlier steps; 4 lanes per row
+  // forward substitution of block s0 for the first solve (right-hand side
+  // -G in the band numbering): y_s0 = (-G)_s0 - sum_j L_s0,j y_j over the
+  // band, with L_s0,j and y_j final since earlier steps; 4 lanes per row.
+  // The last item of the cell; in panel mode the trailing-block item runs it
+  // after its blocks (no workgroup of its own in the many-cell launches).
+  auto forward_block = [&]() {
     if (!fwd.y) return;
     const T *A = K + p.k_off;
     const long long ld = p.npad;
@@ -934,6 +935,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       const int i = s0 * NB + r, dof = fwd.kinv[p.sys_off + i];
       y[i] = (dof >= 0 ? (float)(-fwd.G[dof]) : 0.0f) - part;
     }
+  };
+  const int t_last = 1 + nsolve + (panel ? 0 : ntile);
+  if (t == t_last && !(panel && ntile > 0)) {
+    forward_block();
     return;
   }
   int role, bi, bj;
@@ -960,6 +965,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       step_tile<T>(acc, A, ld, ti, tj, k, k >= 0 && ti <= reach && tj <= reach, ls, us);
       acc_store64<T, 0>(acc, A + (long long)(ti * NB) * ld + tj * NB, ld);
     }
+    forward_block();
     return;
   }
   if (role >= 2) {

@@ -294,8 +294,9 @@ struct LuPlan {
         const int w = std::min(nb - s0 - 1, p.bw);
         const long long npl = k >= 0 ? std::max(0, std::min(nb - 1, k + p.bw) - s0) : 0;
         const long long tiles = p.sym ? npl * (npl + 1) / 2 : npl * npl;
-        // + 1: forward substitution of block s0 for the first solve
-        if (np == 2) return 2 + (p.sym ? 0 : w) + (tiles > 0 ? 1 : 0);  // panel item, U items, one for the tiles
+        // the forward substitution of block s0 for the first solve: an item
+        // of its own, in panel mode with the trailing blocks when there are any
+        if (np == 2) return 2 + (p.sym ? 0 : w);  // panel item, U items, one for the tiles and the forward block
         return 2 + w + (p.sym ? 0 : w) + tiles;
       }
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
