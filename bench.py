@@ -178,8 +178,9 @@ def main():
                     help="only time a plain device copy of MIB MiB (HBM probe / counter reference) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (independent cells would be "
-              "partitioned across GPUs with no exchange, see DESIGN.md)" % a.gpus, file=sys.stderr)
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the cells of a batch are "
+              "independent systems: several GPUs would each take a share of them, with no exchange)" % a.gpus,
+              file=sys.stderr)
         sys.exit(2)
 
     import wholecellfe_amd as W
