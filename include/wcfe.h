@@ -199,7 +199,10 @@ int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path);
  * strain update, time += dt).  stats may be NULL (then nothing is copied back
  * and the call does not synchronise, unless the batch has CSR cells, whose
  * PCG checks convergence on the host every few iterations; such batches are
- * not captured in a graph). */
+ * not captured in a graph).  With stats, the step's errors are returned:
+ * WCFE_ERR_DEGENERATE, WCFE_ERR_ZERO_PIVOT, and WCFE_ERR_NOT_CONVERGED when a
+ * cell's linear solve missed lin_rtol within max_refine sweeps (stats are
+ * filled in every case; n_failed counts those cells). */
 int wcfe_newton_step(wcfe_ctx *ctx, wcfe_step_stats *stats);
 /* Newton iterations until every cell has committed one load step. */
 int wcfe_load_step(wcfe_ctx *ctx, int32_t *iterations, wcfe_step_stats *stats);
@@ -282,6 +285,11 @@ int wcfe_frame_get(wcfe_ctx *ctx, int32_t frame, int32_t cell, double *X /* 2N *
 int wcfe_frame_write_vtk(wcfe_ctx *ctx, int32_t frame, int32_t cell, const char *path, int32_t style,
                          const double *cA /* N */, int32_t law, double kint);
 int wcfe_get_springs(wcfe_ctx *ctx, int32_t cell, double *state /* n_spr */, double *anchor /* 2 n_spr */);
+/* Number of cells of the context and one cell's sizes (after wcfe_add_cell or
+ * wcfe_checkpoint_load; no wcfe_finalize needed): callers size the host
+ * buffers of the getters below from these. */
+int wcfe_num_cells(const wcfe_ctx *ctx, int32_t *n_cells);
+int wcfe_get_cell_sizes(wcfe_ctx *ctx, int32_t cell, int32_t *n_nodes, int32_t *n_tris, int32_t *n_spr);
 int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step, int32_t *newton_iter,
                        int32_t *n_colors, int32_t *ndof);
 /* FP64 residual G(x) = R_int(x) - F_spring(x) at the current positions (2N). */

@@ -83,11 +83,14 @@ template <class Mesh> std::vector<int32_t> flat_tris(const Mesh &mesh) {
 // strain increment and lumped mass of the cell law (mat = 1) or the
 // substrate law (mat = 2) on the GPU.
 // Res_Adif / Res_Cdif (diffusion terms the reference computes but never uses,
-// UL:617-618) are returned as zero vectors.
-template <class Vec3, class VecX, class Mesh>
+// UL:617-618) are returned as zero vectors.  `centroid` takes the reference's
+// Eigen::Vector3d (UL:632) unchanged; its value is not read: it only sets the
+// fibre direction a0 of the active stress t_act (kappa I + (1 - 2 kappa) a0 a0),
+// and kappa = 0.5 (UL:872) makes that term vanish.
+template <class Vec3, class VecX, class Mesh, class Centroid>
 void calculateRes(const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t, std::vector<VecX> &ea_t,
                   const std::vector<double> &c_A, const std::vector<double> &c_C, const Mesh &mesh, double dt_real,
-                  VecX &Res, VecX &Res_ea, VecX &MM, VecX &Res_Adif, VecX &Res_Cdif, int mat, const void *centroid,
+                  VecX &Res, VecX &Res_ea, VecX &MM, VecX &Res_Adif, VecX &Res_Cdif, int mat, const Centroid &centroid,
                   double time) {
   (void)c_A; (void)c_C; (void)dt_real; (void)centroid;
   const int n = (int)X_t.size();
@@ -119,7 +122,8 @@ void calculateRes(const std::vector<Vec3> &X_t, const std::vector<Vec3> &x_t, st
                   const Mesh &mesh, double dt_real, VecX &Res, VecX &Res_ea, VecX &MM, int mat, double time) {
   VecX RA, RC;
   const std::vector<double> none(X_t.size(), 0.0);
-  calculateRes(X_t, x_t, ea_t, none, none, mesh, dt_real, Res, Res_ea, MM, RA, RC, mat, nullptr, time);
+  const double no_centroid[3] = {0.0, 0.0, 0.0};
+  calculateRes(X_t, x_t, ea_t, none, none, mesh, dt_real, Res, Res_ea, MM, RA, RC, mat, no_centroid, time);
 }
 
 // calculateForce_FA (UL_growth_explicit.cpp:1198 / UL1:1196).

@@ -81,7 +81,8 @@ int main(int argc, char **argv) {
       }
       std::vector<double> cA(X.size(), 0.0), cC(X.size(), 0.0);
       VecX Res, Rea, MM, RA, RC;
-      wcfe_compat::calculateRes(X, x, ea, cA, cC, m, meta[1], Res, Rea, MM, RA, RC, 1, nullptr, meta[0]);
+      Vec3 centroid = {{0.0, 0.0, 0.0}};  // stands for the reference's Eigen::Vector3d lvalue (UL:543)
+      wcfe_compat::calculateRes(X, x, ea, cA, cC, m, meta[1], Res, Rea, MM, RA, RC, 1, centroid, meta[0]);
       store("Res", Res.d);
       store("Res_ea", Rea.d);
       store("MM", MM.d);

@@ -231,7 +231,7 @@ def test_node_terms_second_load_step_draws_new_ap_stream(tmp_path):
     assert it >= 0
     assert np.max(np.abs(x2 - x_o)) <= 1e-8 * np.max(np.abs(x_o - X1))
     h = W.Context(precision=W.PREC_FP64, t0=1.0, dt=0.1, newton_rtol=1e-11, newton_atol=5e-8, max_refine=16)
-    h.checkpoint_load(path, [(len(d["X"]) // 2, len(d["tris"]) // 3, 0)])
+    h.checkpoint_load(path)
     h.finalize()
     h.load_step()
     assert np.array_equal(h.positions(0), x2)

@@ -135,7 +135,7 @@ def test_set_external_forces_and_checkpoint(tmp_path):
     assert it >= 0
     assert np.max(np.abs(x2 - x_o)) <= 1e-6 * np.max(np.abs(x_o - X1))
     h = W.Context(precision=W.PREC_MIXED, t0=1.0, dt=DT, newton_rtol=1e-11, newton_atol=1e-6, max_refine=16)
-    h.checkpoint_load(path, [(len(d["X"]) // 2, len(d["tris"]) // 3, 0)])
+    h.checkpoint_load(path)
     h.finalize()
     h.load_step()
     assert np.array_equal(h.positions(0), x2)

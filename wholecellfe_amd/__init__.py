@@ -36,7 +36,7 @@ EXPORTS = [
     "wcfe_write_vtk", "wcfe_frame_capture", "wcfe_frame_get", "wcfe_frame_write_vtk", "wcfe_profile_step",
     "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state", "wcfe_replace_cell", "wcfe_checkpoint_save",
     "wcfe_checkpoint_load", "wcfe_area_tot", "wcfe_bending_force_2d", "wcfe_area_constraint_force",
-    "wcfe_force_ap", "wcfe_get_schedule",
+    "wcfe_force_ap", "wcfe_get_schedule", "wcfe_num_cells", "wcfe_get_cell_sizes",
 ]
 TUNE_SOLVER, TUNE_LU = 1, 2
 # reference constants of the boundary and body forces (UL_growth_explicit.cpp)
@@ -214,11 +214,21 @@ class Context:
     def checkpoint_save(self, path):
         self._check(self._L.wcfe_checkpoint_save(self._h, str(path).encode()))
 
-    def checkpoint_load(self, path, sizes):
-        """Load a checkpoint into this (empty) context; `sizes` lists
-        (n_nodes, n_tris, n_spr) per cell for the host-side bookkeeping."""
+    def checkpoint_load(self, path):
+        """Load a checkpoint into this (empty) context; the per-cell sizes
+        that size the getters' buffers come from the library."""
         self._check(self._L.wcfe_checkpoint_load(self._h, str(path).encode()))
-        self.cells = [dict(n_nodes=a, n_tris=b, n_spr=c) for a, b, c in sizes]
+        self.cells = self._library_sizes()
+
+    def _library_sizes(self):
+        n = C.c_int32(0)
+        self._check(self._L.wcfe_num_cells(self._h, C.byref(n)))
+        out = []
+        for c in range(n.value):
+            a, b, s = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+            self._check(self._L.wcfe_get_cell_sizes(self._h, c, C.byref(a), C.byref(b), C.byref(s)))
+            out.append(dict(n_nodes=a.value, n_tris=b.value, n_spr=s.value))
+        return out
 
     def _desc(self, X, tris, law=LAW_CST, x=None, mu=1000.0, E=1000.0, nu=0.3, thick=0.2, ea=None,
               node_springs=0, cC=None, fau=None, rho_max=100.0, spr_tri=None, spr_bary=None, spr_anchor=None,
@@ -257,13 +267,20 @@ class Context:
         self._check(self._L.wcfe_finalize(self._h))
         self._keep = []
 
-    def newton_step(self, stats=True):
+    def newton_step(self, stats=True, check=True):
+        """One batched Newton iteration.  With stats, the step's statistics;
+        check=False returns them (with "error": code) instead of raising when
+        the step reports an error (e.g. a linear solve that missed lin_rtol)."""
         if not stats:
             self._check(self._L.wcfe_newton_step(self._h, None))
             return None
         st = StepStats()
-        self._check(self._L.wcfe_newton_step(self._h, C.byref(st)))
-        return st.as_dict()
+        rc = self._L.wcfe_newton_step(self._h, C.byref(st))
+        if check:
+            self._check(rc)
+        d = st.as_dict()
+        d["error"] = rc
+        return d
 
     def load_step(self):
         st = StepStats()

@@ -1490,13 +1490,14 @@ void collect_stats(wcfe_ctx *ctx, wcfe_step_stats *st, int *err_code) {
   wcfe_step_stats s;
   memset(&s, 0, sizeof s);
   s.n_cells = b.ncell;
-  int degenerate = -1, zero = -1, stall = -1;
+  int degenerate = -1, zero = -1, stall = -1, missed = -1;
   for (int c = 0; c < b.ncell; ++c) {
     const CellState &q = cs[c];
     s.n_converged += q.converged;
     s.max_refine_used = std::max(s.max_refine_used, q.ir_iters);
     s.n_static_pivots += q.n_static;
     s.n_failed += q.ir_done ? 0 : 1;
+    if (!q.ir_done && missed < 0) missed = c;
     s.max_lin_relres = std::max(s.max_lin_relres, q.lin_relres);
     const double rel = q.g0 > 0 ? q.gnorm / q.g0 : 0.0;
     s.max_newton_rel = std::max(s.max_newton_rel, rel);
@@ -1517,6 +1518,15 @@ void collect_stats(wcfe_ctx *ctx, wcfe_step_stats *st, int *err_code) {
   } else if (stall >= 0) {
     *err_code = WCFE_ERR_HIP;
     ctx->err = "cell " + std::to_string(stall) + ": triangular-solve wait exceeded its bound";
+  } else if (missed >= 0) {
+    // the Newton update of such a cell is not the Newton step (the host may
+    // go on, e.g. with more refinement sweeps, but is told)
+    *err_code = WCFE_ERR_NOT_CONVERGED;
+    char buf[160];
+    snprintf(buf, sizeof buf, "cell %d: linear solve stopped at relative residual %.3e > lin_rtol %.1e after %d "
+             "refinement sweeps (%d cells)", missed, cs[missed].lin_relres, ctx->opt.lin_rtol, cs[missed].ir_iters,
+             s.n_failed);
+    ctx->err = buf;
   }
 }
 
@@ -1663,6 +1673,9 @@ namespace {
 HostCell host_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d) {
     if (d->n_nodes <= 0 || d->n_tris <= 0 || !d->X || !d->tris) fail(WCFE_ERR_ARG, "empty mesh");
     if (d->law != WCFE_LAW_UL && d->law != WCFE_LAW_CST) fail(WCFE_ERR_ARG, "unknown law %d", d->law);
+    if (d->spring_law != WCFE_SPRING_LINEAR && d->spring_law != WCFE_SPRING_TABLE)
+      fail(WCFE_ERR_ARG, "unknown spring law %d", d->spring_law);
+    if (d->kinetics < WCFE_KIN_NONE || d->kinetics > WCFE_KIN_BELL) fail(WCFE_ERR_ARG, "unknown kinetics %d", d->kinetics);
     if (d->node_springs && d->law != WCFE_LAW_UL)
       fail(WCFE_ERR_ARG, "node-attached springs follow the reference model and need the UL law");
     if (d->node_springs && (!d->cC || !d->fau)) fail(WCFE_ERR_ARG, "node springs need cC and fau");
@@ -2296,6 +2309,24 @@ int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step,
   });
 }
 
+int wcfe_num_cells(const wcfe_ctx *ctx, int32_t *n_cells) {
+  if (!ctx || !n_cells) return WCFE_ERR_ARG;
+  *n_cells = (int32_t)ctx->cells.size();
+  return WCFE_OK;
+}
+
+int wcfe_get_cell_sizes(wcfe_ctx *ctx, int32_t cell, int32_t *n_nodes, int32_t *n_tris, int32_t *n_spr) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (cell < 0 || cell >= (int)ctx->cells.size())
+      fail(WCFE_ERR_ARG, "cell %d out of range [0, %d)", cell, (int)ctx->cells.size());
+    const HostCell &h = ctx->cells[cell];
+    if (n_nodes) *n_nodes = h.n_nodes;
+    if (n_tris) *n_tris = h.n_tris;
+    if (n_spr) *n_spr = h.n_spr;
+  });
+}
+
 int wcfe_residual_fp64(wcfe_ctx *ctx, int32_t cell, double *G) {
   if (!ctx || !G) return WCFE_ERR_ARG;
   return guarded(ctx, [&] {
@@ -2912,9 +2943,12 @@ int wcfe_force_ap(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t 
 
 namespace {
 const char CKP_MAGIC[8] = {'W', 'C', 'F', 'E', 'C', 'K', 'P', '1'};
-// version 2 adds the boundary and body force parameters after the scalars,
-// version 3 the substrate form (passive flag, constant nodal load)
-const int CKP_VERSION = 3;
+// version 2 added the boundary and body force parameters after the scalars,
+// version 3 the substrate form (passive flag, constant nodal load), version 4
+// the size of the per-cell solver state record (stored as raw bytes): a file
+// whose record size differs from this build's is refused.  Versions 1-3 carry
+// no record size and are refused.
+const int CKP_VERSION = 4;
 struct CkpWriter {
   FILE *f;
   void raw(const void *p, size_t n) {
@@ -2974,6 +3008,7 @@ int wcfe_checkpoint_save(wcfe_ctx *ctx, const char *path) {
     try {
       w.raw(CKP_MAGIC, 8);
       w.i32(CKP_VERSION);
+      w.i32((int)sizeof(CellState));
       w.i32((int)ctx->cells.size());
       w.i32((int)ctx->tables.size());
       for (auto &kv : ctx->tables) {
@@ -3029,7 +3064,12 @@ int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path) {
       r.raw(magic, 8);
       if (memcmp(magic, CKP_MAGIC, 8) != 0) fail(WCFE_ERR_ARG, "%s is not a wcfe checkpoint", path);
       const int version = r.i32();
-      if (version < 1 || version > 3) fail(WCFE_ERR_ARG, "unsupported checkpoint version %d", version);
+      if (version != CKP_VERSION)
+        fail(WCFE_ERR_ARG, "unsupported checkpoint version %d (this build reads version %d)", version, CKP_VERSION);
+      const int state_bytes = r.i32();
+      if (state_bytes != (int)sizeof(CellState))
+        fail(WCFE_ERR_ARG, "checkpoint solver-state record is %d bytes, this build's is %d", state_bytes,
+             (int)sizeof(CellState));
       const int ncell = r.i32(), ntab = r.i32();
       if (ncell < 1 || ncell >= (1 << 11) || ntab < 0 || ntab > 1024) fail(WCFE_ERR_ARG, "bad checkpoint header");
       for (int t = 0; t < ntab; ++t) {
@@ -3048,16 +3088,11 @@ int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path) {
         for (int *v : iv) *v = r.i32();
         double *dv[] = {&h.mu, &h.E, &h.nu, &h.thick, &h.rho_max, &h.k_lin, &h.time};
         for (double *v : dv) *v = r.f64();
-        if (version >= 2) {
-          h.ap = r.i32();
-          double *bv[] = {&h.k_bend, &h.k_area, &h.area_0, &h.drag, &h.ap_rate, &h.ap_quantum, &h.ap_density};
-          for (double *v : bv) *v = r.f64();
-        }
-        int has_ext = 0;
-        if (version >= 3) {
-          h.passive = r.i32() ? 1 : 0;
-          has_ext = r.i32();
-        }
+        h.ap = r.i32();
+        double *bv[] = {&h.k_bend, &h.k_area, &h.area_0, &h.drag, &h.ap_rate, &h.ap_quantum, &h.ap_density};
+        for (double *v : bv) *v = r.f64();
+        h.passive = r.i32() ? 1 : 0;
+        const int has_ext = r.i32();
         if (h.n_nodes <= 0 || h.n_tris <= 0 || h.n_spr < 0) fail(WCFE_ERR_ARG, "bad cell sizes in checkpoint");
         const long long N = h.n_nodes, T = h.n_tris, S = h.n_spr;
         if (has_ext) r.vec(h.f_ext, 2 * N);
@@ -3072,11 +3107,6 @@ int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path) {
         r.vec(h.spr_anchor, 2 * S);
         r.vec(h.spr_state, S);
         r.raw(&h.state, sizeof(CellState));
-        h.has_state = true;
-        for (int v : h.tris)
-          if (v < 0 || v >= h.n_nodes) fail(WCFE_ERR_ARG, "checkpoint triangle index out of range");
-        for (int t : h.spr_tri)
-          if (t < 0 || t >= h.n_tris) fail(WCFE_ERR_ARG, "checkpoint spring triangle out of range");
         cells.push_back(std::move(h));
       }
     } catch (...) {
@@ -3084,8 +3114,38 @@ int wcfe_checkpoint_load(wcfe_ctx *ctx, const char *path) {
       throw;
     }
     fclose(f);
+    // every cell passes the checks wcfe_add_cell applies (enumerations, index
+    // ranges, table ids, parameter signs) before the context takes the file
+    std::map<int, Table> keep_tables;
+    std::swap(keep_tables, ctx->tables);
     ctx->tables = tables;
-    ctx->cells = std::move(cells);
+    std::vector<HostCell> valid;
+    try {
+      for (HostCell &h : cells) {
+        wcfe_cell_desc d;
+        memset(&d, 0, sizeof d);
+        d.n_nodes = h.n_nodes; d.n_tris = h.n_tris;
+        d.X = h.X.data(); d.x = h.x.data(); d.tris = h.tris.data();
+        d.law = h.law; d.mu = h.mu; d.E = h.E; d.nu = h.nu; d.thick = h.thick; d.ea = h.ea.data();
+        d.node_springs = h.node_springs; d.cC = h.cC.data(); d.fau = h.fau.data(); d.rho_max = h.rho_max;
+        d.n_spr = h.n_spr; d.spr_tri = h.spr_tri.data(); d.spr_bary = h.spr_bary.data();
+        d.spr_anchor = h.spr_anchor.data(); d.spr_state = h.spr_state.data();
+        d.spring_law = h.spring_law; d.k_lin = h.k_lin; d.table_id = h.table_id; d.kinetics = h.kinetics;
+        d.solver = h.solver; d.k_bend = h.k_bend; d.k_area = h.k_area; d.area_0 = h.area_0; d.drag = h.drag;
+        d.ap = h.ap; d.ap_rate = h.ap_rate; d.ap_quantum = h.ap_quantum; d.ap_density = h.ap_density;
+        d.passive = h.passive; d.f_ext = h.f_ext.empty() ? nullptr : h.f_ext.data();
+        HostCell v = host_cell(ctx, &d);
+        v.time = h.time;
+        v.step = h.step;
+        v.state = h.state;
+        v.has_state = true;
+        valid.push_back(std::move(v));
+      }
+    } catch (...) {
+      ctx->tables = std::move(keep_tables);
+      throw;
+    }
+    ctx->cells = std::move(valid);
   });
 }
 
