@@ -233,6 +233,11 @@ int wcfe_get_schedule(wcfe_ctx *ctx, int32_t *lu_depth, int32_t *lu_groups, int3
  * HBM bandwidth probe and the known-size reference for byte counters. */
 int wcfe_copy_probe(wcfe_ctx *ctx, int64_t bytes, int32_t reps, double *ms_per_copy);
 
+/* Diagnostics: the library's FP64 exp (the one the bond kinetics use:
+ * Cody-Waite reduction, degree-13 Taylor polynomial in fused multiply-adds,
+ * within 1 ulp of exp) evaluated on the device over n host values. */
+int wcfe_eval_exp(wcfe_ctx *ctx, int64_t n, const double *x, double *y);
+
 /* Phases of a Newton step, for wcfe_profile_step. */
 #define WCFE_PH_FILL 0     /* zero the dense systems and residuals */
 #define WCFE_PH_ASSEMBLE 1 /* colour-ordered element assembly of K and G (work = elements) */

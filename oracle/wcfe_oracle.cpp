@@ -9,7 +9,15 @@
 // independent derivation from the GPU kernels' hand-derived tangents.
 //
 // Compiled with -ffp-contract=off: the reference build had no FMA.
+//
+// exp: the kinetics (mean-field k_off, Bell probabilities, actin
+// polymerisation) use the library's shared exp (wcfe_exp.hpp: Cody-Waite
+// reduction and a Taylor polynomial in fused multiply-adds), the one the GPU
+// kinetics use, so both round every bond decision identically; it is pinned
+// against the C library's exp to 1 ulp (tests/test_oracle_pins.py).
 #include "wcfe_oracle.h"
+
+#include "../wholecellfe_amd/csrc/wcfe_exp.hpp"
 
 #include <math.h>
 #include <stdlib.h>
@@ -459,7 +467,7 @@ extern "C" void orc_update_fa(int n_nodes, int n_tris, const int32_t *tris, cons
       force_FA = ::sqrt(fs[0] * fs[0] + fs[1] * fs[1] + fs[2] * fs[2]);
     }
     const double k_on = 1, ka = 0.004, kb = 10, Fa = 15, Fb = 15;  // UL:2139-2143
-    double k_off = ka * ::exp(force_FA / Fa) + kb * ::exp(-force_FA / Fb);
+    double k_off = ka * wcfe::shared_exp(force_FA / Fa) + kb * wcfe::shared_exp(-force_FA / Fb);
     double c_C_old = cC[ni];
     cC[ni] = cC[ni] + dt * k_on * (1 - cC[ni]) - dt * k_off * cC[ni];
     if (cC[ni] < 0) cC[ni] = 0;
@@ -701,7 +709,7 @@ extern "C" int orc_force_ap(int n_nodes, int n_tris, const int32_t *tris, const 
     // adds a quantum when the draw is below rate dt exp(-rate dt)
     double factin = 0;
     for (int ii = 0; ii < density * L / 2; ii++)
-      if (u[ni] < rate * dt * ::exp(-rate * dt)) factin += quantum;
+      if (u[ni] < rate * dt * wcfe::shared_exp(-rate * dt)) factin += quantum;
     for (int c = 0; c < 3; ++c) F[3 * ni + c] = factin * n[c];
   }
   return open;
@@ -1167,6 +1175,10 @@ extern "C" void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_i
   for (int i = 0; i < 4; ++i) out[i] = c[i];
 }
 
+extern "C" void orc_exp_array(int n, const double *x, double *y) {
+  for (int i = 0; i < n; ++i) y[i] = wcfe::shared_exp(x[i]);
+}
+
 extern "C" double orc_uniform(uint32_t key_lo, uint32_t key_hi, uint32_t a, uint32_t b, uint32_t cc) {
   uint32_t ctr[4] = {a, b, cc, 0}, key[2] = {key_lo, key_hi}, o[4];
   orc_philox4x32_10(ctr, key, o);
@@ -1190,11 +1202,11 @@ extern "C" int orc_bell_update(const orc_cell *c, double *state, double *anchor,
       double d0 = p[0] - anchor[2 * s], d1 = p[1] - anchor[2 * s + 1];
       double D = ::sqrt(d0 * d0 + d1 * d1), f, df;
       orc_spring_law(c, D, &f, &df);
-      double koff = k_off0 * ::exp(fabs(f) / f_b);
-      prob = 1.0 - ::exp(-koff * dt);
+      double koff = k_off0 * wcfe::shared_exp(fabs(f) / f_b);
+      prob = 1.0 - wcfe::shared_exp(-koff * dt);
       if (u < prob) { state[s] = 0.0; ++transitions; }
     } else {
-      prob = 1.0 - ::exp(-k_on * dt);
+      prob = 1.0 - wcfe::shared_exp(-k_on * dt);
       if (u < prob) {
         state[s] = 1.0;
         anchor[2 * s] = p[0];

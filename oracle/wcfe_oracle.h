@@ -156,6 +156,9 @@ int orc_pcg_csr(int n, const int32_t *rowptr, const int32_t *col, const double *
  * res_hist (optional, maxit+1 entries) receives ||G|| per iterate. */
 int orc_newton_solve(orc_cell *c, double *x, double rtol, double atol, int maxit, double *res_hist);
 
+/* The shared FP64 exp of the kinetics (wholecellfe_amd/csrc/wcfe_exp.hpp). */
+void orc_exp_array(int n, const double *x, double *y);
+
 /* Philox4x32-10 (Salmon et al., SC'11). */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* Uniform in (0,1) from the first word of Philox(ctr=(a,b,c,0), key). */

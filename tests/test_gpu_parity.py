@@ -430,8 +430,25 @@ def test_bell_kinetics_bit_exact():
         transitions += nt
         assert np.array_equal(st1, st_o), step
         assert np.array_equal(an1, an_o), step
+        # no decision within rounding distance of its probability: the states
+        # would agree even with an exp that differed in the last bits
+        assert margin > 1e-13, (step, margin)
     g.close()
     assert transitions > 0
+
+
+def test_gpu_exp_equals_cpu_exp_bitwise():
+    """The kinetics' exp on the GPU and the same function compiled for the CPU
+    (oracle) round identically, over the kinetics' arguments and the whole
+    range."""
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(-50.0, 50.0, 200000), rng.uniform(-745.0, 709.7, 200000),
+                        np.array([0.0, -0.0, 1e-300, 710.5, -746.5, np.inf, -np.inf])])
+    with W.Context() as g:
+        y = g.eval_exp(x)
+    y_cpu = np.zeros_like(x)
+    O.lib().orc_exp_array(len(x), O.dp(x), O.dp(y_cpu))
+    assert np.array_equal(y.view(np.int64), y_cpu.view(np.int64))
 
 
 def test_degenerate_element_is_reported():

@@ -37,6 +37,7 @@ EXPORTS = [
     "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state", "wcfe_replace_cell", "wcfe_checkpoint_save",
     "wcfe_checkpoint_load", "wcfe_area_tot", "wcfe_bending_force_2d", "wcfe_area_constraint_force",
     "wcfe_force_ap", "wcfe_get_schedule", "wcfe_num_cells", "wcfe_get_cell_sizes",
+    "wcfe_eval_exp",
 ]
 TUNE_SOLVER, TUNE_LU = 1, 2
 # reference constants of the boundary and body forces (UL_growth_explicit.cpp)
@@ -326,6 +327,13 @@ class Context:
         ms = C.c_double(0)
         self._check(self._L.wcfe_copy_probe(self._h, C.c_int64(nbytes), reps, C.byref(ms)))
         return ms.value
+
+    def eval_exp(self, x):
+        """The library's FP64 exp (the kinetics' exp), evaluated on the GPU."""
+        x = _f64(x).reshape(-1)
+        y = np.zeros_like(x)
+        self._check(self._L.wcfe_eval_exp(self._h, C.c_int64(len(x)), _dp(x), _dp(y)))
+        return y
 
     def synchronize(self):
         self._check(self._L.wcfe_synchronize(self._h))

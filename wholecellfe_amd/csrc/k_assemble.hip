@@ -527,6 +527,7 @@ __global__ __launch_bounds__(256) void k_commit_nodes(View v, const int *node_ce
     u.x = u.x + (xi.x - Xi.x);
     u.y = u.y + (xi.y - Xi.y);
     if (p.kinetics == KIN_MEANFIELD) {
+#pragma clang fp contract(off)
       double force;
       if (p.spring_law == SPRING_LINEAR) {
         force = p.k_lin * sqrt(u.x * u.x + u.y * u.y);  // 2D norm, UL1:2077
@@ -536,7 +537,7 @@ __global__ __launch_bounds__(256) void k_commit_nodes(View v, const int *node_ce
         force = sqrt(f[0] * f[0] + f[1] * f[1]);  // UL:2137
       }
       const double k_on = 1.0, ka = 0.004, kb = 10.0, Fa = 15.0, Fb = 15.0;  // UL:2139-2143
-      const double k_off = ka * exp(force / Fa) + kb * exp(-force / Fb);
+      const double k_off = ka * shared_exp(force / Fa) + kb * shared_exp(-force / Fb);
       const double cold = cC[i];
       double cnew = cold + dt * k_on * (1.0 - cold) - dt * k_off * cold;
       if (cnew < 0.0) cnew = 0.0;
@@ -564,7 +565,8 @@ void launch_commit_nodes(hipStream_t s, const View &v, const int *node_cell, dou
 }
 
 // Bell kinetics per barycentric spring after a converged load step:
-// bound -> unbound with p = 1 - exp(-k_off0 e^{|F|/f_b} dt), unbound -> bound
+// bound -> unbound with p = 1 - exp(-k_off0 e^{|F(D)|/f_b} dt), D = |u|,
+// unbound -> bound
 // with p = 1 - exp(-k_on dt), re-anchored at the current attachment point.
 // Uniform u = (w + 0.5) 2^-32, w = Philox(ctr = (replica, spring, step, 0)).
 __global__ __launch_bounds__(256) void k_commit_springs(View v, const int *sp_cell, const int *sp_elem,
@@ -594,18 +596,23 @@ __global__ __launch_bounds__(256) void k_commit_springs(View v, const int *sp_ce
   unsigned ctr[4] = {(unsigned)replica_of_cell[c], (unsigned)sp_local[s], (unsigned)cst.step, 0u};
   philox4x32_10(ctr, key_lo, key_hi);
   const double u = ((double)ctr[0] + 0.5) * (1.0 / 4294967296.0);
+  // the force and both probabilities rounded as written, with the shared exp
   if (state[s] != 0.0) {
-    const double ux = px - anchor[2 * s], uy = py - anchor[2 * s + 1];
-    double f[2], J[4];
-    spring_force(v, p, ux, uy, f, J);
-    const double F = sqrt(f[0] * f[0] + f[1] * f[1]);
-    const double prob = 1.0 - exp(-k_off0 * exp(F / f_b) * dt);
+    double ux, uy;
+    {
+#pragma clang fp contract(off)
+      ux = px - anchor[2 * s];
+      uy = py - anchor[2 * s + 1];
+    }
+    const double F = spring_magnitude(v, p, ux, uy);
+    const double koff = k_off0 * shared_exp(F / f_b);
+    const double prob = 1.0 - shared_exp(-koff * dt);
     if (u < prob) {
       state[s] = 0.0;
       atomicAdd(transitions, 1);
     }
   } else {
-    const double prob = 1.0 - exp(-k_on * dt);
+    const double prob = 1.0 - shared_exp(-k_on * dt);
     if (u < prob) {
       state[s] = 1.0;
       anchor[2 * s] = px;

@@ -107,7 +107,7 @@ __device__ double bend_scale(const CurveGeom &g, double k_bend) {
 // count is capped at 2^24 so that non-finite positions cannot spin the loop.
 __device__ double ap_scale(const CurveGeom &g, double u, double dt, double rate, double quantum, double density) {
 #pragma clang fp contract(off)
-  const double L = g.l1 + g.l2, lim = density * L / 2, pf = rate * dt * exp(-rate * dt);
+  const double L = g.l1 + g.l2, lim = density * L / 2, pf = rate * dt * shared_exp(-rate * dt);
   const int trials = lim > 0 ? (int)fmin(ceil(lim), 16777216.0) : 0;
   double f = 0;
   if (u < pf)

@@ -100,6 +100,7 @@ __global__ __launch_bounds__(256) void k_ref_update_fa(int n_nodes, const double
                                                        double *cC, double *fau, double dt, PointGrid grid,
                                                        const int *stris, const double *SX, const double *Sx, int law,
                                                        double kint, int *n_unlocated) {
+#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_nodes) return;
   double cell_u[3], sX[3], subs_u[3] = {0.0, 0.0, 0.0};
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void k_ref_update_fa(int n_nodes, const double
     force = sqrt(fs[0] * fs[0] + fs[1] * fs[1] + fs[2] * fs[2]);
   }
   const double k_on = 1.0, ka = 0.004, kb = 10.0, Fa = 15.0, Fb = 15.0;  // UL:2139-2143
-  const double k_off = ka * exp(force / Fa) + kb * exp(-force / Fb);
+  const double k_off = ka * shared_exp(force / Fa) + kb * shared_exp(-force / Fb);
   const double cold = cC[i];
   double cnew = cold + dt * k_on * (1.0 - cold) - dt * k_off * cold;
   if (cnew < 0.0) cnew = 0.0;
@@ -242,6 +243,17 @@ void launch_dense_residual(hipStream_t s, const CellParams *cp, const CellState 
                            const double *A64, const long long *a_off, const double *b, const double *x, double *r) {
   if (ncell <= 0) return;
   hipLaunchKernelGGL(k_dense_residual, dim3(ncell), dim3(256), 0, s, cp, cs, cells, A64, a_off, b, x, r);
+}
+
+// The shared FP64 exp over an array (diagnostics: bit-for-bit comparison with
+// the same function compiled for the CPU).
+__global__ __launch_bounds__(256) void k_eval_exp(long long n, const double *x, double *y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = shared_exp(x[i]);
+}
+void launch_eval_exp(hipStream_t s, long long n, const double *x, double *y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_eval_exp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, y);
 }
 
 }  // namespace wcfe

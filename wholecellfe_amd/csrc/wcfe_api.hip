@@ -2675,6 +2675,21 @@ void check_mesh(int n_nodes, int n_tris, const int32_t *tris) {
 
 extern "C" {
 
+int wcfe_eval_exp(wcfe_ctx *ctx, int64_t n, const double *x, double *y) {
+  if (!ctx || n < 0 || n > (1LL << 28) || (n > 0 && (!x || !y))) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    HIPCHECK(hipSetDevice(ctx->opt.device));
+    if (n == 0) return;
+    Scratch sc;
+    const double *dx = sc.up(x, (size_t)n);
+    double *dy = sc.zeros<double>((size_t)n);
+    launch_eval_exp(ctx->stream, n, dx, dy);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy(y, dy, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+  });
+}
+
 int wcfe_dense_solve(wcfe_ctx *ctx, int32_t n_sys, const int32_t *n, const double *A, const double *b, double *x,
                      double *relres, int32_t *iters, int32_t *ipiv_out) {
   if (!ctx || n_sys <= 0 || !n || !A || !b || !x) return WCFE_ERR_ARG;

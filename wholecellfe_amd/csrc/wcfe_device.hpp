@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "wcfe_exp.hpp"
+
 namespace wcfe {
 
 constexpr int NB = 64;          // dense LU block size
@@ -175,6 +177,25 @@ __device__ __forceinline__ void spring_force(const View &v, const CellParams &p,
   J[1] = (dF - s) * nx * ny;
   J[2] = J[1];
   J[3] = dF * ny * ny + s * (1.0 - ny * ny);
+}
+
+// Force magnitude |F(D)| of a spring at extension u, D = |u|, with every
+// operation rounded as written (no contraction) and the table segment rule of
+// UL:1258-1264 (segment j for px_j <= D < px_{j+1}, the last one
+// extrapolating): the force the Bell off-rate sees, rounded as a plain FP64
+// CPU evaluation of the same expressions rounds it.
+__device__ inline double spring_magnitude(const View &v, const CellParams &p, double ux, double uy) {
+#pragma clang fp contract(off)
+  const double D = sqrt(ux * ux + uy * uy);
+  if (p.spring_law == SPRING_LINEAR) return fabs(p.k_lin * D);
+  const double *px = v.tab_px + p.table_off, *py = v.tab_py + p.table_off, *kk = v.tab_k + p.table_off;
+  int lo = 0, hi = p.table_n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (px[mid] <= D) lo = mid + 1; else hi = mid;
+  }
+  const int j = lo > 0 ? lo - 1 : 0;
+  return fabs(py[j] + kk[j] * (D - px[j]));
 }
 
 // Element residual contribution Ge[6] and (if K) tangent Ke[36] (row-major,

@@ -177,6 +177,7 @@ void launch_pcg_finish(hipStream_t s, const int4 *chunks, int nchunk, const Cell
                        int accumulate);
 
 // ---- reference-shaped stateless kernels (k_reference.hip) ----
+void launch_eval_exp(hipStream_t s, long long n, const double *x, double *y);
 void launch_ref_force_fa(hipStream_t s, int n_nodes, const int *ring_start, const int *ring, const int *tris,
                          const double *x, const double *cC, const double *fau, int law, double kint, double *F);
 // uniform grid over substrate triangles (point location); list[start[c] ..
