@@ -22,6 +22,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include <algorithm>
 #include <map>
@@ -1248,4 +1249,304 @@ extern "C" void orc_commit_ul_strain(const orc_cell *c, double *ea_out) {
     ea_out[3 * i + 1] = c->ea[3 * i + 1] + Rea[6 * i + 1] / MM[i];
     ea_out[3 * i + 2] = c->ea[3 * i + 2] + Rea[6 * i + 3] / MM[i];
   }
+}
+
+// ==========================================================================
+// Part 2, sparse forms: CSR assembly, Newton with Jacobi-PCG, and one Newton
+// step with a reverse Cuthill-McKee band LU (the CPU baseline's solver: the
+// same numbering idea and factorization the GPU's band LU uses, in FP64)
+// ==========================================================================
+namespace {
+
+std::vector<std::vector<int>> node_neighbours(int n_nodes, int n_tris, const int32_t *tris) {
+  std::vector<std::vector<int>> nb(n_nodes);
+  for (int i = 0; i < n_nodes; ++i) nb[i].push_back(i);
+  for (int e = 0; e < n_tris; ++e)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) nb[tris[3 * e + a]].push_back(tris[3 * e + b]);
+  for (auto &v : nb) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  return nb;
+}
+
+bool csr_supported(const orc_cell *c) { return c->k_bend == 0 && c->k_area == 0; }
+
+void csr_add(const int32_t *rowptr, const int32_t *col, double *val, int r, int cidx, double v) {
+  const int32_t *b = col + rowptr[r], *e = col + rowptr[r + 1];
+  const int32_t *p = std::lower_bound(b, e, cidx);
+  val[p - col] += v;
+}
+
+}  // namespace
+
+extern "C" int orc_csr_pattern(int n_nodes, int n_tris, const int32_t *tris, int32_t *rowptr, int32_t *col, int cap) {
+  std::vector<std::vector<int>> nb = node_neighbours(n_nodes, n_tris, tris);
+  long long nnz = 0;
+  for (auto &v : nb) nnz += 4LL * (long long)v.size();
+  if (nnz > 0x7fffffff) return -1;
+  if (!rowptr || !col || cap < nnz) return (int)nnz;
+  int k = 0;
+  rowptr[0] = 0;
+  for (int i = 0; i < n_nodes; ++i) {
+    for (int q = 0; q < 2; ++q) {
+      for (int j : nb[i]) {
+        col[k++] = 2 * j;
+        col[k++] = 2 * j + 1;
+      }
+      rowptr[2 * i + q + 1] = k;
+    }
+  }
+  return (int)nnz;
+}
+
+extern "C" int orc_assemble_csr(const orc_cell *c, const int32_t *rowptr, const int32_t *col, double *val, double *G) {
+  if (!csr_supported(c)) return -1;
+  const int n = 2 * c->n_nodes;
+  memset(G, 0, sizeof(double) * n);
+  memset(val, 0, sizeof(double) * rowptr[n]);
+  std::vector<int> sst, sli;
+  spring_lists(c, sst, sli);
+  for (int e = 0; e < c->n_tris; ++e) {
+    double Ge[6], Ke[36];
+    element_system_impl(c, e, sst, sli, Ge, Ke);
+    const int32_t *t = c->tris + 3 * e;
+    for (int r = 0; r < 6; ++r) {
+      const int gr = 2 * t[r / 2] + (r % 2);
+      G[gr] += Ge[r];
+      for (int k = 0; k < 6; ++k) csr_add(rowptr, col, val, gr, 2 * t[k / 2] + (k % 2), Ke[6 * r + k]);
+    }
+  }
+  add_node_terms(c, G, nullptr);  // G parts (external load, drag, actin polymerisation)
+  if (c->drag != 0) {
+    const double cd = c->drag / c->dt;
+    for (int i = 0; i < n; ++i) csr_add(rowptr, col, val, i, i, cd);
+  }
+  return 0;
+}
+
+extern "C" int orc_newton_solve_pcg(orc_cell *c, double *x, double rtol, double atol, int maxit, double cg_rtol,
+                                    int cg_maxit, double *res_hist) {
+  if (!csr_supported(c)) return -2;
+  const int n = 2 * c->n_nodes;
+  const int nnz = orc_csr_pattern(c->n_nodes, c->n_tris, c->tris, nullptr, nullptr, 0);
+  if (nnz < 0) return -2;
+  std::vector<int32_t> rowptr(n + 1), col(nnz);
+  orc_csr_pattern(c->n_nodes, c->n_tris, c->tris, rowptr.data(), col.data(), nnz);
+  std::vector<double> val(nnz), G(n), dx(n);
+  const double *xsave = c->x;
+  c->x = x;
+  double r0 = -1;
+  int ret = -1;
+  for (int it = 0; it <= maxit; ++it) {
+    orc_assemble_csr(c, rowptr.data(), col.data(), val.data(), G.data());
+    double rn = 0;
+    for (int i = 0; i < n; ++i) rn += G[i] * G[i];
+    rn = ::sqrt(rn);
+    if (r0 < 0) r0 = rn;
+    if (res_hist) res_hist[it] = rn;
+    if (rn <= rtol * r0 || rn <= atol) { ret = it; break; }
+    if (it == maxit) break;
+    for (int i = 0; i < n; ++i) { G[i] = -G[i]; dx[i] = 0; }
+    if (orc_pcg_csr(n, rowptr.data(), col.data(), val.data(), G.data(), dx.data(), cg_rtol, cg_maxit) < 0) break;
+    for (int i = 0; i < n; ++i) x[i] += dx[i];
+  }
+  c->x = xsave;
+  return ret;
+}
+
+namespace {
+
+// Reverse Cuthill-McKee node order (George and Liu, "Computer Solution of
+// Large Sparse Positive Definite Systems", 1981): each connected component
+// starts at a pseudo-peripheral node (repeated breadth-first searches from a
+// node of least degree in the last level), is numbered level by level with
+// unnumbered neighbours in increasing degree, and the order is reversed.
+std::vector<int> rcm_order(const std::vector<std::vector<int>> &nb) {
+  const int N = (int)nb.size();
+  std::vector<int> deg(N), order, level(N, -1);
+  for (int i = 0; i < N; ++i) deg[i] = (int)nb[i].size() - 1;
+  std::vector<char> done(N, 0);
+  order.reserve(N);
+  auto bfs_last_level = [&](int s, int &depth) {
+    std::vector<int> cur{s}, nxt, seen;
+    level[s] = 0;
+    seen.push_back(s);
+    depth = 0;
+    std::vector<int> last = cur;
+    while (!cur.empty()) {
+      nxt.clear();
+      for (int v : cur)
+        for (int w : nb[v])
+          if (level[w] < 0 && !done[w]) { level[w] = depth + 1; nxt.push_back(w); seen.push_back(w); }
+      if (nxt.empty()) break;
+      last = nxt;
+      cur.swap(nxt);
+      ++depth;
+    }
+    for (int v : seen) level[v] = -1;
+    return last;
+  };
+  for (int s0 = 0; s0 < N; ++s0) {
+    if (done[s0]) continue;
+    // least-degree node of this component
+    int s = s0;
+    {
+      std::vector<int> stack{s0}, comp;
+      std::vector<char> mark(N, 0);
+      mark[s0] = 1;
+      while (!stack.empty()) {
+        int v = stack.back();
+        stack.pop_back();
+        comp.push_back(v);
+        for (int w : nb[v])
+          if (!mark[w] && !done[w]) { mark[w] = 1; stack.push_back(w); }
+      }
+      for (int v : comp)
+        if (deg[v] < deg[s] || (deg[v] == deg[s] && v < s)) s = v;
+    }
+    int depth = 0;
+    for (int iter = 0; iter < 8; ++iter) {
+      std::vector<int> last = bfs_last_level(s, depth);
+      int best = last[0];
+      for (int v : last)
+        if (deg[v] < deg[best] || (deg[v] == deg[best] && v < best)) best = v;
+      int d2 = 0;
+      bfs_last_level(best, d2);
+      if (d2 <= depth) break;
+      s = best;
+    }
+    // Cuthill-McKee from s
+    const size_t first = order.size();
+    order.push_back(s);
+    done[s] = 1;
+    for (size_t h = first; h < order.size(); ++h) {
+      const int v = order[h];
+      std::vector<int> nx;
+      for (int w : nb[v])
+        if (!done[w]) { done[w] = 1; nx.push_back(w); }
+      std::sort(nx.begin(), nx.end(), [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+      order.insert(order.end(), nx.begin(), nx.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+// One cell's band system in RCM numbering (2 unknowns per node): row r of
+// the permuted matrix holds columns r - w .. r + w at band[r (2w+1) + (c - r + w)].
+struct BandPlan {
+  int n = 0, w = 0;
+  std::vector<int> pos;  // node -> position in the RCM order
+  std::vector<int32_t> rowptr, col;
+  std::vector<double> val, G, band, rhs;
+};
+
+BandPlan band_plan(const orc_cell *c) {
+  BandPlan p;
+  const int N = c->n_nodes;
+  p.n = 2 * N;
+  std::vector<std::vector<int>> nb = node_neighbours(N, c->n_tris, c->tris);
+  std::vector<int> ord = rcm_order(nb);
+  p.pos.assign(N, 0);
+  for (int k = 0; k < N; ++k) p.pos[ord[k]] = k;
+  int wn = 0;
+  for (int i = 0; i < N; ++i)
+    for (int j : nb[i]) wn = std::max(wn, std::abs(p.pos[i] - p.pos[j]));
+  p.w = 2 * wn + 1;
+  const int nnz = orc_csr_pattern(N, c->n_tris, c->tris, nullptr, nullptr, 0);
+  p.rowptr.resize(p.n + 1);
+  p.col.resize(nnz);
+  orc_csr_pattern(N, c->n_tris, c->tris, p.rowptr.data(), p.col.data(), nnz);
+  p.val.resize(nnz);
+  p.G.resize(p.n);
+  p.band.resize((size_t)p.n * (2 * p.w + 1));
+  p.rhs.resize(p.n);
+  return p;
+}
+
+// Newton iteration: assemble, band LU without pivoting (FP64), solve, update.
+int band_newton_step(orc_cell *c, BandPlan &p, double *x) {
+  const double *xsave = c->x;
+  c->x = x;
+  int rc = orc_assemble_csr(c, p.rowptr.data(), p.col.data(), p.val.data(), p.G.data());
+  c->x = xsave;
+  if (rc) return rc;
+  const int n = p.n, w = p.w, W = 2 * w + 1;
+  std::fill(p.band.begin(), p.band.end(), 0.0);
+  auto P = [&](int r) { return 2 * p.pos[r / 2] + (r % 2); };
+  for (int r = 0; r < n; ++r) {
+    const int pr = P(r);
+    for (int k = p.rowptr[r]; k < p.rowptr[r + 1]; ++k) {
+      const int pc = P(p.col[k]);
+      p.band[(size_t)pr * W + (pc - pr + w)] += p.val[k];
+    }
+    p.rhs[pr] = -p.G[r];
+  }
+  double *B = p.band.data();
+  for (int k = 0; k < n; ++k) {
+    const double piv = B[(size_t)k * W + w];
+    if (piv == 0.0) return k + 1;
+    const int iend = std::min(n - 1, k + w);
+    for (int i = k + 1; i <= iend; ++i) {
+      double *Ri = B + (size_t)i * W;
+      const double l = Ri[k - i + w] / piv;
+      Ri[k - i + w] = l;
+      if (l == 0.0) continue;
+      const double *Rk = B + (size_t)k * W;
+      for (int j = k + 1; j <= iend; ++j) Ri[j - i + w] -= l * Rk[j - k + w];
+    }
+  }
+  double *y = p.rhs.data();
+  for (int i = 0; i < n; ++i) {
+    double s = y[i];
+    const double *Ri = B + (size_t)i * W;
+    for (int j = std::max(0, i - w); j < i; ++j) s -= Ri[j - i + w] * y[j];
+    y[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = y[i];
+    const double *Ri = B + (size_t)i * W;
+    for (int j = i + 1; j <= std::min(n - 1, i + w); ++j) s -= Ri[j - i + w] * y[j];
+    y[i] = s / Ri[w];
+  }
+  for (int r = 0; r < n; ++r) x[r] += y[P(r)];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int orc_band_width(int n_nodes, int n_tris, const int32_t *tris, int32_t *order) {
+  std::vector<std::vector<int>> nb = node_neighbours(n_nodes, n_tris, tris);
+  std::vector<int> ord = rcm_order(nb);
+  std::vector<int> pos(n_nodes);
+  for (int k = 0; k < n_nodes; ++k) pos[ord[k]] = k;
+  if (order)
+    for (int k = 0; k < n_nodes; ++k) order[k] = ord[k];
+  int wn = 0;
+  for (int i = 0; i < n_nodes; ++i)
+    for (int j : nb[i]) wn = std::max(wn, std::abs(pos[i] - pos[j]));
+  return wn;
+}
+
+extern "C" int orc_band_newton_step(orc_cell *c, double *x) {
+  BandPlan p = band_plan(c);
+  return band_newton_step(c, p, x);
+}
+
+extern "C" double orc_batch_band_newton_step(int n_cells, orc_cell *cells, double *const *x, int n_threads, int *err) {
+  std::vector<BandPlan> plans(n_cells);
+#pragma omp parallel for schedule(dynamic) num_threads(n_threads)
+  for (int i = 0; i < n_cells; ++i) plans[i] = band_plan(&cells[i]);
+  int bad = 0;
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  // largest cells first: their plans are ordered by the caller's list, the
+  // dynamic schedule hands out one cell at a time
+#pragma omp parallel for schedule(dynamic) num_threads(n_threads) reduction(+ : bad)
+  for (int i = 0; i < n_cells; ++i) bad += band_newton_step(&cells[i], plans[i], x[i]) != 0;
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  if (err) *err = bad;
+  return (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
 }

@@ -156,6 +156,30 @@ int orc_pcg_csr(int n, const int32_t *rowptr, const int32_t *col, const double *
  * res_hist (optional, maxit+1 entries) receives ||G|| per iterate. */
 int orc_newton_solve(orc_cell *c, double *x, double rtol, double atol, int maxit, double *res_hist);
 
+/* ---------------- Part 2, sparse forms (2 DOF per node) ------------------- */
+/* Scalar CSR pattern of the tangent: rows 2i and 2i+1 hold columns 2j, 2j+1
+ * of every node j sharing a triangle with i (i included), increasing.
+ * rowptr: 2N+1, col: capacity cap.  Returns nnz; writes only when cap >= nnz. */
+int orc_csr_pattern(int n_nodes, int n_tris, const int32_t *tris, int32_t *rowptr, int32_t *col, int cap);
+/* orc_assemble into that pattern (element order).  Cells with bending or
+ * area terms (whose tangent leaves the pattern) are refused: returns -1. */
+int orc_assemble_csr(const orc_cell *c, const int32_t *rowptr, const int32_t *col, double *val, double *G);
+/* Newton solve with FP64 Jacobi-PCG linear solves (cg_rtol per solve);
+ * returns iterations or -1 (no convergence), -2 (unsupported cell). */
+int orc_newton_solve_pcg(orc_cell *c, double *x, double rtol, double atol, int maxit, double cg_rtol, int cg_maxit,
+                         double *res_hist);
+/* Reverse Cuthill-McKee node order (order: N, optional); returns the node
+ * bandwidth max |pos(i) - pos(j)| over neighbours. */
+int orc_band_width(int n_nodes, int n_tris, const int32_t *tris, int32_t *order);
+/* One Newton iteration (assemble, FP64 band LU in RCM numbering, solve,
+ * x += dx); returns 0, -1 (unsupported cell) or k+1 at a zero pivot. */
+int orc_band_newton_step(orc_cell *c, double *x);
+/* orc_band_newton_step on every cell (x[i]: 2N each) with OpenMP over the
+ * cells, n_threads threads; numbering and storage are prepared before the
+ * timed part.  Returns the seconds of the Newton iterations; err receives the
+ * number of cells that failed. */
+double orc_batch_band_newton_step(int n_cells, orc_cell *cells, double *const *x, int n_threads, int *err);
+
 /* The shared FP64 exp of the kinetics (wholecellfe_amd/csrc/wcfe_exp.hpp). */
 void orc_exp_array(int n, const double *x, double *y);
 

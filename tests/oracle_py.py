@@ -297,3 +297,57 @@ def commit_ul_strain(cell):
     ea = np.zeros(3 * cell.n_nodes)
     lib().orc_commit_ul_strain(C.byref(cell), dp(ea))
     return ea
+
+
+# ---------------- Part 2, sparse forms ----------------
+
+def csr_pattern(n_nodes, tris):
+    tris = i32(tris).reshape(-1)
+    L = lib()
+    nnz = L.orc_csr_pattern(C.c_int(n_nodes), C.c_int(len(tris) // 3), ip(tris), None, None, C.c_int(0))
+    rowptr = np.zeros(2 * n_nodes + 1, dtype=np.int32)
+    col = np.zeros(nnz, dtype=np.int32)
+    L.orc_csr_pattern(C.c_int(n_nodes), C.c_int(len(tris) // 3), ip(tris), ip(rowptr), ip(col), C.c_int(nnz))
+    return rowptr, col
+
+
+def assemble_csr(cell, rowptr, col):
+    val = np.zeros(len(col))
+    G = np.zeros(2 * cell.n_nodes)
+    rc = lib().orc_assemble_csr(C.byref(cell), ip(rowptr), ip(col), dp(val), dp(G))
+    if rc != 0:
+        raise ValueError("cell not supported by the CSR assembly")
+    return G, val
+
+
+def newton_solve_pcg(cell, x0, rtol=1e-12, atol=0.0, maxit=30, cg_rtol=1e-13, cg_maxit=200000):
+    x = f64(x0).copy()
+    hist = np.zeros(maxit + 1)
+    it = lib().orc_newton_solve_pcg(C.byref(cell), dp(x), C.c_double(rtol), C.c_double(atol), C.c_int(maxit),
+                                    C.c_double(cg_rtol), C.c_int(cg_maxit), dp(hist))
+    return x, it, hist
+
+
+def band_width(n_nodes, tris):
+    tris = i32(tris).reshape(-1)
+    order = np.zeros(n_nodes, dtype=np.int32)
+    w = lib().orc_band_width(C.c_int(n_nodes), C.c_int(len(tris) // 3), ip(tris), ip(order))
+    return w, order
+
+
+def band_newton_step(cell, x0):
+    x = f64(x0).copy()
+    rc = lib().orc_band_newton_step(C.byref(cell), dp(x))
+    return x, rc
+
+
+def batch_band_newton_step(cells_, xs, n_threads=1):
+    """One Newton iteration per cell (FP64 RCM band LU), OpenMP over cells;
+    returns (seconds of the timed iterations, failures); xs updated in place."""
+    L = lib()
+    L.orc_batch_band_newton_step.restype = C.c_double
+    arr = (Cell * len(cells_))(*cells_)
+    ptrs = (D * len(xs))(*[dp(x) for x in xs])
+    err = C.c_int(0)
+    t = L.orc_batch_band_newton_step(C.c_int(len(cells_)), arr, ptrs, C.c_int(n_threads), C.byref(err))
+    return t, err.value

@@ -34,13 +34,28 @@ def single_cell(T_target=905, n_spr=300, E=1000.0, nu=0.3, thick=0.2, spring="li
     return [cell], {TABLE_ID: mesh.stiffening_table()}
 
 
-# the named workloads: family 2 (one 20k-triangle cell, dense FP32 LU),
-# family 3 (256 uneven cells, the headline batch), family 4 (one 500k-triangle
-# cell, block-CSR PCG), family 5 (1024 Bell replicas)
-WORKLOADS = ("batched256", "single20k", "single500k", "replicas1024")
+# the named workloads: family 1 (one 905-triangle-class cell with 300 linear
+# integrins on a rigid 50 x 50 substrate grid, 200 load steps, FP64), family 2
+# (one 20k-triangle cell, dense FP32 LU), family 3 (256 uneven cells, the
+# headline batch), family 4 (one 500k-triangle cell, block-CSR PCG), family 5
+# (1024 Bell replicas)
+WORKLOADS = ("single905", "batched256", "single20k", "single500k", "replicas1024")
+
+# family 1: rigid substrate, a regular 50 x 50 node grid over [-20, 20]^2 um
+C1_SUBSTRATE = dict(n_side=50, half=20.0)
+
+
+def c1_substrate():
+    """The rigid substrate of family 1 (nodes, triangles).  Rigid, it only
+    holds the anchors: each spring's anchor is the substrate point under its
+    attachment point (rest length 0), located in the grid's triangles."""
+    return mesh.substrate_grid(**C1_SUBSTRATE)
 
 
 def workload(name):
+    if name == "single905":
+        # 300 integrins, k = 1 pN/nm, anchored on the rigid substrate grid
+        return single_cell(T_target=905, n_spr=300, spring="linear", k_lin=1000.0, seed=1, solver=1)
     if name == "batched256":
         return batched_cells(256)
     if name == "single20k":
