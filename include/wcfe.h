@@ -295,6 +295,10 @@ int wcfe_get_springs(wcfe_ctx *ctx, int32_t cell, double *state /* n_spr */, dou
  * buffers of the getters below from these. */
 int wcfe_num_cells(const wcfe_ctx *ctx, int32_t *n_cells);
 int wcfe_get_cell_sizes(wcfe_ctx *ctx, int32_t cell, int32_t *n_nodes, int32_t *n_tris, int32_t *n_spr);
+/* How a cell is solved after wcfe_finalize: solver (WCFE_SOLVER_DENSE or
+ * WCFE_SOLVER_CSR), 64-row blocks of its dense system, their block bandwidth
+ * in the band numbering (nblk - 1: full matrix), symmetric factorization. */
+int wcfe_get_cell_layout(wcfe_ctx *ctx, int32_t cell, int32_t *solver, int32_t *nblk, int32_t *bw, int32_t *sym);
 int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step, int32_t *newton_iter,
                        int32_t *n_colors, int32_t *ndof);
 /* FP64 residual G(x) = R_int(x) - F_spring(x) at the current positions (2N). */

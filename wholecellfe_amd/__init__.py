@@ -37,7 +37,7 @@ EXPORTS = [
     "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state", "wcfe_replace_cell", "wcfe_checkpoint_save",
     "wcfe_checkpoint_load", "wcfe_area_tot", "wcfe_bending_force_2d", "wcfe_area_constraint_force",
     "wcfe_force_ap", "wcfe_get_schedule", "wcfe_num_cells", "wcfe_get_cell_sizes",
-    "wcfe_eval_exp",
+    "wcfe_eval_exp", "wcfe_get_cell_layout",
 ]
 TUNE_SOLVER, TUNE_LU = 1, 2
 # reference constants of the boundary and body forces (UL_growth_explicit.cpp)
@@ -392,6 +392,13 @@ class Context:
         self._check(self._L.wcfe_get_cell_info(self._h, cell, C.byref(t), C.byref(step), C.byref(nit), C.byref(nc),
                                                C.byref(ndof)))
         return dict(time=t.value, step=step.value, newton_iter=nit.value, n_colors=nc.value, ndof=ndof.value)
+
+    def cell_layout(self, cell):
+        """solver (SOLVER_DENSE / SOLVER_CSR), nblk (64-row blocks), bw (block
+        bandwidth in the band numbering), sym (symmetric factorization)."""
+        v = [C.c_int32(0) for _ in range(4)]
+        self._check(self._L.wcfe_get_cell_layout(self._h, cell, *[C.byref(x) for x in v]))
+        return dict(solver=v[0].value, nblk=v[1].value, bw=v[2].value, sym=v[3].value)
 
     def residual(self, cell):
         G = np.zeros(2 * self.cells[cell]["n_nodes"])

@@ -2309,6 +2309,18 @@ int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step,
   });
 }
 
+int wcfe_get_cell_layout(wcfe_ctx *ctx, int32_t cell, int32_t *solver, int32_t *nblk, int32_t *bw, int32_t *sym) {
+  if (!ctx) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    const CellParams &p = ctx->b.cp_h[cell];
+    if (solver) *solver = p.csr ? WCFE_SOLVER_CSR : WCFE_SOLVER_DENSE;
+    if (nblk) *nblk = p.nblk;
+    if (bw) *bw = p.bw;
+    if (sym) *sym = p.sym;
+  });
+}
+
 int wcfe_num_cells(const wcfe_ctx *ctx, int32_t *n_cells) {
   if (!ctx || !n_cells) return WCFE_ERR_ARG;
   *n_cells = (int32_t)ctx->cells.size();

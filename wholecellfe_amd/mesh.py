@@ -73,36 +73,64 @@ def jitter(X, T, bnd, frac, rng):
 
 
 def locate(X, T, P, cell=None):
-    """Host triangle and barycentric coordinates of points P (uniform-grid buckets)."""
+    """Host triangle and barycentric coordinates of points P: uniform-grid
+    buckets (about two triangles each) listing the triangles whose bounding
+    box meets them in increasing index; a point takes the first triangle of
+    its bucket that contains it (barycentric coordinates >= -1e-12).
+    Vectorised over the points, one candidate slot of every bucket at a time."""
+    X = np.asarray(X, dtype=np.float64)
+    T = np.asarray(T)
+    P = np.asarray(P, dtype=np.float64).reshape(-1, 2)
     a = X[T]
     lo, hi = a.min(axis=1), a.max(axis=1)
     xmin, ymin = X.min(axis=0) - 1e-9
     xmax, ymax = X.max(axis=0) + 1e-9
     nb = max(1, int(math.sqrt(len(T) / 2)))
     sx, sy = (xmax - xmin) / nb, (ymax - ymin) / nb
-    buckets = {}
-    ix0 = np.floor((lo[:, 0] - xmin) / sx).astype(int)
-    ix1 = np.floor((hi[:, 0] - xmin) / sx).astype(int)
-    iy0 = np.floor((lo[:, 1] - ymin) / sy).astype(int)
-    iy1 = np.floor((hi[:, 1] - ymin) / sy).astype(int)
-    for t in range(len(T)):
-        for bx in range(ix0[t], ix1[t] + 1):
-            for by in range(iy0[t], iy1[t] + 1):
-                buckets.setdefault((bx, by), []).append(t)
+    ix0 = np.floor((lo[:, 0] - xmin) / sx).astype(np.int64)
+    ix1 = np.floor((hi[:, 0] - xmin) / sx).astype(np.int64)
+    iy0 = np.floor((lo[:, 1] - ymin) / sy).astype(np.int64)
+    iy1 = np.floor((hi[:, 1] - ymin) / sy).astype(np.int64)
+    # (bucket, triangle) pairs; the bucket key packs (bx, by) with an offset
+    # wide enough for every coordinate a point or box can produce
+    span = nb + 4
+    keys, tids = [], []
+    tri_ids = np.arange(len(T), dtype=np.int64)
+    for dx in range(int((ix1 - ix0).max()) + 1):
+        for dy in range(int((iy1 - iy0).max()) + 1):
+            m = (ix0 + dx <= ix1) & (iy0 + dy <= iy1)
+            keys.append((ix0[m] + dx + 2) * span + (iy0[m] + dy + 2))
+            tids.append(tri_ids[m])
+    keys = np.concatenate(keys)
+    tids = np.concatenate(tids)
+    order = np.lexsort((tids, keys))  # by bucket, then triangle index
+    keys, tids = keys[order], tids[order]
+    ukeys, start, count = np.unique(keys, return_index=True, return_counts=True)
+    pk = (np.floor_divide(P[:, 0] - xmin, sx).astype(np.int64) + 2) * span + \
+        (np.floor_divide(P[:, 1] - ymin, sy).astype(np.int64) + 2)
+    slot = np.searchsorted(ukeys, pk)
+    has = (slot < len(ukeys)) & (ukeys[np.minimum(slot, len(ukeys) - 1)] == pk)
+    b0 = np.where(has, start[np.minimum(slot, len(ukeys) - 1)], 0)
+    nc = np.where(has, count[np.minimum(slot, len(ukeys) - 1)], 0)
     tri = np.full(len(P), -1, dtype=np.int32)
     bary = np.zeros((len(P), 3))
-    for k, p in enumerate(P):
-        key = (int((p[0] - xmin) // sx), int((p[1] - ymin) // sy))
-        for t in buckets.get(key, ()):
-            p0, p1, p2 = a[t]
-            det = (p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1])
-            l1 = ((p[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p[1] - p0[1])) / det
-            l2 = ((p1[0] - p0[0]) * (p[1] - p0[1]) - (p[0] - p0[0]) * (p1[1] - p0[1])) / det
-            l0 = 1.0 - l1 - l2
-            if l0 >= -1e-12 and l1 >= -1e-12 and l2 >= -1e-12:
-                tri[k] = t
-                bary[k] = (l0, l1, l2)
-                break
+    todo = np.nonzero(nc > 0)[0]
+    k = 0
+    while len(todo):
+        t = tids[b0[todo] + k]
+        p0, p1, p2 = a[t, 0], a[t, 1], a[t, 2]
+        q = P[todo]
+        det = (p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p2[:, 0] - p0[:, 0]) * (p1[:, 1] - p0[:, 1])
+        l1 = ((q[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p2[:, 0] - p0[:, 0]) * (q[:, 1] - p0[:, 1])) / det
+        l2 = ((p1[:, 0] - p0[:, 0]) * (q[:, 1] - p0[:, 1]) - (q[:, 0] - p0[:, 0]) * (p1[:, 1] - p0[:, 1])) / det
+        l0 = 1.0 - l1 - l2
+        hit = (l0 >= -1e-12) & (l1 >= -1e-12) & (l2 >= -1e-12)
+        idx = todo[hit]
+        tri[idx] = t[hit]
+        bary[idx] = np.stack([l0[hit], l1[hit], l2[hit]], axis=1)
+        k += 1
+        todo = todo[~hit]
+        todo = todo[nc[todo] > k]
     return tri, bary
 
 
