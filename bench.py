@@ -147,9 +147,9 @@ def oracle_check(cells, tables, common, n_sample=16):
         if it < 0:
             raise RuntimeError("oracle Newton did not converge")
         xe = max(xe, float(np.max(np.abs(x - x_o)) / np.max(np.abs(x_o - X))))
+    sizes = [len(c["tris"]) for c in sub]
     return {"x_err_vs_oracle": xe, "force_err_vs_oracle": fe,
-            "oracle_sample": "%d cells, %d..%d triangles, one load step each" % (
-                len(sub), len(sub[0]["tris"]), len(sub[-1]["tris"]))}
+            "oracle_sample": "%d cells, %d..%d triangles, one load step each" % (len(sub), min(sizes), max(sizes))}
 
 
 # ---------------------------------------------------------------------------
@@ -301,7 +301,7 @@ def main():
     ap.add_argument("--pivoting", choices=["static", "partial"], default="static")
     ap.add_argument("--autotune", choices=["none", "solver", "lu", "all"], default="none",
                     help="run wcfe_autotune on the headline batch before timing")
-    ap.add_argument("--fused-step", type=int, choices=[0, 1, 2, 3], default=None,
+    ap.add_argument("--fused-step", type=int, choices=[0, 1, 2, 3, 4], default=None,
                     help="band-LU block-step mode (library default when omitted)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (HBM probe / counter reference) and exit")

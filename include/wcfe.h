@@ -97,7 +97,10 @@ typedef struct wcfe_options {
                             chain) or, above 2 items per compute unit, one panel item per cell (one
                             inverse, then every L block: least work) and one item for the
                             cell's trailing blocks; 2: L items only; 3: panel items
-                            after the first step; 0: separate launches */
+                            after the first step; 4: one launch for the whole
+                            factorization, each cell's block steps in order in one
+                            workgroup (no launch between two steps of a cell's chain);
+                            0: separate launches */
 } wcfe_options;
 
 typedef struct wcfe_cell_desc {
@@ -213,10 +216,12 @@ int wcfe_synchronize(wcfe_ctx *ctx);
  * threshold (dense_max_dof, cells with solver AUTO; candidates that would not
  * fit in free device memory are skipped); WCFE_TUNE_LU: delayed-update depth
  * (1, 2, 4, 6) x concurrent cell groups (1, 2, 4), at depth 1 with and without
- * the fused block step.  Every trial starts from the
+ * the fused block step, and (mixed precision, one group) the whole
+ * factorization in one launch (fused_step 4).  Every trial starts from the
  * state the cells were added with, and so does the batch afterwards: call it
- * after wcfe_finalize and before any Newton step.  chosen (optional, 3):
- * lu_depth, lu_groups, dense_max_dof; ms_best (optional): ms per step. */
+ * after wcfe_finalize and before any Newton step.  chosen (optional, 4):
+ * lu_depth, lu_groups, dense_max_dof, fused_step in effect; ms_best
+ * (optional): ms per step. */
 #define WCFE_TUNE_SOLVER 1
 #define WCFE_TUNE_LU 2
 int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, double *ms_best);
@@ -224,7 +229,8 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
  * automatic choice): delayed-update depth, concurrent cell groups, the widest
  * block band of the dense cells (64-row blocks) and whether the triangular
  * sweeps run one workgroup per cell along the band (1) or spread over
- * workgroups (0), and whether each block step is one fused launch (1). */
+ * workgroups (0), and whether each block step is one fused launch (1) or one
+ * launch runs every block step, one workgroup per cell (2). */
 int wcfe_get_schedule(wcfe_ctx *ctx, int32_t *lu_depth, int32_t *lu_groups, int32_t *max_band, int32_t *band_solve,
                       int32_t *fused_step);
 

@@ -26,7 +26,8 @@ def _run(ds, **opt):
 
 
 @pytest.mark.parametrize("sym,depth,fused", [(1, 4, 1), (0, 4, 1), (1, 1, 1), (0, 1, 1), (1, 1, 0), (0, 1, 0),
-                                             (0, 2, 1), (1, 3, 1), (1, 1, 2), (0, 1, 2), (1, 1, 3), (0, 1, 3)])
+                                             (0, 2, 1), (1, 3, 1), (1, 1, 2), (0, 1, 2), (1, 1, 3), (0, 1, 3),
+                                             (1, 1, 4), (0, 1, 4)])
 def test_band_lu_matches_full_matrix(sym, depth, fused):
     """Band LU in the reverse Cuthill-McKee numbering (the default) against the
     full-matrix LU in the natural numbering, for symmetric and full
@@ -59,24 +60,28 @@ def test_band_lu_matches_full_matrix(sym, depth, fused):
 
 @pytest.mark.parametrize("sym", [1, 0])
 def test_fused_step_forms_agree_bit_for_bit(sym):
-    """The fused block step's two forms -- L items that each invert the
-    diagonal block, or one panel item per cell that inverts it once and solves
-    every L block -- run the same products in the same order: one Newton step
-    gives identical positions and statistics."""
+    """The fused block step's forms -- L items that each invert the diagonal
+    block, one panel item per cell that inverts it once and solves every L
+    block, and the whole factorization in one launch with each cell's steps
+    in one workgroup (the first step in L items, the others in panel form) --
+    run the same products in the same order: one Newton step gives identical
+    positions and statistics."""
     print("case sym", sym, flush=True)
-    ds = [cells.cst_cell(K=k, n_spr=20 + 10 * k, law_table=(k % 2 == 1), seed=k, time=1.0) for k in (9, 16, 22)]
+    ds = [cells.cst_cell(K=k, n_spr=20 + 10 * k, law_table=(k % 2 == 1), seed=k, time=1.0) for k in (1, 2, 9, 16, 22)]
     out = []
-    for fused in (2, 3):
+    for fused in (2, 3, 4):
         c = W.Context(precision=W.PREC_MIXED, t0=1.0, max_refine=12, symmetric=sym, lu_depth=1, fused_step=fused)
         for d in ds:
             cells.add_to_context(c, d)
         c.finalize()
         st = c.newton_step()
+        assert c.schedule()["fused_step"] == (2 if fused == 4 else 1)
         out.append(([c.positions(i) for i in range(len(ds))], st["max_lin_relres"]))
         c.close()
-    for a, b in zip(out[0][0], out[1][0]):
-        assert np.array_equal(a, b)
-    assert out[0][1] == out[1][1]
+    for o in out[1:]:
+        for a, b in zip(out[0][0], o[0]):
+            assert np.array_equal(a, b)
+        assert out[0][1] == o[1]
 
 
 def test_band_assembled_system_is_natural_order():

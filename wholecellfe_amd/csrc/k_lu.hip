@@ -876,19 +876,30 @@ __device__ __forceinline__ void acc_store64(const typename Acc4<T>::type (&acc)[
       for (int r = 0; r < 4; ++r) C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
 }
 
+// Items of cell p in block step k (the host's LuPlan::items for ST_STEP):
+// panel form (k >= 0): the panel item, the U items (non-symmetric cells) and
+// one item for the trailing blocks and the forward block; L-item form: the
+// diagonal item, the L items, the U items, one item per trailing block and
+// the forward block.
+__device__ __forceinline__ int lu_step_items(const CellParams &p, int k, int panel) {
+  const int s0 = k + 1;
+  if (s0 >= p.nblk) return 0;
+  const int w = min(p.nblk - s0 - 1, p.bw);
+  const int npl = lu_step_np(p, k);
+  const int tiles = p.sym ? npl * (npl + 1) / 2 : npl * npl;
+  if (panel && k >= 0) return 2 + (p.sym ? 0 : w);
+  return 2 + w + (p.sym ? 0 : w) + tiles;
+}
+
+// One item t of cell c in block step k (see the comment above step_tile).
+// All threads of the workgroup call it with the same arguments (uniform
+// barriers); ls, us, xs are the workgroup's three LDS tiles.
 template <class T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_lu_step(
-    const CellParams *cp, CellState *cs, const int *map, int k, T *K, T *Dinv, int panel, StepFwd fwd) {
+__device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs, int c, int t, int k, T *K, T *Dinv,
+                                             int panel, StepFwd fwd, T *ls, T *us, T *xs) {
   constexpr int P = Pitch<T>::v, PA = P;
   typedef typename Acc4<T>::type acc_t;
-  __shared__ __attribute__((aligned(16))) T ls[64 * P];
-  __shared__ __attribute__((aligned(16))) T us[64 * P];
-  __shared__ __attribute__((aligned(16))) T xs[64 * P];
-  const int item = xcd_item(blockIdx.x, gridDim.x);
-  const int v = map[item];
-  const int c = v >> 20;
   const CellParams p = cp[c];
-  const int t = v & 0xFFFFF;
   const int s0 = k + 1, wl = lu_step_w(p, k), npl = lu_step_np(p, k);
   if (k < 0) panel = 0;  // the first step always runs L items
   // panel mode: item 0 inverts block s0 once and solves every L block of
@@ -1151,6 +1162,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   mma_64<T, P, PA>(xs, a, acc, T(1));
   acc_store64<T, 0>(acc, C, ld);
 }
+
+// One launch per block step k: every item of every cell in the launch.
+template <class T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_lu_step(
+    const CellParams *cp, CellState *cs, const int *map, int k, T *K, T *Dinv, int panel, StepFwd fwd) {
+  constexpr int P = Pitch<T>::v;
+  __shared__ __attribute__((aligned(16))) T ls[64 * P];
+  __shared__ __attribute__((aligned(16))) T us[64 * P];
+  __shared__ __attribute__((aligned(16))) T xs[64 * P];
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int v = map[item];
+  lu_step_item<T>(cp, cs, v >> 20, v & 0xFFFFF, k, K, Dinv, panel, fwd, ls, us, xs);
+}
+
+// The whole band LU of a cell in one workgroup: block steps k = -1 .. nblk-2
+// in order, each step's items one after another (the first step in the
+// L-item form, the others in the panel form: one inverse per step), with a
+// barrier between items.  The items of one step are independent (they run
+// concurrently in k_lu_step), so any order inside a step gives the same
+// results as the launch-per-step schedule in panel form.  One launch factors
+// every cell of the batch; a cell never waits for another cell's steps, and
+// no launch sits between two steps of the chain.  map: (cell << 20), largest
+// cells first.
+template <class T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_lu_cell(
+    const CellParams *cp, CellState *cs, const int *map, T *K, T *Dinv, StepFwd fwd) {
+  constexpr int P = Pitch<T>::v;
+  __shared__ __attribute__((aligned(16))) T ls[64 * P];
+  __shared__ __attribute__((aligned(16))) T us[64 * P];
+  __shared__ __attribute__((aligned(16))) T xs[64 * P];
+  const int c = map[xcd_item(blockIdx.x, gridDim.x)] >> 20;
+  const int nblk = cp[c].nblk;
+  for (int k = -1; k <= nblk - 2; ++k) {
+    const int panel = k >= 0 ? 1 : 0;
+    const int nt = lu_step_items(cp[c], k, panel);
+    for (int t = 0; t < nt; ++t) {
+      lu_step_item<T>(cp, cs, c, t, k, K, Dinv, panel, fwd, ls, us, xs);
+      __syncthreads();  // the item's global writes are visible to the next item
+    }
+  }
+}
+
+template <class T>
+void launch_lu_cell(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, T *K, T *Dinv,
+                    StepFwd fwd, hipEvent_t ev_start, hipEvent_t ev_stop) {
+  if (wl.total <= 0) return;
+  if constexpr (sizeof(T) == 4) {
+    if (ev_start)
+      hipExtLaunchKernelGGL(k_lu_cell<T>, dim3(wl.total), dim3(256), 0, s, ev_start, ev_stop, 0, cp, cs, wl.map, K,
+                            Dinv, fwd);
+    else
+      hipLaunchKernelGGL(k_lu_cell<T>, dim3(wl.total), dim3(256), 0, s, cp, cs, wl.map, K, Dinv, fwd);
+  }
+}
+
 template <class T>
 void launch_lu_step(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
                     int panel, StepFwd fwd, hipEvent_t ev_start, hipEvent_t ev_stop) {
@@ -1821,6 +1887,8 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
                                   int, int, int, hipEvent_t, hipEvent_t);                                             \
   template void launch_lu_step<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *, int, \
                                   StepFwd,                                                                         \
+                                  hipEvent_t, hipEvent_t);                                                            \
+  template void launch_lu_cell<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, T *, T *, StepFwd,  \
                                   hipEvent_t, hipEvent_t);                                                            \
   template void launch_lu_solve<T>(hipStream_t, const CellParams *, const CellState *, const int *, int, const T *,   \
                                    const T *, const int *, int, const double *, double *, int, int);                  \

@@ -245,7 +245,7 @@ int block_bandwidth(int n_tris, const int *tris, const std::vector<int> &pos) {
 // LU schedule: a list of stages (kernel kind, block step k, panel count),
 // each with the exclusive prefix sums of its work items over the cells of a
 // list (all cells batched, or one cell at a time).
-enum StageKind { ST_DIAG, ST_TRSM, ST_STRIP, ST_GEMM1, ST_GEMM, ST_PANEL, ST_SWP, ST_STEP };
+enum StageKind { ST_DIAG, ST_TRSM, ST_STRIP, ST_GEMM1, ST_GEMM, ST_PANEL, ST_SWP, ST_STEP, ST_CELL };
 struct Stage {
   int kind, k, np, off, tot;
   long long work;  // 64 x 64 x 64 block products (GEMM and strip stages)
@@ -299,6 +299,7 @@ struct LuPlan {
         if (np == 2) return 2 + (p.sym ? 0 : w);  // panel item, U items, one for the tiles and the forward block
         return 2 + w + (p.sym ? 0 : w) + tiles;
       }
+      case ST_CELL: return nb > 0 ? 1 : 0;  // the whole factorization, one workgroup
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
     }
   }
@@ -313,6 +314,11 @@ struct LuPlan {
           k >= 0 ? 1 + (p.sym ? 1 : 2) * std::min(w, p.bw - 1) + (p.sym ? npl * (npl + 1) / 2 : npl * npl) : 0;
       return upd + w + 1;
     }
+    if (kind == ST_CELL) {  // every block step: the first in the L-item form, the others in the panel form
+      long long tot = 0;
+      for (int kk = -1; kk < p.nblk - 1; ++kk) tot += block_products(p, ST_STEP, kk, kk >= 0 ? 2 : 1, pivoting);
+      return tot;
+    }
     return 0;
   }
   // static pivoting: LU_DEPTH block steps per round -- panel k, then the
@@ -324,10 +330,13 @@ struct LuPlan {
   // update depth): the trailing matrix is read and written once per `depth`
   // block steps
   int depth = 4;
-  bool fused = false;  // static pivoting, depth 1: one fused launch per block step
+  bool fused = false;      // static pivoting, depth 1: one fused launch per block step
+  bool cell_loop = false;  // ... or one launch for every step, one workgroup per cell
   std::vector<std::array<int, 3>> schedule(int nblk, int pivoting) const {
     std::vector<std::array<int, 3>> s;
-    if (!pivoting && fused) {
+    if (!pivoting && cell_loop) {
+      if (nblk > 0) s.push_back({ST_CELL, -1, 0});
+    } else if (!pivoting && fused) {
       for (int k = -1; k < nblk - 1; ++k) s.push_back({ST_STEP, k, 1});
     } else if (!pivoting) {
       s.push_back({ST_DIAG, 0, 0});
@@ -369,12 +378,15 @@ struct LuPlan {
 
   // fused block steps (fuse = the fused_step option): 1 panel items in the
   // launches with more than panel_min items, L items in the others; 2 L items
-  // only; 3 panel items after the first step
+  // only; 3 panel items after the first step; 4 one launch for the whole
+  // factorization (k_lu_cell: every cell's steps in one workgroup, largest
+  // cells first)
   void build(const std::vector<CellParams> &cp, int piv, int groups, int lu_depth, int fuse = 0, int panel_min = 512) {
     if (lu_depth < 1 || groups < 1) throw ApiError{WCFE_ERR_ARG, "LU schedule needs depth and groups >= 1"};
     pivoting = piv;
     depth = lu_depth;
     fused = fuse && !piv && lu_depth == 1;
+    cell_loop = fused && fuse == 4;
     // the step's form: L items (np 1) or one panel item per cell (np 2)
     auto step_form = [&](const std::vector<int> &cells, std::array<int, 3> &st) {
       if (st[0] != ST_STEP || st[1] < 0 || fuse == 2) return;
@@ -394,10 +406,12 @@ struct LuPlan {
     for (auto &p : cp) max_nblk = std::max(max_nblk, p.nblk);
     if (n >= (1 << 11)) throw ApiError{WCFE_ERR_ARG, "at most 2047 systems per batch"};
     batched.clear();
+    std::vector<int> by_size(all);  // the cell launch takes the longest chains first
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int a, int b) { return cp[a].nblk > cp[b].nblk; });
     for (auto st : schedule(max_nblk, pivoting)) {
       step_form(all, st);
       Stage g{st[0], st[1], st[2], (int)buf.size(), 0, 0};
-      for (int c = 0; c < n; ++c) {
+      for (int c : (st[0] == ST_CELL ? by_size : all)) {
         const long long t = items(cp[c], st[0], st[1], st[2], pivoting);
         if (t >= (1 << 20)) throw ApiError{WCFE_ERR_ARG, "too many LU work items per cell"};
         for (int i = 0; i < t; ++i) buf.push_back((c << 20) | i);
@@ -1294,6 +1308,12 @@ void enqueue_lu(wcfe_ctx *ctx, hipStream_t s, const LuPlan &plan, const std::vec
         launch_lu_step<T>(s, cp, cs, wl, g.k, K, Dinv, g.np == 2 ? 1 : 0, fwd, kt.a, kt.b);
         break;
       }
+      case ST_CELL: {
+        if (sizeof(T) != 4) fail(WCFE_ERR_STATE, "the fused block step is built for FP32 factors only");
+        KernelTimer kt(ctx, WCFE_PH_LU_GEMM, g.work);
+        launch_lu_cell<T>(s, cp, cs, wl, K, Dinv, fwd, kt.a, kt.b);
+        break;
+      }
     }
   }
 }
@@ -1597,7 +1617,7 @@ int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     if (ctx->opt.lu_groups < 0 || ctx->opt.lu_groups > 16) fail(WCFE_ERR_ARG, "lu_groups must be in [0, 16]");
     if (ctx->opt.lu_depth < 0 || ctx->opt.lu_depth > 8) fail(WCFE_ERR_ARG, "lu_depth must be in [0, 8]");
-    if (ctx->opt.fused_step < 0 || ctx->opt.fused_step > 3) fail(WCFE_ERR_ARG, "fused_step must be in [0, 3]");
+    if (ctx->opt.fused_step < 0 || ctx->opt.fused_step > 4) fail(WCFE_ERR_ARG, "fused_step must be in [0, 4]");
     if (ctx->opt.band != 0 && ctx->opt.band != 1) fail(WCFE_ERR_ARG, "band must be 0 or 1");
     HIPCHECK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     for (int g = 0; g < std::max(ctx->opt.lu_groups, AUTO_GROUPS_MAX); ++g) {
@@ -1941,7 +1961,8 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
       const wcfe_options base = best;
       for (int d : depths)
         for (int g : groups)
-          for (int f = 1; f >= (d == 1 ? 0 : 1); --f) {
+          for (int f = (d == 1 && g == 1 && mixed(ctx)) ? 4 : 1; f >= (d == 1 ? 0 : 1); --f) {
+            if (f == 3 || f == 2) continue;  // the per-launch choice (1) covers the L-item and panel forms
             ctx->opt = base;
             ctx->opt.lu_depth = d;
             ctx->opt.lu_groups = g;
@@ -1980,6 +2001,7 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
       chosen[0] = ctx->b.lu_depth;
       chosen[1] = ctx->b.lu_groups;
       chosen[2] = ctx->opt.dense_max_dof;
+      chosen[3] = ctx->b.plan.cell_loop ? 4 : ctx->b.plan.fused ? ctx->opt.fused_step : 0;
     }
     if (ms_best) *ms_best = best_ms < 1e300 ? best_ms : 0.0;
   });
@@ -2880,7 +2902,7 @@ int wcfe_get_schedule(wcfe_ctx *ctx, int32_t *lu_depth, int32_t *lu_groups, int3
     if (lu_groups) *lu_groups = ctx->b.lu_groups;
     if (max_band) *max_band = ctx->b.max_bw;
     if (band_solve) *band_solve = ctx->b.solve_wg ? 1 : 0;
-    if (fused_step) *fused_step = ctx->b.plan.fused ? 1 : 0;
+    if (fused_step) *fused_step = ctx->b.plan.cell_loop ? 2 : ctx->b.plan.fused ? 1 : 0;
   });
 }
 

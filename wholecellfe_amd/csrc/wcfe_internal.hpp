@@ -117,6 +117,10 @@ struct StepFwd {
 template <class T>
 void launch_lu_step(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, int k, T *K, T *Dinv,
                     int panel, StepFwd fwd, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
+// every block step of every listed cell in one launch, one workgroup per cell
+template <class T>
+void launch_lu_cell(hipStream_t s, const CellParams *cp, CellState *cs, const WorkList &wl, T *K, T *Dinv,
+                    StepFwd fwd, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 template <class T>
 void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                      const T *K, const T *Dinv, const int *ipiv, int pivoting, const double *r, double *delta,
