@@ -339,6 +339,20 @@ int wcfe_calculate_res(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int
  * x: 3N, cC: N, fau: 3N, F: 3N. */
 int wcfe_force_fa(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *tris, const double *x,
                   const double *cC, const double *fau, int32_t law, double kint, double *F);
+/* calculateForce_FA on device-resident arrays, allocating nothing: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (stream capture, serving loops, harnesses that forbid hipMalloc).
+ * The arguments mean what wcfe_force_fa's do, but every array is a device
+ * pointer (d_tris 3T, d_x 3N, d_cC N, d_fau 3N, d_F 3N); d_n_bad (optional,
+ * device) receives the number of triangle corners outside [0, n_nodes)
+ * (skipped).  The one-ring lists live in a static per-device scratch array
+ * of the library: n_nodes <= 262144 and n_tris <= 524288 (WCFE_ERR_ARG
+ * beyond).  The call is enqueued on the context's stream and returns without
+ * waiting (wcfe_synchronize waits); the result equals wcfe_force_fa's bit for
+ * bit.  Calls that overlap on one device share the scratch array and race. */
+int wcfe_force_fa_noalloc(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *d_tris, const double *d_x,
+                          const double *d_cC, const double *d_fau, int32_t law, double kint, double *d_F,
+                          int32_t *d_n_bad);
 /* updateFA 11-argument form (UL:2026 kMD, law 1; UL1:2005 k1/k31, law 0).
  * cA, cC, fau are updated in place.  n_unlocated (optional) receives the
  * number of anchors outside every substrate triangle (subs_u := 0 there; the

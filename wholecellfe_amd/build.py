@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["k_assemble.hip", "k_lu.hip", "k_csr.hip", "k_reference.hip", "k_boundary.hip", "wcfe_vtk.hip",
-           "wcfe_api.hip"]
+           "wcfe_api.hip", "wcfe_noalloc.hip"]
 HEADERS = ["wcfe_device.hpp", "wcfe_internal.hpp"]
 
 FLAGS = [

@@ -1614,6 +1614,7 @@ int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->opt.device) == hipSuccess && ncu > 0)
       ctx->n_cu = ncu;
     HIPCHECK(lu_prepare_attributes(ctx->max_lds));
+    HIPCHECK(noalloc_init(ctx->opt.device));  // the device's scratch array address (wcfe_force_fa_noalloc)
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     if (ctx->opt.lu_groups < 0 || ctx->opt.lu_groups > 16) fail(WCFE_ERR_ARG, "lu_groups must be in [0, 16]");
     if (ctx->opt.lu_depth < 0 || ctx->opt.lu_depth > 8) fail(WCFE_ERR_ARG, "lu_depth must be in [0, 8]");
@@ -2819,6 +2820,19 @@ int wcfe_force_fa(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t 
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     HIPCHECK(hipMemcpy(F, dF, sizeof(double) * 3 * n_nodes, hipMemcpyDeviceToHost));
   });
+}
+
+int wcfe_force_fa_noalloc(wcfe_ctx *ctx, int32_t n_nodes, int32_t n_tris, const int32_t *d_tris, const double *d_x,
+                          const double *d_cC, const double *d_fau, int32_t law, double kint, double *d_F,
+                          int32_t *d_n_bad) {
+  if (!ctx || !d_tris || !d_x || !d_cC || !d_fau || !d_F) return WCFE_ERR_ARG;
+  if (law != 0 && law != 1) return set_err(ctx, WCFE_ERR_ARG, "law must be 0 (linear kint) or 1 (MD table)");
+  if (hipSetDevice(ctx->opt.device) != hipSuccess) return set_err(ctx, WCFE_ERR_HIP, "hipSetDevice failed");
+  std::string err;
+  const int rc = noalloc_force_fa(ctx->opt.device, ctx->stream, n_nodes, n_tris, d_tris, d_x, d_cC, d_fau, law, kint,
+                                  d_F, d_n_bad, err);
+  if (rc != WCFE_OK) return set_err(ctx, rc, err);
+  return WCFE_OK;
 }
 
 int wcfe_update_fa(wcfe_ctx *ctx, int32_t n_nodes, const double *X, const double *x, double *cA, double *cC,

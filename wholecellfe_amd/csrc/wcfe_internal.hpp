@@ -180,6 +180,12 @@ template <class T>
 void launch_pcg_finish(hipStream_t s, const int4 *chunks, int nchunk, const CellState *cs, const T *x, double *delta,
                        int accumulate);
 
+// ---- calculateForce_FA on device pointers with no allocation (wcfe_noalloc.hip) ----
+hipError_t noalloc_init(int device);
+int noalloc_force_fa(int device, hipStream_t s, int n_nodes, int n_tris, const int *d_tris, const double *d_x,
+                     const double *d_cC, const double *d_fau, int law, double kint, double *d_F, int *d_n_bad,
+                     std::string &err);
+
 // ---- reference-shaped stateless kernels (k_reference.hip) ----
 void launch_eval_exp(hipStream_t s, long long n, const double *x, double *y);
 void launch_ref_force_fa(hipStream_t s, int n_nodes, const int *ring_start, const int *ring, const int *tris,
