@@ -8,7 +8,7 @@
 //
 // Usage:  refharness_<variant> <mode> <case_dir>
 //   modes: elem | calcres | forcefa | updatefa | areatot | bend | areacon | ap | forcefasubs | calcressubs |
-//          vtkcell | vtksubs
+//          vtkcell | vtksubs | time (per-call seconds of the per-step residual work)
 // Inputs are read from <case_dir>/<name>.bin (float64 or int64, row-major),
 // outputs written to <case_dir>/out_<name>.bin.  Scalars come from
 // <case_dir>/meta.txt ("key value" lines).
@@ -29,6 +29,7 @@
 #include "vec.h"
 #include "nondestructivetrimesh.h"
 #include <Eigen/Dense>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -284,6 +285,47 @@ int main(int argc, char **argv) {
     store("cA", cA.data(), n);
     store("cC", cC.data(), n);
     store("fau", fo.data(), 3 * n);
+    return 0;
+  }
+  if (mode == "time") {
+    // the reference's per-step residual work on these inputs, `reps` times:
+    // calculateAreaTot + calculateRes (cell law), calculateForce_FA, updateFA
+    // against the substrate (each timed on its own; wall clock, one thread)
+    std::vector<double> ea_raw = load<double>("ea");
+    std::vector<double> cA = load<double>("cA"), cC = load<double>("cC");
+    std::vector<Vec3d> fau = vec3(load<double>("fau"));
+    std::vector<Vec3d> SX = vec3(load<double>("SX")), Sx = vec3(load<double>("Sx"));
+    NonDestructiveTriMesh smesh;
+    build_mesh(smesh, load<long long>("Stris"));
+    std::vector<Eigen::VectorXd> ea(n);
+    for (size_t i = 0; i < n; ++i) {
+      ea[i].resize(6);
+      for (int q = 0; q < 6; ++q) ea[i](q) = ea_raw[6 * i + q];
+    }
+    const int reps = (int)meta["reps"];
+    const double dt = meta["dt"], time = meta["time"], kint = meta["kint"];
+    double t_res = 0, t_fa = 0, t_upd = 0, sink = 0;
+    for (int r = 0; r < reps; ++r) {
+      auto t0 = std::chrono::steady_clock::now();
+      Eigen::Vector4d area = calculateAreaTot(x, mesh);
+      Eigen::Vector3d centroid(area(1), area(2), area(3));
+      Eigen::VectorXd Res(3 * n), Res_ea(6 * n), MM(n), Ad(n), Cd(n);
+      calculateRes(X, x, ea, cA, cC, mesh, dt, Res, Res_ea, MM, Ad, Cd, 1, centroid, time);
+      auto t1 = std::chrono::steady_clock::now();
+      Eigen::VectorXd F(3 * n);
+      calculateForce_FA(F, X, x, cC, fau, mesh, kint);
+      auto t2 = std::chrono::steady_clock::now();
+      std::vector<double> cA2(cA), cC2(cC);
+      std::vector<Vec3d> fau2(fau);
+      updateFA(cA2, cC2, fau2, dt, X, x, mesh, SX, Sx, smesh, kint);
+      auto t3 = std::chrono::steady_clock::now();
+      t_res += std::chrono::duration<double>(t1 - t0).count();
+      t_fa += std::chrono::duration<double>(t2 - t1).count();
+      t_upd += std::chrono::duration<double>(t3 - t2).count();
+      sink += Res(0) + F(0) + cC2[0];
+    }
+    double out[4] = {t_res / reps, t_fa / reps, t_upd / reps, sink};
+    store("time", out, 4);
     return 0;
   }
   std::fprintf(stderr, "refharness: unknown mode %s\n", mode.c_str());
