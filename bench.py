@@ -121,7 +121,7 @@ def oracle_check(cells, tables, common, n_sample=16):
     pick = sorted({order[int(round(k * (len(order) - 1) / (n_sample - 1)))] for k in range(n_sample)})
     sub = [cells[i] for i in pick]
     opts = dict(common, newton_rtol=1e-10, newton_atol=0.0)
-    ctx = W.Context(precision=W.PREC_MIXED, batched=1, use_graph=1, max_refine=8, **opts)
+    ctx = W.Context(precision=W.PREC_MIXED, batched=1, use_graph=1, max_refine=4, **opts)
     configs.load(ctx, sub, tables)
     # each cell's positions right after the Newton iteration that committed
     # its first load step (cells that converge first go on to the next one)
@@ -263,7 +263,7 @@ def timed_steps(ctx, steps, warmup):
 def run_workload(W, configs, name, cells, tables, steps, warmup, common, precision, **opt):
     """Time one workload family; returns its entry of "workloads"."""
     t_setup = time.perf_counter()
-    ctx = W.Context(precision=precision, batched=1, use_graph=1, max_refine=8, **dict(common, **opt))
+    ctx = W.Context(precision=precision, batched=1, use_graph=1, max_refine=4, **dict(common, **opt))
     configs.load(ctx, cells, tables)
     setup = time.perf_counter() - t_setup
     dt = timed_steps(ctx, steps, warmup)
@@ -338,7 +338,7 @@ def main():
     all_dense = dict(dense_max_dof=1 << 30) if a.workload == "replicas1024" else {}
 
     # ---- headline: batched mixed precision, one hipGraph replay per Newton step ----
-    ctx = W.Context(precision=W.PREC_MIXED, batched=1, use_graph=1, max_refine=8, **dict(common, **all_dense))
+    ctx = W.Context(precision=W.PREC_MIXED, batched=1, use_graph=1, max_refine=4, **dict(common, **all_dense))
     configs.load(ctx, cells, tables)
     tuned = None
     if a.autotune != "none":

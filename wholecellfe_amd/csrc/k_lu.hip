@@ -1432,6 +1432,17 @@ template <class T> __device__ __forceinline__ T sum8(T v) {
   }
 }
 
+// 8 consecutive LDS values (32-byte aligned) as two 16-byte reads (FP32)
+template <class T> __device__ __forceinline__ void lds8(T (&v)[8], const T *p) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 a = *reinterpret_cast<const float4 *>(p), b = *reinterpret_cast<const float4 *>(p + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = p[e];
+  }
+}
+
 template <class T, int BWM, bool FWD>
 __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb, int b, int bw, int nb) {
   const int r = threadIdx.x >> 3, q = threadIdx.x & 7;
@@ -1440,8 +1451,10 @@ __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb
   for (int u = 0; u < BWM; ++u) {
     const int j = FWD ? b - bw + u : b + 1 + u;
     if (u < bw && j >= 0 && j < nb) {
+      T yv[8];
+      lds8<T>(yv, y + j * NB + q * 8);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) part += R.a[u][e] * y[j * NB + q * 8 + e];
+      for (int e = 0; e < 8; ++e) part += R.a[u][e] * yv[e];
     }
   }
   part = sum8(part);
@@ -1453,8 +1466,10 @@ __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb
   if (q == 0) sb[r] = y[b * NB + r] - part;
   lds_barrier();
   T t = T(0);
+  T sv[8];
+  lds8<T>(sv, sb + q * 8);
 #pragma unroll
-  for (int e = 0; e < 8; ++e) t += R.d[e] * sb[q * 8 + e];
+  for (int e = 0; e < 8; ++e) t += R.d[e] * sv[e];
   t = sum8(t);
   if (q == 0) y[b * NB + r] = t;
   lds_barrier();
@@ -1466,7 +1481,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams
                                                                 const int *kpos, const double *rhs, double *delta,
                                                                 int accumulate, const float *yfwd) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  __shared__ T sb[64];
+  __shared__ __attribute__((aligned(32))) T sb[64];
   T *y = reinterpret_cast<T *>(smem);
   const int c = cells[blockIdx.x];
   if (cs[c].ir_done) return;
