@@ -188,6 +188,28 @@ int wcfe_finalize(wcfe_ctx *ctx);
  * other cell keeps its current state; the batch is coloured and laid out
  * again. */
 int wcfe_replace_cell(wcfe_ctx *ctx, int32_t cell, const wcfe_cell_desc *desc);
+/* After the host's remeshing of one cell (El Topo's edge flips, splits and
+ * collapses, then defragmentation: main.cpp:190-191): the cell's new
+ * triangles, with its state mapped on the device from its old nodes.  New
+ * node i takes w[i] old[src0[i]] + (1 - w[i]) old[src1[i]] of the positions
+ * X and x, the strain ea, the bound fraction cC and the spring extension fau:
+ * a kept (or renumbered) node has src0 = src1 = its old index and w = 1, a
+ * vertex split from edge (a, b) has a, b and w = 1/2 (El Topo's midpoint
+ * interpolation, edgesplitter.cpp:563-577).  src0, src1, w NULL: the nodes
+ * are kept as they are.  x_new (2N, optional) replaces the mapped current
+ * positions (El Topo's smoothed position of a new vertex).  spr_tri,
+ * spr_bary (optional, one per existing spring): the springs' new host
+ * triangles and weights; bond states and anchors stay with their springs.
+ * The topology is compared by a hash of the triangle list (then element by
+ * element): when neither the triangles nor the maps nor the springs change,
+ * only x_new is written and nothing is laid out again (relaid = 0);
+ * otherwise the cell starts its load step again on the new mesh, the batch is
+ * laid out again and every other cell keeps its state bit for bit
+ * (relaid = 1).  A constant nodal load (f_ext) is reset to zero: set it again
+ * with wcfe_set_external_forces. */
+int wcfe_remesh_cell(wcfe_ctx *ctx, int32_t cell, int32_t n_nodes, int32_t n_tris, const int32_t *tris,
+                     const int32_t *src0, const int32_t *src1, const double *w, const double *x_new,
+                     const int32_t *spr_tri, const double *spr_bary, int32_t *relaid);
 /* Binary checkpoint of every cell (mesh, material, positions, strains, node
  * springs, bond states and anchors, load time and step -- the Bell stream
  * position) and the force tables.  Options are not stored: load into a

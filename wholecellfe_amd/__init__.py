@@ -37,7 +37,7 @@ EXPORTS = [
     "wcfe_copy_probe", "wcfe_autotune", "wcfe_set_ul_state", "wcfe_replace_cell", "wcfe_checkpoint_save",
     "wcfe_checkpoint_load", "wcfe_area_tot", "wcfe_bending_force_2d", "wcfe_area_constraint_force",
     "wcfe_force_ap", "wcfe_get_schedule", "wcfe_num_cells", "wcfe_get_cell_sizes",
-    "wcfe_eval_exp", "wcfe_get_cell_layout", "wcfe_force_fa_noalloc",
+    "wcfe_eval_exp", "wcfe_get_cell_layout", "wcfe_force_fa_noalloc", "wcfe_remesh_cell",
 ]
 TUNE_SOLVER, TUNE_LU = 1, 2
 # reference constants of the boundary and body forces (UL_growth_explicit.cpp)
@@ -211,6 +211,23 @@ class Context:
         self._check(self._L.wcfe_replace_cell(self._h, cell, C.byref(d)))
         self.cells[cell] = dict(n_nodes=d.n_nodes, n_tris=d.n_tris, n_spr=d.n_spr)
         self._keep = []
+
+    def remesh_cell(self, cell, tris, src0=None, src1=None, w=None, x_new=None, spr_tri=None, spr_bary=None):
+        """One cell's new triangles after remeshing, its state mapped on the
+        device (new node i = w_i old[src0_i] + (1 - w_i) old[src1_i]);
+        returns 1 when the batch was laid out again, 0 when the topology did
+        not change."""
+        tris = _i32(tris).reshape(-1)
+        n = self.cells[cell]["n_nodes"] if src0 is None else len(src0)
+        arrs = [_i32(src0), _i32(src1), _f64(w), None if x_new is None else _f64(x_new).reshape(-1),
+                None if spr_tri is None else _i32(spr_tri).reshape(-1),
+                None if spr_bary is None else _f64(spr_bary).reshape(-1)]
+        relaid = C.c_int32(-1)
+        self._check(self._L.wcfe_remesh_cell(self._h, cell, n, len(tris) // 3, _ip(tris), _ip(arrs[0]), _ip(arrs[1]),
+                                             _dp(arrs[2]), _dp(arrs[3]), _ip(arrs[4]), _dp(arrs[5]),
+                                             C.byref(relaid)))
+        self.cells[cell] = dict(n_nodes=n, n_tris=len(tris) // 3, n_spr=self.cells[cell]["n_spr"])
+        return relaid.value
 
     def checkpoint_save(self, path):
         self._check(self._L.wcfe_checkpoint_save(self._h, str(path).encode()))

@@ -631,6 +631,36 @@ void launch_commit_springs(hipStream_t s, const View &v, const int *sp_cell, con
                      anchor, s_begin, s_end, dt, k_on, k_off0, f_b, key_lo, key_hi, replica_of_cell, transitions);
 }
 
+// A remeshed cell's nodes from its old nodes (the host's remeshing maps): new
+// node i takes w_i old[src0_i] + (1 - w_i) old[src1_i] of the start-of-step
+// and current positions, the strain, the bound fraction and the spring
+// extension -- El Topo's midpoint interpolation of a split vertex
+// (edgesplitter.cpp:563-577) is w = 1/2, a kept or renumbered vertex
+// (defragmentation, surftrack.cpp:487-600) w = 1.  Rounded as written.
+__global__ __launch_bounds__(256) void k_remap_nodes(int n_new, const int *src0, const int *src1, const double *w,
+                                                     int old_off, const double2 *X, const double2 *x, const double *ea,
+                                                     const double *cC, const double2 *fau, double2 *Xn, double2 *xn,
+                                                     double *ean, double *cCn, double2 *faun) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_new) return;
+  const int a = old_off + src0[i], b = old_off + src1[i];
+  const double wa = w[i], wb = 1.0 - w[i];
+  auto mix2 = [&](const double2 *v) { return make_double2(wa * v[a].x + wb * v[b].x, wa * v[a].y + wb * v[b].y); };
+  Xn[i] = mix2(X);
+  xn[i] = mix2(x);
+  faun[i] = mix2(fau);
+  cCn[i] = wa * cC[a] + wb * cC[b];
+  for (int q = 0; q < 3; ++q) ean[3 * i + q] = wa * ea[3 * a + q] + wb * ea[3 * b + q];
+}
+void launch_remap_nodes(hipStream_t s, int n_new, const int *src0, const int *src1, const double *w, int old_off,
+                        const double2 *X, const double2 *x, const double *ea, const double *cC, const double2 *fau,
+                        double2 *Xn, double2 *xn, double *ean, double *cCn, double2 *faun) {
+  if (n_new <= 0) return;
+  hipLaunchKernelGGL(k_remap_nodes, dim3((n_new + 255) / 256), dim3(256), 0, s, n_new, src0, src1, w, old_off, X, x,
+                     ea, cC, fau, Xn, xn, ean, cCn, faun);
+}
+
 __global__ void k_commit_cells(CellState *cs, const int *cells, int ncell, double dt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncell) return;

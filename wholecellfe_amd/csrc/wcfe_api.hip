@@ -72,7 +72,24 @@ struct HostCell {
   std::vector<double> f_ext;
   std::vector<double> X, x, ea, cC, fau, spr_bary, spr_anchor, spr_state;
   std::vector<int> tris, spr_tri;
+  uint64_t topo_hash = 0;  // of the triangle list (topology_hash)
 };
+
+// FNV-1a over the triangle list: a cell's topology as the host's remeshing
+// leaves it (wcfe_remesh_cell lays a cell out again only when it changes).
+uint64_t topology_hash(int n_nodes, int n_tris, const int *tris) {
+  uint64_t h = 1469598103934665603ULL;
+  auto mix = [&](uint32_t v) {
+    for (int k = 0; k < 4; ++k) {
+      h ^= (v >> (8 * k)) & 0xFFu;
+      h *= 1099511628211ULL;
+    }
+  };
+  mix((uint32_t)n_nodes);
+  mix((uint32_t)n_tris);
+  for (long long i = 0; i < 3LL * n_tris; ++i) mix((uint32_t)tris[i]);
+  return h;
+}
 
 struct Table {
   std::vector<double> px, py, k;
@@ -1744,6 +1761,7 @@ HostCell host_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d) {
     h.tris.assign(d->tris, d->tris + 3 * d->n_tris);
     for (int v : h.tris)
       if (v < 0 || v >= d->n_nodes) fail(WCFE_ERR_ARG, "triangle node index %d out of range", v);
+    h.topo_hash = topology_hash(d->n_nodes, d->n_tris, d->tris);
     if (d->ea) h.ea.assign(d->ea, d->ea + 3 * d->n_nodes);
     if (d->cC) h.cC.assign(d->cC, d->cC + d->n_nodes);
     if (d->fau) h.fau.assign(d->fau, d->fau + 2 * d->n_nodes);
@@ -3059,6 +3077,95 @@ int wcfe_replace_cell(wcfe_ctx *ctx, int32_t cell, const wcfe_cell_desc *d) {
 }
 
 
+
+int wcfe_remesh_cell(wcfe_ctx *ctx, int32_t cell, int32_t n_nodes, int32_t n_tris, const int32_t *tris,
+                     const int32_t *src0, const int32_t *src1, const double *w, const double *x_new,
+                     const int32_t *spr_tri, const double *spr_bary, int32_t *relaid) {
+  if (!ctx || !tris || n_nodes <= 0 || n_tris <= 0) return WCFE_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_cell(ctx, cell);
+    HostCell &old = ctx->cells[cell];
+    Batch &b = ctx->b;
+    const CellParams &p = b.cp_h[cell];
+    if ((src0 == nullptr) != (src1 == nullptr) || (src0 == nullptr) != (w == nullptr))
+      fail(WCFE_ERR_ARG, "src0, src1 and w come together (or all NULL: the nodes are kept as they are)");
+    if (!src0 && n_nodes != old.n_nodes) fail(WCFE_ERR_ARG, "a changed node count needs the node maps");
+    for (long long i = 0; i < 3LL * n_tris; ++i)
+      if (tris[i] < 0 || tris[i] >= n_nodes) fail(WCFE_ERR_ARG, "triangle node index %d out of range", tris[i]);
+    bool identity = true;
+    if (src0) {
+      for (int i = 0; i < n_nodes; ++i) {
+        if (src0[i] < 0 || src0[i] >= old.n_nodes || src1[i] < 0 || src1[i] >= old.n_nodes)
+          fail(WCFE_ERR_ARG, "node map of new node %d outside the old cell's %d nodes", i, old.n_nodes);
+        if (!(w[i] >= 0.0 && w[i] <= 1.0)) fail(WCFE_ERR_ARG, "interpolation weight of node %d outside [0, 1]", i);
+        identity &= src0[i] == i && (w[i] == 1.0 || src1[i] == i);
+      }
+      identity &= n_nodes == old.n_nodes;
+    }
+    if ((spr_tri == nullptr) != (spr_bary == nullptr)) fail(WCFE_ERR_ARG, "spr_tri and spr_bary come together");
+    for (int k = 0; spr_tri && k < old.n_spr; ++k)
+      if (spr_tri[k] < 0 || spr_tri[k] >= n_tris) fail(WCFE_ERR_ARG, "spring host triangle %d out of range", spr_tri[k]);
+    const uint64_t hash = topology_hash(n_nodes, n_tris, tris);
+    const bool same_mesh = hash == old.topo_hash && n_tris == old.n_tris && n_nodes == old.n_nodes &&
+                           std::equal(tris, tris + 3LL * n_tris, old.tris.begin());
+    const bool same_springs =
+        !spr_tri || (std::equal(spr_tri, spr_tri + old.n_spr, old.spr_tri.begin()) &&
+                     std::equal(spr_bary, spr_bary + 3LL * old.n_spr, old.spr_bary.begin()));
+    if (same_mesh && identity && same_springs) {
+      // the topology did not change: positions (if given) in place, no layout
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+      if (x_new)
+        HIPCHECK(hipMemcpy(b.x + p.node_off, x_new, sizeof(double2) * p.n_nodes, hipMemcpyHostToDevice));
+      if (relaid) *relaid = 0;
+      return;
+    }
+    // the cell's new nodes from its old ones, on the device
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    Scratch sc;
+    std::vector<int> s0(n_nodes), s1(n_nodes);
+    std::vector<double> ww(n_nodes, 1.0);
+    for (int i = 0; i < n_nodes; ++i) {
+      s0[i] = src0 ? src0[i] : i;
+      s1[i] = src1 ? src1[i] : i;
+      if (w) ww[i] = w[i];
+    }
+    const int *ds0 = sc.up(s0.data(), s0.size()), *ds1 = sc.up(s1.data(), s1.size());
+    const double *dw = sc.up(ww.data(), ww.size());
+    double2 *Xn = sc.zeros<double2>(n_nodes), *xn = sc.zeros<double2>(n_nodes), *fn = sc.zeros<double2>(n_nodes);
+    double *en = sc.zeros<double>(3 * (size_t)n_nodes), *cn = sc.zeros<double>(n_nodes);
+    launch_remap_nodes(ctx->stream, n_nodes, ds0, ds1, dw, p.node_off, b.X, b.x, b.ea, b.cC, b.fau, Xn, xn, en, cn, fn);
+    HIPCHECK(hipGetLastError());
+    state_to_host(ctx);  // every cell's state (this cell's springs too)
+    HostCell h = old;
+    h.n_nodes = n_nodes;
+    h.n_tris = n_tris;
+    h.tris.assign(tris, tris + 3LL * n_tris);
+    h.topo_hash = hash;
+    h.X.resize(2 * (size_t)n_nodes);
+    h.x.resize(2 * (size_t)n_nodes);
+    h.ea.resize(3 * (size_t)n_nodes);
+    h.cC.resize(n_nodes);
+    h.fau.resize(2 * (size_t)n_nodes);
+    HIPCHECK(hipMemcpy(h.X.data(), Xn, sizeof(double2) * n_nodes, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(h.x.data(), xn, sizeof(double2) * n_nodes, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(h.ea.data(), en, sizeof(double) * 3 * n_nodes, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(h.cC.data(), cn, sizeof(double) * n_nodes, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(h.fau.data(), fn, sizeof(double2) * n_nodes, hipMemcpyDeviceToHost));
+    if (x_new) h.x.assign(x_new, x_new + 2 * (size_t)n_nodes);
+    if (!h.f_ext.empty()) {  // a constant load per node does not interpolate: the host sets it again
+      h.f_ext.assign(2 * (size_t)n_nodes, 0.0);
+    }
+    if (spr_tri) {  // bond states and anchors stay with their springs
+      h.spr_tri.assign(spr_tri, spr_tri + old.n_spr);
+      h.spr_bary.assign(spr_bary, spr_bary + 3LL * old.n_spr);
+    }
+    h.has_state = false;  // the cell starts its load step again on the new mesh
+    ctx->cells[cell] = std::move(h);
+    rebuild_batch(ctx);
+    ctx->stepped = true;
+    if (relaid) *relaid = 1;
+  });
+}
 
 int wcfe_checkpoint_save(wcfe_ctx *ctx, const char *path) {
   if (!ctx || !path) return WCFE_ERR_ARG;

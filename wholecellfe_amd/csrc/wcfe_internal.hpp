@@ -66,6 +66,9 @@ void launch_commit_springs(hipStream_t s, const View &v, const int *sp_cell, con
                            double k_off0, double f_b, unsigned key_lo, unsigned key_hi, const int *replica_of_cell,
                            int *transitions);
 void launch_commit_cells(hipStream_t s, CellState *cs, const int *cells, int ncell, double dt);
+void launch_remap_nodes(hipStream_t s, int n_new, const int *src0, const int *src1, const double *w, int old_off,
+                        const double2 *X, const double2 *x, const double *ea, const double *cC, const double2 *fau,
+                        double2 *Xn, double2 *xn, double *ean, double *cCn, double2 *faun);
 template <class T>
 void launch_diag_tau(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell, const T *K);
 
