@@ -197,17 +197,18 @@ def _nnz_blocks(c):
 
 
 def pcg_roofline(cells, prof, st):
-    """Block-CSR PCG (FP32): HBM-bound; algorithmic bytes per iteration
-    20 nnzb (2 x 2 FP32 values + column index per block) + 116 N (row
-    pointers, the five PCG vectors, the 2 x 2 diagonal inverses) over the PCG
-    iterations of the solve phase."""
+    """Block-CSR PCG (FP32, single-reduction form): HBM-bound; algorithmic
+    bytes per iteration 20 nnzb (2 x 2 FP32 values + column index per block)
+    + 124 N (row pointers 4; update: p, s, w, x, r read and p, s, x, r, u
+    written, 8 each, the 2 x 2 diagonal inverse 16; SpMV: u, r read and w
+    written, 8 each) over the PCG iterations of the solve phase."""
     nnzb = sum(_nnz_blocks(c) for c in cells if c.get("solver", 0) == 2)
     nodes = sum(len(c["X"]) for c in cells if c.get("solver", 0) == 2)
     iters = max(st.get("max_cg_iters", 0), 1) * max(st.get("max_refine_used", 1), 1)
-    bytes_iter = 20.0 * nnzb + 116.0 * nodes
+    bytes_iter = 20.0 * nnzb + 124.0 * nodes
     ms = prof["solve"]["ms"]
     achieved = bytes_iter * iters / (ms * 1e-3) if ms > 0 else 0.0
-    return {"kernel": "block-CSR PCG (k_pcg_spmv_dot + k_pcg_update + k_pcg_paxpy)", "bound": "hbm",
+    return {"kernel": "block-CSR PCG (k_pcg_update + k_pcg_spmv_dot + k_pcg_scalars)", "bound": "hbm",
             "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
             "traffic": None, "nnz_blocks": nnzb, "nodes": nodes, "pcg_iterations": iters,
             "bytes_per_iteration": bytes_iter}

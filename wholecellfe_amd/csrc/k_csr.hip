@@ -66,114 +66,111 @@ __device__ __forceinline__ void apply2(const T *M, T x0, T x1, T &y0, T &y1) {
   y1 = M[2] * x0 + M[3] * x1;
 }
 
-// r = rhs (FP64 -> T), x = 0, z = M^-1 r, p = z; partials (r.z, r.r)
+// Preconditioned conjugate gradients in the single-reduction form of
+// Chronopoulos and Gear ("s-step iterative methods for symmetric linear
+// systems", J. Comput. Appl. Math. 25, 1989): with u = M^-1 r and w = K u,
+// gamma = r.u and delta = w.u come from ONE reduction per iteration, and
+//   beta = gamma_new / gamma,  alpha = gamma_new / (delta - beta gamma_new / alpha)
+//   p = u + beta p,  s = w + beta s  (s = K p),  x += alpha p,  r -= alpha s
+// -- the same iterates as the two-reduction form in exact arithmetic.  Per
+// iteration: one element-wise kernel (p, s, x, r, u), one SpMV with the
+// three partial dot products (r.u, w.u, r.r), one per-cell scalar kernel.
+
+// r = rhs (FP64 -> T), x = 0, u = M^-1 r, p = s = 0 (so the first update
+// gives p = u, s = w)
 template <class T>
-__global__ __launch_bounds__(CG_ROWS) void k_pcg_init(const int4 *chunks, int cbase, const CellState *cs,
-                                                      const double *rhs, const T *dinv, T *x, T *r, T *z, T *p,
-                                                      double *partials) {
-  __shared__ double red[CG_ROWS];
+__global__ __launch_bounds__(CG_ROWS) void k_pcg_init(const int4 *chunks, const CellState *cs, const double *rhs,
+                                                      const T *dinv, T *x, T *r, T *u, T *p, T *sv) {
   const int4 ch = chunks[blockIdx.x];
   if (cs[ch.x].ir_done) return;
   const int i = ch.y + threadIdx.x;
-  double rz = 0.0, rr = 0.0;
-  if (i < ch.z) {
-    const T r0 = (T)rhs[2LL * i], r1 = (T)rhs[2LL * i + 1];
-    T z0, z1;
-    apply2(dinv + 4LL * i, r0, r1, z0, z1);
-    r[2LL * i] = r0; r[2LL * i + 1] = r1;
-    z[2LL * i] = z0; z[2LL * i + 1] = z1;
-    p[2LL * i] = z0; p[2LL * i + 1] = z1;
-    x[2LL * i] = T(0); x[2LL * i + 1] = T(0);
-    rz = (double)(r0 * z0 + r1 * z1);
-    rr = (double)(r0 * r0 + r1 * r1);
-  }
-  rz = block_sum256(rz, red);
-  rr = block_sum256(rr, red);
-  if (threadIdx.x == 0) {
-    partials[2LL * (cbase + blockIdx.x)] = rz;
-    partials[2LL * (cbase + blockIdx.x) + 1] = rr;
-  }
+  if (i >= ch.z) return;
+  const long long k = 2LL * i;
+  const T r0 = (T)rhs[k], r1 = (T)rhs[k + 1];
+  T u0, u1;
+  apply2(dinv + 4LL * i, r0, r1, u0, u1);
+  r[k] = r0; r[k + 1] = r1;
+  u[k] = u0; u[k + 1] = u1;
+  x[k] = T(0); x[k + 1] = T(0);
+  p[k] = T(0); p[k + 1] = T(0);
+  sv[k] = T(0); sv[k + 1] = T(0);
 }
 
-// q = K p; partial p.q
+// w = K u; partials (r.u, w.u, r.r) of the chunk
 template <class T>
 __global__ __launch_bounds__(CG_ROWS) void k_pcg_spmv_dot(const int4 *chunks, int cbase, const CgState *cg,
                                                           const int *row_ptr, const int *col, const T *val,
-                                                          const T *p, T *q, double *partials) {
+                                                          const T *r, const T *u, T *w, double *partials) {
   __shared__ double red[CG_ROWS];
   const int4 ch = chunks[blockIdx.x];
   if (cg[ch.x].done) return;
   const int i = ch.y + threadIdx.x;
-  double pq = 0.0;
+  double ru = 0.0, wu = 0.0, rr = 0.0;
   if (i < ch.z) {
     typedef typename V2<T>::type v2;
-    const v2 *p2 = reinterpret_cast<const v2 *>(p);
+    const v2 *u2 = reinterpret_cast<const v2 *>(u);
     T y0 = T(0), y1 = T(0);
     for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
-      const v2 pj = p2[col[k]];
+      const v2 uj = u2[col[k]];
       const T *B = val + 4LL * k;
-      y0 += B[0] * pj.x + B[1] * pj.y;
-      y1 += B[2] * pj.x + B[3] * pj.y;
+      y0 += B[0] * uj.x + B[1] * uj.y;
+      y1 += B[2] * uj.x + B[3] * uj.y;
     }
-    q[2LL * i] = y0;
-    q[2LL * i + 1] = y1;
-    const v2 pi = p2[i];
-    pq = (double)(pi.x * y0 + pi.y * y1);
-  }
-  pq = block_sum256(pq, red);
-  if (threadIdx.x == 0) partials[2LL * (cbase + blockIdx.x)] = pq;
-}
-
-// x += alpha p, r -= alpha q, z = M^-1 r; partials (r.z, r.r)
-template <class T>
-__global__ __launch_bounds__(CG_ROWS) void k_pcg_update(const int4 *chunks, int cbase, const CgState *cg,
-                                                        const T *dinv, const T *p, const T *q, T *x, T *r, T *z,
-                                                        double *partials) {
-  __shared__ double red[CG_ROWS];
-  const int4 ch = chunks[blockIdx.x];
-  if (cg[ch.x].done) return;
-  const T alpha = (T)cg[ch.x].alpha;
-  const int i = ch.y + threadIdx.x;
-  double rz = 0.0, rr = 0.0;
-  if (i < ch.z) {
-    const long long k = 2LL * i;
-    x[k] += alpha * p[k];
-    x[k + 1] += alpha * p[k + 1];
-    const T r0 = r[k] - alpha * q[k], r1 = r[k + 1] - alpha * q[k + 1];
-    r[k] = r0;
-    r[k + 1] = r1;
-    T z0, z1;
-    apply2(dinv + 4LL * i, r0, r1, z0, z1);
-    z[k] = z0;
-    z[k + 1] = z1;
-    rz = (double)(r0 * z0 + r1 * z1);
+    w[2LL * i] = y0;
+    w[2LL * i + 1] = y1;
+    const v2 ui = u2[i];
+    const T r0 = r[2LL * i], r1 = r[2LL * i + 1];
+    ru = (double)(r0 * ui.x + r1 * ui.y);
+    wu = (double)(y0 * ui.x + y1 * ui.y);
     rr = (double)(r0 * r0 + r1 * r1);
   }
-  rz = block_sum256(rz, red);
+  ru = block_sum256(ru, red);
+  wu = block_sum256(wu, red);
   rr = block_sum256(rr, red);
   if (threadIdx.x == 0) {
-    partials[2LL * (cbase + blockIdx.x)] = rz;
-    partials[2LL * (cbase + blockIdx.x) + 1] = rr;
+    double *o = partials + 3LL * (cbase + blockIdx.x);
+    o[0] = ru;
+    o[1] = wu;
+    o[2] = rr;
   }
 }
 
-// p = z + beta p
+// p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = M^-1 r
 template <class T>
-__global__ __launch_bounds__(CG_ROWS) void k_pcg_paxpy(const int4 *chunks, const CgState *cg, const T *z, T *p) {
+__global__ __launch_bounds__(CG_ROWS) void k_pcg_update(const int4 *chunks, const CgState *cg, const T *dinv, T *p,
+                                                        T *sv, const T *w, T *x, T *r, T *u) {
   const int4 ch = chunks[blockIdx.x];
-  if (cg[ch.x].done) return;
-  const T beta = (T)cg[ch.x].beta;
+  const CgState &g = cg[ch.x];
+  if (g.done) return;
+  const T alpha = (T)g.alpha, beta = (T)g.beta;
   const int i = ch.y + threadIdx.x;
   if (i >= ch.z) return;
-  p[2LL * i] = z[2LL * i] + beta * p[2LL * i];
-  p[2LL * i + 1] = z[2LL * i + 1] + beta * p[2LL * i + 1];
+  const long long k = 2LL * i;
+  const T p0 = u[k] + beta * p[k], p1 = u[k + 1] + beta * p[k + 1];
+  const T s0 = w[k] + beta * sv[k], s1 = w[k + 1] + beta * sv[k + 1];
+  p[k] = p0; p[k + 1] = p1;
+  sv[k] = s0; sv[k + 1] = s1;
+  x[k] += alpha * p0;
+  x[k + 1] += alpha * p1;
+  const T r0 = r[k] - alpha * s0, r1 = r[k + 1] - alpha * s1;
+  r[k] = r0; r[k + 1] = r1;
+  T u0, u1;
+  apply2(dinv + 4LL * i, r0, r1, u0, u1);
+  u[k] = u0; u[k + 1] = u1;
 }
 
-// Per-cell scalars, one workgroup per sparse cell of the list; partial sums
-// of the cell's chunks in chunk order.
-//   CG_INIT:  rz, rr0 from (r.z, r.r); done if refinement is finished or r = 0
-//   CG_ALPHA: alpha = rz / p.q
-//   CG_BETA:  beta = rz_new / rz; convergence ||r|| <= rtol ||r0|| or iteration cap
+// a new solve: cells still refining run it, the others stay done
+__global__ void k_pcg_reset_done(const int *cells, int ncell, const CellState *cs, CgState *cg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncell) cg[cells[i]].done = cs[cells[i]].ir_done ? 1 : 0;
+}
+
+// Per-cell scalars, one workgroup per sparse cell of the list, from the
+// partial sums of the cell's chunks in chunk order.
+//   CG_INIT: gamma, delta of the initial u; alpha = gamma / delta, beta = 0;
+//            done if refinement is finished or r = 0
+//   CG_STEP: the iteration's scalars; convergence ||r|| <= rtol ||r0|| (r of
+//            the update just made) or the iteration cap
 __global__ __launch_bounds__(256) void k_pcg_scalars(const int *cells, const CellState *cs, CgState *cg,
                                                      const double *partials, int mode, double rtol, int max_iter) {
   __shared__ double red[256];
@@ -183,36 +180,53 @@ __global__ __launch_bounds__(256) void k_pcg_scalars(const int *cells, const Cel
     if (mode == CG_INIT && threadIdx.x == 0) g.done = 1;
     return;
   }
-  double s0 = 0.0, s1 = 0.0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   for (int k = threadIdx.x; k < g.nchunk; k += 256) {
-    s0 += partials[2LL * (g.chunk_off + k)];
-    s1 += partials[2LL * (g.chunk_off + k) + 1];
+    const double *o = partials + 3LL * (g.chunk_off + k);
+    s0 += o[0];
+    s1 += o[1];
+    s2 += o[2];
   }
   s0 = block_sum256(s0, red);
-  if (mode != CG_ALPHA) s1 = block_sum256(s1, red);
+  s1 = block_sum256(s1, red);
+  s2 = block_sum256(s2, red);
   if (threadIdx.x != 0) return;
+  const double gamma = s0, delta = s1, rr = s2;
   if (mode == CG_INIT) {
-    g.rz = s0;
-    g.rr = g.rr0 = s1;
     g.iters = 0;
     g.breakdown = 0;
-    g.done = s1 > 0.0 ? 0 : 1;
-  } else if (mode == CG_ALPHA) {
-    g.pq = s0;
-    if (s0 > 0.0) {
-      g.alpha = g.rz / s0;
-    } else {  // K p . p <= 0: the system is not positive definite along p
-      g.alpha = 0.0;
-      g.breakdown = 1;
-      g.done = 1;
+    g.rr = g.rr0 = rr;
+    g.rz = gamma;
+    g.beta = 0.0;
+    g.done = rr > 0.0 ? 0 : 1;
+    if (!g.done) {
+      if (delta > 0.0) {
+        g.alpha = gamma / delta;
+      } else {  // K u . u <= 0: the system is not positive definite along u
+        g.alpha = 0.0;
+        g.breakdown = 1;
+        g.done = 1;
+      }
     }
-  } else {
-    g.iters += 1;
-    g.beta = g.rz > 0.0 ? s0 / g.rz : 0.0;
-    g.rz = s0;
-    g.rr = s1;
-    if (s1 <= rtol * rtol * g.rr0 || g.iters >= max_iter || !(s1 == s1)) g.done = 1;
+    return;
   }
+  g.iters += 1;
+  g.rr = rr;
+  if (rr <= rtol * rtol * g.rr0 || g.iters >= max_iter || !(rr == rr)) {
+    g.done = 1;
+    return;
+  }
+  const double beta = g.rz > 0.0 ? gamma / g.rz : 0.0;
+  const double den = delta - beta * gamma / g.alpha;
+  g.pq = den;
+  if (!(den > 0.0)) {  // the step's curvature p.Kp <= 0
+    g.breakdown = 1;
+    g.done = 1;
+    return;
+  }
+  g.beta = beta;
+  g.alpha = gamma / den;
+  g.rz = gamma;
 }
 
 // delta (FP64) = x, or += x for the later refinement sweeps
@@ -236,11 +250,18 @@ void launch_bsr_diag_inv(hipStream_t s, const int4 *chunks, int nchunk, const in
 }
 
 template <class T>
-void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const CellState *cs,
-                     const double *rhs, const CgVectors<T> &v, double *partials) {
+void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const int *cells, int ncell,
+                     const CellState *cs, CgState *cg, const int *row_ptr, const int *col, const T *val,
+                     const double *rhs, const CgVectors<T> &v, double *partials, double rtol, int max_iter) {
   if (nchunk <= 0) return;
-  hipLaunchKernelGGL(k_pcg_init<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks + cbase, cbase, cs, rhs, v.dinv, v.x,
-                     v.r, v.z, v.p, partials);
+  hipLaunchKernelGGL(k_pcg_reset_done, dim3((ncell + 63) / 64), dim3(64), 0, s, cells, ncell, cs, cg);
+  hipLaunchKernelGGL(k_pcg_init<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks + cbase, cs, rhs, v.dinv, v.x, v.r,
+                     v.z, v.p, v.s);
+  // w = K u and the partials of the first step
+  hipLaunchKernelGGL(k_pcg_spmv_dot<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks + cbase, cbase, cg, row_ptr, col,
+                     val, v.r, v.z, v.q, partials);
+  hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(256), 0, s, cells, cs, cg, partials, (int)CG_INIT, rtol,
+                     max_iter);
 }
 
 template <class T>
@@ -249,15 +270,12 @@ void launch_pcg_iteration(hipStream_t s, const int4 *chunks, int cbase, int nchu
                           const CgVectors<T> &v, double *partials, double rtol, int max_iter) {
   if (nchunk <= 0) return;
   chunks += cbase;
-  hipLaunchKernelGGL(k_pcg_spmv_dot<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks, cbase, cg, row_ptr, col, val,
-                     v.p, v.q, partials);
-  hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(256), 0, s, cells, cs, cg, partials, (int)CG_ALPHA, rtol,
+  hipLaunchKernelGGL(k_pcg_update<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks, cg, v.dinv, v.p, v.s, v.q, v.x, v.r,
+                     v.z);
+  hipLaunchKernelGGL(k_pcg_spmv_dot<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks, cbase, cg, row_ptr, col, val, v.r,
+                     v.z, v.q, partials);
+  hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(256), 0, s, cells, cs, cg, partials, (int)CG_STEP, rtol,
                      max_iter);
-  hipLaunchKernelGGL(k_pcg_update<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks, cbase, cg, v.dinv, v.p, v.q, v.x,
-                     v.r, v.z, partials);
-  hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(256), 0, s, cells, cs, cg, partials, (int)CG_BETA, rtol,
-                     max_iter);
-  hipLaunchKernelGGL(k_pcg_paxpy<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks, cg, v.z, v.p);
 }
 
 void launch_pcg_scalars(hipStream_t s, const int *cells, int ncell, const CellState *cs, CgState *cg,
@@ -276,8 +294,9 @@ void launch_pcg_finish(hipStream_t s, const int4 *chunks, int nchunk, const Cell
 #define WCFE_CSR_INST(T)                                                                                          \
   template void launch_zero<T>(hipStream_t, T *, long long);                                                      \
   template void launch_bsr_diag_inv<T>(hipStream_t, const int4 *, int, const int *, const T *, T *);              \
-  template void launch_pcg_init<T>(hipStream_t, const int4 *, int, int, const CellState *, const double *,        \
-                                   const CgVectors<T> &, double *);                                               \
+  template void launch_pcg_init<T>(hipStream_t, const int4 *, int, int, const int *, int, const CellState *,       \
+                                   CgState *, const int *, const int *, const T *, const double *,                \
+                                   const CgVectors<T> &, double *, double, int);                                  \
   template void launch_pcg_iteration<T>(hipStream_t, const int4 *, int, int, const int *, int, const CellState *, \
                                         CgState *, const int *, const int *, const T *, const CgVectors<T> &,     \
                                         double *, double, int);                                                   \

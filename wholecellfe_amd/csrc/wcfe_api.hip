@@ -547,7 +547,7 @@ struct Batch {
   long long nnzb = 0;
   int *row_ptr = nullptr, *bcol = nullptr, *diag_pos = nullptr, *el_bpos = nullptr, *csr_cells = nullptr;
   int4 *chunks = nullptr;
-  void *Kb = nullptr, *cgvec = nullptr;         // block values; x, r, z, p, q (2N each) and dinv (4N)
+  void *Kb = nullptr, *cgvec = nullptr;         // block values; x, r, z, p, q, s (2N each) and dinv (4N)
   CgState *cg = nullptr;
   double *partials = nullptr;
   // dense numbering: per node its position in the cell's dense system; the
@@ -1118,7 +1118,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.chunks = dupload(ch);
     b.csr_cells = dupload(b.csr_cells_h);
     b.Kb = (void *)dalloc<unsigned char>((size_t)std::max<long long>(b.nnzb, 1) * 4 * elem_size);
-    b.cgvec = (void *)dalloc<unsigned char>((size_t)std::max(N, 1) * 14 * elem_size);
+    b.cgvec = (void *)dalloc<unsigned char>((size_t)std::max(N, 1) * 16 * elem_size);
     std::vector<CgState> cg(b.ncell);
     for (int c = 0; c < b.ncell; ++c) {
       memset(&cg[c], 0, sizeof cg[c]);
@@ -1127,7 +1127,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
       cg[c].done = 1;
     }
     b.cg = dupload(cg);
-    b.partials = dalloc<double>(2 * (size_t)std::max(b.nchunk, 1));
+    b.partials = dalloc<double>(3 * (size_t)std::max(b.nchunk, 1));
   }
   {
     // node gather plan: for node n, its distinct column nodes in order of
@@ -1228,7 +1228,7 @@ bool scope_sparse(const Batch &b, const Scope &sc, int &cbase, int &nch, const i
 template <class T> CgVectors<T> cg_vectors(const Batch &b) {
   T *base = static_cast<T *>(b.cgvec);
   const long long n2 = 2LL * b.N;
-  return CgVectors<T>{base, base + n2, base + 2 * n2, base + 3 * n2, base + 4 * n2, base + 5 * n2};
+  return CgVectors<T>{base, base + n2, base + 2 * n2, base + 3 * n2, base + 4 * n2, base + 5 * n2, base + 6 * n2};
 }
 
 BndView bnd_view(const wcfe_ctx *ctx, const Batch &b) {
@@ -1376,8 +1376,8 @@ template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, in
   const CgVectors<T> v = cg_vectors<T>(b);
   const double rtol = ctx->opt.cg_rtol;
   const int maxit = std::max(1, ctx->opt.cg_max_iter);
-  launch_pcg_init<T>(s, b.chunks, cb, nch, b.cs, b.r, v, b.partials);
-  launch_pcg_scalars(s, cl, ncl, b.cs, b.cg, b.partials, CG_INIT, rtol, maxit);
+  launch_pcg_init<T>(s, b.chunks, cb, nch, cl, ncl, b.cs, b.cg, b.row_ptr, b.bcol, static_cast<const T *>(b.Kb),
+                     b.r, v, b.partials, rtol, maxit);
   std::vector<int> ids = sc.cell < 0 ? b.csr_cells_h : std::vector<int>{sc.cell};
   std::vector<CgState> h(b.ncell);
   for (int it = 0; it < maxit; it += CG_CHECK) {

@@ -155,7 +155,7 @@ hipError_t lu_prepare_attributes(size_t max_lds);
 
 // ---- sparse path: block-CSR systems and batched PCG (k_csr.hip) ----
 constexpr int CG_ROWS = 256;  // nodes per chunk (one thread per node)
-enum CgMode : int { CG_INIT = 0, CG_ALPHA = 1, CG_BETA = 2 };
+enum CgMode : int { CG_INIT = 0, CG_STEP = 1 };
 // per-cell conjugate-gradient state (one entry per cell; used by sparse cells)
 struct CgState {
   double rz, rr, rr0, pq, alpha, beta;
@@ -164,15 +164,16 @@ struct CgState {
   int pad_[2];
 };
 template <class T> struct CgVectors {
-  T *x, *r, *z, *p, *q, *dinv;  // 2 N each (dinv: 4 N)
+  T *x, *r, *z, *p, *q, *s, *dinv;  // 2 N each (dinv: 4 N); z = M^-1 r, q = K z, s = K p
 };
 template <class T> void launch_zero(hipStream_t s, T *p, long long n);
 template <class T>
 void launch_bsr_diag_inv(hipStream_t s, const int4 *chunks, int nchunk, const int *diag_pos, const T *val, T *dinv);
 // chunks [cbase, cbase + nchunk) of the batch's chunk list
 template <class T>
-void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const CellState *cs,
-                     const double *rhs, const CgVectors<T> &v, double *partials);
+void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const int *cells, int ncell,
+                     const CellState *cs, CgState *cg, const int *row_ptr, const int *col, const T *val,
+                     const double *rhs, const CgVectors<T> &v, double *partials, double rtol, int max_iter);
 template <class T>
 void launch_pcg_iteration(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const int *cells, int ncell,
                           const CellState *cs, CgState *cg, const int *row_ptr, const int *col, const T *val,
