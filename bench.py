@@ -208,7 +208,7 @@ def pcg_roofline(cells, prof, st):
     bytes_iter = 20.0 * nnzb + 124.0 * nodes
     ms = prof["solve"]["ms"]
     achieved = bytes_iter * iters / (ms * 1e-3) if ms > 0 else 0.0
-    return {"kernel": "block-CSR PCG (k_pcg_update + k_pcg_spmv_dot + k_pcg_scalars)", "bound": "hbm",
+    return {"kernel": "block-CSR PCG (k_pcg_update + k_pcg_spmv_dot with the per-cell scalars)", "bound": "hbm",
             "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
             "traffic": None, "nnz_blocks": nnzb, "nodes": nodes, "pcg_iterations": iters,
             "bytes_per_iteration": bytes_iter}
@@ -271,6 +271,7 @@ def run_workload(W, configs, name, cells, tables, steps, warmup, common, precisi
     st = ctx.newton_step(stats=True, check=False)
     prof = ctx.profile_step()
     sched = ctx.schedule()
+    sched["two_chain_cells"] = sum(ctx.cell_layout(i)["chain_top"] > 0 for i in range(len(cells)))
     ctx.close()
     n_tri = sum(len(c["tris"]) for c in cells)
     fp64 = precision == W.PREC_FP64
@@ -354,6 +355,7 @@ def main():
     prof = ctx.profile_step()
     sched = ctx.schedule()
     layouts = [ctx.cell_layout(i) for i in range(len(cells))]
+    sched["two_chain_cells"] = sum(L["chain_top"] > 0 for L in layouts)
     ctx.close()
     log("check step:", st)
     log("phases:", {k: round(v["ms"], 3) for k, v in prof.items()})

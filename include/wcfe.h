@@ -101,6 +101,12 @@ typedef struct wcfe_options {
                             factorization, each cell's block steps in order in one
                             workgroup (no launch between two steps of a cell's chain);
                             0: separate launches */
+  int32_t twist;         /* with fused block steps and one-workgroup sweeps in FP32 (block bandwidths
+                            up to 4): cells whose chain of block steps is longer than the batch needs
+                            are factored as two chains at once (top down and bottom up, meeting in a
+                            middle window of bw blocks), about half the dependent block steps.
+                            1: when the batch's block steps are latency bound (at most 8 block
+                            products per compute unit and step); 2: always; 0: one chain per cell */
 } wcfe_options;
 
 typedef struct wcfe_cell_desc {
@@ -325,8 +331,11 @@ int wcfe_num_cells(const wcfe_ctx *ctx, int32_t *n_cells);
 int wcfe_get_cell_sizes(wcfe_ctx *ctx, int32_t cell, int32_t *n_nodes, int32_t *n_tris, int32_t *n_spr);
 /* How a cell is solved after wcfe_finalize: solver (WCFE_SOLVER_DENSE or
  * WCFE_SOLVER_CSR), 64-row blocks of its dense system, their block bandwidth
- * in the band numbering (nblk - 1: full matrix), symmetric factorization. */
-int wcfe_get_cell_layout(wcfe_ctx *ctx, int32_t cell, int32_t *solver, int32_t *nblk, int32_t *bw, int32_t *sym);
+ * in the band numbering (nblk - 1: full matrix), symmetric factorization, and
+ * for a two-chain factorization (wcfe_options.twist) the top blocks of chain
+ * A and the bottom blocks of chain B (0, 0: one chain). */
+int wcfe_get_cell_layout(wcfe_ctx *ctx, int32_t cell, int32_t *solver, int32_t *nblk, int32_t *bw, int32_t *sym,
+                         int32_t *chain_top, int32_t *chain_bottom);
 int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step, int32_t *newton_iter,
                        int32_t *n_colors, int32_t *ndof);
 /* FP64 residual G(x) = R_int(x) - F_spring(x) at the current positions (2N). */

@@ -57,6 +57,7 @@ class Options(C.Structure):
         ("k_off0", C.c_double), ("f_bell", C.c_double), ("kappa", C.c_double), ("symmetric", C.c_int32),
         ("dense_max_dof", C.c_int32), ("cg_rtol", C.c_double), ("cg_max_iter", C.c_int32),
         ("lu_groups", C.c_int32), ("lu_depth", C.c_int32), ("band", C.c_int32), ("fused_step", C.c_int32),
+        ("twist", C.c_int32),
     ]
 
 
@@ -412,10 +413,13 @@ class Context:
 
     def cell_layout(self, cell):
         """solver (SOLVER_DENSE / SOLVER_CSR), nblk (64-row blocks), bw (block
-        bandwidth in the band numbering), sym (symmetric factorization)."""
-        v = [C.c_int32(0) for _ in range(4)]
+        bandwidth in the band numbering), sym (symmetric factorization),
+        chain_top / chain_bottom (two-chain band LU: blocks of each chain; 0:
+        one chain)."""
+        v = [C.c_int32(0) for _ in range(6)]
         self._check(self._L.wcfe_get_cell_layout(self._h, cell, *[C.byref(x) for x in v]))
-        return dict(solver=v[0].value, nblk=v[1].value, bw=v[2].value, sym=v[3].value)
+        return dict(solver=v[0].value, nblk=v[1].value, bw=v[2].value, sym=v[3].value, chain_top=v[4].value,
+                    chain_bottom=v[5].value)
 
     def residual(self, cell):
         G = np.zeros(2 * self.cells[cell]["n_nodes"])

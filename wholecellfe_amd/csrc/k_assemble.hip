@@ -19,7 +19,10 @@ template <> struct Vec16<double> { typedef double2 type; static constexpr int n 
 template <class T>
 // lower_only: symmetric cells get the blocks up to the diagonal only (the
 // fused band LU writes every upper band block, U = L^T, before anything
-// reads it)
+// reads it); a symmetric two-chain cell's bottom chain factors the block
+// reversal, whose lower blocks are the real upper ones: block rows of the
+// bottom part get the blocks from the diagonal on, the middle window's rows
+// the whole envelope
 __global__ __launch_bounds__(256) void k_fill_systems(const CellParams *cp, const int *cells, int ncell, T *K,
                                                       int lower_only) {
   const int ci = blockIdx.y;
@@ -32,8 +35,10 @@ __global__ __launch_bounds__(256) void k_fill_systems(const CellParams *cp, cons
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int r = 4 * blockIdx.x + wave; r < p.npad; r += 4 * gridDim.x) {
     const int rb = r / NB;
-    const int c0 = max(0, rb - p.env) * (NB / VN),
-              c1 = min(p.nblk, rb + 1 + (lower_only && p.sym ? 0 : p.env)) * (NB / VN);
+    const bool lo = lower_only && p.sym;
+    const bool bottom = lo && p.tw_m > 0 && rb >= p.tw_m + p.bw, top = lo && (p.tw_m == 0 || rb < p.tw_m);
+    const int c0 = (bottom ? rb : max(0, rb - p.env)) * (NB / VN),
+              c1 = min(p.nblk, rb + 1 + (top ? 0 : p.env)) * (NB / VN);
     V *row = reinterpret_cast<V *>(K + p.k_off + (long long)r * p.npad);
     for (int c = c0 + lane; c < c1; c += 64) {
       T vals[VN];
@@ -232,8 +237,12 @@ __global__ __launch_bounds__(256) void k_group_gather(View v, const int *node_ce
   }
   const CellParams &p = v.cp[node_cell[n]];
   // lower_only: a symmetric cell's blocks above the diagonal block are
-  // written by the fused band LU (U = L^T) before any read
-  if (lower_only && p.sym && !p.csr && (v.kpos[n] >> 5) < (v.kpos[gp.col[g]] >> 5)) return;
+  // written by the fused band LU (U = L^T) before any read; a two-chain
+  // cell's blocks that involve its bottom part: those below the diagonal
+  if (lower_only && p.sym && !p.csr) {
+    const int bi = v.kpos[n] >> 5, bj = v.kpos[gp.col[g]] >> 5;
+    if (p.tw_m > 0 && max(bi, bj) >= p.tw_m + p.bw ? bi > bj : bi < bj) return;
+  }
   T *b0, *b1;
   if (p.csr) {
     b0 = Kb + 4LL * gp.bpos[g];
@@ -307,19 +316,46 @@ void launch_element_dump(hipStream_t s, const View &v, int e_begin, int e_end, d
 //        iterate), 1 = refinement residual (rnorm), 2 = residual after the
 //        update (gnorm)
 // ---------------------------------------------------------------------------
+__device__ void norm_finish(CellState &s, int which, double nrm, double lin_rtol) {
+  if (which == 0) {
+    s.gnorm = nrm;
+    s.bnorm = nrm;
+    if (s.newton_it == 0) s.g0 = nrm;
+  } else if (which == 1) {
+    s.rnorm = nrm;
+    if (lin_rtol >= 0.0 && !s.ir_done) {  // refinement control (k_ir_control) for this cell
+      s.ir_iters += 1;
+      s.lin_relres = s.bnorm > 0.0 ? s.rnorm / s.bnorm : 0.0;
+      if (s.rnorm <= lin_rtol * s.bnorm) s.ir_done = 1;
+    }
+  } else {
+    s.gnorm = nrm;
+  }
+}
+
+// One workgroup per cell (blockIdx.x) and chunk of NORM_CHUNK entries
+// (blockIdx.y).  A cell of one chunk is summed by its one workgroup; a larger
+// cell's chunks write partial sums and the last chunk to arrive adds them in
+// chunk order (fixed order: bitwise reproducible).
 __global__ __launch_bounds__(256) void k_cell_norm(const CellParams *cp, CellState *cs, const int *cells,
-                                                   const double *vec, int which, int only_active, double lin_rtol) {
+                                                   const double *vec, int which, int only_active, double lin_rtol,
+                                                   NormWork nw) {
   const int c = cells[blockIdx.x];
   if (only_active && cs[c].ir_done) return;
   const CellParams &p = cp[c];
+  const int nch = nw.part ? (p.ndof + NORM_CHUNK - 1) / NORM_CHUNK : 1;
+  const int chunk = blockIdx.y;
+  if (chunk >= nch) return;
   const double *v = vec + (long long)p.vec_off;
+  const int lo = nch > 1 ? chunk * NORM_CHUNK : 0;
+  const int hi = nch > 1 ? min(p.ndof, lo + NORM_CHUNK) : p.ndof;
   // four partial sums per thread: four loads in flight
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int i0 = threadIdx.x; i0 < p.ndof; i0 += 4 * 256) {
+  for (int i0 = lo + threadIdx.x; i0 < hi; i0 += 4 * 256) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int i = i0 + 256 * u;
-      const double x = i < p.ndof ? v[i] : 0.0;
+      const double x = i < hi ? v[i] : 0.0;
       acc[u] += x * x;
     }
   }
@@ -330,29 +366,26 @@ __global__ __launch_bounds__(256) void k_cell_norm(const CellParams *cp, CellSta
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const double nrm = sqrt(red[0]);
-    CellState &s = cs[c];
-    if (which == 0) {
-      s.gnorm = nrm;
-      s.bnorm = nrm;
-      if (s.newton_it == 0) s.g0 = nrm;
-    } else if (which == 1) {
-      s.rnorm = nrm;
-      if (lin_rtol >= 0.0 && !s.ir_done) {  // refinement control (k_ir_control) for this cell
-        s.ir_iters += 1;
-        s.lin_relres = s.bnorm > 0.0 ? s.rnorm / s.bnorm : 0.0;
-        if (s.rnorm <= lin_rtol * s.bnorm) s.ir_done = 1;
-      }
-    } else {
-      s.gnorm = nrm;
-    }
+  if (threadIdx.x != 0) return;
+  if (nch == 1) {
+    norm_finish(cs[c], which, sqrt(red[0]), lin_rtol);
+    return;
   }
+  double *part = nw.part + (long long)c * nw.stride;
+  part[chunk] = red[0];
+  const unsigned prev = __hip_atomic_fetch_add(nw.cnt + c, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (prev != (unsigned)(nch - 1)) return;  // not the last chunk of the cell to arrive
+  __hip_atomic_store(nw.cnt + c, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  double sum = 0.0;
+  for (int k = 0; k < nch; ++k) sum += __hip_atomic_load(part + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  norm_finish(cs[c], which, sqrt(sum), lin_rtol);
 }
 void launch_cell_norm(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell,
-                      const double *vec, int which, int only_active, double lin_rtol) {
+                      const double *vec, int which, int only_active, double lin_rtol, const NormWork &nw) {
   if (ncell <= 0) return;
-  hipLaunchKernelGGL(k_cell_norm, dim3(ncell), dim3(256), 0, s, cp, cs, cells, vec, which, only_active, lin_rtol);
+  const int nx = nw.part ? std::max(1, nw.stride) : 1;
+  hipLaunchKernelGGL(k_cell_norm, dim3(ncell, nx), dim3(256), 0, s, cp, cs, cells, vec, which, only_active, lin_rtol,
+                     nw);
 }
 
 // ---------------------------------------------------------------------------

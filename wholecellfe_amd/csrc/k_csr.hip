@@ -17,6 +17,9 @@ namespace wcfe {
 template <class T> struct V2;
 template <> struct V2<float> { typedef float2 type; };
 template <> struct V2<double> { typedef double2 type; };
+template <class T> struct V4;
+template <> struct V4<float> { typedef float4 type; };
+template <> struct V4<double> { typedef double4 type; };
 
 // fixed-order tree sum over the 256 threads of the workgroup
 __device__ __forceinline__ double block_sum256(double v, double *red) {
@@ -29,6 +32,26 @@ __device__ __forceinline__ double block_sum256(double v, double *red) {
   const double r = red[0];
   __syncthreads();
   return r;
+}
+
+// three fixed-order tree sums at once (the same bits as three block_sum256)
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double (*red)[CG_ROWS]) {
+  red[0][threadIdx.x] = a;
+  red[1][threadIdx.x] = b;
+  red[2][threadIdx.x] = c;
+  __syncthreads();
+  for (int w = CG_ROWS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+      red[2][threadIdx.x] += red[2][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  a = red[0][0];
+  b = red[1][0];
+  c = red[2][0];
+  __syncthreads();
 }
 
 template <class T>
@@ -96,42 +119,72 @@ __global__ __launch_bounds__(CG_ROWS) void k_pcg_init(const int4 *chunks, const 
   sv[k] = T(0); sv[k + 1] = T(0);
 }
 
-// w = K u; partials (r.u, w.u, r.r) of the chunk
+// w = K u and the partial sums (r.u, w.u, r.r) of one eighth of a chunk (32
+// rows: workgroup 8 c + part takes rows 32 part .. 32 part + 31 of chunk c).
+// Eight lanes per block row: lane (row, slot) takes the row's blocks slot,
+// slot + 8, ..., so a wave reads its eight rows' blocks as one contiguous
+// range; the eight lane sums are added by an xor butterfly (every lane gets
+// the same bits).  The partial sums: xor butterfly over a wave's eight rows,
+// then the four waves in order (fixed order: bitwise reproducible).
 template <class T>
-__global__ __launch_bounds__(CG_ROWS) void k_pcg_spmv_dot(const int4 *chunks, int cbase, const CgState *cg,
-                                                          const int *row_ptr, const int *col, const T *val,
-                                                          const T *r, const T *u, T *w, double *partials) {
-  __shared__ double red[CG_ROWS];
-  const int4 ch = chunks[blockIdx.x];
+__global__ __launch_bounds__(256) void k_pcg_spmv_dot(const int4 *chunks, int cbase, const CgState *cg,
+                                                      const int *row_ptr, const int *col, const T *val, const T *r,
+                                                      const T *u, T *w, double *partials) {
+  __shared__ double red[4][3];
+  const int4 ch = chunks[blockIdx.x >> 3];
   if (cg[ch.x].done) return;
-  const int i = ch.y + threadIdx.x;
-  double ru = 0.0, wu = 0.0, rr = 0.0;
+  typedef typename V2<T>::type v2;
+  typedef typename V4<T>::type v4;
+  const v2 *u2 = reinterpret_cast<const v2 *>(u);
+  const v2 *r2 = reinterpret_cast<const v2 *>(r);
+  const v4 *B4 = reinterpret_cast<const v4 *>(val);
+  v2 *w2 = reinterpret_cast<v2 *>(w);
+  const int lane = threadIdx.x & 63, slot = lane & 7, wave = threadIdx.x >> 6;
+  const int i = ch.y + (blockIdx.x & 7) * 32 + wave * 8 + (lane >> 3);
+  T y0 = T(0), y1 = T(0);
   if (i < ch.z) {
-    typedef typename V2<T>::type v2;
-    const v2 *u2 = reinterpret_cast<const v2 *>(u);
-    T y0 = T(0), y1 = T(0);
-    for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+    const int ke = row_ptr[i + 1];
+    for (int k = row_ptr[i] + slot; k < ke; k += 8) {
       const v2 uj = u2[col[k]];
-      const T *B = val + 4LL * k;
-      y0 += B[0] * uj.x + B[1] * uj.y;
-      y1 += B[2] * uj.x + B[3] * uj.y;
+      const v4 B = B4[k];
+      y0 += B.x * uj.x + B.y * uj.y;
+      y1 += B.z * uj.x + B.w * uj.y;
     }
-    w[2LL * i] = y0;
-    w[2LL * i + 1] = y1;
-    const v2 ui = u2[i];
-    const T r0 = r[2LL * i], r1 = r[2LL * i + 1];
-    ru = (double)(r0 * ui.x + r1 * ui.y);
-    wu = (double)(y0 * ui.x + y1 * ui.y);
-    rr = (double)(r0 * r0 + r1 * r1);
   }
-  ru = block_sum256(ru, red);
-  wu = block_sum256(wu, red);
-  rr = block_sum256(rr, red);
+  y0 += __shfl_xor(y0, 1);
+  y1 += __shfl_xor(y1, 1);
+  y0 += __shfl_xor(y0, 2);
+  y1 += __shfl_xor(y1, 2);
+  y0 += __shfl_xor(y0, 4);
+  y1 += __shfl_xor(y1, 4);
+  double ru = 0.0, wu = 0.0, rr = 0.0;
+  if (slot == 0 && i < ch.z) {
+    v2 wi;
+    wi.x = y0;
+    wi.y = y1;
+    w2[i] = wi;
+    const v2 ui = u2[i], ri = r2[i];
+    ru = (double)(ri.x * ui.x + ri.y * ui.y);
+    wu = (double)(y0 * ui.x + y1 * ui.y);
+    rr = (double)(ri.x * ri.x + ri.y * ri.y);
+  }
+#pragma unroll
+  for (int o = 8; o < 64; o <<= 1) {
+    ru += __shfl_xor(ru, o);
+    wu += __shfl_xor(wu, o);
+    rr += __shfl_xor(rr, o);
+  }
+  if (lane == 0) {
+    red[wave][0] = ru;
+    red[wave][1] = wu;
+    red[wave][2] = rr;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double *o = partials + 3LL * (cbase + blockIdx.x);
-    o[0] = ru;
-    o[1] = wu;
-    o[2] = rr;
+    double *o = partials + 3LL * (8LL * cbase + blockIdx.x);
+    o[0] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+    o[1] = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
+    o[2] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
   }
 }
 
@@ -166,14 +219,14 @@ __global__ void k_pcg_reset_done(const int *cells, int ncell, const CellState *c
 }
 
 // Per-cell scalars, one workgroup per sparse cell of the list, from the
-// partial sums of the cell's chunks in chunk order.
+// partial sums of the cell's chunks (eight per chunk) in order.
 //   CG_INIT: gamma, delta of the initial u; alpha = gamma / delta, beta = 0;
 //            done if refinement is finished or r = 0
 //   CG_STEP: the iteration's scalars; convergence ||r|| <= rtol ||r0|| (r of
 //            the update just made) or the iteration cap
 __global__ __launch_bounds__(256) void k_pcg_scalars(const int *cells, const CellState *cs, CgState *cg,
                                                      const double *partials, int mode, double rtol, int max_iter) {
-  __shared__ double red[256];
+  __shared__ double red[3][CG_ROWS];
   CgState &g = cg[cells[blockIdx.x]];
   const bool skip = mode == CG_INIT ? cs[cells[blockIdx.x]].ir_done != 0 : g.done != 0;
   if (skip) {
@@ -181,15 +234,13 @@ __global__ __launch_bounds__(256) void k_pcg_scalars(const int *cells, const Cel
     return;
   }
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int k = threadIdx.x; k < g.nchunk; k += 256) {
-    const double *o = partials + 3LL * (g.chunk_off + k);
+  for (int k = threadIdx.x; k < 8 * g.nchunk; k += 256) {
+    const double *o = partials + 3LL * (8LL * g.chunk_off + k);
     s0 += o[0];
     s1 += o[1];
     s2 += o[2];
   }
-  s0 = block_sum256(s0, red);
-  s1 = block_sum256(s1, red);
-  s2 = block_sum256(s2, red);
+  block_sum3(s0, s1, s2, red);
   if (threadIdx.x != 0) return;
   const double gamma = s0, delta = s1, rr = s2;
   if (mode == CG_INIT) {
@@ -258,7 +309,7 @@ void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, c
   hipLaunchKernelGGL(k_pcg_init<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks + cbase, cs, rhs, v.dinv, v.x, v.r,
                      v.z, v.p, v.s);
   // w = K u and the partials of the first step
-  hipLaunchKernelGGL(k_pcg_spmv_dot<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks + cbase, cbase, cg, row_ptr, col,
+  hipLaunchKernelGGL(k_pcg_spmv_dot<T>, dim3(8 * nchunk), dim3(256), 0, s, chunks + cbase, cbase, cg, row_ptr, col,
                      val, v.r, v.z, v.q, partials);
   hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(256), 0, s, cells, cs, cg, partials, (int)CG_INIT, rtol,
                      max_iter);
@@ -272,16 +323,10 @@ void launch_pcg_iteration(hipStream_t s, const int4 *chunks, int cbase, int nchu
   chunks += cbase;
   hipLaunchKernelGGL(k_pcg_update<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks, cg, v.dinv, v.p, v.s, v.q, v.x, v.r,
                      v.z);
-  hipLaunchKernelGGL(k_pcg_spmv_dot<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks, cbase, cg, row_ptr, col, val, v.r,
+  hipLaunchKernelGGL(k_pcg_spmv_dot<T>, dim3(8 * nchunk), dim3(256), 0, s, chunks, cbase, cg, row_ptr, col, val, v.r,
                      v.z, v.q, partials);
   hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(256), 0, s, cells, cs, cg, partials, (int)CG_STEP, rtol,
                      max_iter);
-}
-
-void launch_pcg_scalars(hipStream_t s, const int *cells, int ncell, const CellState *cs, CgState *cg,
-                        const double *partials, int mode, double rtol, int max_iter) {
-  if (ncell <= 0) return;
-  hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(256), 0, s, cells, cs, cg, partials, mode, rtol, max_iter);
 }
 
 template <class T>

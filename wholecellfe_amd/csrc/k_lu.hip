@@ -821,13 +821,13 @@ void launch_lu_gemm(hipStream_t s, const CellParams *cp, CellState *cs, const Wo
 // lower blocks; an L item also stores U_{s0,i} = A_{i,s0}^T.
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ void step_tile(typename Acc4<T>::type (&acc)[2][2], const T *A, long long ld, int bi,
-                                          int bj, int k, bool upd, T *ls, T *us) {
+__device__ __forceinline__ void step_tile(typename Acc4<T>::type (&acc)[2][2], const Chain &v, const T *A,
+                                          long long ld, int bi, int bj, int k, bool upd, T *ls, T *us) {
   constexpr int P = Pitch<T>::v;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int lk = lane >> 4, lr = lane & 15;
-  const T *C = A + (long long)(bi * NB) * ld + bj * NB;
+  const T *C = A + v.off(bi, bj, ld);
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -837,8 +837,8 @@ __device__ __forceinline__ void step_tile(typename Acc4<T>::type (&acc)[2][2], c
         acc[ti][tj][r] = C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr];
   if (!upd) return;
   T lt[TileShape<T>::E], ut[TileShape<T>::E];
-  tile_load<T>(lt, A + (long long)(bi * NB) * ld + k * NB, ld);
-  tile_load<T>(ut, A + (long long)(k * NB) * ld + bj * NB, ld);
+  tile_load<T>(lt, A + v.off(bi, k, ld), ld);
+  tile_load<T>(ut, A + v.off(k, bj, ld), ld);
   __syncthreads();  // earlier users of ls / us are done
   tile_to_lds<T, P>(lt, ls);
   tile_to_lds<T, P>(ut, us);
@@ -846,9 +846,9 @@ __device__ __forceinline__ void step_tile(typename Acc4<T>::type (&acc)[2][2], c
   mma_64<T, P, P>(ls, us, acc, T(-1));
 }
 
-__device__ __forceinline__ int lu_step_w(const CellParams &p, int k) { return min(p.nblk - k - 2, p.bw); }
-__device__ __forceinline__ int lu_step_np(const CellParams &p, int k) {
-  return k >= 0 ? max(0, min(p.nblk - 1, k + p.bw) - (k + 1)) : 0;
+__device__ __forceinline__ int lu_step_w(const Chain &v, int k) { return min(v.nblk - k - 2, v.bw); }
+__device__ __forceinline__ int lu_step_np(const Chain &v, int k) {
+  return k >= 0 ? max(0, min(v.nblk - 1, k + v.bw) - (k + 1)) : 0;
 }
 
 // acc <- the 64 x 64 block at C (leading dimension ld), MFMA accumulator layout
@@ -876,19 +876,18 @@ __device__ __forceinline__ void acc_store64(const typename Acc4<T>::type (&acc)[
       for (int r = 0; r < 4; ++r) C[(long long)(wr + ti * 16 + acc_row(T(0), lk, r)) * ld + wc + tj * 16 + lr] = acc[ti][tj][r];
 }
 
-// Items of cell p in block step k (the host's LuPlan::items for ST_STEP):
-// panel form (k >= 0): the panel item, the U items (non-symmetric cells) and
-// one item for the trailing blocks and the forward block; L-item form: the
-// diagonal item, the L items, the U items, one item per trailing block and
-// the forward block.
-__device__ __forceinline__ int lu_step_items(const CellParams &p, int k, int panel) {
-  const int s0 = k + 1;
-  if (s0 >= p.nblk) return 0;
-  const int w = min(p.nblk - s0 - 1, p.bw);
-  const int npl = lu_step_np(p, k);
-  const int tiles = p.sym ? npl * (npl + 1) / 2 : npl * npl;
-  if (panel && k >= 0) return 2 + (p.sym ? 0 : w);
-  return 2 + w + (p.sym ? 0 : w) + tiles;
+// Chain B of a symmetric two-chain cell updates the middle window's blocks
+// below the diagonal in transposed position: its virtual lower block (i, j),
+// i > j, is real block (rmax - i, rmax - j) above the diagonal, and
+// L_jk U_ki = (L_ik U_kj)^T for a symmetric matrix, so B applies the update
+// of virtual (j, i) = real lower block instead -- the blocks that chain A
+// updates and factors.
+__device__ __forceinline__ void mid_swap(const Chain &v, int &ti, int &tj) {
+  if (v.flip && v.sym && tj >= v.elim && ti > tj) {
+    const int x = ti;
+    ti = tj;
+    tj = x;
+  }
 }
 
 // One item t of cell c in block step k (see the comment above step_tile).
@@ -900,15 +899,47 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
   constexpr int P = Pitch<T>::v, PA = P;
   typedef typename Acc4<T>::type acc_t;
   const CellParams p = cp[c];
-  const int s0 = k + 1, wl = lu_step_w(p, k), npl = lu_step_np(p, k);
+  // the item's chain: A (or the cell's only chain), then B's items
+  Chain v = chain_of(p, 0);
+  if (p.tw_m > 0) {
+    const int na = chain_items(v, k, panel);
+    if (t >= na) {
+      v = chain_of(p, 1);
+      t -= na;
+    }
+  }
+  const int s0 = k + 1, wl = lu_step_w(v, k), npl = lu_step_np(v, k);
+  T *A = K + p.k_off;
+  const long long ld = p.npad;
+  if (s0 == v.elim) {
+    // update-only step of a chain that stops before the middle window:
+    // block t of the window [s0, k + bw] takes panel k's update
+    const int nu = min(v.nblk - 1, k + v.bw) - s0 + 1;
+    int ti, tj;
+    if (v.sym) {
+      int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+      while (i * (i + 1) / 2 > t) --i;
+      while ((i + 1) * (i + 2) / 2 <= t) ++i;
+      ti = s0 + i;
+      tj = s0 + (t - i * (i + 1) / 2);
+    } else {
+      ti = s0 + t / nu;
+      tj = s0 + t % nu;
+    }
+    mid_swap(v, ti, tj);
+    acc_t acc[2][2];
+    step_tile<T>(acc, v, A, ld, ti, tj, k, true, ls, us);
+    acc_store64<T, 0>(acc, A + v.off(ti, tj, ld), ld);
+    return;
+  }
   if (k < 0) panel = 0;  // the first step always runs L items
   // panel mode: item 0 inverts block s0 once and solves every L block of
   // the panel (no L items); one item takes all trailing blocks of the window
   const int nl = panel ? 0 : wl;
-  const int nsolve = nl + (p.sym ? 0 : wl);
+  const int nsolve = nl + (v.sym ? 0 : wl);
   // trailing tile q of the window (lower triangle for symmetric cells)
   auto plain_tile = [&](int q, int &ti, int &tj) {
-    if (p.sym) {
+    if (v.sym) {
       int i = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
       while (i * (i + 1) / 2 > q) --i;
       while ((i + 1) * (i + 2) / 2 <= q) ++i;
@@ -921,16 +952,14 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
   };
   // 0 diagonal, 1 L block, 2 U block (stored as updated), 3 other, 4 every
   // other block of the window (panel mode: one item per cell)
-  const int ntile = p.sym ? npl * (npl + 1) / 2 : npl * npl;
+  const int ntile = v.sym ? npl * (npl + 1) / 2 : npl * npl;
   // forward substitution of block s0 for the first solve (right-hand side
   // -G in the band numbering): y_s0 = (-G)_s0 - sum_j L_s0,j y_j over the
   // band, with L_s0,j and y_j final since earlier steps; 4 lanes per row.
   // The last item of the cell; in panel mode the trailing-block item runs it
   // after its blocks (no workgroup of its own in the many-cell launches).
   auto forward_block = [&]() {
-    if (!fwd.y) return;
-    const T *A = K + p.k_off;
-    const long long ld = p.npad;
+    if (!fwd.y || p.tw_m > 0) return;  // two-chain cells: the sweep kernel runs the whole forward sweep
     float *y = fwd.y + p.sys_off;
     const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
     float part = 0.0f;
@@ -960,27 +989,27 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
   else {
     role = 3;
     plain_tile(t - 1 - nsolve, bi, bj);
+    mid_swap(v, bi, bj);
   }
-  T *A = K + p.k_off;
-  const long long ld = p.npad;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int lk = lane >> 4, lr = lane & 15;
-  const int reach = k + p.bw;  // panel k's last block row / column
-  T *C = A + (long long)(bi * NB) * ld + bj * NB;
+  const int reach = k + v.bw;  // panel k's last block row / column
+  T *C = A + v.off(bi, bj, ld);
   acc_t acc[2][2];
   if (role == 4) {  // the window's tiles one after another, as items of role 3 take them
     for (int q = 0; q < ntile; ++q) {
       int ti, tj;
       plain_tile(q, ti, tj);
-      step_tile<T>(acc, A, ld, ti, tj, k, k >= 0 && ti <= reach && tj <= reach, ls, us);
-      acc_store64<T, 0>(acc, A + (long long)(ti * NB) * ld + tj * NB, ld);
+      mid_swap(v, ti, tj);
+      step_tile<T>(acc, v, A, ld, ti, tj, k, k >= 0 && ti <= reach && tj <= reach, ls, us);
+      acc_store64<T, 0>(acc, A + v.off(ti, tj, ld), ld);
     }
     forward_block();
     return;
   }
   if (role >= 2) {
-    step_tile<T>(acc, A, ld, bi, bj, k, k >= 0 && bi <= reach && bj <= reach, ls, us);
+    step_tile<T>(acc, v, A, ld, bi, bj, k, k >= 0 && bi <= reach && bj <= reach, ls, us);
     acc_store64<T, 0>(acc, C, ld);
     return;
   }
@@ -996,9 +1025,9 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
     // (then its inverse), ls L_{s0,k} (then L_{bi,k}, then the own block),
     // us U_{k,s0}.  Every tile's MFMA sequence is mma_64's.
     T lt[TileShape<T>::E], ld2[TileShape<T>::E], ut[TileShape<T>::E];
-    tile_load<T>(lt, A + (long long)(s0 * NB) * ld + s0 * NB, ld);
-    tile_load<T>(ld2, A + (long long)(s0 * NB) * ld + k * NB, ld);
-    tile_load<T>(ut, A + (long long)(k * NB) * ld + s0 * NB, ld);
+    tile_load<T>(lt, A + v.off(s0, s0, ld), ld);
+    tile_load<T>(ld2, A + v.off(s0, k, ld), ld);
+    tile_load<T>(ut, A + v.off(k, s0, ld), ld);
     acc_t ow[6];
     if (own && w > 0) {
 #pragma unroll
@@ -1014,7 +1043,7 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
     tile_to_lds<T, P>(lt, xs);
     tile_to_lds<T, P>(ld2, ls);
     tile_to_lds<T, P>(ut, us);
-    if (upd_own) tile_load<T>(lt, A + (long long)(bi * NB) * ld + k * NB, ld);
+    if (upd_own) tile_load<T>(lt, A + v.off(bi, k, ld), ld);
     __syncthreads();
     auto pre = [&]() {
       acc_t d;
@@ -1022,7 +1051,7 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
       mmneg_tile<T, P>(ls, us, 0, 0, d);
       acc16_store<T, PA>(d, xs);
     };
-    T *Ut = A + (long long)(s0 * NB) * ld + bi * NB;  // U_{s0,bi} (symmetric cells)
+    T *Ut = A + v.off(s0, bi, ld);  // U_{s0,bi} (symmetric cells)
     auto side = [&](int kb) {
       if (kb == 0) {
 #pragma unroll
@@ -1042,7 +1071,7 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
         if (tt >= 16) continue;
         const int I = tt >> 2, J = tt & 3;
         if (upd_own) mmneg_tile<T, P>(ls, us, I, J, ow[sl]);
-        if (p.sym) {  // U_{s0,bi} = A_{bi,s0}^T: four consecutive entries of one row per lane
+        if (v.sym) {  // U_{s0,bi} = A_{bi,s0}^T: four consecutive entries of one row per lane
           const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
           *reinterpret_cast<float4 *>(Ut + (long long)(J * 16 + lr) * ld + I * 16 + 4 * lk) =
               make_float4(ow[sl][0], ow[sl][1], ow[sl][2], ow[sl][3]);
@@ -1059,7 +1088,7 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
         }
       }
     };
-    T *D = Dinv + dinv_off(p, s0);
+    T *D = Dinv + dinv_off(p, v.rb(s0));
     if (store) {
       const int fixed = invert_diag_ov<T, PA, true>(xs, D, D + 4096, (T)cs[c].tau, pre, side, post);
       if (fixed) atomicAdd(&cs[c].n_static, fixed);
@@ -1080,11 +1109,11 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
     // computes it (same products in the same order)
     for (int t2 = 2; t2 <= wl; ++t2) {
       const int b2 = s0 + t2;
-      T *C2 = A + (long long)(b2 * NB) * ld + s0 * NB;
+      T *C2 = A + v.off(b2, s0, ld);
       acc_load64<T>(acc, C2, ld);
       __syncthreads();  // the previous block's product is done with ls
       if (b2 <= reach) {
-        tile_load<T>(lt, A + (long long)(b2 * NB) * ld + k * NB, ld);
+        tile_load<T>(lt, A + v.off(b2, k, ld), ld);
         tile_to_lds<T, P>(lt, ls);
         __syncthreads();
         mma_64<T, P, P>(ls, us, acc, T(-1));  // us: U_{k,s0}
@@ -1097,8 +1126,8 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
 #pragma unroll
           for (int r = 0; r < 4; ++r) ls[(wr + ti * 16 + acc_row(T(0), lk, r)) * P + wc + tj * 16 + lr] = acc[ti][tj][r];
       __syncthreads();
-      if (p.sym) {  // U_{s0,b2} = A_{b2,s0}^T
-        T *U2 = A + (long long)(s0 * NB) * ld + b2 * NB;
+      if (v.sym) {  // U_{s0,b2} = A_{b2,s0}^T
+        T *U2 = A + v.off(s0, b2, ld);
         for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
           const int r = idx >> 6, cc = idx & 63;
           U2[(long long)r * ld + cc] = ls[cc * P + r];
@@ -1117,7 +1146,7 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
   }
   // first step (no panel before it): block s0 and the own block as they are
   acc_t dacc[2][2];
-  acc_load64<T>(dacc, A + (long long)(s0 * NB) * ld + s0 * NB, ld);
+  acc_load64<T>(dacc, A + v.off(s0, s0, ld), ld);
   if (own) acc_load64<T>(acc, C, ld);
   // the own block staged in xs for the product after the inverse; for a
   // symmetric cell U_{s0,bi} = A_{bi,s0}^T is stored now (its stores overlap
@@ -1138,14 +1167,14 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
 #pragma unroll
       for (int r = 0; r < 4; ++r) a[(wr + ti * 16 + acc_row(T(0), lk, r)) * PA + wc + tj * 16 + lr] = dacc[ti][tj][r];
   __syncthreads();
-  if (own && p.sym) {
-    T *U = A + (long long)(s0 * NB) * ld + bi * NB;
+  if (own && v.sym) {
+    T *U = A + v.off(s0, bi, ld);
     for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
       const int r = idx >> 6, cc = idx & 63;
       U[(long long)r * ld + cc] = xs[cc * P + r];
     }
   }
-  T *D = Dinv + dinv_off(p, s0);
+  T *D = Dinv + dinv_off(p, v.rb(s0));
   if (!own) {
     const int fixed = invert_diag<T, PA, true>(a, D, D + 4096, (T)cs[c].tau);
     if (fixed) atomicAdd(&cs[c].n_static, fixed);
@@ -1193,10 +1222,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ __attribute__((aligned(16))) T us[64 * P];
   __shared__ __attribute__((aligned(16))) T xs[64 * P];
   const int c = map[xcd_item(blockIdx.x, gridDim.x)] >> 20;
-  const int nblk = cp[c].nblk;
+  const int nblk = chain_steps(cp[c]);
   for (int k = -1; k <= nblk - 2; ++k) {
     const int panel = k >= 0 ? 1 : 0;
-    const int nt = lu_step_items(cp[c], k, panel);
+    const int nt = cell_step_items(cp[c], k, panel);
     for (int t = 0; t < nt; ++t) {
       lu_step_item<T>(cp, cs, c, t, k, K, Dinv, panel, fwd, ls, us, xs);
       __syncthreads();  // the item's global writes are visible to the next item
@@ -1475,6 +1504,162 @@ __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb
   lds_barrier();
 }
 
+// Sweeps of a two-chain cell (tw_m > 0).  The factorization is the block
+// LU of Q A Q^T, Q the order [top blocks 0 .. m-1, bottom blocks n-1 down
+// to m+bw, middle window m .. m+bw-1]: top and bottom do not couple, so
+//   forward   chain A: top rows 0 .. m-1; chain B: bottom rows (virtual
+//             order), then B's L blocks of the middle rows (X_B y_B)
+//   middle    the middle rows' forward with A's L blocks, then their
+//             backward with A's U blocks inside the window
+//   backward  chain A: top rows m-1 .. 0; chain B: bottom rows nb-1 .. 0
+// Each chain runs in a workgroup of its own (a compute unit streams one
+// chain's factors): a forward launch, then a backward launch in which both
+// workgroups of the cell solve the middle window (the same operations on the
+// same data: the same bits) and then their own chain.  Nothing waits across
+// workgroups.  One job is one block row: virtual row `row` of a chain (real
+// block rb(row)), column blocks j0 .. j0+nj-1 (virtual), and for the
+// backward sweep the diagonal inverse.
+struct SweepJob {
+  int row, j0, nj, flip, rmax;
+  __device__ int rb(int v) const { return flip ? rmax - v : v; }
+};
+
+template <class T, int BWM, bool FWD>
+__device__ __forceinline__ void job_load(BandRegs<T, BWM> &R, const SweepJob &s, const CellParams &p, const T *A,
+                                         const T *Dinv) {
+  const int r = threadIdx.x >> 3, q = threadIdx.x & 7;
+  const long long ld = p.npad;
+#pragma unroll
+  for (int u = 0; u < BWM; ++u) {
+    if (s.row >= 0 && u < s.nj) {
+      const T *src = A + (long long)(s.rb(s.row) * NB + r) * ld + s.rb(s.j0 + u) * NB + q * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) R.a[u][e] = src[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) R.a[u][e] = T(0);
+    }
+  }
+  if (s.row >= 0 && !FWD) {
+    const T *src = Dinv + dinv_off(p, s.rb(s.row)) + 4096 + r * 64 + q * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) R.d[e] = src[e];
+  }
+}
+
+// steps 0 .. L-1 of one chain; job(s) gives step s's block row (row -1:
+// none); a ring of PF block rows in registers as in the one-chain sweep
+template <class T, int BWM, bool FWD, class J>
+__device__ __forceinline__ void sweep_chain(const CellParams &p, const T *A, const T *Dinv, T *y, T *sb, int L,
+                                            J job) {
+  const int r = threadIdx.x >> 3, q = threadIdx.x & 7;
+  constexpr int PF = (sizeof(T) == 4 && BWM <= 4) ? 4 : 2;
+  BandRegs<T, BWM> R[PF];
+#pragma unroll
+  for (int i = 0; i < PF; ++i) job_load<T, BWM, FWD>(R[i], job(i), p, A, Dinv);
+  lds_barrier();
+  for (int s0 = 0; s0 < L; s0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int st = s0 + i;
+      if (st < L) {
+        const SweepJob a = job(st);
+        T part = T(0);
+#pragma unroll
+        for (int u = 0; u < BWM; ++u) {
+          if (a.row >= 0 && u < a.nj) {
+            T yv[8];
+            lds8<T>(yv, y + a.rb(a.j0 + u) * NB + q * 8);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) part += R[i].a[u][e] * yv[e];
+          }
+        }
+        part = sum8(part);
+        if (a.row >= 0) {
+          const int o = a.rb(a.row) * NB + r;
+          if (FWD) {  // identity diagonal blocks (Li = I)
+            if (q == 0) y[o] -= part;
+            lds_barrier();
+          } else {
+            if (q == 0) sb[r] = y[o] - part;
+            lds_barrier();
+            T t = T(0);
+            T sv[8];
+            lds8<T>(sv, sb + q * 8);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) t += R[i].d[e] * sv[e];
+            t = sum8(t);
+            if (q == 0) y[o] = t;
+            lds_barrier();
+          }
+        }
+      }
+      job_load<T, BWM, FWD>(R[i], job(st + PF), p, A, Dinv);
+    }
+  }
+}
+
+// the right-hand side in the cell's band numbering: node n's rows go to
+// 2 kpos[n] (the padding rows stay zero)
+template <class T>
+__device__ __forceinline__ void band_rhs(const CellParams &p, const int *kpos, const double *rhs, T *y) {
+  for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = T(0);
+  lds_barrier();
+  for (int m = threadIdx.x; m < p.n_nodes; m += BAND_THREADS) {
+    const int n = p.node_off + m, o = 2 * kpos[n];
+    y[o] = (T)rhs[2LL * n];
+    y[o + 1] = (T)rhs[2LL * n + 1];
+  }
+}
+
+// back to the batch numbering for the nodes whose band rows lie in
+// [row_lo, row_hi); accumulate: delta += the solution
+template <class T>
+__device__ __forceinline__ void band_delta(const CellParams &p, const int *kpos, const T *y, double *delta,
+                                           int accumulate, int row_lo, int row_hi) {
+  for (int m = threadIdx.x; m < p.n_nodes; m += BAND_THREADS) {
+    const int n = p.node_off + m, o = 2 * kpos[n];
+    if (o < row_lo || o >= row_hi) continue;
+    double *d = delta + 2LL * n;
+    const double d0 = (double)y[o], d1 = (double)y[o + 1];
+    d[0] = accumulate ? d[0] + d0 : d0;
+    d[1] = accumulate ? d[1] + d1 : d1;
+  }
+}
+
+// the one-chain sweeps of a plain cell: forward (skipped when the fused
+// block steps ran it), backward
+template <class T, int BWM, bool FWD>
+__device__ __forceinline__ void band_sweep_plain(const CellParams &p, const T *A, const T *Dinv, T *y, T *sb) {
+  const int bw = p.bw, nb = p.nblk;
+  // a ring of PF block rows in registers: the loads of block b + PF are in
+  // flight while blocks b + 1 .. b + PF - 1 are applied
+  constexpr int PF = (sizeof(T) == 4 && BWM <= 4) ? 4 : 2;
+  BandRegs<T, BWM> R[PF];
+  if (FWD) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) band_load<T, BWM, true>(R[i], p, A, Dinv, i, bw);
+    lds_barrier();
+    for (int b = 0; b < nb; b += PF) {
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        if (b + i < nb) band_step<T, BWM, true>(R[i], y, sb, b + i, bw, nb);
+        band_load<T, BWM, true>(R[i], p, A, Dinv, b + i + PF, bw);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) band_load<T, BWM, false>(R[i], p, A, Dinv, nb - 1 - i, bw);
+    for (int b = nb - 1; b >= 0; b -= PF) {
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        if (b - i >= 0) band_step<T, BWM, false>(R[i], y, sb, b - i, bw, nb);
+        band_load<T, BWM, false>(R[i], p, A, Dinv, b - i - PF, bw);
+      }
+    }
+  }
+}
+
 template <class T, int BWM>
 __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams *cp, const CellState *cs,
                                                                 const int *cells, const T *K, const T *Dinv,
@@ -1487,66 +1672,148 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams
   if (cs[c].ir_done) return;
   const CellParams p = cp[c];
   if (p.npad == 0) return;  // sparse cell: solved by PCG
-  const T *A = K + p.k_off;
-  const int bw = p.bw;      // <= BWM (checked by the launcher's caller)
-  const int nb = p.nblk;
-  // the right-hand side in the cell's band numbering: node n's rows go to
-  // 2 kpos[n] (the padding rows stay zero); yfwd: the forward sweep already
-  // ran inside the fused block steps, y is its result
+  const T *A = K + p.k_off;  // bw <= BWM (checked by the launcher's caller)
+  // yfwd: the forward sweep already ran inside the fused block steps, y is its result
   if (yfwd) {
     for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = (T)yfwd[p.sys_off + i];
   } else {
-    for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = T(0);
+    band_rhs<T>(p, kpos, rhs, y);
     lds_barrier();
-    for (int m = threadIdx.x; m < p.n_nodes; m += BAND_THREADS) {
-      const int n = p.node_off + m, o = 2 * kpos[n];
-      y[o] = (T)rhs[2LL * n];
-      y[o + 1] = (T)rhs[2LL * n + 1];
-    }
+    band_sweep_plain<T, BWM, true>(p, A, Dinv, y, sb);
   }
-  // a ring of PF block rows in registers: the loads of block b + PF are in
-  // flight while blocks b + 1 .. b + PF - 1 are applied
-  constexpr int PF = (sizeof(T) == 4 && BWM <= 4) ? 4 : 2;
-  BandRegs<T, BWM> R[PF];
-  if (!yfwd) {
-#pragma unroll
-    for (int i = 0; i < PF; ++i) band_load<T, BWM, true>(R[i], p, A, Dinv, i, bw);
-    lds_barrier();
-    for (int b = 0; b < nb; b += PF) {
-#pragma unroll
-      for (int i = 0; i < PF; ++i) {
-        if (b + i < nb) band_step<T, BWM, true>(R[i], y, sb, b + i, bw, nb);
-        band_load<T, BWM, true>(R[i], p, A, Dinv, b + i + PF, bw);
-      }
+  lds_barrier();
+  band_sweep_plain<T, BWM, false>(p, A, Dinv, y, sb);
+  band_delta<T>(p, kpos, y, delta, accumulate, 0, p.npad);
+}
+
+// Batches with two-chain cells: a forward launch and a backward launch, two
+// workgroups per cell (the second returns at once for a one-chain cell); the
+// forward result goes through ysw (per cell at sys_off, band numbering).
+template <class T, int BWM>
+__global__ __launch_bounds__(BAND_THREADS) void k_lu_sweep_fwd(const CellParams *cp, const CellState *cs,
+                                                               const int *cells, const T *K, const T *Dinv,
+                                                               const int *kpos, const double *rhs,
+                                                               const float *yfwd, T *ysw) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ __attribute__((aligned(32))) T sb[64];
+  T *y = reinterpret_cast<T *>(smem);
+  const int c = cells[blockIdx.x >> 1], part = blockIdx.x & 1;
+  if (cs[c].ir_done) return;
+  const CellParams p = cp[c];
+  if (p.npad == 0) return;
+  if (p.tw_m == 0 && (part || yfwd)) return;  // one-chain cell: one workgroup; its first forward ran in the LU
+  const T *A = K + p.k_off;
+  band_rhs<T>(p, kpos, rhs, y);
+  lds_barrier();
+  int lo = 0, hi = p.npad;
+  if (p.tw_m == 0) {
+    band_sweep_plain<T, BWM, true>(p, A, Dinv, y, sb);
+  } else {
+    const int n = p.nblk, bw = p.bw, m = p.tw_m, nb = p.tw_nb;
+    if (part == 0) {  // chain A: the top rows
+      sweep_chain<T, BWM, true>(p, A, Dinv, y, sb, m, [&](int s) {
+        if (s >= m) return SweepJob{-1, 0, 0, 0, 0};
+        const int j0 = max(0, s - bw);
+        return SweepJob{s, j0, s - j0, 0, 0};
+      });
+      hi = m * NB;
+    } else {  // chain B: the bottom rows, then its L blocks of the middle rows
+      sweep_chain<T, BWM, true>(p, A, Dinv, y, sb, nb + bw, [&](int s) {
+        if (s >= nb + bw) return SweepJob{-1, 0, 0, 1, n - 1};
+        const int j0 = max(0, s - bw);
+        return SweepJob{s, j0, min(s, nb) - j0, 1, n - 1};
+      });
+      lo = m * NB;
     }
   }
   lds_barrier();
-#pragma unroll
-  for (int i = 0; i < PF; ++i) band_load<T, BWM, false>(R[i], p, A, Dinv, nb - 1 - i, bw);
-  for (int b = nb - 1; b >= 0; b -= PF) {
-#pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      if (b - i >= 0) band_step<T, BWM, false>(R[i], y, sb, b - i, bw, nb);
-      band_load<T, BWM, false>(R[i], p, A, Dinv, b - i - PF, bw);
-    }
+  T *g = ysw + p.sys_off;
+  for (int i = lo + threadIdx.x; i < hi; i += BAND_THREADS) g[i] = y[i];
+}
+
+template <class T, int BWM>
+__global__ __launch_bounds__(BAND_THREADS) void k_lu_sweep_bwd(const CellParams *cp, const CellState *cs,
+                                                               const int *cells, const T *K, const T *Dinv,
+                                                               const int *kpos, double *delta, int accumulate,
+                                                               const float *yfwd, const T *ysw) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ __attribute__((aligned(32))) T sb[64];
+  T *y = reinterpret_cast<T *>(smem);
+  const int c = cells[blockIdx.x >> 1], part = blockIdx.x & 1;
+  if (cs[c].ir_done) return;
+  const CellParams p = cp[c];
+  if (p.npad == 0) return;
+  if (p.tw_m == 0 && part) return;
+  const T *A = K + p.k_off;
+  if (p.tw_m == 0 && yfwd) {
+    for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = (T)yfwd[p.sys_off + i];
+  } else {
+    const T *g = ysw + p.sys_off;
+    for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = g[i];
   }
-  // back to the batch numbering; accumulate: delta += the solution
-  for (int m = threadIdx.x; m < p.n_nodes; m += BAND_THREADS) {
-    const int n = p.node_off + m, o = 2 * kpos[n];
-    double *d = delta + 2LL * n;
-    const double d0 = (double)y[o], d1 = (double)y[o + 1];
-    d[0] = accumulate ? d[0] + d0 : d0;
-    d[1] = accumulate ? d[1] + d1 : d1;
+  lds_barrier();
+  if (p.tw_m == 0) {
+    band_sweep_plain<T, BWM, false>(p, A, Dinv, y, sb);
+    band_delta<T>(p, kpos, y, delta, accumulate, 0, p.npad);
+    return;
+  }
+  const int n = p.nblk, bw = p.bw, m = p.tw_m, nb = p.tw_nb, mid = m + bw;
+  // the middle window (both workgroups): forward with A's L blocks, then backward
+  sweep_chain<T, BWM, true>(p, A, Dinv, y, sb, bw, [&](int s) {
+    if (s >= bw) return SweepJob{-1, 0, 0, 0, 0};
+    const int i = m + s, j0 = max(0, i - bw);
+    return SweepJob{i, j0, i - j0, 0, 0};
+  });
+  lds_barrier();
+  sweep_chain<T, BWM, false>(p, A, Dinv, y, sb, bw, [&](int s) {
+    if (s >= bw) return SweepJob{-1, 0, 0, 0, 0};
+    const int i = mid - 1 - s;
+    return SweepJob{i, i + 1, min(i + bw, mid - 1) - i, 0, 0};
+  });
+  lds_barrier();
+  if (part == 0) {  // chain A: top rows m-1 .. 0; writes the top and middle nodes
+    sweep_chain<T, BWM, false>(p, A, Dinv, y, sb, m, [&](int s) {
+      const int i = m - 1 - s;
+      if (i < 0) return SweepJob{-1, 0, 0, 0, 0};
+      return SweepJob{i, i + 1, min(i + bw, mid - 1) - i, 0, 0};
+    });
+    lds_barrier();
+    band_delta<T>(p, kpos, y, delta, accumulate, 0, mid * NB);
+  } else {  // chain B: bottom rows nb-1 .. 0 (virtual); writes the bottom nodes
+    sweep_chain<T, BWM, false>(p, A, Dinv, y, sb, nb, [&](int s) {
+      const int v = nb - 1 - s;
+      if (v < 0) return SweepJob{-1, 0, 0, 1, n - 1};
+      return SweepJob{v, v + 1, min(v + bw, nb + bw - 1) - v, 1, n - 1};
+    });
+    lds_barrier();
+    band_delta<T>(p, kpos, y, delta, accumulate, mid * NB, p.npad);
   }
 }
 
 template <class T>
 void launch_lu_solve_band(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                           const T *K, const T *Dinv, const int *kpos, const double *r, double *delta, int accumulate,
-                          int max_npad, int max_bw, const float *yfwd) {
+                          int max_npad, int max_bw, const float *yfwd, T *ysw) {
   if (ncell <= 0) return;
   const size_t lds = lu_solve_lds_bytes(max_npad, sizeof(T));
-  if (max_bw <= 4)
+  if (ysw) {  // two-chain cells in the batch (FP32, bw <= 4)
+    if (max_bw <= 2) {
+      hipLaunchKernelGGL((k_lu_sweep_fwd<T, 2>), dim3(2 * ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv,
+                         kpos, r, yfwd, ysw);
+      hipLaunchKernelGGL((k_lu_sweep_bwd<T, 2>), dim3(2 * ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv,
+                         kpos, delta, accumulate, yfwd, ysw);
+    } else {
+      hipLaunchKernelGGL((k_lu_sweep_fwd<T, 4>), dim3(2 * ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv,
+                         kpos, r, yfwd, ysw);
+      hipLaunchKernelGGL((k_lu_sweep_bwd<T, 4>), dim3(2 * ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv,
+                         kpos, delta, accumulate, yfwd, ysw);
+    }
+    return;
+  }
+  if (max_bw <= 2)
+    hipLaunchKernelGGL((k_lu_solve_band<T, 2>), dim3(ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv, kpos, r,
+                       delta, accumulate, yfwd);
+  else if (max_bw <= 4)
     hipLaunchKernelGGL((k_lu_solve_band<T, 4>), dim3(ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv, kpos, r,
                        delta, accumulate, yfwd);
   else
@@ -1891,7 +2158,7 @@ void launch_lu_solve(hipStream_t s, const CellParams *cp, const CellState *cs, c
 #define WCFE_INST(T)                                                                                                 \
   template void launch_lu_solve_band<T>(hipStream_t, const CellParams *, const CellState *, const int *, int,        \
                                         const T *, const T *, const int *, const double *, double *, int, int, int,  \
-                                        const float *);                                                              \
+                                        const float *, T *);                                                              \
   template void launch_lu_diag_inv<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *, T *); \
   template void launch_lu_panel_pp<T>(hipStream_t, const CellParams *, CellState *, const WorkList &, int, T *,       \
                                       int *, T *);                                                                    \

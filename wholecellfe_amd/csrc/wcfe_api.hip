@@ -305,35 +305,32 @@ struct LuPlan {
       case ST_GEMM1: return k < nb ? tiles(p, m) : 0;
       case ST_GEMM: return nb - k - np >= 1 ? tiles(p, std::min(nb - k - np, p.bw)) : 0;
       case ST_PANEL: return k < nb ? 1 : 0;
-      case ST_STEP: {  // fused block step k (k = -1 .. nb - 2), see k_lu_step; np 2: panel mode
-        const int s0 = k + 1;
-        if (s0 >= nb) return 0;
-        const int w = std::min(nb - s0 - 1, p.bw);
-        const long long npl = k >= 0 ? std::max(0, std::min(nb - 1, k + p.bw) - s0) : 0;
-        const long long tiles = p.sym ? npl * (npl + 1) / 2 : npl * npl;
-        // the forward substitution of block s0 for the first solve: an item
-        // of its own, in panel mode with the trailing blocks when there are any
-        if (np == 2) return 2 + (p.sym ? 0 : w);  // panel item, U items, one for the tiles and the forward block
-        return 2 + w + (p.sym ? 0 : w) + tiles;
-      }
+      case ST_STEP:  // fused block step k (k = -1 .. chain_steps - 2), see k_lu_step; np 2: panel mode
+        return cell_step_items(p, k, np == 2 ? 1 : 0);
       case ST_CELL: return nb > 0 ? 1 : 0;  // the whole factorization, one workgroup
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
     }
   }
+  static long long chain_products(const Chain &v, int k) {
+    const int s0 = k + 1;
+    if (s0 >= v.nblk || s0 > v.elim) return 0;
+    if (s0 == v.elim) return chain_items(v, k, 0);  // update-only: one product per block
+    const int w = std::min(v.nblk - s0 - 1, v.bw);
+    const long long npl = k >= 0 ? std::max(0, std::min(v.nblk - 1, k + v.bw) - s0) : 0;
+    const long long upd =
+        k >= 0 ? 1 + (v.sym ? 1 : 2) * std::min(w, v.bw - 1) + (v.sym ? npl * (npl + 1) / 2 : npl * npl) : 0;
+    return upd + w + 1;
+  }
   static long long block_products(const CellParams &p, int kind, int k, int np, int pivoting) {
     if (kind == ST_STRIP || kind == ST_GEMM || kind == ST_GEMM1) return (long long)np * items(p, kind, k, np, pivoting);
     if (kind == ST_STEP) {  // useful 64^3 products: updated blocks, L block solves, the inverse
-      const int nb = p.nblk, s0 = k + 1;
-      if (s0 >= nb) return 0;
-      const int w = std::min(nb - s0 - 1, p.bw);
-      const long long npl = k >= 0 ? std::max(0, std::min(nb - 1, k + p.bw) - s0) : 0;
-      const long long upd =
-          k >= 0 ? 1 + (p.sym ? 1 : 2) * std::min(w, p.bw - 1) + (p.sym ? npl * (npl + 1) / 2 : npl * npl) : 0;
-      return upd + w + 1;
+      long long tot = chain_products(chain_of(p, 0), k);
+      if (p.tw_m > 0) tot += chain_products(chain_of(p, 1), k);
+      return tot;
     }
     if (kind == ST_CELL) {  // every block step: the first in the L-item form, the others in the panel form
       long long tot = 0;
-      for (int kk = -1; kk < p.nblk - 1; ++kk) tot += block_products(p, ST_STEP, kk, kk >= 0 ? 2 : 1, pivoting);
+      for (int kk = -1; kk < chain_steps(p) - 1; ++kk) tot += block_products(p, ST_STEP, kk, kk >= 0 ? 2 : 1, pivoting);
       return tot;
     }
     return 0;
@@ -388,7 +385,17 @@ struct LuPlan {
       const int b0 = g.off + x * q + std::min(x, r), nx = q + (x < r ? 1 : 0);
       std::stable_partition(buf.begin() + b0, buf.begin() + b0 + nx, [&](int v) {
         const CellParams &p = cp[v >> 20];
-        return (v & 0xFFFFF) <= (g.np == 2 ? 0 : std::min(p.nblk - g.k - 2, p.bw));
+        int t = v & 0xFFFFF;
+        Chain ch = chain_of(p, 0);
+        if (p.tw_m > 0) {
+          const int na = chain_items(ch, g.k, g.np == 2 ? 1 : 0);
+          if (t >= na) {
+            ch = chain_of(p, 1);
+            t -= na;
+          }
+        }
+        if (g.k + 1 >= ch.elim) return false;  // update-only items
+        return t <= (g.np == 2 ? 0 : std::min(ch.nblk - g.k - 2, ch.bw));
       });
     }
   }
@@ -420,11 +427,12 @@ struct LuPlan {
     const int n = (int)cp.size();
     std::vector<int> buf;
     max_nblk = 0;
-    for (auto &p : cp) max_nblk = std::max(max_nblk, p.nblk);
+    for (auto &p : cp) max_nblk = std::max(max_nblk, chain_steps(p));
     if (n >= (1 << 11)) throw ApiError{WCFE_ERR_ARG, "at most 2047 systems per batch"};
     batched.clear();
     std::vector<int> by_size(all);  // the cell launch takes the longest chains first
-    std::stable_sort(by_size.begin(), by_size.end(), [&](int a, int b) { return cp[a].nblk > cp[b].nblk; });
+    std::stable_sort(by_size.begin(), by_size.end(),
+                     [&](int a, int b) { return chain_steps(cp[a]) > chain_steps(cp[b]); });
     for (auto st : schedule(max_nblk, pivoting)) {
       step_form(all, st);
       Stage g{st[0], st[1], st[2], (int)buf.size(), 0, 0};
@@ -440,7 +448,7 @@ struct LuPlan {
     }
     single.assign(n, {});
     for (int c = 0; c < n; ++c)
-      for (auto st : schedule(cp[c].nblk, pivoting)) {
+      for (auto st : schedule(chain_steps(cp[c]), pivoting)) {
         step_form({c}, st);
         const long long t = items(cp[c], st[0], st[1], st[2], pivoting);
         if (t == 0) continue;
@@ -477,7 +485,7 @@ struct LuPlan {
         std::vector<int> mine;
         for (int q = g; q < n; q += G) mine.push_back(order[q]);
         int gmax = 0;
-        for (int c : mine) gmax = std::max(gmax, cp[c].nblk);
+        for (int c : mine) gmax = std::max(gmax, chain_steps(cp[c]));
         std::vector<Stage> st;
         for (auto sk : schedule(gmax, pivoting)) {
           step_form(mine, sk);
@@ -550,6 +558,19 @@ struct Batch {
   void *Kb = nullptr, *cgvec = nullptr;         // block values; x, r, z, p, q, s (2N each) and dinv (4N)
   CgState *cg = nullptr;
   double *partials = nullptr;
+  // per-cell norms of cells above NORM_CHUNK entries (chunk partials, counters)
+  double *norm_part = nullptr;
+  unsigned *norm_cnt = nullptr;
+  int norm_stride = 1;
+  NormWork norm_work() const {
+    NormWork w;
+    if (norm_part) {
+      w.part = norm_part;
+      w.cnt = norm_cnt;
+      w.stride = norm_stride;
+    }
+    return w;
+  }
   // dense numbering: per node its position in the cell's dense system; the
   // permuted right-hand side and correction of the triangular solves
   int *kpos = nullptr;
@@ -557,6 +578,8 @@ struct Batch {
   // band row its batch dof (-1: padding) and the forward result
   int *kinv = nullptr;
   float *yfwd = nullptr;
+  // batches with two-chain cells: the sweeps' forward result (sys elements)
+  void *ysw = nullptr;
   std::vector<std::vector<int>> kpos_h;
   double *rp = nullptr, *dp = nullptr;
   bool solve_wg = false;  // static pivoting: one workgroup per cell walks its band (narrow bands)
@@ -573,7 +596,7 @@ struct Batch {
     void *ptrs[] = {X, x, fau, ea, cC, el_tri, el_cell, el_spr, sp_bary, sp_anchor, sp_state, sp_cell, sp_elem,
                     sp_local, cp, cs, node_cell, adj_start, adj, Ge, g_start, g_col, g_bpos, g_cstart, g_ke, g_node, Ke64, G, r, delta, Rea, MM, K, Dinv, ysol, sync, ipiv, cells,
                     replica, transitions, plan.d_prefix, row_ptr, bcol, diag_pos, el_bpos, csr_cells, chunks, Kb,
-                    cgvec, cg, partials, kpos, kinv, yfwd, rp, dp, bt_bidx, bt_node, bt_cst, crange, bt_cand, bt_frame, bt_g, bt_J, bt_area,
+                    cgvec, cg, partials, norm_part, norm_cnt, kpos, kinv, yfwd, ysw, rp, dp, bt_bidx, bt_node, bt_cst, crange, bt_cand, bt_frame, bt_g, bt_J, bt_area,
                     bt_gd, bt_fext};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -842,6 +865,41 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     p.env = std::min(full, p.bw + std::max(b.lu_depth - 1, b.solve_wg ? 0 : 1));
     if (!p.csr) b.max_bw = std::max(b.max_bw, p.bw);
   }
+  // two-chain band LU: the batch's chain of block steps is its longest cell's;
+  // a cell of n blocks and block bandwidth bw factored as two chains takes
+  // m + bw steps (m = n - bw - nb top blocks, nb = (n - 2 bw) / 2 bottom
+  // blocks: nb <= m - bw, so the chains never update the middle window in the
+  // same step).  Only cells longer than the shortest achievable batch chain
+  // are split (the others keep their factorization bit for bit).
+  if (elem_size == 4 && ctx->opt.twist && ctx->opt.pivoting == WCFE_PIVOT_STATIC && b.lu_depth == 1 &&
+      b.solve_wg && ctx->opt.fused_step >= 1 && b.max_bw <= 4) {
+    auto split = [](const CellParams &p, int &m, int &nb) {
+      if (p.csr || p.bw < 1 || p.bw >= p.nblk - 1 || p.nblk - 2 * p.bw < 2) return false;
+      nb = (p.nblk - 2 * p.bw) / 2;
+      m = p.nblk - p.bw - nb;
+      return m + p.bw < p.nblk;
+    };
+    int chain = 0;
+    double work = 0.0;  // block products of the whole factorization, about nblk (bw+1)(bw+2)/2 per cell
+    for (const CellParams &p : b.cp_h) {
+      if (p.csr) continue;
+      int m, nb;
+      chain = std::max(chain, split(p, m, nb) ? m + p.bw : p.nblk);
+      work += (double)p.nblk * (p.bw + 1) * (p.sym ? (p.bw + 2) * 0.5 : (p.bw + 1));
+    }
+    // automatic (twist 1): only when the shorter chain's launches stay
+    // latency bound -- at most 8 block products per compute unit and step;
+    // a batch whose launches are throughput bound gains nothing from fewer,
+    // fuller launches
+    if (ctx->opt.twist == 1 && work > 8.0 * ctx->n_cu * chain) chain = INT_MAX;
+    for (CellParams &p : b.cp_h) {
+      int m, nb;
+      if (p.nblk > chain && split(p, m, nb)) {
+        p.tw_m = m;
+        p.tw_nb = nb;
+      }
+    }
+  }
   // colour-major element order
   b.color_start.assign(b.ncolor + 1, 0);
   b.cell_color.assign(b.ncell, std::vector<int>(b.ncolor + 1, 0));
@@ -1053,6 +1111,9 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     }
     b.kinv = dupload(kinv);
     b.yfwd = dalloc<float>(std::max(sys, 1));
+    bool twist = false;
+    for (const CellParams &p : b.cp_h) twist |= p.tw_m > 0;
+    if (twist) b.ysw = (void *)dalloc<unsigned char>((size_t)std::max(sys, 1) * elem_size);
   }
   const size_t yb = lu_solve_exchange_bytes(std::max(sys, 1), elem_size);
   b.ysol = (void *)dalloc<unsigned char>(yb);
@@ -1060,6 +1121,16 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   b.nsync = SOLVE_SYNC_INTS;
   b.sync = dalloc<int>(b.nsync);
   HIPCHECK(hipMemset(b.sync, 0, sizeof(int) * b.nsync));
+  {
+    int max_ndof = 0;
+    for (int c = 0; c < b.ncell; ++c) max_ndof = std::max(max_ndof, b.cp_h[c].ndof);
+    if (max_ndof > NORM_CHUNK) {
+      b.norm_stride = (max_ndof + NORM_CHUNK - 1) / NORM_CHUNK;
+      b.norm_part = dalloc<double>((size_t)b.ncell * b.norm_stride);
+      b.norm_cnt = dalloc<unsigned>(b.ncell);
+      HIPCHECK(hipMemset(b.norm_cnt, 0, sizeof(unsigned) * b.ncell));
+    }
+  }
   // sparse cells: block-CSR pattern (each node's neighbours and itself,
   // columns ascending), the element block positions and the PCG chunks
   b.csr_cells_h.clear();
@@ -1127,7 +1198,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
       cg[c].done = 1;
     }
     b.cg = dupload(cg);
-    b.partials = dalloc<double>(3 * (size_t)std::max(b.nchunk, 1));
+    b.partials = dalloc<double>(24 * (size_t)std::max(b.nchunk, 1));  // 3 sums x 8 row groups per chunk
   }
   {
     // node gather plan: for node n, its distinct column nodes in order of
@@ -1424,7 +1495,7 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
           // first solve after a fused factorization: its forward sweep ran in the block steps (r = -G)
           const float *yf = (it == 0 && b.plan.fused) ? b.yfwd : nullptr;
           launch_lu_solve_band<T>(s, b.cp, b.cs, sc.cells, sc.ncell, K, static_cast<const T *>(b.Dinv), b.kpos, b.r,
-                                  b.delta, it > 0, npad, b.max_bw, yf);
+                                  b.delta, it > 0, npad, b.max_bw, yf, static_cast<T *>(b.ysw));
         } else {
           launch_perm_gather(s, b.node_cell, b.cp, b.kpos, b.r, b.rp, sc.node_begin, sc.node_end);
           launch_lu_solve_mw<T>(s, b.cp, b.cs, sc.cells, sc.ncell, WorkList{b.plan.d_prefix + L.fwd_off, L.tot},
@@ -1440,7 +1511,7 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
     if (b.any_area) launch_cell_gdot(s, b.cp, sc.cells, sc.ncell, bv, b.delta);
     launch_matvec_ir(s, v, b.node_cell, b.adj_start, b.adj, b.Ke64, b.G, b.delta, b.r, sc.node_begin, sc.node_end,
                      b.cs, bv);
-    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.r, 1, 1, ctx->opt.lin_rtol);
+    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.r, 1, 1, ctx->opt.lin_rtol, b.norm_work());
   }
 }
 
@@ -1464,7 +1535,7 @@ template <class T> void enqueue_newton(wcfe_ctx *ctx, Batch &b, const Scope &sc)
   {
     PhaseScope ph(ctx, WCFE_PH_ASSEMBLE, sc.cell < 0 ? b.T : b.cp_h[sc.cell].n_elems);
     enqueue_assemble<T>(ctx, b, sc, ASM_FULL);
-    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.G, 0, 0);
+    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.G, 0, 0, -1.0, b.norm_work());
   }
   // 2. factor
   enqueue_factor<T>(ctx, b, sc);
@@ -1476,7 +1547,7 @@ template <class T> void enqueue_newton(wcfe_ctx *ctx, Batch &b, const Scope &sc)
     launch_update_x(s, b.x, b.delta, sc.node_begin, sc.node_end);
     launch_fill(s, b.G + v0, vn, 0.0);
     enqueue_assemble<T>(ctx, b, sc, ASM_RESIDUAL);
-    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.G, 2, 0);
+    launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.G, 2, 0, -1.0, b.norm_work());
     launch_newton_control(s, b.cs, sc.cells, sc.ncell, ctx->opt.newton_rtol, ctx->opt.newton_atol);
   }
   // 5. commit converged cells
@@ -1602,6 +1673,7 @@ void wcfe_default_options(wcfe_options *o) {
   o->lu_depth = 0;   // auto
   o->band = 1;
   o->fused_step = 1;
+  o->twist = 1;
 }
 
 int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
@@ -1636,6 +1708,7 @@ int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
     if (ctx->opt.lu_groups < 0 || ctx->opt.lu_groups > 16) fail(WCFE_ERR_ARG, "lu_groups must be in [0, 16]");
     if (ctx->opt.lu_depth < 0 || ctx->opt.lu_depth > 8) fail(WCFE_ERR_ARG, "lu_depth must be in [0, 8]");
     if (ctx->opt.fused_step < 0 || ctx->opt.fused_step > 4) fail(WCFE_ERR_ARG, "fused_step must be in [0, 4]");
+    if (ctx->opt.twist < 0 || ctx->opt.twist > 2) fail(WCFE_ERR_ARG, "twist must be 0, 1 or 2");
     if (ctx->opt.band != 0 && ctx->opt.band != 1) fail(WCFE_ERR_ARG, "band must be 0 or 1");
     HIPCHECK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     for (int g = 0; g < std::max(ctx->opt.lu_groups, AUTO_GROUPS_MAX); ++g) {
@@ -2350,7 +2423,8 @@ int wcfe_get_cell_info(wcfe_ctx *ctx, int32_t cell, double *time, int32_t *step,
   });
 }
 
-int wcfe_get_cell_layout(wcfe_ctx *ctx, int32_t cell, int32_t *solver, int32_t *nblk, int32_t *bw, int32_t *sym) {
+int wcfe_get_cell_layout(wcfe_ctx *ctx, int32_t cell, int32_t *solver, int32_t *nblk, int32_t *bw, int32_t *sym,
+                         int32_t *chain_top, int32_t *chain_bottom) {
   if (!ctx) return WCFE_ERR_ARG;
   return guarded(ctx, [&] {
     check_cell(ctx, cell);
@@ -2359,6 +2433,8 @@ int wcfe_get_cell_layout(wcfe_ctx *ctx, int32_t cell, int32_t *solver, int32_t *
     if (nblk) *nblk = p.nblk;
     if (bw) *bw = p.bw;
     if (sym) *sym = p.sym;
+    if (chain_top) *chain_top = p.tw_m;
+    if (chain_bottom) *chain_bottom = p.tw_nb;
   });
 }
 

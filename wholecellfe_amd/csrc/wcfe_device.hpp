@@ -37,12 +37,86 @@ struct CellParams {
   int env;                        // blocks |i - j| <= env are zeroed before assembly (bw + delayed
                                   // panels - 1: the trailing update reads that far off the band)
   int passive;                    // 1: no active stress (the reference's substrate, mat = 2)
+  int tw_m, tw_nb;                // two-chain band LU (tw_m > 0): chain A eliminates blocks
+                                  // 0 .. tw_m-1, chain B blocks nblk-1 down to nblk-tw_nb, at
+                                  // the same time; then A eliminates the middle window of bw blocks
   long long k_off;                // offset of the dense matrix (elements)
   double mu, E, nu, thick, k_lin, rho_max;
   double k_bend, k_area, area_0, drag, ap_rate, ap_quantum, ap_density;
 };
 
 enum : int { BT_BEND = 1, BT_AREA = 2, BT_AP = 4, BT_DRAG = 8, BT_EXT = 16 };
+
+// One chain of fused band-LU block steps over a cell's dense system, in
+// virtual block numbering: real block rb(v) = flip ? rmax - v : v (64 x 64
+// blocks keep their orientation: the chain factors the block-reversed matrix
+// P A P, P the block reversal, which has the same band).  Block steps
+// k = -1 .. eliminate blocks 0 .. elim-1; when elim < nblk the step
+// k = elim - 1 only applies the last panel's update to the blocks past it
+// (the middle window that the other chain eliminates).
+//   plain cell      one chain: all nblk blocks
+//   two-chain cell  A = blocks 0 .. tw_m + bw - 1 (the top, then the middle
+//                   window), B = blocks nblk-1 down to tw_m + bw (the bottom,
+//                   reversed) with the middle window as its virtual blocks
+//                   tw_nb .. tw_nb + bw - 1, updated and not eliminated.
+// A and B touch disjoint blocks except the middle window, which B updates
+// up to its step tw_nb - 1 and A from its step tw_m - bw on (tw_nb <=
+// tw_m - bw): never in the same launch.
+struct Chain {
+  int nblk, bw, sym, elim, flip, rmax;
+  __host__ __device__ int rb(int v) const { return flip ? rmax - v : v; }
+  __host__ __device__ long long off(int bi, int bj, long long ld) const {
+    return (long long)(rb(bi) * NB) * ld + (long long)rb(bj) * NB;
+  }
+};
+__host__ __device__ inline Chain chain_of(const CellParams &p, int which) {
+  Chain v;
+  v.bw = p.bw;
+  v.sym = p.sym;
+  if (p.tw_m <= 0) {
+    v.nblk = v.elim = p.nblk;
+    v.flip = v.rmax = 0;
+  } else if (which == 0) {
+    v.nblk = v.elim = p.tw_m + p.bw;
+    v.flip = v.rmax = 0;
+  } else {
+    v.nblk = p.tw_nb + p.bw;
+    v.elim = p.tw_nb;
+    v.flip = 1;
+    v.rmax = p.nblk - 1;
+  }
+  return v;
+}
+// block steps of the cell's longest chain (steps k = -1 .. chain_steps - 2)
+__host__ __device__ inline int chain_steps(const CellParams &p) { return p.tw_m > 0 ? p.tw_m + p.bw : p.nblk; }
+// Work items of one chain in fused block step k (s0 = k + 1): panel form
+// (k >= 0): the panel item, the U items (non-symmetric cells) and one item for
+// the trailing blocks and the forward block; L-item form: the diagonal item,
+// the L items, the U items, one item per trailing block and the forward
+// block; the update-only step: one item per updated block.
+__host__ __device__ inline int chain_items(const Chain &v, int k, int panel) {
+  const int s0 = k + 1;
+  if (s0 >= v.nblk || s0 > v.elim) return 0;
+  if (s0 == v.elim) {
+    if (k < 0) return 0;
+    const int nu = (k + v.bw < v.nblk - 1 ? k + v.bw : v.nblk - 1) - s0 + 1;
+    return nu <= 0 ? 0 : (v.sym ? nu * (nu + 1) / 2 : nu * nu);
+  }
+  const int w = (v.nblk - s0 - 1 < v.bw) ? v.nblk - s0 - 1 : v.bw;
+  int npl = 0;
+  if (k >= 0) {
+    const int e = (v.nblk - 1 < k + v.bw) ? v.nblk - 1 : k + v.bw;
+    npl = e - s0 > 0 ? e - s0 : 0;
+  }
+  const int tiles = v.sym ? npl * (npl + 1) / 2 : npl * npl;
+  if (panel && k >= 0) return 2 + (v.sym ? 0 : w);
+  return 2 + w + (v.sym ? 0 : w) + tiles;
+}
+__host__ __device__ inline int cell_step_items(const CellParams &p, int k, int panel) {
+  int t = chain_items(chain_of(p, 0), k, panel);
+  if (p.tw_m > 0) t += chain_items(chain_of(p, 1), k, panel);
+  return t;
+}
 
 // Boundary tables and per-evaluation scratch of the node terms (batch wide;
 // bidx == nullptr when no cell has boundary or body forces).

@@ -43,10 +43,20 @@ void launch_assemble_gather(hipStream_t s, const View &v, int e_begin, int e_end
                             const int *adj_start, const int *adj, const GatherPlan &gp, int n0, int n1, double *Ge,
                             double *Ke64, double *G, T *K, T *Kb, int lower_only = 0);
 void launch_element_dump(hipStream_t s, const View &v, int e_begin, int e_end, double *Ge, double *Ke);
+// per-cell norms of cells above NORM_CHUNK entries: partial sums per chunk
+// (stride per cell = the batch's largest chunk count) and an arrival counter
+// per cell (zero between launches)
+constexpr int NORM_CHUNK = 32768;
+struct NormWork {
+  double *part = nullptr;
+  unsigned *cnt = nullptr;
+  int stride = 1;
+};
 // which 1 with lin_rtol >= 0 also applies the refinement control to each
 // refining cell (as launch_ir_control with reset 0)
 void launch_cell_norm(hipStream_t s, const CellParams *cp, CellState *cs, const int *cells, int ncell,
-                      const double *vec, int which, int only_active, double lin_rtol = -1.0);
+                      const double *vec, int which, int only_active, double lin_rtol = -1.0,
+                      const NormWork &nw = NormWork());
 void launch_neg_copy(hipStream_t s, const double *src, double *dst, long long n);
 void launch_matvec_ir(hipStream_t s, const View &v, const int *node_cell, const int *adj_start, const int *adj,
                       const double *Ke64, const double *G, const double *delta, double *r, int node_begin,
@@ -142,11 +152,13 @@ size_t lu_solve_exchange_bytes(long long rows, int elem_size);
 size_t lu_solve_lds_bytes(int npad, int elem_size);
 size_t lu_solve_max_dynamic(size_t max_lds);
 // one workgroup per cell, band-limited left-looking sweeps with the next
-// block's data prefetched (static pivoting, max_bw <= SOLVE_WG_MAX_BW)
+// block's data prefetched (static pivoting, max_bw <= SOLVE_WG_MAX_BW); ysw
+// (batches with two-chain cells): a forward and a backward launch with one
+// workgroup per chain, the forward result kept there (sys elements)
 template <class T>
 void launch_lu_solve_band(hipStream_t s, const CellParams *cp, const CellState *cs, const int *cells, int ncell,
                           const T *K, const T *Dinv, const int *kpos, const double *r, double *delta, int accumulate,
-                          int max_npad, int max_bw, const float *yfwd = nullptr);
+                          int max_npad, int max_bw, const float *yfwd = nullptr, T *ysw = nullptr);
 // block bandwidth up to which one workgroup per cell runs the triangular sweeps
 constexpr int SOLVE_WG_MAX_BW = 8;
 // concurrent cell groups of the dense LU chosen automatically for wide bands
@@ -178,8 +190,7 @@ template <class T>
 void launch_pcg_iteration(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const int *cells, int ncell,
                           const CellState *cs, CgState *cg, const int *row_ptr, const int *col, const T *val,
                           const CgVectors<T> &v, double *partials, double rtol, int max_iter);
-void launch_pcg_scalars(hipStream_t s, const int *cells, int ncell, const CellState *cs, CgState *cg,
-                        const double *partials, int mode, double rtol, int max_iter);
+
 template <class T>
 void launch_pcg_finish(hipStream_t s, const int4 *chunks, int nchunk, const CellState *cs, const T *x, double *delta,
                        int accumulate);
