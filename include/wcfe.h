@@ -245,11 +245,12 @@ int wcfe_synchronize(wcfe_ctx *ctx);
  * fit in free device memory are skipped); WCFE_TUNE_LU: delayed-update depth
  * (1, 2, 4, 6) x concurrent cell groups (1, 2, 4), at depth 1 with and without
  * the fused block step, and (mixed precision, one group) the whole
- * factorization in one launch (fused_step 4).  Every trial starts from the
- * state the cells were added with, and so does the batch afterwards: call it
- * after wcfe_finalize and before any Newton step.  chosen (optional, 4):
- * lu_depth, lu_groups, dense_max_dof, fused_step in effect; ms_best
- * (optional): ms per step. */
+ * factorization in one launch (fused_step 4); then, with fused block steps,
+ * one chain per cell against two-chain cells (twist 0 / 2).  Every trial
+ * starts from the state the cells were added with, and so does the batch
+ * afterwards: call it after wcfe_finalize and before any Newton step.
+ * chosen (optional, 5): lu_depth, lu_groups, dense_max_dof, fused_step and
+ * the number of two-chain cells in effect; ms_best (optional): ms per step. */
 #define WCFE_TUNE_SOLVER 1
 #define WCFE_TUNE_LU 2
 int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, double *ms_best);

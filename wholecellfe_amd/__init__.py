@@ -321,11 +321,11 @@ class Context:
     def autotune(self, flags=TUNE_SOLVER | TUNE_LU, steps=2):
         """Time candidate settings on this batch and keep the fastest (call
         after finalize, before any Newton step)."""
-        chosen = np.zeros(4, dtype=np.int32)
+        chosen = np.zeros(5, dtype=np.int32)
         ms = C.c_double(0.0)
         self._check(self._L.wcfe_autotune(self._h, flags, steps, _ip(chosen), C.byref(ms)))
         return dict(lu_depth=int(chosen[0]), lu_groups=int(chosen[1]), dense_max_dof=int(chosen[2]),
-                    fused_step=int(chosen[3]), ms_per_step=ms.value)
+                    fused_step=int(chosen[3]), two_chain_cells=int(chosen[4]), ms_per_step=ms.value)
 
     def set_ul_state(self, cell, X=None, ea=None, cC=None, fau=None, time=float("nan")):
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (X, ea, cC, fau)]

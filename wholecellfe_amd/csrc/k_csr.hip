@@ -21,39 +21,6 @@ template <class T> struct V4;
 template <> struct V4<float> { typedef float4 type; };
 template <> struct V4<double> { typedef double4 type; };
 
-// fixed-order tree sum over the 256 threads of the workgroup
-__device__ __forceinline__ double block_sum256(double v, double *red) {
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  const double r = red[0];
-  __syncthreads();
-  return r;
-}
-
-// three fixed-order tree sums at once (the same bits as three block_sum256)
-__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double (*red)[CG_ROWS]) {
-  red[0][threadIdx.x] = a;
-  red[1][threadIdx.x] = b;
-  red[2][threadIdx.x] = c;
-  __syncthreads();
-  for (int w = CG_ROWS / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + w];
-      red[1][threadIdx.x] += red[1][threadIdx.x + w];
-      red[2][threadIdx.x] += red[2][threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  a = red[0][0];
-  b = red[1][0];
-  c = red[2][0];
-  __syncthreads();
-}
-
 template <class T>
 __global__ __launch_bounds__(256) void k_zero_t(T *p, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -129,7 +96,7 @@ __global__ __launch_bounds__(CG_ROWS) void k_pcg_init(const int4 *chunks, const 
 template <class T>
 __global__ __launch_bounds__(256) void k_pcg_spmv_dot(const int4 *chunks, int cbase, const CgState *cg,
                                                       const int *row_ptr, const int *col, const T *val, const T *r,
-                                                      const T *u, T *w, double *partials) {
+                                                      const T *u, T *w, double *partials, long long pstride) {
   __shared__ double red[4][3];
   const int4 ch = chunks[blockIdx.x >> 3];
   if (cg[ch.x].done) return;
@@ -180,11 +147,11 @@ __global__ __launch_bounds__(256) void k_pcg_spmv_dot(const int4 *chunks, int cb
     red[wave][2] = rr;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double *o = partials + 3LL * (8LL * cbase + blockIdx.x);
+  if (threadIdx.x == 0) {  // three arrays of pstride partial sums
+    double *o = partials + 8LL * cbase + blockIdx.x;
     o[0] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
-    o[1] = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
-    o[2] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
+    o[pstride] = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
+    o[2 * pstride] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
   }
 }
 
@@ -218,15 +185,18 @@ __global__ void k_pcg_reset_done(const int *cells, int ncell, const CellState *c
   if (i < ncell) cg[cells[i]].done = cs[cells[i]].ir_done ? 1 : 0;
 }
 
-// Per-cell scalars, one workgroup per sparse cell of the list, from the
-// partial sums of the cell's chunks (eight per chunk) in order.
+// Per-cell scalars, one workgroup of 1024 threads per sparse cell of the
+// list, from the partial sums of the cell's chunks (eight per chunk; three
+// arrays, read coalesced) by a fixed-order tree.
 //   CG_INIT: gamma, delta of the initial u; alpha = gamma / delta, beta = 0;
 //            done if refinement is finished or r = 0
 //   CG_STEP: the iteration's scalars; convergence ||r|| <= rtol ||r0|| (r of
 //            the update just made) or the iteration cap
-__global__ __launch_bounds__(256) void k_pcg_scalars(const int *cells, const CellState *cs, CgState *cg,
-                                                     const double *partials, int mode, double rtol, int max_iter) {
-  __shared__ double red[3][CG_ROWS];
+constexpr int SCALAR_THREADS = 1024;
+__global__ __launch_bounds__(SCALAR_THREADS) void k_pcg_scalars(const int *cells, const CellState *cs, CgState *cg,
+                                                                const double *partials, long long pstride, int mode,
+                                                                double rtol, int max_iter) {
+  __shared__ double red[3][SCALAR_THREADS];
   CgState &g = cg[cells[blockIdx.x]];
   const bool skip = mode == CG_INIT ? cs[cells[blockIdx.x]].ir_done != 0 : g.done != 0;
   if (skip) {
@@ -234,13 +204,27 @@ __global__ __launch_bounds__(256) void k_pcg_scalars(const int *cells, const Cel
     return;
   }
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int k = threadIdx.x; k < 8 * g.nchunk; k += 256) {
-    const double *o = partials + 3LL * (8LL * g.chunk_off + k);
-    s0 += o[0];
-    s1 += o[1];
-    s2 += o[2];
+  const double *o = partials + 8LL * g.chunk_off;
+  for (int k = threadIdx.x; k < 8 * g.nchunk; k += SCALAR_THREADS) {
+    s0 += o[k];
+    s1 += o[pstride + k];
+    s2 += o[2 * pstride + k];
   }
-  block_sum3(s0, s1, s2, red);
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  red[2][threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = SCALAR_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+      red[2][threadIdx.x] += red[2][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  s0 = red[0][0];
+  s1 = red[1][0];
+  s2 = red[2][0];
   if (threadIdx.x != 0) return;
   const double gamma = s0, delta = s1, rr = s2;
   if (mode == CG_INIT) {
@@ -310,9 +294,9 @@ void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, c
                      v.z, v.p, v.s);
   // w = K u and the partials of the first step
   hipLaunchKernelGGL(k_pcg_spmv_dot<T>, dim3(8 * nchunk), dim3(256), 0, s, chunks + cbase, cbase, cg, row_ptr, col,
-                     val, v.r, v.z, v.q, partials);
-  hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(256), 0, s, cells, cs, cg, partials, (int)CG_INIT, rtol,
-                     max_iter);
+                     val, v.r, v.z, v.q, partials, v.pstride);
+  hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(SCALAR_THREADS), 0, s, cells, cs, cg, partials, v.pstride,
+                     (int)CG_INIT, rtol, max_iter);
 }
 
 template <class T>
@@ -324,9 +308,9 @@ void launch_pcg_iteration(hipStream_t s, const int4 *chunks, int cbase, int nchu
   hipLaunchKernelGGL(k_pcg_update<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks, cg, v.dinv, v.p, v.s, v.q, v.x, v.r,
                      v.z);
   hipLaunchKernelGGL(k_pcg_spmv_dot<T>, dim3(8 * nchunk), dim3(256), 0, s, chunks, cbase, cg, row_ptr, col, val, v.r,
-                     v.z, v.q, partials);
-  hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(256), 0, s, cells, cs, cg, partials, (int)CG_STEP, rtol,
-                     max_iter);
+                     v.z, v.q, partials, v.pstride);
+  hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(SCALAR_THREADS), 0, s, cells, cs, cg, partials, v.pstride,
+                     (int)CG_STEP, rtol, max_iter);
 }
 
 template <class T>

@@ -1198,7 +1198,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
       cg[c].done = 1;
     }
     b.cg = dupload(cg);
-    b.partials = dalloc<double>(24 * (size_t)std::max(b.nchunk, 1));  // 3 sums x 8 row groups per chunk
+    b.partials = dalloc<double>(24 * (size_t)std::max(b.nchunk, 1));  // 3 sums, 8 row groups per chunk
   }
   {
     // node gather plan: for node n, its distinct column nodes in order of
@@ -1299,7 +1299,8 @@ bool scope_sparse(const Batch &b, const Scope &sc, int &cbase, int &nch, const i
 template <class T> CgVectors<T> cg_vectors(const Batch &b) {
   T *base = static_cast<T *>(b.cgvec);
   const long long n2 = 2LL * b.N;
-  return CgVectors<T>{base, base + n2, base + 2 * n2, base + 3 * n2, base + 4 * n2, base + 5 * n2, base + 6 * n2};
+  return CgVectors<T>{base,          base + n2,     base + 2 * n2, base + 3 * n2,
+                      base + 4 * n2, base + 5 * n2, base + 6 * n2, 8LL * std::max(b.nchunk, 1)};
 }
 
 BndView bnd_view(const wcfe_ctx *ctx, const Batch &b) {
@@ -2067,6 +2068,24 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
             }
           }
     }
+    // 3. with fused block steps in FP32: one chain per cell against two-chain
+    //    cells for every cell longer than the batch needs (the automatic
+    //    rule splits only when the block steps are latency bound)
+    if ((flags & WCFE_TUNE_LU) && mixed(ctx) && best.fused_step >= 1 &&
+        (best.lu_depth == 1 || (best.lu_depth == 0 && ctx->b.lu_depth == 1))) {
+      const wcfe_options base = best;
+      for (int tw : {0, 2}) {
+        if (tw == base.twist) continue;
+        ctx->opt = base;
+        ctx->opt.twist = tw;
+        const double ms = trial();
+        if (ms < 1e300) tried.push_back({ms, ctx->opt});
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = ctx->opt;
+        }
+      }
+    }
     // the fastest candidate that can be laid out again (a setting near the
     // memory limit may not fit a second time); else the batch's own
     std::stable_sort(tried.begin(), tried.end(),
@@ -2094,6 +2113,9 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
       chosen[1] = ctx->b.lu_groups;
       chosen[2] = ctx->opt.dense_max_dof;
       chosen[3] = ctx->b.plan.cell_loop ? 4 : ctx->b.plan.fused ? ctx->opt.fused_step : 0;
+      int ntw = 0;
+      for (const CellParams &p : ctx->b.cp_h) ntw += p.tw_m > 0;
+      chosen[4] = ntw;
     }
     if (ms_best) *ms_best = best_ms < 1e300 ? best_ms : 0.0;
   });

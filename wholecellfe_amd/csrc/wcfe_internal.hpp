@@ -177,6 +177,7 @@ struct CgState {
 };
 template <class T> struct CgVectors {
   T *x, *r, *z, *p, *q, *s, *dinv;  // 2 N each (dinv: 4 N); z = M^-1 r, q = K z, s = K p
+  long long pstride;                // partial sums: three arrays of pstride (8 per chunk of the batch)
 };
 template <class T> void launch_zero(hipStream_t s, T *p, long long n);
 template <class T>
