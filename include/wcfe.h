@@ -85,7 +85,8 @@ typedef struct wcfe_options {
   double cg_rtol;        /* PCG (working precision) stops at ||r|| <= cg_rtol ||r0|| per refinement sweep */
   int32_t cg_max_iter;   /* PCG iteration cap per refinement sweep */
   int32_t lu_groups;     /* batched dense LU: cells split in this many groups factored on concurrent
-                            streams (1..16; 0 = automatic: 1 when every dense band is narrow, else 4) */
+                            streams (1..16; 0 = automatic: 1 when every dense band is narrow -- 2 when
+                            the batch has two-chain cells -- else 4) */
   int32_t lu_depth;      /* static pivoting: block steps per delayed trailing update (1..8; 0 = automatic:
                             1 when every dense band is narrow, else 4) */
   int32_t band;          /* 1: dense cells under static pivoting are numbered to narrow their band

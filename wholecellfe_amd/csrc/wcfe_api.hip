@@ -897,6 +897,9 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
       if (p.nblk > chain && split(p, m, nb)) {
         p.tw_m = m;
         p.tw_nb = nb;
+        // automatic schedule: two concurrent cell groups, so one group's
+        // latency-bound launches overlap the other's fuller ones
+        if (ctx->opt.lu_groups == 0) b.lu_groups = 2;
       }
     }
   }
