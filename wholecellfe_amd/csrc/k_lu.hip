@@ -2126,14 +2126,24 @@ size_t lu_solve_max_dynamic(size_t max_lds) { return max_lds > 1024 ? max_lds - 
 
 hipError_t lu_prepare_attributes(size_t max_lds) {
   const int dyn = (int)lu_solve_max_dynamic(max_lds);
-  const void *fns[8] = {reinterpret_cast<const void *>(&k_lu_solve<float, true>),
-                        reinterpret_cast<const void *>(&k_lu_solve<float, false>),
-                        reinterpret_cast<const void *>(&k_lu_solve<double, true>),
-                        reinterpret_cast<const void *>(&k_lu_solve<double, false>),
-                        reinterpret_cast<const void *>(&k_lu_solve_band<float, 4>),
-                        reinterpret_cast<const void *>(&k_lu_solve_band<float, 8>),
-                        reinterpret_cast<const void *>(&k_lu_solve_band<double, 4>),
-                        reinterpret_cast<const void *>(&k_lu_solve_band<double, 8>)};
+  const void *fns[] = {reinterpret_cast<const void *>(&k_lu_solve<float, true>),
+                       reinterpret_cast<const void *>(&k_lu_solve<float, false>),
+                       reinterpret_cast<const void *>(&k_lu_solve<double, true>),
+                       reinterpret_cast<const void *>(&k_lu_solve<double, false>),
+                       reinterpret_cast<const void *>(&k_lu_solve_band<float, 2>),
+                       reinterpret_cast<const void *>(&k_lu_solve_band<float, 4>),
+                       reinterpret_cast<const void *>(&k_lu_solve_band<float, 8>),
+                       reinterpret_cast<const void *>(&k_lu_solve_band<double, 2>),
+                       reinterpret_cast<const void *>(&k_lu_solve_band<double, 4>),
+                       reinterpret_cast<const void *>(&k_lu_solve_band<double, 8>),
+                       reinterpret_cast<const void *>(&k_lu_sweep_fwd<float, 2>),
+                       reinterpret_cast<const void *>(&k_lu_sweep_fwd<float, 4>),
+                       reinterpret_cast<const void *>(&k_lu_sweep_fwd<double, 2>),
+                       reinterpret_cast<const void *>(&k_lu_sweep_fwd<double, 4>),
+                       reinterpret_cast<const void *>(&k_lu_sweep_bwd<float, 2>),
+                       reinterpret_cast<const void *>(&k_lu_sweep_bwd<float, 4>),
+                       reinterpret_cast<const void *>(&k_lu_sweep_bwd<double, 2>),
+                       reinterpret_cast<const void *>(&k_lu_sweep_bwd<double, 4>)};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
     if (e != hipSuccess) return e;
