@@ -911,10 +911,40 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
   const int s0 = k + 1, wl = lu_step_w(v, k), npl = lu_step_np(v, k);
   T *A = K + p.k_off;
   const long long ld = p.npad;
+  // forward substitution of the first solve (right-hand side -G in the band
+  // numbering) for the chain's block row s: y_s = base - sum_{j in [jlo,
+  // jhi)} L_s,j y_j with L_s,j and y_j final since earlier steps; base = -G,
+  // or (chain A's rows of a two-chain cell's window) y_s as chain B left it;
+  // 4 lanes per row
+  auto fwd_row = [&](int s, int jlo, int jhi, bool from_y) {
+    float *y = fwd.y + p.sys_off;
+    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+    float part = 0.0f;
+    for (int j = jlo; j < jhi; ++j) {
+      const T *Lr = A + v.off(s, j, ld) + (long long)r * ld + q * 16;
+      const float *yj = y + v.rb(j) * NB + q * 16;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) part += (float)Lr[e] * yj[e];
+    }
+    part += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(part), 0xB1, 0xF, 0xF, false));
+    part += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(part), 0x4E, 0xF, 0xF, false));
+    if (q == 0) {
+      const int i = v.rb(s) * NB + r, dof = fwd.kinv[p.sys_off + i];
+      const float base = from_y ? y[i] : (dof >= 0 ? (float)(-fwd.G[dof]) : 0.0f);
+      y[i] = base - part;
+    }
+  };
   if (s0 == v.elim) {
     // update-only step of a chain that stops before the middle window:
     // block t of the window [s0, k + bw] takes panel k's update
     const int nu = min(v.nblk - 1, k + v.bw) - s0 + 1;
+    if (t == (v.sym ? nu * (nu + 1) / 2 : nu * nu)) {
+      // the window rows' forward with this chain's L blocks (columns before
+      // the window); the other chain adds its own and finishes them
+      if (fwd.y)
+        for (int s = s0; s < v.nblk; ++s) fwd_row(s, max(0, s - v.bw), s0, false);
+      return;
+    }
     int ti, tj;
     if (v.sym) {
       int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
@@ -959,22 +989,8 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
   // The last item of the cell; in panel mode the trailing-block item runs it
   // after its blocks (no workgroup of its own in the many-cell launches).
   auto forward_block = [&]() {
-    if (!fwd.y || p.tw_m > 0) return;  // two-chain cells: the sweep kernel runs the whole forward sweep
-    float *y = fwd.y + p.sys_off;
-    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-    float part = 0.0f;
-    for (int j = max(0, s0 - p.bw); j < s0; ++j) {
-      const T *Lr = A + (long long)(s0 * NB + r) * ld + j * NB + q * 16;
-      const float *yj = y + j * NB + q * 16;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) part += (float)Lr[e] * yj[e];
-    }
-    part += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(part), 0xB1, 0xF, 0xF, false));
-    part += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(part), 0x4E, 0xF, 0xF, false));
-    if (q == 0) {
-      const int i = s0 * NB + r, dof = fwd.kinv[p.sys_off + i];
-      y[i] = (dof >= 0 ? (float)(-fwd.G[dof]) : 0.0f) - part;
-    }
+    if (!fwd.y) return;
+    fwd_row(s0, max(0, s0 - v.bw), s0, p.tw_m > 0 && !v.flip && s0 >= p.tw_m);
   };
   const int t_last = 1 + nsolve + (panel ? 0 : ntile);
   if (t == t_last && !(panel && ntile > 0)) {
@@ -1688,12 +1704,13 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_solve_band(const CellParams
 
 // Batches with two-chain cells: a forward launch and a backward launch, two
 // workgroups per cell (the second returns at once for a one-chain cell); the
-// forward result goes through ysw (per cell at sys_off, band numbering).
+// forward result goes through ysw (per cell at sys_off, band numbering).  The
+// first solve after the fused block steps has its forward result in yfwd
+// (every cell, the window rows included) and no forward launch.
 template <class T, int BWM>
 __global__ __launch_bounds__(BAND_THREADS) void k_lu_sweep_fwd(const CellParams *cp, const CellState *cs,
                                                                const int *cells, const T *K, const T *Dinv,
-                                                               const int *kpos, const double *rhs,
-                                                               const float *yfwd, T *ysw) {
+                                                               const int *kpos, const double *rhs, T *ysw) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   __shared__ __attribute__((aligned(32))) T sb[64];
   T *y = reinterpret_cast<T *>(smem);
@@ -1701,7 +1718,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_sweep_fwd(const CellParams 
   if (cs[c].ir_done) return;
   const CellParams p = cp[c];
   if (p.npad == 0) return;
-  if (p.tw_m == 0 && (part || yfwd)) return;  // one-chain cell: one workgroup; its first forward ran in the LU
+  if (p.tw_m == 0 && part) return;  // one-chain cell: one workgroup
   const T *A = K + p.k_off;
   band_rhs<T>(p, kpos, rhs, y);
   lds_barrier();
@@ -1745,7 +1762,7 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_sweep_bwd(const CellParams 
   if (p.npad == 0) return;
   if (p.tw_m == 0 && part) return;
   const T *A = K + p.k_off;
-  if (p.tw_m == 0 && yfwd) {
+  if (yfwd) {  // the first solve: the fused block steps ran the forward sweep (the window rows' too)
     for (int i = threadIdx.x; i < p.npad; i += BAND_THREADS) y[i] = (T)yfwd[p.sys_off + i];
   } else {
     const T *g = ysw + p.sys_off;
@@ -1758,13 +1775,16 @@ __global__ __launch_bounds__(BAND_THREADS) void k_lu_sweep_bwd(const CellParams 
     return;
   }
   const int n = p.nblk, bw = p.bw, m = p.tw_m, nb = p.tw_nb, mid = m + bw;
-  // the middle window (both workgroups): forward with A's L blocks, then backward
-  sweep_chain<T, BWM, true>(p, A, Dinv, y, sb, bw, [&](int s) {
-    if (s >= bw) return SweepJob{-1, 0, 0, 0, 0};
-    const int i = m + s, j0 = max(0, i - bw);
-    return SweepJob{i, j0, i - j0, 0, 0};
-  });
-  lds_barrier();
+  // the middle window (both workgroups): forward with A's L blocks (unless
+  // the block steps did), then backward
+  if (!yfwd) {
+    sweep_chain<T, BWM, true>(p, A, Dinv, y, sb, bw, [&](int s) {
+      if (s >= bw) return SweepJob{-1, 0, 0, 0, 0};
+      const int i = m + s, j0 = max(0, i - bw);
+      return SweepJob{i, j0, i - j0, 0, 0};
+    });
+    lds_barrier();
+  }
   sweep_chain<T, BWM, false>(p, A, Dinv, y, sb, bw, [&](int s) {
     if (s >= bw) return SweepJob{-1, 0, 0, 0, 0};
     const int i = mid - 1 - s;
@@ -1796,15 +1816,17 @@ void launch_lu_solve_band(hipStream_t s, const CellParams *cp, const CellState *
                           int max_npad, int max_bw, const float *yfwd, T *ysw) {
   if (ncell <= 0) return;
   const size_t lds = lu_solve_lds_bytes(max_npad, sizeof(T));
-  if (ysw) {  // two-chain cells in the batch (FP32, bw <= 4)
+  if (ysw) {  // two-chain cells in the batch (FP32, bw <= 4); no forward launch after the fused block steps
     if (max_bw <= 2) {
-      hipLaunchKernelGGL((k_lu_sweep_fwd<T, 2>), dim3(2 * ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv,
-                         kpos, r, yfwd, ysw);
+      if (!yfwd)
+        hipLaunchKernelGGL((k_lu_sweep_fwd<T, 2>), dim3(2 * ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K,
+                           Dinv, kpos, r, ysw);
       hipLaunchKernelGGL((k_lu_sweep_bwd<T, 2>), dim3(2 * ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv,
                          kpos, delta, accumulate, yfwd, ysw);
     } else {
-      hipLaunchKernelGGL((k_lu_sweep_fwd<T, 4>), dim3(2 * ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv,
-                         kpos, r, yfwd, ysw);
+      if (!yfwd)
+        hipLaunchKernelGGL((k_lu_sweep_fwd<T, 4>), dim3(2 * ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K,
+                           Dinv, kpos, r, ysw);
       hipLaunchKernelGGL((k_lu_sweep_bwd<T, 4>), dim3(2 * ncell), dim3(BAND_THREADS), lds, s, cp, cs, cells, K, Dinv,
                          kpos, delta, accumulate, yfwd, ysw);
     }

@@ -314,7 +314,7 @@ struct LuPlan {
   static long long chain_products(const Chain &v, int k) {
     const int s0 = k + 1;
     if (s0 >= v.nblk || s0 > v.elim) return 0;
-    if (s0 == v.elim) return chain_items(v, k, 0);  // update-only: one product per block
+    if (s0 == v.elim) return std::max(0, chain_items(v, k, 0) - 1);  // update-only: one product per block
     const int w = std::min(v.nblk - s0 - 1, v.bw);
     const long long npl = k >= 0 ? std::max(0, std::min(v.nblk - 1, k + v.bw) - s0) : 0;
     const long long upd =

@@ -93,14 +93,15 @@ __host__ __device__ inline int chain_steps(const CellParams &p) { return p.tw_m 
 // (k >= 0): the panel item, the U items (non-symmetric cells) and one item for
 // the trailing blocks and the forward block; L-item form: the diagonal item,
 // the L items, the U items, one item per trailing block and the forward
-// block; the update-only step: one item per updated block.
+// block; the update-only step: one item per updated block and one for the
+// chain's part of the forward sweep of the window rows.
 __host__ __device__ inline int chain_items(const Chain &v, int k, int panel) {
   const int s0 = k + 1;
   if (s0 >= v.nblk || s0 > v.elim) return 0;
-  if (s0 == v.elim) {
+  if (s0 == v.elim) {  // the window's blocks, then one item for the chain's part of the window rows' forward sweep
     if (k < 0) return 0;
     const int nu = (k + v.bw < v.nblk - 1 ? k + v.bw : v.nblk - 1) - s0 + 1;
-    return nu <= 0 ? 0 : (v.sym ? nu * (nu + 1) / 2 : nu * nu);
+    return nu <= 0 ? 0 : (v.sym ? nu * (nu + 1) / 2 : nu * nu) + 1;
   }
   const int w = (v.nblk - s0 - 1 < v.bw) ? v.nblk - s0 - 1 : v.bw;
   int npl = 0;
