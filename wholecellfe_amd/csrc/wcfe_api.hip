@@ -5,12 +5,14 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/wcfe.h"
@@ -73,6 +75,11 @@ struct HostCell {
   std::vector<double> X, x, ea, cC, fau, spr_bary, spr_anchor, spr_state;
   std::vector<int> tris, spr_tri;
   uint64_t topo_hash = 0;  // of the triangle list (topology_hash)
+  // the band numbering of the mesh, computed at the first layout that needs
+  // it; valid for the triangle list whose topology_hash it carries
+  mutable std::vector<int> rcm_pos;
+  mutable int bw_nat = -1, bw_rcm = -1;
+  mutable uint64_t rcm_hash = 0;
 };
 
 // FNV-1a over the triangle list: a cell's topology as the host's remeshing
@@ -743,6 +750,23 @@ View make_view(const wcfe_ctx *ctx, const Batch &b) {
 // Lay out cells in HBM.  Elements are stored colour-major (colour k of every
 // cell, then colour k+1, ...), so one colour of the whole batch, or of one
 // cell, is a contiguous range.
+// f(c) for every cell on up to 16 host threads (cells dealt out in turn);
+// each f(c) writes only its own cell's parts of the outputs.
+template <class F> void parallel_cells(int ncell, F f) {
+  const int nth = std::max(1, std::min<int>({16, ncell, (int)std::thread::hardware_concurrency()}));
+  if (nth <= 1) {
+    for (int c = 0; c < ncell; ++c) f(c);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nth; ++t)
+    pool.emplace_back([&]() {
+      for (int c = next++; c < ncell; c = next++) f(c);
+    });
+  for (auto &th : pool) th.join();
+}
+
 void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int elem_size) {
   b.ncell = (int)hc.size();
   std::vector<std::vector<int>> colors(b.ncell);
@@ -824,7 +848,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   // the whole boundary) and ctx->opt.band == 0 keep the natural numbering and
   // the full matrix
   b.kpos_h.assign(b.ncell, {});
-  for (int c = 0; c < b.ncell; ++c) {
+  parallel_cells(b.ncell, [&](int c) {
     CellParams &p = b.cp_h[c];
     const HostCell &h = hc[c];
     std::vector<int> ident(h.n_nodes);
@@ -833,17 +857,20 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     p.bw = full;
     b.kpos_h[c] = ident;
     if (!p.csr && ctx->opt.band && ctx->opt.pivoting == WCFE_PIVOT_STATIC && !(p.bterm & BT_AREA) && p.nblk > 1) {
-      const int bw_nat = block_bandwidth(h.n_tris, h.tris.data(), ident);
-      std::vector<int> pos = rcm_positions(h.n_nodes, h.n_tris, h.tris.data());
-      const int bw_rcm = block_bandwidth(h.n_tris, h.tris.data(), pos);
-      if (bw_rcm < bw_nat) {
-        b.kpos_h[c] = pos;
-        p.bw = std::min(full, bw_rcm);
+      if (h.bw_rcm < 0 || h.rcm_hash != h.topo_hash || (int)h.rcm_pos.size() != h.n_nodes) {  // once per mesh
+        h.bw_nat = block_bandwidth(h.n_tris, h.tris.data(), ident);
+        h.rcm_pos = rcm_positions(h.n_nodes, h.n_tris, h.tris.data());
+        h.bw_rcm = block_bandwidth(h.n_tris, h.tris.data(), h.rcm_pos);
+        h.rcm_hash = h.topo_hash;
+      }
+      if (h.bw_rcm < h.bw_nat) {
+        b.kpos_h[c] = h.rcm_pos;
+        p.bw = std::min(full, h.bw_rcm);
       } else {
-        p.bw = std::min(full, bw_nat);
+        p.bw = std::min(full, h.bw_nat);
       }
     }
-  }
+  });
   // automatic LU schedule: narrow bands are latency bound (one block step per
   // round, one stream); wide or full matrices are throughput bound (delayed
   // updates over 4 block steps, 4 concurrent cell groups)
@@ -915,38 +942,40 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
   }
   b.T = T;
   std::vector<int> el_tri(3 * (size_t)T), el_cell(T);
+  // elements per (cell, colour), then the colour-major positions: colour k
+  // of cell 0, of cell 1, ..., then colour k + 1; in increasing element
+  // order inside each (cell, colour)
+  std::vector<std::vector<int>> cnt(b.ncell, std::vector<int>(b.ncolor, 0));
+  for (int c = 0; c < b.ncell; ++c)
+    for (int e = 0; e < hc[c].n_tris; ++e) cnt[c][colors[c][e]]++;
+  std::vector<std::vector<int>> ranges(b.ncell, std::vector<int>(2 * b.ncolor, 0));
   int g = 0;
   for (int k = 0; k < b.ncolor; ++k) {
     b.color_start[k] = g;
     for (int c = 0; c < b.ncell; ++c) {
-      b.cell_color[c][k] = g;
-      const HostCell &h = hc[c];
-      for (int e = 0; e < h.n_tris; ++e) {
-        if (colors[c][e] != k) continue;
-        b.elem_pos[c][e] = g;
-        for (int a = 0; a < 3; ++a) el_tri[3 * (size_t)g + a] = b.cp_h[c].node_off + h.tris[3 * e + a];
-        el_cell[g] = c;
-        ++g;
-      }
+      ranges[c][2 * k] = g;
+      g += cnt[c][k];
+      ranges[c][2 * k + 1] = g;
     }
   }
   b.color_start[b.ncolor] = g;
-  for (int c = 0; c < b.ncell; ++c) b.cell_color[c][b.ncolor] = g;
-  // fix per-cell colour ranges: cell_color[c][k] .. next colour start of the same cell
-  std::vector<std::vector<int>> ranges(b.ncell, std::vector<int>(2 * b.ncolor, 0));
-  for (int c = 0; c < b.ncell; ++c)
-    for (int k = 0; k < b.ncolor; ++k) {
-      int cnt = 0;
-      for (int e = 0; e < hc[c].n_tris; ++e) cnt += colors[c][e] == k;
-      ranges[c][2 * k] = b.cell_color[c][k];
-      ranges[c][2 * k + 1] = b.cell_color[c][k] + cnt;
+  for (int c = 0; c < b.ncell; ++c) {
+    const HostCell &h = hc[c];
+    std::vector<int> next(b.ncolor);
+    for (int k = 0; k < b.ncolor; ++k) next[k] = ranges[c][2 * k];
+    for (int e = 0; e < h.n_tris; ++e) {
+      const int pos = next[colors[c][e]]++;
+      b.elem_pos[c][e] = pos;
+      for (int a = 0; a < 3; ++a) el_tri[3 * (size_t)pos + a] = b.cp_h[c].node_off + h.tris[3 * e + a];
+      el_cell[pos] = c;
     }
-  b.cell_color = ranges;  // now 2 entries (begin, end) per colour
+  }
+  b.cell_color = ranges;  // 2 entries (begin, end) per colour
   // springs: per cell, ordered by host element global id, then original index
   std::vector<int> el_spr(2 * (size_t)T, 0), sp_cell(S), sp_elem(S), sp_local(S);
   std::vector<double> sp_bary(3 * (size_t)S), sp_anchor(2 * (size_t)S), sp_state(S);
   b.spr_pos.assign(b.ncell, {});
-  for (int c = 0; c < b.ncell; ++c) {
+  parallel_cells(b.ncell, [&](int c) {  // disjoint ranges per cell (its springs and elements)
     const HostCell &h = hc[c];
     std::vector<int> order(h.n_spr);
     for (int s = 0; s < h.n_spr; ++s) order[s] = s;
@@ -968,7 +997,7 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
       if (el_spr[2 * (size_t)ge + 1] == 0 && el_spr[2 * (size_t)ge] == 0) el_spr[2 * (size_t)ge] = gs;
       el_spr[2 * (size_t)ge + 1] = gs + 1;
     }
-  }
+  });
   // node -> (element, corner) adjacency in increasing element order
   std::vector<int> adj_start(N + 1, 0), adj(3 * (size_t)T), node_cell(N);
   for (int e = 0; e < T; ++e)
@@ -1208,39 +1237,61 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     // first appearance along its elements (element order) and, for each, the
     // contributing element blocks in element order -- the order in which the
     // colour-by-colour scatter adds them to the same entries
+    // Built per cell (its nodes are one range) on host threads and joined
+    // in cell order: the same plan as one pass over the nodes.
+    struct Part {
+      std::vector<int> gcol, gbp, gcnt, gke, gnode, ncnt;
+    };
+    std::vector<Part> parts(b.ncell);
+    auto plan_cell = [&](int c) {
+      Part &P = parts[c];
+      const int n0 = b.cp_h[c].node_off, n1 = n0 + b.cp_h[c].n_nodes;
+      std::vector<int> cols, bps;
+      std::vector<std::vector<int>> lists;
+      for (int n = n0; n < n1; ++n) {
+        cols.clear();
+        bps.clear();
+        for (auto &l : lists) l.clear();
+        for (int k = adj_start[n]; k < adj_start[n + 1]; ++k) {
+          const int e = adj[k] >> 2, a = adj[k] & 3;
+          for (int q = 0; q < 3; ++q) {
+            const int col = el_tri[3 * (size_t)e + q];
+            size_t g = 0;
+            while (g < cols.size() && cols[g] != col) ++g;
+            if (g == cols.size()) {
+              cols.push_back(col);
+              if (lists.size() < cols.size()) lists.emplace_back();
+              bps.push_back(bpos_all.empty() ? -1 : bpos_all[9 * (size_t)e + 3 * a + q]);
+            }
+            lists[g].push_back(36 * e + 12 * a + 2 * q);
+          }
+        }
+        for (size_t g = 0; g < cols.size(); ++g) {
+          P.gcol.push_back(cols[g]);
+          P.gnode.push_back(n);
+          P.gbp.push_back(bps[g]);
+          P.gke.insert(P.gke.end(), lists[g].begin(), lists[g].end());
+          P.gcnt.push_back((int)lists[g].size());
+        }
+        P.ncnt.push_back((int)cols.size());
+      }
+    };
+    parallel_cells(b.ncell, plan_cell);
     std::vector<int> gs(N + 1, 0), gcol, gbp, gcs(1, 0), gke, gnode;
     gcol.reserve(7 * (size_t)N);
     gbp.reserve(7 * (size_t)N);
     gke.reserve(9 * (size_t)T);
-    std::vector<int> cols;
-    std::vector<std::vector<int>> lists;
-    for (int n = 0; n < N; ++n) {
-      cols.clear();
-      for (auto &l : lists) l.clear();
-      std::vector<int> bps;
-      for (int k = adj_start[n]; k < adj_start[n + 1]; ++k) {
-        const int e = adj[k] >> 2, a = adj[k] & 3;
-        for (int q = 0; q < 3; ++q) {
-          const int col = el_tri[3 * (size_t)e + q];
-          size_t g = 0;
-          while (g < cols.size() && cols[g] != col) ++g;
-          if (g == cols.size()) {
-            cols.push_back(col);
-            if (lists.size() < cols.size()) lists.emplace_back();
-            bps.push_back(bpos_all.empty() ? -1 : bpos_all[9 * (size_t)e + 3 * a + q]);
-          }
-          lists[g].push_back(36 * e + 12 * a + 2 * q);
-        }
-      }
-      for (size_t g = 0; g < cols.size(); ++g) {
-        gcol.push_back(cols[g]);
-        gnode.push_back(n);
-        gbp.push_back(bps[g]);
-        gke.insert(gke.end(), lists[g].begin(), lists[g].end());
-        gcs.push_back((int)gke.size());
-      }
-      gs[n + 1] = (int)gcol.size();
+    for (int c = 0; c < b.ncell; ++c) {
+      const Part &P = parts[c];
+      const int n0 = b.cp_h[c].node_off;
+      for (size_t i = 0; i < P.ncnt.size(); ++i) gs[n0 + i + 1] = P.ncnt[i];
+      gcol.insert(gcol.end(), P.gcol.begin(), P.gcol.end());
+      gbp.insert(gbp.end(), P.gbp.begin(), P.gbp.end());
+      gnode.insert(gnode.end(), P.gnode.begin(), P.gnode.end());
+      gke.insert(gke.end(), P.gke.begin(), P.gke.end());
+      for (int k : P.gcnt) gcs.push_back(gcs.back() + k);
     }
+    for (int n = 0; n < N; ++n) gs[n + 1] += gs[n];
     if (36LL * T >= (1LL << 31)) fail(WCFE_ERR_ARG, "batch of %d elements exceeds the gather plan's 32-bit offsets", T);
     b.g_start = dupload(gs);
     b.g_col = dupload(gcol);
@@ -1858,29 +1909,33 @@ HostCell host_cell(wcfe_ctx *ctx, const wcfe_cell_desc *d) {
 void state_to_host(wcfe_ctx *ctx) {
   Batch &b = ctx->b;
   HIPCHECK(hipStreamSynchronize(ctx->stream));
+  // the batch's arrays in one copy each, then split per cell
   std::vector<CellState> cs(b.ncell);
   HIPCHECK(hipMemcpy(cs.data(), b.cs, sizeof(CellState) * b.ncell, hipMemcpyDeviceToHost));
+  const size_t N = (size_t)b.N, S = (size_t)b.S;
+  std::vector<double> X(2 * N), x(2 * N), ea(3 * N), cC(N), fau(2 * N), st(S), an(2 * S);
+  HIPCHECK(hipMemcpy(X.data(), b.X, sizeof(double2) * N, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(x.data(), b.x, sizeof(double2) * N, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(ea.data(), b.ea, sizeof(double) * 3 * N, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(cC.data(), b.cC, sizeof(double) * N, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(fau.data(), b.fau, sizeof(double2) * N, hipMemcpyDeviceToHost));
+  if (S > 0) {
+    HIPCHECK(hipMemcpy(st.data(), b.sp_state, sizeof(double) * S, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(an.data(), b.sp_anchor, sizeof(double) * 2 * S, hipMemcpyDeviceToHost));
+  }
   for (int c = 0; c < b.ncell; ++c) {
     HostCell &h = ctx->cells[c];
     const CellParams &p = b.cp_h[c];
-    h.X.resize(2 * (size_t)h.n_nodes);
-    h.x.resize(2 * (size_t)h.n_nodes);
-    h.ea.resize(3 * (size_t)h.n_nodes);
-    h.cC.resize(h.n_nodes);
-    h.fau.resize(2 * (size_t)h.n_nodes);
-    HIPCHECK(hipMemcpy(h.X.data(), b.X + p.node_off, sizeof(double2) * h.n_nodes, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(h.x.data(), b.x + p.node_off, sizeof(double2) * h.n_nodes, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(h.ea.data(), b.ea + 3LL * p.node_off, sizeof(double) * 3 * h.n_nodes, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(h.cC.data(), b.cC + p.node_off, sizeof(double) * h.n_nodes, hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(h.fau.data(), b.fau + p.node_off, sizeof(double2) * h.n_nodes, hipMemcpyDeviceToHost));
+    const size_t o = (size_t)p.node_off, n = (size_t)h.n_nodes;
+    h.X.assign(X.begin() + 2 * o, X.begin() + 2 * (o + n));
+    h.x.assign(x.begin() + 2 * o, x.begin() + 2 * (o + n));
+    h.ea.assign(ea.begin() + 3 * o, ea.begin() + 3 * (o + n));
+    h.cC.assign(cC.begin() + o, cC.begin() + o + n);
+    h.fau.assign(fau.begin() + 2 * o, fau.begin() + 2 * (o + n));
     if (h.n_spr > 0) {
-      std::vector<double> st(h.n_spr), an(2 * (size_t)h.n_spr);
-      HIPCHECK(hipMemcpy(st.data(), b.sp_state + p.spr_off, sizeof(double) * h.n_spr, hipMemcpyDeviceToHost));
-      HIPCHECK(hipMemcpy(an.data(), b.sp_anchor + 2LL * p.spr_off, sizeof(double) * 2 * h.n_spr,
-                         hipMemcpyDeviceToHost));
       h.spr_state.resize(h.n_spr);
       for (int k = 0; k < h.n_spr; ++k) {
-        const int g = b.spr_pos[c][k] - p.spr_off;
+        const size_t g = (size_t)b.spr_pos[c][k];
         h.spr_state[k] = st[g];
         h.spr_anchor[2 * k] = an[2 * g];
         h.spr_anchor[2 * k + 1] = an[2 * g + 1];
