@@ -688,6 +688,52 @@ __device__ __forceinline__ void tile_to_lds(const T (&t)[TileShape<T>::E], T *s)
   }
 }
 
+// The fused block step's B operand (the U tile) is kept in LDS unpadded and
+// swizzled: element (k, n) at 64 k + (n ^ 16 (k & 3)).  An MFMA B read takes
+// rows kk .. kk+3 (lane group lk) and 16 consecutive columns (lr): the four
+// row groups land on four different 16-bank groups, so the read is free of
+// bank conflicts (a padded row pitch of 68 puts rows 4 words apart: up to
+// four lanes on one bank).  The same values in the same MFMA order.
+__device__ __forceinline__ int bswz(int k, int n) { return 64 * k + (n ^ ((k & 3) << 4)); }
+template <class T>
+__device__ __forceinline__ void tile_to_lds_bswz(const T (&t)[TileShape<T>::E], T *s) {
+  typedef TileShape<T> R;
+  static_assert(R::VN == 4, "FP32 tiles");
+#pragma unroll
+  for (int j = 0; j < R::N; ++j) {
+    const int idx = threadIdx.x + 256 * j, r = idx / R::PER_ROW, c = (idx % R::PER_ROW) * R::VN;
+    *reinterpret_cast<float4 *>(s + bswz(r, c)) = make_float4(t[4 * j], t[4 * j + 1], t[4 * j + 2], t[4 * j + 3]);
+  }
+}
+template <class T, int P>
+__device__ __forceinline__ void mmneg_tile_bswz(const T *X, const T *Y, int I, int J, typename Acc4<T>::type &acc) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int kk = 0; kk < 64; kk += 4) {
+    const T x = T(-1) * X[(I * 16 + lr) * P + kk + lk];
+    const T y = Y[bswz(kk + lk, J * 16 + lr)];
+    acc = mfma16(x, y, acc);
+  }
+}
+template <class T, int PX>
+__device__ __forceinline__ void mma_64_bswz(const T *xs, const T *ys, typename Acc4<T>::type acc[2][2], T sign) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int kk = 0; kk < 64; kk += 4) {
+    T a[2], b[2];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti) a[ti] = sign * xs[(wr + ti * 16 + lr) * PX + kk + lk];
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) b[tj] = ys[bswz(kk + lk, wc + tj * 16 + lr)];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = mfma16(a[ti], b[tj], acc[ti][tj]);
+  }
+}
+
 // Trailing update C = A_ij - sum_p L_{i,k+p} U_{k+p,j} over np consecutive
 // panels k .. k+np-1 (C is read and written once for np panels); the next
 // panel's tiles are loaded into registers while the current one is on MFMA.
@@ -841,9 +887,9 @@ __device__ __forceinline__ void step_tile(typename Acc4<T>::type (&acc)[2][2], c
   tile_load<T>(ut, A + v.off(k, bj, ld), ld);
   __syncthreads();  // earlier users of ls / us are done
   tile_to_lds<T, P>(lt, ls);
-  tile_to_lds<T, P>(ut, us);
+  tile_to_lds_bswz<T>(ut, us);
   __syncthreads();
-  mma_64<T, P, P>(ls, us, acc, T(-1));
+  mma_64_bswz<T, P>(ls, us, acc, T(-1));
 }
 
 __device__ __forceinline__ int lu_step_w(const Chain &v, int k) { return min(v.nblk - k - 2, v.bw); }
@@ -1058,13 +1104,13 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
     }
     tile_to_lds<T, P>(lt, xs);
     tile_to_lds<T, P>(ld2, ls);
-    tile_to_lds<T, P>(ut, us);
+    tile_to_lds_bswz<T>(ut, us);
     if (upd_own) tile_load<T>(lt, A + v.off(bi, k, ld), ld);
     __syncthreads();
     auto pre = [&]() {
       acc_t d;
       acc16_load<T, PA>(d, xs);
-      mmneg_tile<T, P>(ls, us, 0, 0, d);
+      mmneg_tile_bswz<T, P>(ls, us, 0, 0, d);
       acc16_store<T, PA>(d, xs);
     };
     T *Ut = A + v.off(s0, bi, ld);  // U_{s0,bi} (symmetric cells)
@@ -1075,7 +1121,7 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
           const int u = w + 3 * q, I = u >> 2, J = u & 3;
           acc_t d;
           acc16_load<T, PA>(d, xs + I * 16 * PA + J * 16);
-          mmneg_tile<T, P>(ls, us, I, J, d);
+          mmneg_tile_bswz<T, P>(ls, us, I, J, d);
           acc16_store<T, PA>(d, xs + I * 16 * PA + J * 16);
         }
         return;
@@ -1086,7 +1132,7 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
         const int tt = (w - 1) + 3 * sl;
         if (tt >= 16) continue;
         const int I = tt >> 2, J = tt & 3;
-        if (upd_own) mmneg_tile<T, P>(ls, us, I, J, ow[sl]);
+        if (upd_own) mmneg_tile_bswz<T, P>(ls, us, I, J, ow[sl]);
         if (v.sym) {  // U_{s0,bi} = A_{bi,s0}^T: four consecutive entries of one row per lane
           const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
           *reinterpret_cast<float4 *>(Ut + (long long)(J * 16 + lr) * ld + I * 16 + 4 * lk) =
@@ -1132,7 +1178,7 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
         tile_load<T>(lt, A + v.off(b2, k, ld), ld);
         tile_to_lds<T, P>(lt, ls);
         __syncthreads();
-        mma_64<T, P, P>(ls, us, acc, T(-1));  // us: U_{k,s0}
+        mma_64_bswz<T, P>(ls, us, acc, T(-1));  // us: U_{k,s0}
         __syncthreads();
       }
 #pragma unroll
