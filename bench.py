@@ -305,6 +305,8 @@ def main():
                     help="run wcfe_autotune on the headline batch before timing")
     ap.add_argument("--fused-step", type=int, choices=[0, 1, 2, 3, 4], default=None,
                     help="band-LU block-step mode (library default when omitted)")
+    ap.add_argument("--twist", type=int, choices=[0, 1, 2], default=None,
+                    help="two-chain band LU: 0 off, 1 automatic (default), 2 always")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (HBM probe / counter reference) and exit")
     a = ap.parse_args()
@@ -337,6 +339,8 @@ def main():
     common = dict(lin_rtol=1e-8, newton_rtol=1e-8, newton_atol=1e-12, t0=T0, dt=DT, pivoting=piv)
     if a.fused_step is not None:
         common["fused_step"] = a.fused_step
+    if a.twist is not None:
+        common["twist"] = a.twist
     all_dense = dict(dense_max_dof=1 << 30) if a.workload == "replicas1024" else {}
 
     # ---- headline: batched mixed precision, one hipGraph replay per Newton step ----

@@ -1492,12 +1492,14 @@ template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc)
 
 // Block-Jacobi PCG on the sparse cells of the scope for the refinement
 // right-hand side b.r; the correction goes to (or is added to) b.delta.
-// Convergence is checked on the host every CG_CHECK iterations.
+// Convergence is checked on the host after batches of iterations (each check
+// drains the stream): the first after 16, then as many as the slowest cell's
+// residual reduction so far predicts it still needs (8 .. 128, a quarter
+// more than predicted); iterations after a cell's convergence return at once.
 template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, int accumulate) {
   const int *cl;
   int cb, nch, ncl;
   if (!scope_sparse(b, sc, cb, nch, cl, ncl)) return;
-  constexpr int CG_CHECK = 16;
   hipStream_t s = ctx->stream;
   const CgVectors<T> v = cg_vectors<T>(b);
   const double rtol = ctx->opt.cg_rtol;
@@ -1506,15 +1508,26 @@ template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, in
                      b.r, v, b.partials, rtol, maxit);
   std::vector<int> ids = sc.cell < 0 ? b.csr_cells_h : std::vector<int>{sc.cell};
   std::vector<CgState> h(b.ncell);
-  for (int it = 0; it < maxit; it += CG_CHECK) {
-    for (int k = 0; k < CG_CHECK; ++k)
+  int batch = 16;
+  for (int it = 0; it < maxit; it += batch) {
+    for (int k = 0; k < batch; ++k)
       launch_pcg_iteration<T>(s, b.chunks, cb, nch, cl, ncl, b.cs, b.cg, b.row_ptr, b.bcol,
                               static_cast<const T *>(b.Kb), v, b.partials, rtol, maxit);
     HIPCHECK(hipMemcpyAsync(h.data(), b.cg, sizeof(CgState) * b.ncell, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     bool all = true;
-    for (int c : ids) all = all && h[c].done;
+    double need = 0.0;  // iterations the slowest open cell still needs at its rate so far
+    for (int c : ids) {
+      const CgState &g = h[c];
+      all = all && g.done;
+      if (g.done || g.iters <= 0 || !(g.rr0 > 0.0) || !(g.rr > 0.0)) continue;
+      const double red = g.rr / g.rr0;  // squared residual reduction so far
+      const double per = std::log(red) / g.iters;
+      const double left = per < 0.0 ? std::log(rtol * rtol / red) / per : 1e9;
+      need = std::max(need, left);
+    }
     if (all) break;
+    batch = (int)std::min(128.0, std::max(8.0, std::ceil(1.25 * need)));
   }
   launch_pcg_finish<T>(s, b.chunks + cb, nch, b.cs, v.x, b.delta, accumulate);
   for (int c : ids) {
