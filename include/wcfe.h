@@ -70,7 +70,9 @@ typedef struct wcfe_options {
   int32_t max_refine;    /* max iterative-refinement sweeps per linear solve */
   double lin_rtol;       /* ||b - K d|| <= lin_rtol ||b|| (FP64) ends refinement */
   double newton_rtol;    /* ||G(x)|| <= newton_rtol ||G(x0)|| ends a load step */
-  double newton_atol;    /* ... or ||G(x)|| <= newton_atol */
+  double newton_atol;    /* ... or ||G(x)|| <= newton_atol; keep it above the FP64
+                            rounding floor of G when a load step can start in
+                            equilibrium (a load that has stopped changing) */
   int32_t max_newton;    /* a load step that needs more iterations is reported */
   double dt;             /* load/time step [s] */
   double t0;             /* initial time [s] (active stress schedule UL:743-757) */
