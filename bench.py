@@ -353,12 +353,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle check")
     ap.add_argument("--no-extra", action="store_true", help="time the headline workload only")
     ap.add_argument("--pivoting", choices=["static", "partial"], default="static")
-    ap.add_argument("--autotune", choices=["none", "solver", "lu", "all"], default="none",
+    ap.add_argument("--autotune", choices=["none", "solver", "lu", "pcg", "all"], default="none",
                     help="run wcfe_autotune on the headline batch before timing")
     ap.add_argument("--fused-step", type=int, choices=[0, 1, 2, 3, 4], default=None,
                     help="band-LU block-step mode (library default when omitted)")
     ap.add_argument("--twist", type=int, choices=[0, 1, 2], default=None,
                     help="two-chain band LU: 0 off, 1 automatic (default), 2 always")
+    ap.add_argument("--pcg-lanes", type=int, choices=[4, 8, 16], default=None,
+                    help="block-CSR PCG: lanes per block row in the SpMV (default 8)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (HBM probe / counter reference) and exit")
     a = ap.parse_args()
@@ -393,6 +395,8 @@ def main():
         common["fused_step"] = a.fused_step
     if a.twist is not None:
         common["twist"] = a.twist
+    if a.pcg_lanes is not None:
+        common["pcg_lanes"] = a.pcg_lanes
     all_dense = dict(dense_max_dof=1 << 30) if a.workload == "replicas1024" else {}
 
     # ---- headline: batched mixed precision, one hipGraph replay per Newton step ----
@@ -400,7 +404,8 @@ def main():
     configs.load(ctx, cells, tables)
     tuned = None
     if a.autotune != "none":
-        flags = {"solver": W.TUNE_SOLVER, "lu": W.TUNE_LU, "all": W.TUNE_SOLVER | W.TUNE_LU}[a.autotune]
+        flags = {"solver": W.TUNE_SOLVER, "lu": W.TUNE_LU, "pcg": W.TUNE_PCG,
+                 "all": W.TUNE_SOLVER | W.TUNE_LU | W.TUNE_PCG}[a.autotune]
         tuned = ctx.autotune(flags=flags, steps=2)
         log("autotune (%s): %s" % (a.autotune, tuned))
     dt_run = timed_steps(ctx, a.steps, a.warmup)

@@ -110,6 +110,8 @@ typedef struct wcfe_options {
                             middle window of bw blocks), about half the dependent block steps.
                             1: when the batch's block steps are latency bound (at most 8 block
                             products per compute unit and step); 2: always; 0: one chain per cell */
+  int32_t pcg_lanes;     /* block-CSR PCG: lanes per block row in the SpMV, 4, 8 or 16 (0: 4); a
+                            workgroup of 1024 threads per chunk of 256 block rows */
 } wcfe_options;
 
 typedef struct wcfe_cell_desc {
@@ -249,13 +251,16 @@ int wcfe_synchronize(wcfe_ctx *ctx);
  * (1, 2, 4, 6) x concurrent cell groups (1, 2, 4), at depth 1 with and without
  * the fused block step, and (mixed precision, one group) the whole
  * factorization in one launch (fused_step 4); then, with fused block steps,
- * one chain per cell against two-chain cells (twist 0 / 2).  Every trial
- * starts from the state the cells were added with, and so does the batch
- * afterwards: call it after wcfe_finalize and before any Newton step.
- * chosen (optional, 5): lu_depth, lu_groups, dense_max_dof, fused_step and
- * the number of two-chain cells in effect; ms_best (optional): ms per step. */
+ * one chain per cell against two-chain cells (twist 0 / 2); WCFE_TUNE_PCG
+ * (batches with block-CSR cells): lanes per block row of the PCG's SpMV
+ * (pcg_lanes 4, 8, 16).  Every trial starts from the state the cells were
+ * added with, and so does the batch afterwards: call it after wcfe_finalize
+ * and before any Newton step.  chosen (optional, 6 entries): lu_depth,
+ * lu_groups, dense_max_dof, fused_step, the number of two-chain cells and
+ * pcg_lanes in effect; ms_best (optional): ms per step. */
 #define WCFE_TUNE_SOLVER 1
 #define WCFE_TUNE_LU 2
+#define WCFE_TUNE_PCG 4
 int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, double *ms_best);
 /* The dense-LU schedule in effect after wcfe_finalize (the options, or their
  * automatic choice): delayed-update depth, concurrent cell groups, the widest

@@ -85,3 +85,62 @@ def test_csr_pcg_deterministic_and_auto_threshold():
     assert st["max_cg_iters"] > 0  # the K=10 cell (> 200 unknowns) went through PCG
     for a, b in zip(runs[0], runs[1]):
         assert np.array_equal(a, b)
+
+
+def test_pcg_lanes_agree_and_are_deterministic():
+    """Lanes per block row of the SpMV (4, 8, 16): a batch of two sparse cells
+    of several 256-row chunks each and a dense cell between them reaches the
+    oracle's FP64 Newton solution with every lane count; a repeated run is
+    bit for bit the same.  The per-cell scalars come from each cell's own
+    chunk partials, so a wrong chunk range or parity shows as a wrong
+    solution of one of the two sparse cells."""
+    ds = [cells.cst_cell(K=k, n_spr=12 * k, law_table=(k % 2 == 0), seed=70 + k, time=1.1) for k in (14, 5, 18)]
+    solvers = [W.SOLVER_CSR, W.SOLVER_DENSE, W.SOLVER_CSR]
+    ref = [O.newton_solve_pcg(O.make_cell(d), d["x"], rtol=1e-12)[0] for d in ds]
+    for lanes in (4, 8, 16):
+        runs = []
+        for _ in range(2):
+            c = _ctx(ds, solvers, precision=W.PREC_MIXED, t0=1.1, newton_rtol=1e-11, newton_atol=1e-11,
+                     max_refine=12, pcg_lanes=lanes)
+            # each cell's positions right after the iteration that commits its
+            # first load step (a cell that converges first goes on to the next)
+            xs = [None] * len(ds)
+            for _ in range(20):
+                st = c.newton_step()
+                assert st["max_lin_relres"] <= 1e-8 and st["n_cg_breakdown"] == 0
+                for i in range(len(ds)):
+                    if xs[i] is None and c.cell_info(i)["step"] >= 1:
+                        xs[i] = c.positions(i)
+                if all(x is not None for x in xs):
+                    break
+            c.close()
+            assert all(x is not None for x in xs)
+            runs.append(xs)
+        for a, b in zip(runs[0], runs[1]):
+            assert np.array_equal(a, b), lanes
+        for d, x, xo in zip(ds, runs[0], ref):
+            err = np.max(np.abs(x - xo)) / np.max(np.abs(xo - d["X"]))
+            print("lanes %d, cell of %d triangles: |x - x_oracle| / |u| = %.2e" % (lanes, len(d["tris"]) // 3, err),
+                  flush=True)
+            assert err <= 1e-8
+
+
+def test_autotune_picks_pcg_lanes():
+    ds = [cells.cst_cell(K=16, n_spr=200, law_table=True, seed=81, time=1.0)]
+    c = _ctx(ds, [W.SOLVER_CSR], precision=W.PREC_MIXED, t0=1.0, newton_rtol=1e-11, newton_atol=1e-11, max_refine=12)
+    ch = c.autotune(flags=W.TUNE_PCG, steps=1)
+    assert ch["pcg_lanes"] in (4, 8, 16) and ch["ms_per_step"] > 0
+    st = c.load_step()
+    x = c.positions(0)
+    c.close()
+    assert st["max_lin_relres"] <= 1e-8
+    xo = O.newton_solve_pcg(O.make_cell(ds[0]), ds[0]["x"], rtol=1e-12)[0]
+    assert np.max(np.abs(x - xo)) <= 1e-8 * np.max(np.abs(xo - ds[0]["X"]))
+
+
+def test_pcg_lanes_out_of_range_is_an_argument_error():
+    d = cells.cst_cell(K=4, n_spr=20, seed=3, time=1.0)
+    with pytest.raises(W.WcfeError) as e:
+        c = _ctx([d], [W.SOLVER_CSR], pcg_lanes=5)
+        c.close()
+    assert e.value.code == W.ERR_ARG

@@ -39,7 +39,7 @@ EXPORTS = [
     "wcfe_force_ap", "wcfe_get_schedule", "wcfe_num_cells", "wcfe_get_cell_sizes",
     "wcfe_eval_exp", "wcfe_get_cell_layout", "wcfe_force_fa_noalloc", "wcfe_remesh_cell",
 ]
-TUNE_SOLVER, TUNE_LU = 1, 2
+TUNE_SOLVER, TUNE_LU, TUNE_PCG = 1, 2, 4
 # reference constants of the boundary and body forces (UL_growth_explicit.cpp)
 K_BEND_REF = 20.0              # pN/um, :1653
 K_AREA_REF = 1.0               # pN/um^3, :1776
@@ -57,7 +57,7 @@ class Options(C.Structure):
         ("k_off0", C.c_double), ("f_bell", C.c_double), ("kappa", C.c_double), ("symmetric", C.c_int32),
         ("dense_max_dof", C.c_int32), ("cg_rtol", C.c_double), ("cg_max_iter", C.c_int32),
         ("lu_groups", C.c_int32), ("lu_depth", C.c_int32), ("band", C.c_int32), ("fused_step", C.c_int32),
-        ("twist", C.c_int32),
+        ("twist", C.c_int32), ("pcg_lanes", C.c_int32),
     ]
 
 
@@ -318,14 +318,15 @@ class Context:
         return dict(lu_depth=v[0].value, lu_groups=v[1].value, max_band=v[2].value, band_solve=v[3].value,
                     fused_step=v[4].value)
 
-    def autotune(self, flags=TUNE_SOLVER | TUNE_LU, steps=2):
+    def autotune(self, flags=TUNE_SOLVER | TUNE_LU | TUNE_PCG, steps=2):
         """Time candidate settings on this batch and keep the fastest (call
         after finalize, before any Newton step)."""
-        chosen = np.zeros(5, dtype=np.int32)
+        chosen = np.zeros(6, dtype=np.int32)
         ms = C.c_double(0.0)
         self._check(self._L.wcfe_autotune(self._h, flags, steps, _ip(chosen), C.byref(ms)))
         return dict(lu_depth=int(chosen[0]), lu_groups=int(chosen[1]), dense_max_dof=int(chosen[2]),
-                    fused_step=int(chosen[3]), two_chain_cells=int(chosen[4]), ms_per_step=ms.value)
+                    fused_step=int(chosen[3]), two_chain_cells=int(chosen[4]), pcg_lanes=int(chosen[5]),
+                    ms_per_step=ms.value)
 
     def set_ul_state(self, cell, X=None, ea=None, cC=None, fau=None, time=float("nan")):
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (X, ea, cC, fau)]

@@ -1222,15 +1222,15 @@ void build_batch(wcfe_ctx *ctx, Batch &b, const std::vector<HostCell> &hc, int e
     b.csr_cells = dupload(b.csr_cells_h);
     b.Kb = (void *)dalloc<unsigned char>((size_t)std::max<long long>(b.nnzb, 1) * 4 * elem_size);
     b.cgvec = (void *)dalloc<unsigned char>((size_t)std::max(N, 1) * 16 * elem_size);
-    std::vector<CgState> cg(b.ncell);
-    for (int c = 0; c < b.ncell; ++c) {
+    std::vector<CgState> cg(2 * (size_t)b.ncell);  // the scalar state twice (by parity)
+    for (int c = 0; c < 2 * b.ncell; ++c) {
       memset(&cg[c], 0, sizeof cg[c]);
-      cg[c].chunk_off = b.chunk_off_h[c];
-      cg[c].nchunk = b.nchunk_h[c];
+      cg[c].chunk_off = b.chunk_off_h[c % b.ncell];
+      cg[c].nchunk = b.nchunk_h[c % b.ncell];
       cg[c].done = 1;
     }
     b.cg = dupload(cg);
-    b.partials = dalloc<double>(24 * (size_t)std::max(b.nchunk, 1));  // 3 sums, 8 row groups per chunk
+    b.partials = dalloc<double>(3 * (size_t)std::max(b.nchunk, 1));  // 3 sums per chunk
   }
   {
     // node gather plan: for node n, its distinct column nodes in order of
@@ -1354,7 +1354,7 @@ template <class T> CgVectors<T> cg_vectors(const Batch &b) {
   T *base = static_cast<T *>(b.cgvec);
   const long long n2 = 2LL * b.N;
   return CgVectors<T>{base,          base + n2,     base + 2 * n2, base + 3 * n2,
-                      base + 4 * n2, base + 5 * n2, base + 6 * n2, 8LL * std::max(b.nchunk, 1)};
+                      base + 4 * n2, base + 5 * n2, base + 6 * n2, (long long)std::max(b.nchunk, 1)};
 }
 
 BndView bnd_view(const wcfe_ctx *ctx, const Batch &b) {
@@ -1496,6 +1496,10 @@ template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc)
 // drains the stream): the first after 16, then as many as the slowest cell's
 // residual reduction so far predicts it still needs (8 .. 128, a quarter
 // more than predicted); iterations after a cell's convergence return at once.
+// In a batch the first update uses the scalars of the start or of the last
+// check, each later update forms its own from the SpMV before it (the scalar
+// state alternates between the two halves of b.cg), and the batch ends with
+// the scalars of its last SpMV, which the host reads.
 template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, int accumulate) {
   const int *cl;
   int cb, nch, ncl;
@@ -1504,16 +1508,23 @@ template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, in
   const CgVectors<T> v = cg_vectors<T>(b);
   const double rtol = ctx->opt.cg_rtol;
   const int maxit = std::max(1, ctx->opt.cg_max_iter);
-  launch_pcg_init<T>(s, b.chunks, cb, nch, cl, ncl, b.cs, b.cg, b.row_ptr, b.bcol, static_cast<const T *>(b.Kb),
-                     b.r, v, b.partials, rtol, maxit);
+  const int lanes = ctx->opt.pcg_lanes > 0 ? ctx->opt.pcg_lanes : 4;
+  CgState *half[2] = {b.cg, b.cg + b.ncell};
+  launch_pcg_init<T>(s, b.chunks, cb, nch, cl, ncl, b.cs, half[0], b.row_ptr, b.bcol, static_cast<const T *>(b.Kb),
+                     b.r, v, b.partials, rtol, maxit, lanes);
+  int cur = 0;
   std::vector<int> ids = sc.cell < 0 ? b.csr_cells_h : std::vector<int>{sc.cell};
   std::vector<CgState> h(b.ncell);
   int batch = 16;
   for (int it = 0; it < maxit; it += batch) {
-    for (int k = 0; k < batch; ++k)
-      launch_pcg_iteration<T>(s, b.chunks, cb, nch, cl, ncl, b.cs, b.cg, b.row_ptr, b.bcol,
-                              static_cast<const T *>(b.Kb), v, b.partials, rtol, maxit);
-    HIPCHECK(hipMemcpyAsync(h.data(), b.cg, sizeof(CgState) * b.ncell, hipMemcpyDeviceToHost, s));
+    for (int k = 0; k < batch; ++k) {
+      launch_pcg_iteration<T>(s, b.chunks, cb, nch, half[cur], half[1 - cur], k > 0, b.row_ptr, b.bcol,
+                              static_cast<const T *>(b.Kb), v, b.partials, rtol, maxit, lanes);
+      if (k > 0) cur = 1 - cur;
+    }
+    launch_pcg_check(s, cl, ncl, b.cs, half[cur], half[1 - cur], b.partials, v.pstride, rtol, maxit);
+    cur = 1 - cur;
+    HIPCHECK(hipMemcpyAsync(h.data(), half[cur], sizeof(CgState) * b.ncell, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     bool all = true;
     double need = 0.0;  // iterations the slowest open cell still needs at its rate so far
@@ -1742,6 +1753,7 @@ void wcfe_default_options(wcfe_options *o) {
   o->band = 1;
   o->fused_step = 1;
   o->twist = 1;
+  o->pcg_lanes = 0;
 }
 
 int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
@@ -1777,6 +1789,8 @@ int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
     if (ctx->opt.lu_depth < 0 || ctx->opt.lu_depth > 8) fail(WCFE_ERR_ARG, "lu_depth must be in [0, 8]");
     if (ctx->opt.fused_step < 0 || ctx->opt.fused_step > 4) fail(WCFE_ERR_ARG, "fused_step must be in [0, 4]");
     if (ctx->opt.twist < 0 || ctx->opt.twist > 2) fail(WCFE_ERR_ARG, "twist must be 0, 1 or 2");
+    if (ctx->opt.pcg_lanes != 0 && ctx->opt.pcg_lanes != 4 && ctx->opt.pcg_lanes != 8 && ctx->opt.pcg_lanes != 16)
+      fail(WCFE_ERR_ARG, "pcg_lanes must be 0, 4, 8 or 16");
     if (ctx->opt.band != 0 && ctx->opt.band != 1) fail(WCFE_ERR_ARG, "band must be 0 or 1");
     HIPCHECK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     for (int g = 0; g < std::max(ctx->opt.lu_groups, AUTO_GROUPS_MAX); ++g) {
@@ -2157,6 +2171,22 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
         }
       }
     }
+    // 4. block-CSR PCG: lanes per block row in the SpMV (batches with sparse
+    //    cells under the best setting so far)
+    if (flags & WCFE_TUNE_PCG) {
+      const wcfe_options base = best;
+      for (int lanes : {4, 8, 16}) {
+        ctx->opt = base;
+        ctx->opt.pcg_lanes = lanes;
+        const double ms = trial();
+        if (ms < 1e300) tried.push_back({ms, ctx->opt});
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = ctx->opt;
+        }
+        if (!ctx->b.any_csr) break;  // no sparse cell: the lane count does not matter
+      }
+    }
     // the fastest candidate that can be laid out again (a setting near the
     // memory limit may not fit a second time); else the batch's own
     std::stable_sort(tried.begin(), tried.end(),
@@ -2187,6 +2217,7 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
       int ntw = 0;
       for (const CellParams &p : ctx->b.cp_h) ntw += p.tw_m > 0;
       chosen[4] = ntw;
+      chosen[5] = ctx->opt.pcg_lanes > 0 ? ctx->opt.pcg_lanes : 4;
     }
     if (ms_best) *ms_best = best_ms < 1e300 ? best_ms : 0.0;
   });

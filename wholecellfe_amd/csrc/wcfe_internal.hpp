@@ -177,20 +177,27 @@ struct CgState {
 };
 template <class T> struct CgVectors {
   T *x, *r, *z, *p, *q, *s, *dinv;  // 2 N each (dinv: 4 N); z = M^-1 r, q = K z, s = K p
-  long long pstride;                // partial sums: three arrays of pstride (8 per chunk of the batch)
+  long long pstride;                // partial sums: three arrays of pstride (one per chunk of the batch)
 };
 template <class T> void launch_zero(hipStream_t s, T *p, long long n);
 template <class T>
 void launch_bsr_diag_inv(hipStream_t s, const int4 *chunks, int nchunk, const int *diag_pos, const T *val, T *dinv);
 // chunks [cbase, cbase + nchunk) of the batch's chunk list
+// lanes: 4, 8 or 16 lanes per block row in the SpMV
 template <class T>
 void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const int *cells, int ncell,
                      const CellState *cs, CgState *cg, const int *row_ptr, const int *col, const T *val,
-                     const double *rhs, const CgVectors<T> &v, double *partials, double rtol, int max_iter);
+                     const double *rhs, const CgVectors<T> &v, double *partials, double rtol, int max_iter,
+                     int lanes);
+// one iteration: the update (step: with the scalars formed from the last
+// SpMV's partials, gin -> gout; else with gin's), then the SpMV
 template <class T>
-void launch_pcg_iteration(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const int *cells, int ncell,
-                          const CellState *cs, CgState *cg, const int *row_ptr, const int *col, const T *val,
-                          const CgVectors<T> &v, double *partials, double rtol, int max_iter);
+void launch_pcg_iteration(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const CgState *gin,
+                          CgState *gout, int step, const int *row_ptr, const int *col, const T *val,
+                          const CgVectors<T> &v, double *partials, double rtol, int max_iter, int lanes);
+// the scalars of the last SpMV by one workgroup per cell, gin -> gout
+void launch_pcg_check(hipStream_t s, const int *cells, int ncell, const CellState *cs, const CgState *gin,
+                      CgState *gout, const double *partials, long long pstride, double rtol, int max_iter);
 
 template <class T>
 void launch_pcg_finish(hipStream_t s, const int4 *chunks, int nchunk, const CellState *cs, const T *x, double *delta,
