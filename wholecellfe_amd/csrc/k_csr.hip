@@ -258,8 +258,25 @@ __global__ __launch_bounds__(CG_ROWS) void k_pcg_update(const int4 *chunks, int 
                                                         double rtol, int max_iter, const T *dinv, T *p, T *sv,
                                                         const T *w, T *x, T *r, T *u) {
   __shared__ double red[3][CG_ROWS];
+  typedef typename V2<T>::type v2;
+  typedef typename V4<T>::type v4;
   const int4 ch = chunks[blockIdx.x];
   CgState g = gin[ch.x];
+  if (g.done && !STEP) return;
+  // the node's vectors first: their loads do not wait for the scalars
+  const int i = ch.y + threadIdx.x;
+  const bool own = i < ch.z && !g.done;
+  v2 pv{}, sv2{}, uv{}, wv{}, xv{}, rv{};
+  v4 dv{};
+  if (own) {
+    pv = reinterpret_cast<const v2 *>(p)[i];
+    sv2 = reinterpret_cast<const v2 *>(sv)[i];
+    uv = reinterpret_cast<const v2 *>(u)[i];
+    wv = reinterpret_cast<const v2 *>(w)[i];
+    xv = reinterpret_cast<const v2 *>(x)[i];
+    rv = reinterpret_cast<const v2 *>(r)[i];
+    dv = reinterpret_cast<const v4 *>(dinv)[i];
+  }
   if (STEP) {
     if (!g.done) {
       double s0, s1, s2;
@@ -268,22 +285,19 @@ __global__ __launch_bounds__(CG_ROWS) void k_pcg_update(const int4 *chunks, int 
     }
     if (cbase + (int)blockIdx.x == g.chunk_off && threadIdx.x == 0) gout[ch.x] = g;
   }
-  if (g.done) return;
+  if (g.done || i >= ch.z) return;
   const T alpha = (T)g.alpha, beta = (T)g.beta;
-  const int i = ch.y + threadIdx.x;
-  if (i >= ch.z) return;
-  const long long k = 2LL * i;
-  const T p0 = u[k] + beta * p[k], p1 = u[k + 1] + beta * p[k + 1];
-  const T s0 = w[k] + beta * sv[k], s1 = w[k + 1] + beta * sv[k + 1];
-  p[k] = p0; p[k + 1] = p1;
-  sv[k] = s0; sv[k + 1] = s1;
-  x[k] += alpha * p0;
-  x[k + 1] += alpha * p1;
-  const T r0 = r[k] - alpha * s0, r1 = r[k + 1] - alpha * s1;
-  r[k] = r0; r[k + 1] = r1;
+  const T p0 = uv.x + beta * pv.x, p1 = uv.y + beta * pv.y;
+  const T s0 = wv.x + beta * sv2.x, s1 = wv.y + beta * sv2.y;
+  reinterpret_cast<v2 *>(p)[i] = v2{p0, p1};
+  reinterpret_cast<v2 *>(sv)[i] = v2{s0, s1};
+  reinterpret_cast<v2 *>(x)[i] = v2{xv.x + alpha * p0, xv.y + alpha * p1};
+  const T r0 = rv.x - alpha * s0, r1 = rv.y - alpha * s1;
+  reinterpret_cast<v2 *>(r)[i] = v2{r0, r1};
+  const T d[4] = {dv.x, dv.y, dv.z, dv.w};
   T u0, u1;
-  apply2(dinv + 4LL * i, r0, r1, u0, u1);
-  u[k] = u0; u[k + 1] = u1;
+  apply2(d, r0, r1, u0, u1);
+  reinterpret_cast<v2 *>(u)[i] = v2{u0, u1};
 }
 
 // a new solve: cells still refining run it, the others stay done
