@@ -355,7 +355,7 @@ def main():
     ap.add_argument("--pivoting", choices=["static", "partial"], default="static")
     ap.add_argument("--autotune", choices=["none", "solver", "lu", "pcg", "all"], default="none",
                     help="run wcfe_autotune on the headline batch before timing")
-    ap.add_argument("--fused-step", type=int, choices=[0, 1, 2, 3, 4], default=None,
+    ap.add_argument("--fused-step", type=int, choices=[0, 1, 2, 3, 4, 5], default=None,
                     help="band-LU block-step mode (library default when omitted)")
     ap.add_argument("--twist", type=int, choices=[0, 1, 2], default=None,
                     help="two-chain band LU: 0 off, 1 automatic (default), 2 always")

@@ -97,12 +97,15 @@ typedef struct wcfe_options {
   int32_t fused_step;    /* with one panel per round (lu_depth 1) in FP32, one launch per block step:
                             trailing update, next diagonal inverse and its block solves together.
                             1: per launch, L items (each inverts the diagonal block itself: shortest
-                            chain) or, above 2 items per compute unit, one panel item per cell (one
-                            inverse, then every L block: least work) and one item for the
-                            cell's trailing blocks; 2: L items only; 3: panel items
-                            after the first step; 4: one launch for the whole
-                            factorization, each cell's block steps in order in one
-                            workgroup (no launch between two steps of a cell's chain);
+                            chain) while they are at most 2 per compute unit; else split panel
+                            items while those are (one item inverts and solves the first L block,
+                            the other L blocks are L items, one item takes the cell's trailing
+                            blocks); else one panel item per cell (one inverse, then every L
+                            block: least work) and one item for the trailing blocks;
+                            2: L items only; 3: panel items after the first step; 4: one
+                            launch for the whole factorization, each cell's block steps in
+                            order in one workgroup (no launch between two steps of a cell's
+                            chain); 5: as 1, split panel items wherever 1 takes panel items;
                             0: separate launches */
   int32_t twist;         /* with fused block steps and one-workgroup sweeps in FP32 (block bandwidths
                             up to 4): cells whose chain of block steps is longer than the batch needs
@@ -249,8 +252,9 @@ int wcfe_synchronize(wcfe_ctx *ctx);
  * threshold (dense_max_dof, cells with solver AUTO; candidates that would not
  * fit in free device memory are skipped); WCFE_TUNE_LU: delayed-update depth
  * (1, 2, 4, 6) x concurrent cell groups (1, 2, 4), at depth 1 with and without
- * the fused block step, and (mixed precision, one group) the whole
- * factorization in one launch (fused_step 4); then, with fused block steps,
+ * the fused block step, with split panel items (fused_step 5) and (mixed
+ * precision, one group) the whole factorization in one launch (fused_step
+ * 4); then, with fused block steps,
  * one chain per cell against two-chain cells (twist 0 / 2); WCFE_TUNE_PCG
  * (batches with block-CSR cells): lanes per block row of the PCG's SpMV
  * (pcg_lanes 4, 8, 16).  Every trial starts from the state the cells were

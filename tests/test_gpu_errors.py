@@ -121,7 +121,7 @@ def _first_solve(ds, fused, sym):
 def test_fused_forward_block_first_solve(sym):
     """The first solve's forward sweep runs inside the fused block steps
     (every form: per-launch choice, L items, panel items, one workgroup per
-    cell for the whole factorization).  With one solve
+    cell for the whole factorization, split panel items).  With one solve
     sweep and no refinement, its delta must be as accurate as the unfused
     factorization's.  The batch mixes one-block cells (no block step),
     two-block cells (a last step without trailing tiles) and wider cells, and
@@ -132,11 +132,16 @@ def test_fused_forward_block_first_solve(sym):
                          law_table=(i % 3 == 0), seed=100 + i, time=1.0) for i in range(270)]
     print("case: sym %d, %d cells" % (sym, len(ds)), flush=True)
     res = {}
-    for fused in (0, 1, 2, 3, 4):
+    for fused in (0, 1, 2, 3, 4, 5):
         xs, sched = _first_solve(ds, fused, sym)
         assert sched["fused_step"] == (2 if fused == 4 else 1 if fused else 0)
         res[fused] = xs
-    worst = {f: 0.0 for f in (1, 2, 3, 4)}
+    # every fused form runs the same products in the same order (the split
+    # panel form of the large first launches included)
+    for f in (2, 3, 4, 5):
+        for a, b in zip(res[1], res[f]):
+            assert np.array_equal(a, b), f
+    worst = {f: 0.0 for f in (1, 2, 3, 4, 5)}
     for i, d in enumerate(ds):
         G, K = O.assemble(O.make_cell(d))
         nb = np.linalg.norm(G)
@@ -145,7 +150,7 @@ def test_fused_forward_block_first_solve(sym):
         d_exact = np.linalg.solve(K, -G)
         r0 = np.linalg.norm(-G - K @ (res[0][i] - d["x"])) / nb
         e0 = np.linalg.norm(res[0][i] - d["x"] - d_exact) / np.linalg.norm(d_exact)
-        for f in (1, 2, 3, 4):
+        for f in (1, 2, 3, 4, 5):
             df = res[f][i] - d["x"]
             rf = np.linalg.norm(-G - K @ df) / nb
             ef = np.linalg.norm(df - d_exact) / np.linalg.norm(d_exact)

@@ -122,7 +122,7 @@ def test_autotune_picks_valid_settings_and_keeps_results():
     c.finalize()
     ch = c.autotune(steps=1)
     assert ch["lu_depth"] in (1, 2, 4, 6) and ch["lu_groups"] in (1, 2, 4) and ch["ms_per_step"] > 0
-    assert ch["fused_step"] in (0, 1, 4)
+    assert ch["fused_step"] in (0, 1, 4, 5)
     # the reported two-chain count is the batch's layout after the tuning
     assert ch["two_chain_cells"] == sum(c.cell_layout(i)["chain_top"] > 0 for i in range(len(ds)))
     # the batch restarts from the state the cells were added with

@@ -102,11 +102,12 @@ def test_two_chain_first_solve_accuracy():
 
 
 def test_fused_forms_agree_bit_for_bit_with_two_chains():
-    """L items, panel items, the per-launch choice and one workgroup per cell
-    run the same products in the same order on the split cells too."""
+    """L items, panel items, split panel items, the per-launch choice and one
+    workgroup per cell run the same products in the same order on the split
+    cells too."""
     ds = _batch()
     ref = None
-    for fused in (1, 2, 3, 4):
+    for fused in (1, 2, 3, 4, 5):
         c = _ctx(ds, fused_step=fused, max_refine=3)
         assert c.cell_layout(len(ds) - 1)["chain_top"] > 0
         c.newton_step(stats=True, check=False)

@@ -1009,9 +1009,12 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
     return;
   }
   if (k < 0) panel = 0;  // the first step always runs L items
-  // panel mode: item 0 inverts block s0 once and solves every L block of
-  // the panel (no L items); one item takes all trailing blocks of the window
-  const int nl = panel ? 0 : wl;
+  // panel mode (panel 1): item 0 inverts block s0 once and solves every L
+  // block of the panel (no L items); one item takes all trailing blocks of
+  // the window.  Split panel mode (panel 2): item 0 inverts block s0 and
+  // solves the first L block, the others are L items (each inverts block s0
+  // itself), one item takes the trailing blocks.
+  const int nl = panel == 1 ? 0 : panel == 2 ? (wl > 1 ? wl - 1 : 0) : wl;
   const int nsolve = nl + (v.sym ? 0 : wl);
   // trailing tile q of the window (lower triangle for symmetric cells)
   auto plain_tile = [&](int q, int &ti, int &tj) {
@@ -1045,7 +1048,7 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
   }
   int role, bi, bj;
   if (t == 0) { role = 0; bi = panel && wl > 0 ? s0 + 1 : s0; bj = s0; }
-  else if (t <= nl) { role = 1; bi = s0 + t; bj = s0; }
+  else if (t <= nl) { role = 1; bi = s0 + t + (panel == 2 ? 1 : 0); bj = s0; }
   else if (t <= nsolve) { role = 2; bi = s0; bj = s0 + (t - nl); }
   else if (panel) { role = 4; bi = s0; bj = s0; }
   else {
@@ -1106,6 +1109,11 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
     tile_to_lds<T, P>(ld2, ls);
     tile_to_lds_bswz<T>(ut, us);
     if (upd_own) tile_load<T>(lt, A + v.off(bi, k, ld), ld);
+    // panel item: the second L block (and, inside panel k's reach, its L
+    // block of column k, below in post) are loaded while the inverse runs
+    const bool pf2 = store && panel == 1 && wl >= 2, pf2_upd = pf2 && s0 + 2 <= reach;
+    acc_t pf[2][2];
+    if (pf2) acc_load64<T>(pf, A + v.off(s0 + 2, s0, ld), ld);
     __syncthreads();
     auto pre = [&]() {
       acc_t d;
@@ -1142,6 +1150,7 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
     };
     auto post = [&](int kb) {
       if (kb == 0 && upd_own) tile_to_lds<T, P>(lt, ls);  // L_{s0,k} is done with
+      if (kb == 0 && pf2_upd) tile_load<T>(lt, A + v.off(s0 + 2, k, ld), ld);
       if (kb == 3 && own && w > 0) {
 #pragma unroll
         for (int sl = 0; sl < 6; ++sl) {
@@ -1166,20 +1175,43 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
         for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
     mma_64<T, P, PA>(ls, xs, acc, T(1));
     acc_store64<T, 0>(acc, C, ld);
-    if (!(store && panel)) return;
+    if (!(store && panel == 1)) return;
     // panel item: L blocks s0+2 .. s0+w after the inverse, each as an L item
     // computes it (same products in the same order)
     for (int t2 = 2; t2 <= wl; ++t2) {
       const int b2 = s0 + t2;
       T *C2 = A + v.off(b2, s0, ld);
-      acc_load64<T>(acc, C2, ld);
+      if (t2 == 2) {
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = pf[ti][tj];
+      } else {
+        acc_load64<T>(acc, C2, ld);
+      }
       __syncthreads();  // the previous block's product is done with ls
       if (b2 <= reach) {
-        tile_load<T>(lt, A + v.off(b2, k, ld), ld);
+        if (t2 > 2) tile_load<T>(lt, A + v.off(b2, k, ld), ld);
         tile_to_lds<T, P>(lt, ls);
         __syncthreads();
         mma_64_bswz<T, P>(ls, us, acc, T(-1));  // us: U_{k,s0}
         __syncthreads();
+      }
+      if (v.sym) {  // U_{s0,b2} = A_{b2,s0}^T from the registers: four consecutive entries of one row per lane
+        T *U2 = A + v.off(s0, b2, ld);
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < 2; ++tj) {
+            T *u2 = U2 + (long long)(wc + tj * 16 + lr) * ld + wr + ti * 16;
+            if constexpr (sizeof(T) == 4) {
+              *reinterpret_cast<float4 *>(u2 + 4 * lk) =
+                  make_float4(acc[ti][tj][0], acc[ti][tj][1], acc[ti][tj][2], acc[ti][tj][3]);
+            } else {
+#pragma unroll
+              for (int r = 0; r < 4; ++r) u2[acc_row(T(0), lk, r)] = acc[ti][tj][r];
+            }
+          }
       }
 #pragma unroll
       for (int ti = 0; ti < 2; ++ti)
@@ -1188,13 +1220,6 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
 #pragma unroll
           for (int r = 0; r < 4; ++r) ls[(wr + ti * 16 + acc_row(T(0), lk, r)) * P + wc + tj * 16 + lr] = acc[ti][tj][r];
       __syncthreads();
-      if (v.sym) {  // U_{s0,b2} = A_{b2,s0}^T
-        T *U2 = A + v.off(s0, b2, ld);
-        for (int idx = threadIdx.x; idx < 4096; idx += blockDim.x) {
-          const int r = idx >> 6, cc = idx & 63;
-          U2[(long long)r * ld + cc] = ls[cc * P + r];
-        }
-      }
 #pragma unroll
       for (int ti = 0; ti < 2; ++ti)
 #pragma unroll

@@ -312,12 +312,15 @@ struct LuPlan {
       case ST_GEMM1: return k < nb ? tiles(p, m) : 0;
       case ST_GEMM: return nb - k - np >= 1 ? tiles(p, std::min(nb - k - np, p.bw)) : 0;
       case ST_PANEL: return k < nb ? 1 : 0;
-      case ST_STEP:  // fused block step k (k = -1 .. chain_steps - 2), see k_lu_step; np 2: panel mode
-        return cell_step_items(p, k, np == 2 ? 1 : 0);
+      case ST_STEP:  // fused block step k (k = -1 .. chain_steps - 2), see k_lu_step; np: the form
+        return cell_step_items(p, k, step_flag(np));
       case ST_CELL: return nb > 0 ? 1 : 0;  // the whole factorization, one workgroup
       default: return (k < nb && m > 0) ? (m * NB + 255) / 256 : 0;
     }
   }
+  // a block step's form (Stage::np): 1 L items, 2 panel items, 3 split
+  // panel items; the kernel's panel argument: 0, 1, 2
+  static int step_flag(int np) { return np == 2 ? 1 : np == 3 ? 2 : 0; }
   static long long chain_products(const Chain &v, int k) {
     const int s0 = k + 1;
     if (s0 >= v.nblk || s0 > v.elim) return 0;
@@ -395,39 +398,47 @@ struct LuPlan {
         int t = v & 0xFFFFF;
         Chain ch = chain_of(p, 0);
         if (p.tw_m > 0) {
-          const int na = chain_items(ch, g.k, g.np == 2 ? 1 : 0);
+          const int na = chain_items(ch, g.k, step_flag(g.np));
           if (t >= na) {
             ch = chain_of(p, 1);
             t -= na;
           }
         }
         if (g.k + 1 >= ch.elim) return false;  // update-only items
-        return t <= (g.np == 2 ? 0 : std::min(ch.nblk - g.k - 2, ch.bw));
+        const int wl = std::min(ch.nblk - g.k - 2, ch.bw);
+        return t <= (g.np == 2 ? 0 : g.np == 3 ? std::max(wl - 1, 0) : wl);
       });
     }
   }
 
-  // fused block steps (fuse = the fused_step option): 1 panel items in the
-  // launches with more than panel_min items, L items in the others; 2 L items
+  // fused block steps (fuse = the fused_step option): 1 per launch, L items
+  // while they are at most panel_min, else split panel items while those
+  // are, else panel items (one inverse per cell: the least work); 2 L items
   // only; 3 panel items after the first step; 4 one launch for the whole
   // factorization (k_lu_cell: every cell's steps in one workgroup, largest
-  // cells first)
+  // cells first); 5 split panel items (the first L block with the inverse,
+  // the others as L items) in every launch with more than panel_min L items,
+  // L items in the others
   void build(const std::vector<CellParams> &cp, int piv, int groups, int lu_depth, int fuse = 0, int panel_min = 512) {
     if (lu_depth < 1 || groups < 1) throw ApiError{WCFE_ERR_ARG, "LU schedule needs depth and groups >= 1"};
     pivoting = piv;
     depth = lu_depth;
     fused = fuse && !piv && lu_depth == 1;
     cell_loop = fused && fuse == 4;
-    // the step's form: L items (np 1) or one panel item per cell (np 2)
+    // the step's form: L items (np 1), one panel item per cell (np 2) or
+    // split panel items (np 3)
     auto step_form = [&](const std::vector<int> &cells, std::array<int, 3> &st) {
       if (st[0] != ST_STEP || st[1] < 0 || fuse == 2) return;
       if (fuse == 3) {
         st[2] = 2;
         return;
       }
-      long long tot = 0;
-      for (int c : cells) tot += items(cp[c], ST_STEP, st[1], 1, pivoting);
-      if (tot > panel_min) st[2] = 2;
+      long long tot = 0, split = 0;
+      for (int c : cells) {
+        tot += items(cp[c], ST_STEP, st[1], 1, pivoting);
+        split += items(cp[c], ST_STEP, st[1], 3, pivoting);
+      }
+      if (tot > panel_min) st[2] = (fuse == 5 || split <= panel_min) ? 3 : 2;
     };
     std::vector<int> all((size_t)cp.size());
     for (size_t c = 0; c < cp.size(); ++c) all[c] = (int)c;
@@ -1448,7 +1459,7 @@ void enqueue_lu(wcfe_ctx *ctx, hipStream_t s, const LuPlan &plan, const std::vec
       case ST_STEP: {
         if (sizeof(T) != 4) fail(WCFE_ERR_STATE, "the fused block step is built for FP32 factors only");
         KernelTimer kt(ctx, WCFE_PH_LU_GEMM, g.work);
-        launch_lu_step<T>(s, cp, cs, wl, g.k, K, Dinv, g.np == 2 ? 1 : 0, fwd, kt.a, kt.b);
+        launch_lu_step<T>(s, cp, cs, wl, g.k, K, Dinv, LuPlan::step_flag(g.np), fwd, kt.a, kt.b);
         break;
       }
       case ST_CELL: {
@@ -1787,7 +1798,7 @@ int wcfe_create(const wcfe_options *opt, wcfe_ctx **out) {
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     if (ctx->opt.lu_groups < 0 || ctx->opt.lu_groups > 16) fail(WCFE_ERR_ARG, "lu_groups must be in [0, 16]");
     if (ctx->opt.lu_depth < 0 || ctx->opt.lu_depth > 8) fail(WCFE_ERR_ARG, "lu_depth must be in [0, 8]");
-    if (ctx->opt.fused_step < 0 || ctx->opt.fused_step > 4) fail(WCFE_ERR_ARG, "fused_step must be in [0, 4]");
+    if (ctx->opt.fused_step < 0 || ctx->opt.fused_step > 5) fail(WCFE_ERR_ARG, "fused_step must be in [0, 5]");
     if (ctx->opt.twist < 0 || ctx->opt.twist > 2) fail(WCFE_ERR_ARG, "twist must be 0, 1 or 2");
     if (ctx->opt.pcg_lanes != 0 && ctx->opt.pcg_lanes != 4 && ctx->opt.pcg_lanes != 8 && ctx->opt.pcg_lanes != 16)
       fail(WCFE_ERR_ARG, "pcg_lanes must be 0, 4, 8 or 16");
@@ -2139,8 +2150,12 @@ int wcfe_autotune(wcfe_ctx *ctx, int32_t flags, int32_t steps, int32_t *chosen, 
       const wcfe_options base = best;
       for (int d : depths)
         for (int g : groups)
-          for (int f = (d == 1 && g == 1 && mixed(ctx)) ? 4 : 1; f >= (d == 1 ? 0 : 1); --f) {
-            if (f == 3 || f == 2) continue;  // the per-launch choice (1) covers the L-item and panel forms
+          for (int f : {5, 4, 1, 0}) {
+            // the per-launch choice (1) covers the L-item, split panel and
+            // panel forms; 5 takes split panel items wherever 1 takes panel
+            // items; 4 (one workgroup per cell) with one group in FP32
+            if (d > 1 && f != 1) continue;
+            if ((f == 4 && (g != 1 || !mixed(ctx))) || (f == 5 && !mixed(ctx))) continue;
             ctx->opt = base;
             ctx->opt.lu_depth = d;
             ctx->opt.lu_groups = g;

@@ -110,7 +110,10 @@ __host__ __device__ inline int chain_items(const Chain &v, int k, int panel) {
     npl = e - s0 > 0 ? e - s0 : 0;
   }
   const int tiles = v.sym ? npl * (npl + 1) / 2 : npl * npl;
-  if (panel && k >= 0) return 2 + (v.sym ? 0 : w);
+  // panel 1: one item inverts and solves every L block, one takes the
+  // trailing tiles; 2: the same, but the L blocks after the first are L items
+  if (panel == 1 && k >= 0) return 2 + (v.sym ? 0 : w);
+  if (panel == 2 && k >= 0) return 2 + (w > 1 ? w - 1 : 0) + (v.sym ? 0 : w);
   return 2 + w + (v.sym ? 0 : w) + tiles;
 }
 __host__ __device__ inline int cell_step_items(const CellParams &p, int k, int panel) {
