@@ -251,13 +251,30 @@ __global__ __launch_bounds__(256) void k_group_gather(View v, const int *node_ce
     b0 = K + p.k_off + 2LL * v.kpos[n] * p.npad + 2 * v.kpos[gp.col[g]];
     b1 = b0 + p.npad;
   }
-  T a00 = b0[0], a01 = b0[1], a10 = b1[0], a11 = b1[1];
-  for (int q = gp.cstart[g]; q < gp.cstart[g + 1]; ++q) {
-    const double *Ke = Ke64 + gp.ke[q];
-    a00 += T(Ke[0]);
-    a01 += T(Ke[1]);
-    a10 += T(Ke[6]);
-    a11 += T(Ke[7]);
+  // the block's entries are zero (the fill or the sparse zero) before its one
+  // group adds to them: no read.  The contributions four at a time, their
+  // offsets and then their values loaded together, added in order.
+  T a00 = T(0), a01 = T(0), a10 = T(0), a11 = T(0);
+  const int q1 = gp.cstart[g + 1];
+  for (int qb = gp.cstart[g]; qb < q1; qb += 4) {
+    int o[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) o[u] = qb + u < q1 ? gp.ke[qb + u] : -1;
+    double2 x[4], y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (o[u] >= 0) {
+        x[u] = *reinterpret_cast<const double2 *>(Ke64 + o[u]);
+        y[u] = *reinterpret_cast<const double2 *>(Ke64 + o[u] + 6);
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (o[u] >= 0) {
+        a00 += T(x[u].x);
+        a01 += T(x[u].y);
+        a10 += T(y[u].x);
+        a11 += T(y[u].y);
+      }
   }
   b0[0] = a00;
   b0[1] = a01;
