@@ -84,7 +84,10 @@ typedef struct wcfe_options {
                             triangle is updated and U_kj = D_k L_jk^T (half the flops);
                             0: always the full LU */
   int32_t dense_max_dof; /* WCFE_SOLVER_AUTO: cells with more unknowns use the CSR solver */
-  double cg_rtol;        /* PCG (working precision) stops at ||r|| <= cg_rtol ||r0|| per refinement sweep */
+  double cg_rtol;        /* PCG (working precision) stops at ||r|| <= tol ||r0|| per refinement sweep:
+                            tol = cg_rtol in the first sweep; in a later one, when the cell's
+                            FP64 relative residual so far is e, tol = max(cg_rtol,
+                            0.1 lin_rtol / e) capped at 0.1 (the rest of the way to lin_rtol) */
   int32_t cg_max_iter;   /* PCG iteration cap per refinement sweep */
   int32_t lu_groups;     /* batched dense LU: cells split in this many groups factored on concurrent
                             streams (1..16; 0 = automatic: 1 when every dense band is narrow -- 2 when

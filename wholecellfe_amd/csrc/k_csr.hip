@@ -175,14 +175,15 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_pcg_spmv_dot(const int4 *chunk
 }
 
 // The scalars of one step from the sums of the last SpMV: convergence
-// ||r|| <= rtol ||r0|| (r of the update just made) or the iteration cap;
+// ||r|| <= tol ||r0|| (r of the update just made; tol: g.tol) or the iteration cap;
 // alpha and beta of the next update.  A cell already done carries its state.
 __device__ __forceinline__ void pcg_step_scalars(CgState &g, double gamma, double delta, double rr, double rtol,
                                                  int max_iter) {
   if (g.done) return;
   g.iters += 1;
   g.rr = rr;
-  if (rr <= rtol * rtol * g.rr0 || g.iters >= max_iter || !(rr == rr)) {
+  (void)rtol;
+  if (rr <= g.tol * g.tol * g.rr0 || g.iters >= max_iter || !(rr == rr)) {
     g.done = 1;
     return;
   }
@@ -201,7 +202,8 @@ __device__ __forceinline__ void pcg_step_scalars(CgState &g, double gamma, doubl
 
 // the start of a solve: gamma, delta of the initial u; alpha = gamma / delta,
 // beta = 0; done if refinement is finished or r = 0
-__device__ __forceinline__ void pcg_init_scalars(CgState &g, double gamma, double delta, double rr) {
+__device__ __forceinline__ void pcg_init_scalars(CgState &g, double gamma, double delta, double rr, double tol) {
+  g.tol = tol;
   g.iters = 0;
   g.breakdown = 0;
   g.rr = g.rr0 = rr;
@@ -313,7 +315,7 @@ constexpr int SCALAR_THREADS = 1024;
 __global__ __launch_bounds__(SCALAR_THREADS) void k_pcg_scalars(const int *cells, const CellState *cs,
                                                                 const CgState *gin, CgState *gout,
                                                                 const double *partials, long long pstride, int mode,
-                                                                double rtol, int max_iter) {
+                                                                double rtol, double lin_rtol, int max_iter) {
   __shared__ double red[3][SCALAR_THREADS];
   const int c = cells[blockIdx.x];
   CgState g = gin[c];
@@ -328,10 +330,18 @@ __global__ __launch_bounds__(SCALAR_THREADS) void k_pcg_scalars(const int *cells
   double s0, s1, s2;
   sum_partials<SCALAR_THREADS>(partials + g.chunk_off, pstride, g.nchunk, red, s0, s1, s2);
   if (threadIdx.x != 0) return;
-  if (mode == CG_INIT)
-    pcg_init_scalars(g, s0, s1, s2);
-  else
+  if (mode == CG_INIT) {
+    // a later refinement sweep needs only the rest of the way to lin_rtol
+    // (a tenth of it, for the FP32 system's own error): the solve's
+    // tolerance from the cell's relative residual so far (0 before the
+    // first sweep)
+    const double lr = cs[c].lin_relres;
+    double tol = rtol;
+    if (lr > 0.0 && lin_rtol > 0.0) tol = fmin(0.1, fmax(rtol, 0.1 * lin_rtol / lr));
+    pcg_init_scalars(g, s0, s1, s2, tol);
+  } else {
     pcg_step_scalars(g, s0, s1, s2, rtol, max_iter);
+  }
   gout[c] = g;
 }
 
@@ -374,8 +384,8 @@ static void launch_spmv_dot(hipStream_t s, const int4 *chunks, int cbase, int nc
 template <class T>
 void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const int *cells, int ncell,
                      const CellState *cs, CgState *cg, const int *row_ptr, const int *col, const T *val,
-                     const double *rhs, const CgVectors<T> &v, double *partials, double rtol, int max_iter,
-                     int lanes) {
+                     const double *rhs, const CgVectors<T> &v, double *partials, double rtol, double lin_rtol,
+                     int max_iter, int lanes) {
   if (nchunk <= 0) return;
   hipLaunchKernelGGL(k_pcg_reset_done, dim3((ncell + 63) / 64), dim3(64), 0, s, cells, ncell, cs, cg);
   hipLaunchKernelGGL(k_pcg_init<T>, dim3(nchunk), dim3(CG_ROWS), 0, s, chunks + cbase, cs, rhs, v.dinv, v.x, v.r,
@@ -383,7 +393,7 @@ void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, c
   // w = K u and the partials of the first step
   launch_spmv_dot<T>(s, chunks + cbase, cbase, nchunk, cg, row_ptr, col, val, v, partials, lanes);
   hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(SCALAR_THREADS), 0, s, cells, cs, cg, cg, partials, v.pstride,
-                     (int)CG_INIT, rtol, max_iter);
+                     (int)CG_INIT, rtol, lin_rtol, max_iter);
 }
 
 template <class T>
@@ -405,7 +415,7 @@ void launch_pcg_check(hipStream_t s, const int *cells, int ncell, const CellStat
                       CgState *gout, const double *partials, long long pstride, double rtol, int max_iter) {
   if (ncell <= 0) return;
   hipLaunchKernelGGL(k_pcg_scalars, dim3(ncell), dim3(SCALAR_THREADS), 0, s, cells, cs, gin, gout, partials, pstride,
-                     (int)CG_STEP, rtol, max_iter);
+                     (int)CG_STEP, rtol, 0.0, max_iter);
 }
 
 template <class T>
@@ -420,7 +430,7 @@ void launch_pcg_finish(hipStream_t s, const int4 *chunks, int nchunk, const Cell
   template void launch_bsr_diag_inv<T>(hipStream_t, const int4 *, int, const int *, const T *, T *);              \
   template void launch_pcg_init<T>(hipStream_t, const int4 *, int, int, const int *, int, const CellState *,       \
                                    CgState *, const int *, const int *, const T *, const double *,                \
-                                   const CgVectors<T> &, double *, double, int, int);                             \
+                                   const CgVectors<T> &, double *, double, double, int, int);                     \
   template void launch_pcg_iteration<T>(hipStream_t, const int4 *, int, int, const CgState *, CgState *, int,     \
                                         const int *, const int *, const T *, const CgVectors<T> &, double *,      \
                                         double, int, int);                                                        \

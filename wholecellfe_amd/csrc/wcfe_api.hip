@@ -1522,7 +1522,7 @@ template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, in
   const int lanes = ctx->opt.pcg_lanes > 0 ? ctx->opt.pcg_lanes : 4;
   CgState *half[2] = {b.cg, b.cg + b.ncell};
   launch_pcg_init<T>(s, b.chunks, cb, nch, cl, ncl, b.cs, half[0], b.row_ptr, b.bcol, static_cast<const T *>(b.Kb),
-                     b.r, v, b.partials, rtol, maxit, lanes);
+                     b.r, v, b.partials, rtol, ctx->opt.lin_rtol, maxit, lanes);
   int cur = 0;
   std::vector<int> ids = sc.cell < 0 ? b.csr_cells_h : std::vector<int>{sc.cell};
   std::vector<CgState> h(b.ncell);
@@ -1545,7 +1545,7 @@ template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, in
       if (g.done || g.iters <= 0 || !(g.rr0 > 0.0) || !(g.rr > 0.0)) continue;
       const double red = g.rr / g.rr0;  // squared residual reduction so far
       const double per = std::log(red) / g.iters;
-      const double left = per < 0.0 ? std::log(rtol * rtol / red) / per : 1e9;
+      const double left = per < 0.0 ? std::log(g.tol * g.tol / red) / per : 1e9;
       need = std::max(need, left);
     }
     if (all) break;

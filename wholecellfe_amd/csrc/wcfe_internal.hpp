@@ -171,6 +171,7 @@ enum CgMode : int { CG_INIT = 0, CG_STEP = 1 };
 // per-cell conjugate-gradient state (one entry per cell; used by sparse cells)
 struct CgState {
   double rz, rr, rr0, pq, alpha, beta;
+  double tol;             // this solve's relative tolerance (set at its start)
   int iters, done, breakdown, total_iters;
   int chunk_off, nchunk;  // the cell's chunks in the chunk list / partials
   int pad_[2];
@@ -184,11 +185,13 @@ template <class T>
 void launch_bsr_diag_inv(hipStream_t s, const int4 *chunks, int nchunk, const int *diag_pos, const T *val, T *dinv);
 // chunks [cbase, cbase + nchunk) of the batch's chunk list
 // lanes: 4, 8 or 16 lanes per block row in the SpMV
+// rtol: cg_rtol; lin_rtol: the refinement's target (a later sweep's solve
+// stops at max(cg_rtol, 0.1 lin_rtol / the cell's relative residual so far))
 template <class T>
 void launch_pcg_init(hipStream_t s, const int4 *chunks, int cbase, int nchunk, const int *cells, int ncell,
                      const CellState *cs, CgState *cg, const int *row_ptr, const int *col, const T *val,
-                     const double *rhs, const CgVectors<T> &v, double *partials, double rtol, int max_iter,
-                     int lanes);
+                     const double *rhs, const CgVectors<T> &v, double *partials, double rtol, double lin_rtol,
+                     int max_iter, int lanes);
 // one iteration: the update (step: with the scalars formed from the last
 // SpMV's partials, gin -> gout; else with gin's), then the SpMV
 template <class T>
