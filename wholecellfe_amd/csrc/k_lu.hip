@@ -1488,13 +1488,23 @@ __device__ __forceinline__ void lds_barrier() {
 // of the static-pivoting factorisation (Li = I, Ui = A_bb^-1):
 //   forward   y_b = r_b - sum_{j=b-bw}^{b-1} L_bj y_j
 //   backward  x_b = Ui_b (y_b - sum_{j=b+1}^{b+bw} U_bj x_j)
-// Thread (row r, part q) = (tid / 8, tid % 8) owns columns 8q .. 8q+7 of row
-// r of every 64-row block.  The band blocks and the diagonal inverse of the
+// Thread (row r, part q) = (tid / 8, tid % 8) owns eight columns of row r
+// of every 64-row block (scol: 4q .. 4q+3 and 32+4q .. 32+4q+3 in FP32).  The band blocks and the diagonal inverse of the
 // next block are loaded into registers while the current block is solved
 // (they do not depend on the solution), so each block step costs its
 // arithmetic and two LDS barriers, not a memory round trip.
 // ---------------------------------------------------------------------------
 constexpr int BAND_THREADS = 512;
+
+// Column of a sweep thread's e-th entry (e = 0..7) of a 64-wide block row.
+// FP32: columns 4q .. 4q+3 and 32+4q .. 32+4q+3 (two float4), so each load
+// instruction of a wave takes whole 128-byte row segments (with 8q .. 8q+7 a
+// wave's instruction would take every other 16 bytes of 256 and the next
+// one the rest); FP64: 8q .. 8q+7.
+template <class T> __device__ __forceinline__ int scol(int q, int e) {
+  if constexpr (sizeof(T) == 4) return e < 4 ? 4 * q + e : 28 + 4 * q + e;
+  else return 8 * q + e;
+}
 
 template <class T, int BWM>
 struct BandRegs {
@@ -1512,18 +1522,18 @@ __device__ __forceinline__ void band_load(BandRegs<T, BWM> &R, const CellParams 
   for (int u = 0; u < BWM; ++u) {
     const int j = FWD ? b - bw + u : b + 1 + u;
     if (ok && u < bw && j >= 0 && j < p.nblk) {
-      const T *src = A + (long long)(b * NB + r) * ld + j * NB + q * 8;
+      const T *src = A + (long long)(b * NB + r) * ld + j * NB;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) R.a[u][e] = src[e];
+      for (int e = 0; e < 8; ++e) R.a[u][e] = src[scol<T>(q, e)];
     } else {
 #pragma unroll
       for (int e = 0; e < 8; ++e) R.a[u][e] = T(0);
     }
   }
   if (ok && !FWD) {  // forward: the block-LDU lower factor has identity diagonal blocks
-    const T *src = Dinv + dinv_off(p, b) + 4096 + r * 64 + q * 8;
+    const T *src = Dinv + dinv_off(p, b) + 4096 + r * 64;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) R.d[e] = src[e];
+    for (int e = 0; e < 8; ++e) R.d[e] = src[scol<T>(q, e)];
   }
 }
 
@@ -1548,14 +1558,15 @@ template <class T> __device__ __forceinline__ T sum8(T v) {
   }
 }
 
-// 8 consecutive LDS values (32-byte aligned) as two 16-byte reads (FP32)
-template <class T> __device__ __forceinline__ void lds8(T (&v)[8], const T *p) {
+// a sweep thread's 8 entries (columns scol(q, e)) of a 64-entry LDS vector
+// block at p: two 16-byte reads (FP32)
+template <class T> __device__ __forceinline__ void lds8(T (&v)[8], const T *p, int q) {
   if constexpr (sizeof(T) == 4) {
-    const float4 a = *reinterpret_cast<const float4 *>(p), b = *reinterpret_cast<const float4 *>(p + 4);
+    const float4 a = *reinterpret_cast<const float4 *>(p + 4 * q), b = *reinterpret_cast<const float4 *>(p + 32 + 4 * q);
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
   } else {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = p[e];
+    for (int e = 0; e < 8; ++e) v[e] = p[8 * q + e];
   }
 }
 
@@ -1568,7 +1579,7 @@ __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb
     const int j = FWD ? b - bw + u : b + 1 + u;
     if (u < bw && j >= 0 && j < nb) {
       T yv[8];
-      lds8<T>(yv, y + j * NB + q * 8);
+      lds8<T>(yv, y + j * NB, q);
 #pragma unroll
       for (int e = 0; e < 8; ++e) part += R.a[u][e] * yv[e];
     }
@@ -1583,7 +1594,7 @@ __device__ __forceinline__ void band_step(const BandRegs<T, BWM> &R, T *y, T *sb
   lds_barrier();
   T t = T(0);
   T sv[8];
-  lds8<T>(sv, sb + q * 8);
+  lds8<T>(sv, sb, q);
 #pragma unroll
   for (int e = 0; e < 8; ++e) t += R.d[e] * sv[e];
   t = sum8(t);
@@ -1619,18 +1630,18 @@ __device__ __forceinline__ void job_load(BandRegs<T, BWM> &R, const SweepJob &s,
 #pragma unroll
   for (int u = 0; u < BWM; ++u) {
     if (s.row >= 0 && u < s.nj) {
-      const T *src = A + (long long)(s.rb(s.row) * NB + r) * ld + s.rb(s.j0 + u) * NB + q * 8;
+      const T *src = A + (long long)(s.rb(s.row) * NB + r) * ld + s.rb(s.j0 + u) * NB;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) R.a[u][e] = src[e];
+      for (int e = 0; e < 8; ++e) R.a[u][e] = src[scol<T>(q, e)];
     } else {
 #pragma unroll
       for (int e = 0; e < 8; ++e) R.a[u][e] = T(0);
     }
   }
   if (s.row >= 0 && !FWD) {
-    const T *src = Dinv + dinv_off(p, s.rb(s.row)) + 4096 + r * 64 + q * 8;
+    const T *src = Dinv + dinv_off(p, s.rb(s.row)) + 4096 + r * 64;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) R.d[e] = src[e];
+    for (int e = 0; e < 8; ++e) R.d[e] = src[scol<T>(q, e)];
   }
 }
 
@@ -1656,7 +1667,7 @@ __device__ __forceinline__ void sweep_chain(const CellParams &p, const T *A, con
         for (int u = 0; u < BWM; ++u) {
           if (a.row >= 0 && u < a.nj) {
             T yv[8];
-            lds8<T>(yv, y + a.rb(a.j0 + u) * NB + q * 8);
+            lds8<T>(yv, y + a.rb(a.j0 + u) * NB, q);
 #pragma unroll
             for (int e = 0; e < 8; ++e) part += R[i].a[u][e] * yv[e];
           }
@@ -1672,7 +1683,7 @@ __device__ __forceinline__ void sweep_chain(const CellParams &p, const T *A, con
             lds_barrier();
             T t = T(0);
             T sv[8];
-            lds8<T>(sv, sb + q * 8);
+            lds8<T>(sv, sb, q);
 #pragma unroll
             for (int e = 0; e < 8; ++e) t += R[i].d[e] * sv[e];
             t = sum8(t);
