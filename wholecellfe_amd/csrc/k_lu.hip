@@ -715,6 +715,26 @@ __device__ __forceinline__ void mmneg_tile_bswz(const T *X, const T *Y, int I, i
     acc = mfma16(x, y, acc);
   }
 }
+// The four tiles (I, 0..3) and the tile (0, J0) of the same product, one
+// wave: each k step reads two X rows and four Y columns for five MFMAs on
+// five independent accumulators (one tile at a time waits on its LDS reads
+// every four MFMAs).  Each tile's MFMA sequence is mmneg_tile_bswz's.
+template <class T, int P>
+__device__ __forceinline__ void mmneg_row_bswz(const T *X, const T *Y, int I, int J0, typename Acc4<T>::type (&acc)[4],
+                                               typename Acc4<T>::type &acc0) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll 2
+  for (int kk = 0; kk < 64; kk += 4) {
+    const T x = T(-1) * X[(I * 16 + lr) * P + kk + lk];
+    const T x0 = T(-1) * X[lr * P + kk + lk];
+    T y[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) y[j] = Y[bswz(kk + lk, j * 16 + lr)];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = mfma16(x, y[j], acc[j]);
+    acc0 = mfma16(x0, J0 == 1 ? y[1] : J0 == 2 ? y[2] : y[3], acc0);
+  }
+}
 template <class T, int PX>
 __device__ __forceinline__ void mma_64_bswz(const T *xs, const T *ys, typename Acc4<T>::type acc[2][2], T sign) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1123,15 +1143,15 @@ __device__ __forceinline__ void lu_step_item(const CellParams *cp, CellState *cs
     };
     T *Ut = A + v.off(s0, bi, ld);  // U_{s0,bi} (symmetric cells)
     auto side = [&](int kb) {
-      if (kb == 0) {
+      if (kb == 0) {  // wave w: row w of the sub-blocks and sub-block (0, w)
+        acc_t d[4], d0;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          const int u = w + 3 * q, I = u >> 2, J = u & 3;
-          acc_t d;
-          acc16_load<T, PA>(d, xs + I * 16 * PA + J * 16);
-          mmneg_tile_bswz<T, P>(ls, us, I, J, d);
-          acc16_store<T, PA>(d, xs + I * 16 * PA + J * 16);
-        }
+        for (int J = 0; J < 4; ++J) acc16_load<T, PA>(d[J], xs + w * 16 * PA + J * 16);
+        acc16_load<T, PA>(d0, xs + w * 16);
+        mmneg_row_bswz<T, P>(ls, us, w, w, d, d0);
+#pragma unroll
+        for (int J = 0; J < 4; ++J) acc16_store<T, PA>(d[J], xs + w * 16 * PA + J * 16);
+        acc16_store<T, PA>(d0, xs + w * 16);
         return;
       }
       if (!own) return;
