@@ -287,23 +287,39 @@ template <class T> struct Acc4;
 template <> struct Acc4<float> { typedef floatx4 type; };
 template <> struct Acc4<double> { typedef doublex4 type; };
 
+// Software pipeline of the LDS-operand MFMA loops: the operands of k step
+// kk + 4 are read while the MFMAs of step kk run (two register stages; the
+// scheduling barriers keep the compiler from sinking the reads to their
+// first use, which costs one LDS latency per step).  The MFMA sequence of
+// every accumulator is unchanged.
+#define WCFE_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+
 // acc += sign * X(64x64, LDS pitch PX) Y(64x64, LDS pitch PY) for this wave's quadrant
 template <class T, int PX, int PY>
 __device__ __forceinline__ void mma_64(const T *xs, const T *ys, typename Acc4<T>::type acc[2][2], T sign) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-#pragma unroll 4
+  T a[2][2], b[2][2];
+  auto load = [&](int s, int kk) {
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti) a[s][ti] = xs[(wr + ti * 16 + lr) * PX + kk + lk];
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) b[s][tj] = ys[(kk + lk) * PY + wc + tj * 16 + lr];
+  };
+  load(0, 0);
+#pragma unroll
   for (int kk = 0; kk < 64; kk += 4) {
-    T a[2], b[2];
+    const int s = (kk >> 2) & 1;
+    if (kk + 4 < 64) load(s ^ 1, kk + 4);
+    WCFE_SCHED_FENCE();
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti) a[ti] = sign * xs[(wr + ti * 16 + lr) * PX + kk + lk];
+    for (int ti = 0; ti < 2; ++ti) {
+      const T x = sign * a[s][ti];
 #pragma unroll
-    for (int tj = 0; tj < 2; ++tj) b[tj] = ys[(kk + lk) * PY + wc + tj * 16 + lr];
-#pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-      for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = mfma16(a[ti], b[tj], acc[ti][tj]);
+      for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = mfma16(x, b[s][tj], acc[ti][tj]);
+    }
+    WCFE_SCHED_FENCE();
   }
 }
 
@@ -350,11 +366,41 @@ template <int M, class T> __device__ __forceinline__ T masked(T v, int r, int c)
 template <class T, int PA, int MX, int MY>
 __device__ __forceinline__ void mm16(const T *X, const T *Y, typename Acc4<T>::type &acc, T s) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  T xv[4], yv[4];  // every operand read before the four chained MFMAs
 #pragma unroll
-  for (int kk = 0; kk < 16; kk += 4) {
-    const T x = masked<MX>(X[lr * PA + kk + lk], lr, kk + lk);
-    const T y = masked<MY>(Y[(kk + lk) * PA + lr], kk + lk, lr);
+  for (int q = 0; q < 4; ++q) {
+    xv[q] = X[lr * PA + 4 * q + lk];
+    yv[q] = Y[(4 * q + lk) * PA + lr];
+  }
+  WCFE_SCHED_FENCE();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const T x = masked<MX>(xv[q], lr, 4 * q + lk);
+    const T y = masked<MY>(yv[q], 4 * q + lk, lr);
     acc = mfma16(s * x, y, acc);
+  }
+}
+// acc[j] += s X Y_j for three 16 x 16 products with the same X (one wave):
+// the three chains interleaved, every operand read first.  Each
+// accumulator's MFMA sequence is mm16's.
+template <class T, int PA>
+__device__ __forceinline__ void mm16x3(const T *X, const T *Y0, const T *Y1, const T *Y2,
+                                       typename Acc4<T>::type (&acc)[3], T s) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+  const T *Y[3] = {Y0, Y1, Y2};
+  T xv[4], yv[3][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    xv[q] = X[lr * PA + 4 * q + lk];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) yv[j][q] = Y[j][(4 * q + lk) * PA + lr];
+  }
+  WCFE_SCHED_FENCE();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const T x = s * xv[q];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) acc[j] = mfma16(x, yv[j][q], acc[j]);
   }
 }
 template <class T, int PA>
@@ -527,8 +573,13 @@ __device__ __forceinline__ int invert_diag_ov(T *a, T *Li, T *Ui, T tau, Pre pre
     }
     __syncthreads();
     post(kb);
+    const int j0 = kb == 0 ? 1 : 0, j1 = kb <= 1 ? 2 : 1, j2 = kb <= 2 ? 3 : 2;  // the columns j != kb
     if (wave < 3) {  // row i: A'_ik = -A_ik P in place, then A_ij += A'_ik A_kj (old row kb)
       const int i = wave + (wave >= kb ? 1 : 0);
+      acc_t o[3];
+      acc16_load<T, PA>(o[0], sub(a, i, j0));
+      acc16_load<T, PA>(o[1], sub(a, i, j1));
+      acc16_load<T, PA>(o[2], sub(a, i, j2));
       acc_t acc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[r] = T(0);
@@ -539,21 +590,16 @@ __device__ __forceinline__ int invert_diag_ov(T *a, T *Li, T *Ui, T tau, Pre pre
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (j == kb) continue;
-        acc16_load<T, PA>(acc, sub(a, i, j));
-        mm16<T, PA, M_FULL, M_FULL>(sub(a, i, kb), sub(a, kb, j), acc, T(1));
-        acc16_store<T, PA>(acc, sub(a, i, j));
-      }
+      mm16x3<T, PA>(sub(a, i, kb), sub(a, kb, j0), sub(a, kb, j1), sub(a, kb, j2), o, T(1));
+      acc16_store<T, PA>(o[0], sub(a, i, j0));
+      acc16_store<T, PA>(o[1], sub(a, i, j1));
+      acc16_store<T, PA>(o[2], sub(a, i, j2));
     } else {  // wave 3: the new row kb, P A_kj, kept until the next window
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int j = q + (q >= kb ? 1 : 0);
+      for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int r = 0; r < 4; ++r) rowk[q][r] = T(0);
-        mm16<T, PA, M_FULL, M_FULL>(sub(a, kb, kb), sub(a, kb, j), rowk[q], T(1));
-      }
+      mm16x3<T, PA>(sub(a, kb, kb), sub(a, kb, j0), sub(a, kb, j1), sub(a, kb, j2), rowk, T(1));
     }
     __syncthreads();
   }
@@ -740,17 +786,26 @@ __device__ __forceinline__ void mma_64_bswz(const T *xs, const T *ys, typename A
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-#pragma unroll 4
+  T a[2][2], b[2][2];
+  auto load = [&](int s, int kk) {
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti) a[s][ti] = xs[(wr + ti * 16 + lr) * PX + kk + lk];
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) b[s][tj] = ys[bswz(kk + lk, wc + tj * 16 + lr)];
+  };
+  load(0, 0);
+#pragma unroll
   for (int kk = 0; kk < 64; kk += 4) {
-    T a[2], b[2];
+    const int s = (kk >> 2) & 1;
+    if (kk + 4 < 64) load(s ^ 1, kk + 4);
+    WCFE_SCHED_FENCE();
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti) a[ti] = sign * xs[(wr + ti * 16 + lr) * PX + kk + lk];
+    for (int ti = 0; ti < 2; ++ti) {
+      const T x = sign * a[s][ti];
 #pragma unroll
-    for (int tj = 0; tj < 2; ++tj) b[tj] = ys[bswz(kk + lk, wc + tj * 16 + lr)];
-#pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-      for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = mfma16(a[ti], b[tj], acc[ti][tj]);
+      for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = mfma16(x, b[s][tj], acc[ti][tj]);
+    }
+    WCFE_SCHED_FENCE();
   }
 }
 
