@@ -769,7 +769,7 @@ template <class T, int P>
 __device__ __forceinline__ void mmneg_row_bswz(const T *X, const T *Y, int I, int J0, typename Acc4<T>::type (&acc)[4],
                                                typename Acc4<T>::type &acc0) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-#pragma unroll 2
+#pragma unroll 4
   for (int kk = 0; kk < 64; kk += 4) {
     const T x = T(-1) * X[(I * 16 + lr) * P + kk + lk];
     const T x0 = T(-1) * X[lr * P + kk + lk];
