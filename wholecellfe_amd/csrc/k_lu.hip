@@ -754,7 +754,7 @@ __device__ __forceinline__ void tile_to_lds_bswz(const T (&t)[TileShape<T>::E], 
 template <class T, int P>
 __device__ __forceinline__ void mmneg_tile_bswz(const T *X, const T *Y, int I, int J, typename Acc4<T>::type &acc) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-#pragma unroll 4
+#pragma unroll
   for (int kk = 0; kk < 64; kk += 4) {
     const T x = T(-1) * X[(I * 16 + lr) * P + kk + lk];
     const T y = Y[bswz(kk + lk, J * 16 + lr)];
