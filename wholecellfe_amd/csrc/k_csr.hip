@@ -322,7 +322,10 @@ __global__ __launch_bounds__(SCALAR_THREADS) void k_pcg_scalars(const int *cells
   const bool skip = mode == CG_INIT ? cs[c].ir_done != 0 : g.done != 0;
   if (skip) {
     if (threadIdx.x == 0) {
-      if (mode == CG_INIT) g.done = 1;
+      if (mode == CG_INIT) {  // no solve for this cell: no iterations either
+        g.done = 1;
+        g.iters = 0;
+      }
       gout[c] = g;
     }
     return;

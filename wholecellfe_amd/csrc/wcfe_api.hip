@@ -601,6 +601,10 @@ struct Batch {
   std::vector<std::vector<int>> kpos_h;
   double *rp = nullptr, *dp = nullptr;
   bool solve_wg = false;  // static pivoting: one workgroup per cell walks its band (narrow bands)
+  // PCG iterations the batch's slowest sparse cell took in refinement sweep
+  // i of the last Newton step (-1: none yet): the first batch of sweep i's
+  // next solve
+  int pcg_hint[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
   int lu_depth = 4, lu_groups = 4;  // the LU schedule in effect (options, or their automatic choice)
   int max_bw = 0;                    // widest block band of the dense cells
   // boundary and body forces (cells with bterm != 0)
@@ -1504,14 +1508,19 @@ template <class T> void enqueue_factor(wcfe_ctx *ctx, Batch &b, const Scope &sc)
 // Block-Jacobi PCG on the sparse cells of the scope for the refinement
 // right-hand side b.r; the correction goes to (or is added to) b.delta.
 // Convergence is checked on the host after batches of iterations (each check
-// drains the stream): the first after 16, then as many as the slowest cell's
-// residual reduction so far predicts it still needs (8 .. 128, a quarter
-// more than predicted); iterations after a cell's convergence return at once.
+// drains the stream): the first after as many as the same refinement sweep
+// took in the batch's last Newton step (at least 8; one when that sweep had
+// nothing to solve, as the sweeps after every cell's convergence; 16 at
+// first and for single-cell scopes), then as many as the slowest cell's residual reduction so far
+// predicts it still needs (8 .. 128, a quarter more than predicted);
+// iterations after a cell's convergence return at once, so the batch sizes
+// change only the number of checks, not the result.
 // In a batch the first update uses the scalars of the start or of the last
 // check, each later update forms its own from the SpMV before it (the scalar
 // state alternates between the two halves of b.cg), and the batch ends with
 // the scalars of its last SpMV, which the host reads.
-template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, int accumulate) {
+template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, int sweep) {
+  const int accumulate = sweep > 0;
   const int *cl;
   int cb, nch, ncl;
   if (!scope_sparse(b, sc, cb, nch, cl, ncl)) return;
@@ -1526,7 +1535,8 @@ template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, in
   int cur = 0;
   std::vector<int> ids = sc.cell < 0 ? b.csr_cells_h : std::vector<int>{sc.cell};
   std::vector<CgState> h(b.ncell);
-  int batch = 16;
+  int *hint = sc.cell < 0 && sweep < 8 ? &b.pcg_hint[sweep] : nullptr;
+  int batch = !hint || *hint < 0 ? 16 : *hint == 0 ? 1 : std::min(std::max(8, *hint), maxit);
   for (int it = 0; it < maxit; it += batch) {
     for (int k = 0; k < batch; ++k) {
       launch_pcg_iteration<T>(s, b.chunks, cb, nch, half[cur], half[1 - cur], k > 0, b.row_ptr, b.bcol,
@@ -1552,10 +1562,13 @@ template <class T> void enqueue_pcg(wcfe_ctx *ctx, Batch &b, const Scope &sc, in
     batch = (int)std::min(128.0, std::max(8.0, std::ceil(1.25 * need)));
   }
   launch_pcg_finish<T>(s, b.chunks + cb, nch, b.cs, v.x, b.delta, accumulate);
+  int most = 0;
   for (int c : ids) {
     ctx->cg_iters_max = std::max(ctx->cg_iters_max, h[c].iters);
     ctx->cg_breakdowns += h[c].breakdown;
+    most = std::max(most, h[c].iters);
   }
+  if (hint) *hint = most;
 }
 
 // The refinement residual for a mesh batch: r = -G - K d from the FP64
@@ -1594,7 +1607,7 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
           launch_perm_scatter(s, b.node_cell, b.cp, b.cs, b.kpos, b.dp, b.delta, sc.node_begin, sc.node_end, it > 0);
         }
       }
-      enqueue_pcg<T>(ctx, b, sc, it > 0);
+      enqueue_pcg<T>(ctx, b, sc, it);
     }
     PhaseScope ph(ctx, WCFE_PH_REFINE);
     const BndView bv = bnd_view(ctx, b);
