@@ -1615,6 +1615,18 @@ template <class T> void enqueue_solve_refine(wcfe_ctx *ctx, Batch &b, const Scop
     launch_matvec_ir(s, v, b.node_cell, b.adj_start, b.adj, b.Ke64, b.G, b.delta, b.r, sc.node_begin, sc.node_end,
                      b.cs, bv);
     launch_cell_norm(s, b.cp, b.cs, sc.cells, sc.ncell, b.r, 1, 1, ctx->opt.lin_rtol, b.norm_work());
+    // a batch with sparse cells is not captured (its solves check their
+    // convergence on the host): from the second sweep on, stop once every
+    // cell of the scope has met lin_rtol -- the sweeps left would launch
+    // only idle kernels (and a solve's probing iteration)
+    if (b.any_csr && it >= 1 && it + 1 < std::max(1, ctx->opt.max_refine)) {
+      std::vector<CellState> h(b.ncell);
+      HIPCHECK(hipMemcpyAsync(h.data(), b.cs, sizeof(CellState) * b.ncell, hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      bool all = true;
+      for (int c = sc.cell < 0 ? 0 : sc.cell; c < (sc.cell < 0 ? b.ncell : sc.cell + 1); ++c) all = all && h[c].ir_done;
+      if (all) break;
+    }
   }
 }
 
