@@ -166,7 +166,7 @@ constexpr int AUTO_GROUPS_MAX = 4;
 hipError_t lu_prepare_attributes(size_t max_lds);
 
 // ---- sparse path: block-CSR systems and batched PCG (k_csr.hip) ----
-constexpr int CG_ROWS = 256;  // nodes per chunk (one thread per node)
+constexpr int CG_ROWS = 1024;  // nodes per chunk (one thread per node)
 enum CgMode : int { CG_INIT = 0, CG_STEP = 1 };
 // per-cell conjugate-gradient state (one entry per cell; used by sparse cells)
 struct CgState {
